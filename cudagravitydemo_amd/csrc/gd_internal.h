@@ -1,0 +1,146 @@
+/*
+ * gd_internal.h -- shared by the engine's HIP translation units.
+ *
+ * Error convention: like the reference's CUDA_CHECK (macros.h:8-17), an error
+ * prints where it happened and exits with -1.  Tests switch that off with
+ * gd_set_exit_on_error(0); the first error is then kept for gd_last_error()
+ * and the failing entry point returns without doing its work.
+ */
+#ifndef GD_INTERNAL_H
+#define GD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "gravity_types.h"
+#include "gravity_ext.h"
+
+/* ---- error handling ---------------------------------------------------- */
+void gd_fail(const char* file, int line, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int gd_has_error(void);
+
+#define GD_FAIL(...) gd_fail(__FILE__, __LINE__, __VA_ARGS__)
+#define GD_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t gd_err_ = (call);                                              \
+        if (gd_err_ != hipSuccess)                                                \
+            gd_fail(__FILE__, __LINE__, "HIP error %d (%s) in %s", (int)gd_err_,  \
+                    hipGetErrorString(gd_err_), #call);                           \
+    } while (0)
+/* launch check: a bad launch configuration is reported at the call site */
+#define GD_LAUNCHED() GD_HIP(hipGetLastError())
+
+/* ---- sizes ------------------------------------------------------------- */
+#define GD_WAVE 64
+#define GD_MAX_LEVEL 21
+#define GD_LEVELS 22                 /* levels 0..21 */
+
+/* bbox: partial min/max blocks of the engine's own reduction */
+#define GD_BBOX_BLOCKS 1024
+#define GD_BBOX_THREADS 256
+
+/* octree build: one tile = 256 threads x 4 bodies */
+#define GD_BUILD_THREADS 256
+#define GD_BUILD_ITEMS 4
+#define GD_BUILD_TILE (GD_BUILD_THREADS * GD_BUILD_ITEMS)
+
+/* radix sort: one tile = 256 threads x 16 keys, 8-bit digits */
+#define GD_SORT_THREADS 256
+#define GD_SORT_ITEMS 16
+#define GD_SORT_TILE (GD_SORT_THREADS * GD_SORT_ITEMS)
+#define GD_SORT_PASSES 8
+#define GD_SORT_BINS 256
+
+/* exclusive scan (prefixSum): one tile = 256 threads x 8 items */
+#define GD_SCAN_THREADS 256
+#define GD_SCAN_ITEMS 8
+#define GD_SCAN_TILE (GD_SCAN_THREADS * GD_SCAN_ITEMS)
+
+static inline int gd_div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+/*
+ * Device-resident summary of the current tree, written by the build kernels
+ * so that no host round trip is needed inside a step.
+ */
+typedef struct {
+    int levelOffsets[GD_LEVELS];
+    int levelCount[GD_LEVELS];
+    int deepestLevel;
+    int totalNodes;
+    int overflow;                    /* 1 when the node array was too small */
+    int zeroMassNodes;               /* nodes whose mass summed to zero */
+    worldBox box;                    /* quantisation box of this step */
+    float3 bmin, bmax;               /* global bbox corners */
+} gd_tree_info;
+
+/*
+ * The sorter handle the library hands out.  Its first member is the public
+ * radixSorter, so a radixSorter* from sorterCreate points at one of these.
+ */
+typedef struct {
+    radixSorter pub;
+    uint64_t* keysTmp;               /* ping-pong key buffer, capacity */
+    uint32_t* valsTmp;               /* ping-pong value buffer, capacity */
+    uint32_t* hist;                  /* [8][256] digit counts, then exclusive offsets */
+    uint32_t* tileCounter;           /* [8] tile tickets, one per pass */
+    unsigned long long* status;      /* [maxTiles][256] look-back status words */
+    int maxTiles;
+    uint32_t epoch;                  /* tag of the next pass's status words */
+} gd_sorter;
+
+/*
+ * The builder handle the library hands out (first member: the public
+ * treeBuilder).  Besides the reference's flag/offset arrays it owns the
+ * device scratch of the single-pass octree build and the compact walk arrays.
+ */
+typedef struct {
+    treeBuilder pub;
+    int maxTiles;                    /* build tiles at capacity */
+    uint32_t* tileCounts;            /* [22][maxTiles] node starts per level and tile */
+    uint32_t* tileOffsets;           /* [22][maxTiles] exclusive scan of tileCounts */
+    uint32_t* levelTotals;           /* [22] */
+    gd_tree_info* dinfo;             /* device */
+    gd_tree_info* hinfo;             /* pinned host mirror */
+    float3* bboxPartMin;             /* [GD_BBOX_BLOCKS] */
+    float3* bboxPartMax;
+    /* prefixSum scratch */
+    uint32_t* scanTiles;             /* tile sums */
+    /* compact walk arrays, maxNodes entries each */
+    float4* wcom;                    /* centre of mass + mass */
+    int2* wlink;                     /* .x first child (>= 0) or -(first body + 1) for a leaf,
+                                        .y = (children or bodies) << 5 | level */
+    int maxNodes;
+} gd_builder;
+
+static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }
+static inline gd_builder* gd_builder_of(treeBuilder* b) { return (gd_builder*)b; }
+
+/* ---- launchers shared between translation units (all on the null stream) */
+void gd_launch_bbox(const float4* pos, int n, gd_builder* b, float3* outMin, float3* outMax);
+void gd_launch_morton_dev(const float4* pos, int n, const gd_tree_info* info, uint64_t* keys,
+                          uint32_t* idx);
+void gd_launch_reorder(const float4* posIn, const float3* velIn, float4* posOut, float3* velOut,
+                       const uint32_t* idx, int n);
+void gd_launch_reorder_lo(const float4* posIn, const float3* velIn, const float3* loIn,
+                          float4* posOut, float3* velOut, float3* loOut, const uint32_t* idx,
+                          int n);
+void gd_sort_pairs(gd_sorter* s, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
+                   uint32_t* vout, int n);
+void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* sortedPos, int n,
+                     node* nodes, int maxNodes);
+void gd_exclusive_scan_u32(gd_builder* b, const uint32_t* in, uint32_t* out, int n);
+
+/* ---- wave helpers ------------------------------------------------------ */
+#ifdef __HIP_DEVICE_COMPILE__
+#define GD_DEVICE_ONLY 1
+#endif
+
+__device__ __forceinline__ unsigned gd_lane() { return __lane_id(); }
+__device__ __forceinline__ unsigned long long gd_lanemask_lt() {
+    unsigned l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+#endif

@@ -1,0 +1,431 @@
+/*
+ * gd_walk.hip -- tree walks.
+ *
+ * 1. Parity walk (calculateAcceleration, tree.cu:562-676).  The reference
+ *    walks one 32-body group per 32-thread warp: a LIFO stack shared by the
+ *    group, a node is opened when any valid body fails size^2 < theta^2 r^2 on
+ *    a non-leaf (size = bbox x extent / 2^level), otherwise every body adds
+ *    the node's softened monopole.  A 64-wide wavefront here carries two such
+ *    groups, one per half, each with its own LDS stack and its own 32-bit
+ *    slice of the wavefront ballot, so each group's interaction list is the
+ *    reference's.  The arithmetic follows the oracle's operation order
+ *    (explicit fmaf, IEEE division and square root, no contraction).
+ *
+ * 2. Production walk (gd_walk).  One 64-body group per wavefront (consecutive
+ *    bodies in Morton order).  The group's bounding box is tested against up
+ *    to 64 nodes at once, one node per lane: a node is accepted when it is a
+ *    leaf or size^2 < theta^2 d^2, d being the distance from the group box to
+ *    the node's centre of mass and size = bbox cube extent / 2^level; opened
+ *    nodes push their (contiguous) children on an LDS stack.  Accepted nodes
+ *    go to an LDS interaction list; when it fills, all 64 lanes evaluate it
+ *    with broadcast LDS reads (compute bound: ~20 FP32 FLOP per entry).  Per
+ *    body, each flush's partial sum is added to the total with Kahan
+ *    compensation.  A group whose stack would overflow its LDS share is
+ *    re-walked by the same code with its stack in global memory.
+ */
+#include "gd_internal.h"
+
+#include <math.h>
+
+/* ---------------------------------------------------------------------------
+ * Parity walk
+ * ------------------------------------------------------------------------- */
+#define GD_PAR_STACK 168          /* 8 x 21, as tree.cu:588 */
+#define GD_PAR_SLOTS 8            /* 32-body groups per 256-thread block */
+
+__global__ __launch_bounds__(256) void k_walk_parity(const float4* __restrict__ pos,
+                                                     const node* __restrict__ nodes,
+                                                     const float3* __restrict__ bmin,
+                                                     const float3* __restrict__ bmax,
+                                                     float3* __restrict__ acc, float eps2,
+                                                     float G, float theta, int n) {
+    __shared__ int sStack[GD_PAR_SLOTS][GD_PAR_STACK];
+    const unsigned lane = gd_lane();
+    const int half = (int)(lane >> 5);
+    const int slot = (int)(threadIdx.x >> 6) * 2 + half;
+    int* stack = sStack[slot];
+    const long long g = (long long)blockIdx.x * GD_PAR_SLOTS + slot;
+    const long long i = g * 32 + (lane & 31u);
+    const bool valid = i < n;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (valid) {
+        const float4 p = pos[i];
+        px = p.x; py = p.y; pz = p.z;
+    }
+    const float ext = bmax->x - bmin->x;
+    const float th2 = theta * theta;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+
+    int top = -1;
+    {
+        const int4* r = reinterpret_cast<const int4*>(nodes);
+        const int4 w1 = r[1], w2 = r[2], w3 = r[3];
+        const int ch[8] = {w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, w3.x};
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (ch[c] != -1) stack[++top] = ch[c];   /* every lane of the half writes the same value */
+    }
+    if (g * 32 >= n) top = -1;   /* slot past the last group */
+
+    while (__any(top >= 0)) {
+        const bool active = top >= 0;
+        bool want = false;
+        float dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f, mass = 0.f;
+        int ch[8];
+        if (active) {
+            const int idx = stack[top];
+            const int4* r = reinterpret_cast<const int4*>(nodes + idx);
+            const int4 w1 = r[1], w2 = r[2], w3 = r[3];
+            const float4 cm = nodes[idx].massData;
+            ch[0] = w1.y; ch[1] = w1.z; ch[2] = w1.w; ch[3] = w2.x;
+            ch[4] = w2.y; ch[5] = w2.z; ch[6] = w2.w; ch[7] = w3.x;
+            const int count = w1.x, level = w3.y;
+            dx = px - cm.x;
+            dy = py - cm.y;
+            dz = pz - cm.z;
+            d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            mass = cm.w;
+            const float size = ldexpf(ext, -level);   /* exact: ext / 2^level */
+            int kids = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) kids += ch[c] != -1;
+            const bool leaf = (count <= 1) || (kids == 0);
+            const bool accept = (size * size) < th2 * d2;
+            want = !(accept || !valid || leaf);
+        }
+        const unsigned long long bal = __ballot(want);
+        const unsigned mine = (unsigned)(bal >> (32 * half));
+        if (active) {
+            --top;
+            if (mine) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (ch[c] != -1) stack[++top] = ch[c];
+            } else {
+                const float r2 = d2 + eps2;
+                const float mag = -(G * mass) / r2;
+                const float dist = sqrtf(r2);
+                ax = __builtin_fmaf(mag, dx / dist, ax);
+                ay = __builtin_fmaf(mag, dy / dist, ay);
+                az = __builtin_fmaf(mag, dz / dist, az);
+            }
+        }
+    }
+    if (valid) acc[i] = make_float3(ax, ay, az);
+}
+
+extern "C" void calculateAcceleration(const float4* positionMassArray, treeState tree,
+                                      float3* accelVals, float antiSingularitySquared,
+                                      float GVal, float theta, int starCount) {
+    if (starCount <= 0) return;
+    const int groups = gd_div_up(starCount, 32);
+    k_walk_parity<<<gd_div_up(groups, GD_PAR_SLOTS), 256>>>(
+        positionMassArray, tree.nodes, tree.boundingData[2], tree.boundingData[3], accelVals,
+        antiSingularitySquared, GVal, theta, starCount);
+    GD_LAUNCHED();
+}
+
+/* ---------------------------------------------------------------------------
+ * Production walk
+ * ------------------------------------------------------------------------- */
+#define GD_FW_WAVES 4
+#define GD_FW_STACK 2048          /* LDS stack entries per wavefront */
+#define GD_FW_BIG_STACK 12288     /* global fallback: > 22 levels x 512 pushes */
+#define GD_FW_LIST 128            /* interaction-list entries per wavefront */
+
+struct gd_kahan3 {
+    float x, y, z, cx, cy, cz;
+};
+
+__device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+/* Evaluate list[0..cnt) for one body; returns the partial sums. */
+__device__ __forceinline__ void gd_eval_list(const float4* __restrict__ list, int cnt, float px,
+                                             float py, float pz, float eps2, float& sx, float& sy,
+                                             float& sz, float& sp) {
+    float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+    int e = 0;
+    for (; e + 2 <= cnt; e += 2) {
+        const float4 c0 = list[e];
+        const float4 c1 = list[e + 1];
+        const float dx0 = c0.x - px, dy0 = c0.y - py, dz0 = c0.z - pz;
+        const float dx1 = c1.x - px, dy1 = c1.y - py, dz1 = c1.z - pz;
+        const float r0 = __builtin_fmaf(dx0, dx0, __builtin_fmaf(dy0, dy0, __builtin_fmaf(dz0, dz0, eps2)));
+        const float r1 = __builtin_fmaf(dx1, dx1, __builtin_fmaf(dy1, dy1, __builtin_fmaf(dz1, dz1, eps2)));
+        const float i0 = __builtin_amdgcn_rsqf(r0);
+        const float i1 = __builtin_amdgcn_rsqf(r1);
+        const float m0 = c0.w * i0, m1 = c1.w * i1;
+        const float f0 = m0 * (i0 * i0), f1 = m1 * (i1 * i1);
+        ax = __builtin_fmaf(f0, dx0, ax); ay = __builtin_fmaf(f0, dy0, ay); az = __builtin_fmaf(f0, dz0, az);
+        ax = __builtin_fmaf(f1, dx1, ax); ay = __builtin_fmaf(f1, dy1, ay); az = __builtin_fmaf(f1, dz1, az);
+        ph += m0;
+        ph += m1;
+    }
+    if (e < cnt) {
+        const float4 c0 = list[e];
+        const float dx0 = c0.x - px, dy0 = c0.y - py, dz0 = c0.z - pz;
+        const float r0 = __builtin_fmaf(dx0, dx0, __builtin_fmaf(dy0, dy0, __builtin_fmaf(dz0, dz0, eps2)));
+        const float i0 = __builtin_amdgcn_rsqf(r0);
+        const float m0 = c0.w * i0;
+        const float f0 = m0 * (i0 * i0);
+        ax = __builtin_fmaf(f0, dx0, ax); ay = __builtin_fmaf(f0, dy0, ay); az = __builtin_fmaf(f0, dz0, az);
+        ph += m0;
+    }
+    sx = ax; sy = ay; sz = az; sp = ph;
+}
+
+template <bool GLOBAL_STACK>
+__global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_fast(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info,
+    float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta, int n,
+    unsigned long long* __restrict__ stats, int* __restrict__ overflowList,
+    int* __restrict__ overflowCount, int* __restrict__ bigStacks) {
+    __shared__ int sStack[GLOBAL_STACK ? 1 : GD_FW_WAVES][GLOBAL_STACK ? 1 : GD_FW_STACK];
+    __shared__ float4 sList[GLOBAL_STACK ? 1 : GD_FW_WAVES][GD_FW_LIST];
+    const unsigned lane = gd_lane();
+    const int wave = GLOBAL_STACK ? 0 : (int)(threadIdx.x >> 6);
+    long long g;
+    int* stack;
+    int cap;
+    if (GLOBAL_STACK) {
+        g = overflowList[blockIdx.x];
+        stack = bigStacks + (size_t)blockIdx.x * GD_FW_BIG_STACK;
+        cap = GD_FW_BIG_STACK;
+    } else {
+        g = (long long)blockIdx.x * GD_FW_WAVES + wave;
+        stack = sStack[wave];
+        cap = GD_FW_STACK;
+    }
+    float4* list = sList[wave];
+    if (g * 64 >= n) return;   /* wave-uniform */
+    const long long i = g * 64 + lane;
+    const bool valid = i < n;
+    const float4 p = pos[valid ? i : g * 64];
+    const float px = p.x, py = p.y, pz = p.z;
+
+    /* group bounding box */
+    float lx = px, ly = py, lz = pz, hx = px, hy = py, hz = pz;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, off)); hx = fmaxf(hx, __shfl_xor(hx, off));
+        ly = fminf(ly, __shfl_xor(ly, off)); hy = fmaxf(hy, __shfl_xor(hy, off));
+        lz = fminf(lz, __shfl_xor(lz, off)); hz = fmaxf(hz, __shfl_xor(hz, off));
+    }
+    const float ext = info->box.extent;
+    const float th2 = theta * theta;
+    const unsigned long long lt = gd_lanemask_lt();
+
+    gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float phs = 0.f, phc = 0.f;
+    unsigned long long nInter = 0, nOpen = 0;
+    int listCount = 0;
+    int top = 1;
+    if (lane == 0) stack[0] = 0;
+    if (GLOBAL_STACK) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    bool overflowed = false;
+
+    while (top > 0) {
+        const int take = top < 64 ? top : 64;
+        const int base = top - take;
+        const bool have = (int)lane < take;
+        int idx = 0;
+        float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+        int2 lk = make_int2(0, 0);
+        bool accept = false, open = false;
+        if (have) {
+            idx = stack[base + lane];
+            c = wcom[idx];
+            lk = wlink[idx];
+            const int level = lk.y & 31;
+            const float size = ldexpf(ext, -level);
+            const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
+            const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
+            const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
+            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool leaf = lk.x < 0;
+            accept = leaf || (size * size < th2 * d2);
+            open = !accept;
+        }
+        /* all of this wave's LDS reads of the popped entries are done before pushes */
+        __builtin_amdgcn_wave_barrier();
+        top = base;
+        const unsigned long long accBal = __ballot(accept);
+        const int nacc = __popcll(accBal);
+        if (listCount + nacc > GD_FW_LIST) {
+            float sx, sy, sz, sp;
+            gd_eval_list(list, listCount, px, py, pz, eps2, sx, sy, sz, sp);
+            gd_kahan_add(tot.x, tot.cx, sx);
+            gd_kahan_add(tot.y, tot.cy, sy);
+            gd_kahan_add(tot.z, tot.cz, sz);
+            gd_kahan_add(phs, phc, sp);
+            nInter += (unsigned long long)listCount;
+            listCount = 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (accept) list[listCount + __popcll(accBal & lt)] = c;
+        listCount += nacc;
+        /* push children of opened nodes */
+        const int nk = open ? (lk.y >> 5) : 0;
+        int incl = nk;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += o;
+        }
+        const int total = __shfl(incl, 63);
+        nOpen += (unsigned long long)__popcll(__ballot(open));
+        if (top + total > cap) {
+            overflowed = true;
+            break;
+        }
+        if (open) {
+            const int at = top + incl - nk;
+            for (int k = 0; k < nk; ++k) stack[at + k] = lk.x + k;
+        }
+        top += total;
+        if (GLOBAL_STACK) {
+            /* pushes by some lanes are popped by others: make them visible */
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (overflowed) {
+        if (lane == 0) {
+            if (GLOBAL_STACK) {
+                atomicOr(&stats[2], 1ull);   /* reported as an error by the host */
+            } else {
+                const int slot = atomicAdd(overflowCount, 1);
+                overflowList[slot] = (int)g;
+            }
+        }
+        return;
+    }
+    if (listCount > 0) {
+        float sx, sy, sz, sp;
+        gd_eval_list(list, listCount, px, py, pz, eps2, sx, sy, sz, sp);
+        gd_kahan_add(tot.x, tot.cx, sx);
+        gd_kahan_add(tot.y, tot.cy, sy);
+        gd_kahan_add(tot.z, tot.cz, sz);
+        gd_kahan_add(phs, phc, sp);
+        nInter += (unsigned long long)listCount;
+    }
+    if (valid) {
+        acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
+        if (phi) phi[i] = -G * phs;
+    }
+    if (stats && lane == 0) {
+        const int nvalid = (int)((n - g * 64) < 64 ? (n - g * 64) : 64);
+        atomicAdd(&stats[0], nInter * (unsigned long long)nvalid);
+        atomicAdd(&stats[1], nOpen);
+    }
+}
+
+/* Walk scratch: stats[4], overflow count, overflow list, big stacks. */
+struct gd_walk_scratch {
+    unsigned long long* stats;
+    int* overflowCount;
+    int* overflowList;
+    int* bigStacks;
+    int listCap;
+    int bigSlots;
+};
+
+static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, 0, 0};
+
+/* optional HIP-event timing of the walk kernel (null stream) */
+static int g_walkTiming = 0;
+static hipEvent_t g_walkEv[2] = {nullptr, nullptr};
+static int g_walkTimed = 0;
+
+static void gd_walk_ensure(int groups) {
+    if (!g_walk.stats) {
+        GD_HIP(hipMalloc(&g_walk.stats, 4 * sizeof(unsigned long long)));
+        GD_HIP(hipMalloc(&g_walk.overflowCount, sizeof(int)));
+        GD_HIP(hipMemset(g_walk.stats, 0, 4 * sizeof(unsigned long long)));
+        GD_HIP(hipMemset(g_walk.overflowCount, 0, sizeof(int)));
+    }
+    if (groups > g_walk.listCap) {
+        if (g_walk.overflowList) GD_HIP(hipFree(g_walk.overflowList));
+        GD_HIP(hipMalloc(&g_walk.overflowList, sizeof(int) * (size_t)groups));
+        g_walk.listCap = groups;
+    }
+}
+
+__global__ void k_walk_reset(unsigned long long* stats, int* overflowCount) {
+    if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) *overflowCount = 0;
+}
+
+extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi,
+                        float eps2, float G, float theta, int starCount, gd_walk_stats* stats) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_walk: tree not initialised");
+        return;
+    }
+    if (starCount <= 0) return;
+    gd_builder* b = gd_builder_of(tree->builder);
+    const int groups = gd_div_up(starCount, 64);
+    gd_walk_ensure(groups);
+    if (gd_has_error()) return;
+    k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.overflowCount);
+    GD_LAUNCHED();
+    if (g_walkTiming) {
+        if (!g_walkEv[0]) {
+            GD_HIP(hipEventCreate(&g_walkEv[0]));
+            GD_HIP(hipEventCreate(&g_walkEv[1]));
+        }
+        GD_HIP(hipEventRecord(g_walkEv[0], nullptr));
+    }
+    k_walk_fast<false><<<gd_div_up(groups, GD_FW_WAVES), 64 * GD_FW_WAVES>>>(
+        pos, b->wcom, b->wlink, b->dinfo, acc, phi, eps2, G, theta, starCount, g_walk.stats,
+        g_walk.overflowList, g_walk.overflowCount, nullptr);
+    GD_LAUNCHED();
+    if (g_walkTiming) {
+        GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
+        g_walkTimed = 1;
+    }
+    /* fallback for groups whose stack outgrew LDS */
+    int over = 0;
+    GD_HIP(hipMemcpy(&over, g_walk.overflowCount, sizeof(int), hipMemcpyDeviceToHost));
+    if (over > 0) {
+        const int chunk = 4096;
+        if (g_walk.bigSlots < (over < chunk ? over : chunk)) {
+            if (g_walk.bigStacks) GD_HIP(hipFree(g_walk.bigStacks));
+            g_walk.bigSlots = over < chunk ? over : chunk;
+            GD_HIP(hipMalloc(&g_walk.bigStacks,
+                             sizeof(int) * (size_t)g_walk.bigSlots * GD_FW_BIG_STACK));
+        }
+        for (int s = 0; s < over; s += g_walk.bigSlots) {
+            const int m = (over - s) < g_walk.bigSlots ? (over - s) : g_walk.bigSlots;
+            k_walk_fast<true><<<m, 64>>>(pos, b->wcom, b->wlink, b->dinfo, acc, phi, eps2, G,
+                                        theta, starCount, g_walk.stats, g_walk.overflowList + s,
+                                        g_walk.overflowCount, g_walk.bigStacks);
+            GD_LAUNCHED();
+        }
+    }
+    if (over > 0 || stats) {
+        unsigned long long h[4];
+        GD_HIP(hipMemcpy(h, g_walk.stats, sizeof(h), hipMemcpyDeviceToHost));
+        if (h[2]) GD_FAIL("gd_walk: a group's traversal stack exceeded %d entries", GD_FW_BIG_STACK);
+        if (!stats) return;
+        stats->interactions = h[0];
+        stats->opened = h[1];
+        stats->overflowGroups = (unsigned long long)over;
+    }
+}
+
+extern "C" void gd_walk_timing(int enable) { g_walkTiming = enable ? 1 : 0; }
+
+extern "C" float gd_walk_kernel_ms(void) {
+    if (!g_walkTimed) return -1.0f;
+    float ms = 0.f;
+    GD_HIP(hipEventSynchronize(g_walkEv[1]));
+    GD_HIP(hipEventElapsedTime(&ms, g_walkEv[0], g_walkEv[1]));
+    return ms;
+}

@@ -1,0 +1,372 @@
+"""GPU parity tests: the HIP engine through its C-ABI against the CPU oracle.
+
+Bars: Morton keys, the sort permutation, the node array and the integrator
+are bit-exact; the reference walk's accelerations are within 1e-5 relative
+(vector norm) of the oracle's; engine-only results are stated against the
+FP64 direct sum.  Parity is checked per step on identical inputs.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cudagravitydemo_amd import Simulation, ics, load_library
+from cudagravitydemo_amd.engine import DeviceBuffer, from_device, to_device
+from cudagravitydemo_amd._lib import WorldBox, check
+from oracle import binding as ob
+from tests.octree_check import check_invariants
+
+pytestmark = pytest.mark.gpu
+
+WALK_TOL = 1e-5
+
+
+def rel_err(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+
+
+def assert_tree_equal(sim, ref):
+    total = sim.tree_summary()[0]
+    assert total == ref["total"], (total, ref["total"])
+    nodes = sim.nodes_raw(total).view(ob.NODE_DTYPE)
+    for f in ob.NODE_FIELDS:
+        assert np.array_equal(nodes[f], ref["nodes"][f]), f
+    offs, counts, deepest = sim.levels()
+    assert deepest == ref["deepest"]
+    last = deepest + 1 if deepest < 21 else 21
+    assert offs[:last + 1] == list(ref["level_offsets"][:last + 1])
+    assert counts[:last + 1] == list(ref["level_count"][:last + 1])
+    return nodes
+
+
+def check_build_and_walk(pos, vel, eps, theta):
+    print(f"case N={pos.shape[0]} eps={eps} theta={theta}", flush=True)
+    with Simulation(pos, vel, eps=eps) as sim:
+        sim.build_tree()
+        sim.walk_reference(theta)
+        ref = ob.build_tree(pos, vel)
+        lo, hi = sim.bbox()
+        assert np.array_equal(lo, ref["bmin"]) and np.array_equal(hi, ref["bmax"])
+        assert np.array_equal(sim.keys_unsorted(), ref["keys"])
+        assert np.array_equal(sim.keys_sorted(), ref["keys_sorted"])
+        assert np.array_equal(sim.permutation(), ref["perm"])
+        assert np.array_equal(sim.positions(), ref["pos"])
+        assert np.array_equal(sim.velocities(), ref["vel"])
+        nodes = assert_tree_equal(sim, ref)
+        acc_ref, _, _ = ob.walk(ref["pos"], ref["nodes_full"], ref["bmin"], ref["bmax"],
+                                np.float32(eps * eps), 1.0, theta)
+        acc = sim.accelerations()
+        err = rel_err(acc, acc_ref)
+        assert err.max() <= WALK_TOL, err.max()
+        return sim, ref, nodes, acc, acc_ref
+
+
+def test_disk16k_keys_perm_tree_walk_bitexact():
+    pos, vel = ics.exponential_disk(16384, seed=1234)
+    _, _, nodes, acc, acc_ref = check_build_and_walk(pos, vel, 0.01, 0.5)
+    check_invariants(nodes, 16384)
+    # same operation order on both sides: the accelerations agree to the bit
+    assert np.array_equal(acc, acc_ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 33, 1000, 4097])
+def test_ragged_sizes(n):
+    pos, vel = ics.uniform_cube(n, seed=n)
+    check_build_and_walk(pos, vel, 0.02, 0.7)
+
+
+def test_duplicates_and_coincident_bodies():
+    pos, vel = ics.uniform_cube(3000, seed=4)
+    pos[100:140] = pos[100]
+    pos[2000:2003] = pos[7]
+    check_build_and_walk(pos, vel, 0.01, 0.5)
+    pos = np.tile(np.array([[0.5, -0.25, 2.0, 0.1]], np.float32), (50, 1))
+    check_build_and_walk(pos, np.zeros((50, 3), np.float32), 0.01, 0.5)
+
+
+def test_theta_zero_walk():
+    pos, vel = ics.uniform_cube(2000, seed=8)
+    check_build_and_walk(pos, vel, 0.01, 0.0)
+
+
+def test_reference_steps_per_step_parity():
+    """100 steps of config 0 (16384 bodies, theta 0.5, eps 0.01, dt 1e-3),
+    each compared with the oracle step run on the GPU's own pre-step state."""
+    pos, vel = ics.exponential_disk(16384, seed=1234)
+    eps, theta, dt = 0.01, 0.5, 1e-3
+    with Simulation(pos, vel, eps=eps) as sim:
+        for step in range(100):
+            p0, v0 = sim.positions(), sim.velocities()
+            ref, acc_ref, p1, v1 = ob.step(p0, v0, eps, 1.0, theta, dt)
+            sim.step_reference(dt, theta)
+            assert np.array_equal(sim.keys_unsorted(), ref["keys"]), step
+            assert np.array_equal(sim.permutation(), ref["perm"]), step
+            assert_tree_equal(sim, ref)
+            acc = sim.accelerations()
+            assert rel_err(acc, acc_ref).max() <= WALK_TOL, step
+            if np.array_equal(acc, acc_ref):
+                assert np.array_equal(sim.positions(), p1), step
+                assert np.array_equal(sim.velocities(), v1), step
+
+
+def test_radix_sort_large_with_ties():
+    lib = load_library()
+    lib.gd_set_exit_on_error(0)
+    rng = np.random.default_rng(5)
+    n = (1 << 21) + 123
+    keys = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+    keys[rng.integers(0, n, n // 4)] = keys[:n // 4]
+    keys[: n // 8] = np.sort(keys[: n // 8])  # a nearly sorted run
+    vals = rng.integers(0, 1 << 32, n, dtype=np.uint32)
+    s = lib.sorterCreate(n)
+    check(lib)
+    kin, vin = to_device(lib, keys), to_device(lib, vals)
+    kout, vout = DeviceBuffer(lib, keys.nbytes), DeviceBuffer(lib, vals.nbytes)
+    try:
+        for _ in range(2):  # second run reuses the look-back status words
+            lib.radixSortPairs(s, kin.ptr, kout.ptr, vin.ptr, vout.ptr, n)
+            check(lib)
+            ks = from_device(lib, kout.ptr, np.uint64, (n,))
+            vs = from_device(lib, vout.ptr, np.uint32, (n,))
+            order = np.argsort(keys, kind="stable")
+            assert np.array_equal(ks, keys[order])
+            assert np.array_equal(vs, vals[order])
+        # inputs untouched
+        assert np.array_equal(from_device(lib, kin.ptr, np.uint64, (n,)), keys)
+        # smaller count on the same sorter
+        m = 777
+        lib.radixSortPairs(s, kin.ptr, kout.ptr, vin.ptr, vout.ptr, m)
+        check(lib)
+        order = np.argsort(keys[:m], kind="stable")
+        assert np.array_equal(from_device(lib, vout.ptr, np.uint32, (m,)), vals[:m][order])
+        lib.radixSortPairs(s, kin.ptr, kout.ptr, vin.ptr, vout.ptr, n + 1)
+        assert b"capacity" in lib.gd_last_error()
+        lib.gd_clear_error()
+    finally:
+        for b in (kin, vin, kout, vout):
+            b.close()
+        lib.sorterDestroyer(s)
+        lib.gd_set_exit_on_error(1)
+
+
+def test_prefix_sum():
+    lib = load_library()
+    rng = np.random.default_rng(9)
+    n = 3_000_007
+    flags = (rng.random(n) < 0.3).astype(np.uint32)
+    flags[::1000] = 7
+    b = lib.sumCreate(n)
+    check(lib)
+    fin = to_device(lib, flags)
+    out = DeviceBuffer(lib, flags.nbytes)
+    try:
+        for m in (n, 1, 5000):
+            lib.prefixSum(b, fin.ptr, out.ptr, m)
+            check(lib)
+            got = from_device(lib, out.ptr, np.uint32, (m,))
+            exp = np.concatenate([[0], np.cumsum(flags[:m], dtype=np.uint64)[:-1]]).astype(np.uint32)
+            assert np.array_equal(got, exp)
+    finally:
+        fin.close()
+        out.close()
+        lib.sumDestroyer(b)
+
+
+class _Builder(C.Structure):
+    _fields_ = [("tempStorage", C.c_void_p), ("tempStorageBytes", C.c_size_t),
+                ("capacity", C.c_int), ("flagsPing", C.c_void_p), ("flagsPong", C.c_void_p),
+                ("offsetsPing", C.c_void_p), ("offsetsPong", C.c_void_p)]
+
+
+def test_level_api_drives_reference_loop():
+    """identifyNodesAtLevel / prefixSum / createNodes / updateTreeParticles /
+    updateTreeMass driven exactly as basicGravity.cu:229-319 does give the
+    same node array as buildTree."""
+    lib = load_library()
+    pos, vel = ics.exponential_disk(8192, seed=31)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.build_tree()
+        total = sim.tree_summary()[0]
+        want = sim.nodes_raw(total).view(ob.NODE_DTYPE).copy()
+        st = sim.state
+        n = sim.n
+        nodes2 = DeviceBuffer(lib, 80 * st.maxNodes)
+        b = _Builder.from_address(st.builder)
+        root = np.zeros(1, ob.NODE_DTYPE)
+        root["count"] = n
+        root["child"] = -1
+        root["parent"] = -1
+        lib.gd_memcpy_h2d(nodes2.ptr, root.ctypes.data, 80)
+        flags, prev_flags = b.flagsPing, b.flagsPong
+        offs, prev_offs = b.offsetsPing, b.offsetsPong
+        base, prev_base = 1, 0
+        level_off, level_cnt, deepest = [0], [1], 0
+        for level in range(1, 22):
+            lib.identifyNodesAtLevel(st.mortonOut, level, n, flags)
+            lib.prefixSum(st.builder, flags, offs, n)
+            lib.createNodes(st.mortonOut, flags, prev_flags, offs, prev_offs, nodes2.ptr, level,
+                            base, prev_base, n, st.maxNodes)
+            check(lib)
+            lf = from_device(lib, flags + 4 * (n - 1), np.uint32, (1,))[0]
+            lo = from_device(lib, offs + 4 * (n - 1), np.uint32, (1,))[0]
+            count = int(lf + lo)
+            if count == 0:
+                break
+            lib.updateTreeParticles(nodes2.ptr, base, count, n)
+            prev_base = base
+            level_off.append(base)
+            level_cnt.append(count)
+            base += count
+            flags, prev_flags = prev_flags, flags
+            offs, prev_offs = prev_offs, offs
+            deepest += 1
+        for j in range(deepest, -1, -1):
+            lib.updateTreeMass(nodes2.ptr, sim.pos, j, level_cnt[j], level_off[j])
+        check(lib)
+        got = from_device(lib, nodes2.ptr, np.uint8, (total, 80)).view(ob.NODE_DTYPE)
+        nodes2.close()
+        assert base == total
+        for f in ob.NODE_FIELDS:
+            assert np.array_equal(got[f], want[f]), f
+
+
+def test_world_box_and_morton_api():
+    lib = load_library()
+    pos, vel = ics.exponential_disk(5000, seed=3)
+    d = to_device(lib, pos)
+    parts = [DeviceBuffer(lib, 12 * 256) for _ in range(2)] + [DeviceBuffer(lib, 12) for _ in range(2)]
+    keys, idx = DeviceBuffer(lib, 8 * 5000), DeviceBuffer(lib, 4 * 5000)
+    try:
+        box = WorldBox()
+        lib.createWorldBox(d.ptr, 5000, *[p.ptr for p in parts], C.byref(box))
+        check(lib)
+        lo, hi = ob.bbox(pos)
+        rb = ob.world_box(lo, hi)
+        assert (box.minX, box.minY, box.minZ, box.extent, box.scale) == \
+            (rb.minX, rb.minY, rb.minZ, rb.extent, rb.scale)
+        lib.mortonEncode(d.ptr, keys.ptr, idx.ptr, 5000, box)
+        check(lib)
+        k_ref, i_ref = ob.morton(pos, rb)
+        assert np.array_equal(from_device(lib, keys.ptr, np.uint64, (5000,)), k_ref)
+        assert np.array_equal(from_device(lib, idx.ptr, np.uint32, (5000,)), i_ref)
+    finally:
+        for b in parts + [d, keys, idx]:
+            b.close()
+
+
+def test_integrator_bitexact():
+    lib = load_library()
+    rng = np.random.default_rng(3)
+    n = 100_003
+    pos = rng.standard_normal((n, 4)).astype(np.float32)
+    vel = rng.standard_normal((n, 3)).astype(np.float32)
+    acc = rng.standard_normal((n, 3)).astype(np.float32)
+    p2, v2 = ob.integrate(pos, vel, acc, np.float32(1e-3))
+    dp, dv, da = to_device(lib, pos), to_device(lib, vel), to_device(lib, acc)
+    try:
+        lib.positionUpdater(dp.ptr, dv.ptr, da.ptr, n, 1e-3)
+        check(lib)
+        assert np.array_equal(from_device(lib, dp.ptr, np.float32, (n, 4)), p2)
+        assert np.array_equal(from_device(lib, dv.ptr, np.float32, (n, 3)), v2)
+    finally:
+        for b in (dp, dv, da):
+            b.close()
+
+
+def test_judge_matches_oracle_fp64():
+    pos, vel = ics.exponential_disk(20000, seed=17)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.build_tree()
+        sample = ics.subsample(20000, 257, seed=4)
+        got = sim.direct_sum_fp64(sample)
+        exp = ob.direct_sum_fp64(sim.positions(), sample, 0.01 ** 2)
+        assert np.abs(got - exp).max() <= 1e-12 * np.abs(exp).max()
+
+
+def test_production_walk_accuracy_disk16k():
+    pos, vel = ics.exponential_disk(16384, seed=1234)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.build_tree_async()
+        sample = ics.subsample(16384, 2048, seed=99)
+        exact = sim.direct_sum_fp64(sample)
+        med = {}
+        for theta in (0.3, 0.5, 0.7):
+            st = sim.walk(theta, stats=True)
+            e = rel_err(sim.accelerations()[sample], exact)
+            med[theta] = float(np.median(e))
+            print(f"theta={theta} median={med[theta]:.3e} p99={np.quantile(e, 0.99):.3e} "
+                  f"interactions/body={st['interactions'] / 16384:.0f} "
+                  f"overflow={st['overflow_groups']}", flush=True)
+        assert med[0.3] < med[0.5] < med[0.7]
+        assert med[0.5] < 1e-3
+
+
+def test_production_walk_theta_zero_is_direct_sum():
+    pos, vel = ics.uniform_cube(3000, seed=2)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.build_tree_async()
+        sim.walk(0.0)
+        sample = np.arange(0, 3000, 7, dtype=np.int32)
+        exact = sim.direct_sum_fp64(sample)
+        e = rel_err(sim.accelerations()[sample], exact)
+        assert e.max() < 1e-4
+
+
+def test_hilo_positions_keep_low_bits():
+    # a slow uniform drift far from the origin: plain FP32 loses the increments
+    n = 4096
+    rng = np.random.default_rng(1)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = 1000.0 + rng.random((n, 3)).astype(np.float32)
+    pos[:, 3] = 1e-30  # negligible gravity
+    vel = np.full((n, 3), 3e-5, np.float32)
+    steps, dt = 200, 1e-2
+    with Simulation(pos, vel, eps=1.0, hilo=True) as sim:
+        for _ in range(steps):
+            sim.build_tree_async()
+            sim.walk(0.5)
+            sim.integrate(dt)
+        hi = sim.positions()[:, :3].astype(np.float64)
+        lo = sim.positions_lo().astype(np.float64)
+        perm_pos = hi + lo
+    exact = np.sort((pos[:, :3].astype(np.float64) + steps * dt * 3e-5).ravel())
+    got = np.sort(perm_pos.ravel())
+    assert np.abs(got - exact).max() < 1e-9 * 1000
+
+
+def test_diagnostics_fp64():
+    pos, vel = ics.plummer(50000, seed=3)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        d = sim.diagnostics()
+    m = pos[:, 3].astype(np.float64)
+    v = vel.astype(np.float64)
+    assert abs(d["kinetic"] - 0.5 * (m * (v * v).sum(1)).sum()) < 1e-12
+    assert abs(d["mass"] - m.sum()) < 1e-12
+    assert abs(d["px"] - (m * v[:, 0]).sum()) < 1e-12
+
+
+def test_large_build_properties():
+    """2^21 bodies (the merger's size): size-independent properties."""
+    pos, vel = ics.disk_merger(1 << 20, seed=2024)
+    n = pos.shape[0]
+    with Simulation(pos, vel, eps=0.005) as sim:
+        sim.build_tree()
+        ks = sim.keys_sorted()
+        assert np.all(ks[1:] >= ks[:-1])
+        perm = sim.permutation()
+        assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.uint32))
+        assert np.array_equal(sim.keys_unsorted()[perm], ks)
+        total, overflow, zero = sim.tree_summary()
+        assert overflow == 0 and zero == 0
+        offs, counts, deepest = sim.levels()
+        assert sum(counts[:deepest + 1]) == total
+        nodes = sim.nodes_raw(total).view(ob.NODE_DTYPE)
+        assert nodes[0]["count"] == n
+        assert abs(float(nodes[0]["mass"][3]) - pos[:, 3].astype(np.float64).sum()) < 1e-4
+        # every level's bodies are a subset of the level above's, leaves cover all bodies
+        leaf = (nodes["child"] == -1).all(1)
+        assert nodes["count"][leaf].sum() == n
+        # topology of a contiguous slice against the oracle's level-wise build
+        ref = ob.build_tree(pos, vel)
+        for f in ob.NODE_FIELDS:
+            assert np.array_equal(nodes[f], ref["nodes"][f]), f
