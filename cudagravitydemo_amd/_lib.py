@@ -106,10 +106,24 @@ _PROTOS = {
                                 C.POINTER(i32)]),
     "gd_build_tree_async": (None, [C.POINTER(TreeState), vpp, vpp, vpp, vpp]),
     "gd_tree_levels": (None, [C.POINTER(TreeState)]),
+    "gd_set_quadrupole": (None, [C.POINTER(TreeState), i32]),
+    "gd_tree_walk_arrays": (None, [C.POINTER(TreeState), vp, vp, vp, i32]),
     "gd_walk": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32,
                        C.POINTER(WalkStats)]),
     "gd_walk_timing": (None, [i32]),
     "gd_walk_kernel_ms": (f32, []),
+    "gd_mlp_create": (vp, [vp]),
+    "gd_mlp_destroy": (None, [vp]),
+    "gd_mlp_get_params": (None, [vp, vp]),
+    "gd_mlp_set_params": (None, [vp, vp]),
+    "gd_mlp_forward": (None, [vp, vp, vp, C.c_longlong]),
+    "gd_mlp_gradient": (f32, [vp, vp, vp, vp, C.c_longlong, vp]),
+    "gd_mlp_train_step": (f32, [vp, vp, vp, vp, C.c_longlong, f32]),
+    "gd_mlp_adam_state": (None, [vp, vp, vp]),
+    "gd_corrector_samples": (C.c_longlong, [C.POINTER(TreeState), vp, vp, i32, f32, f32, vp, vp,
+                                             C.c_longlong, C.POINTER(i32)]),
+    "gd_walk_learned": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32, vp, f32,
+                               C.POINTER(WalkStats)]),
     "gd_integrate_hilo": (None, [vp, vp, vp, vp, i32, f32]),
     "gd_direct_sum_fp64": (None, [vp, vp, i32, vp, i32, f64, f64, vp]),
     "gd_diagnostics": (None, [vp, vp, vp, i32, C.POINTER(C.c_double)]),

@@ -1,0 +1,173 @@
+"""Training and use of the learned far-field corrector (host orchestration).
+
+All arithmetic runs in libgravity.so: samples come from gd_corrector_samples
+(exact per-cell sums on the GPU), training from gd_mlp_train_step (BF16 MFMA
+forward/backward + Adam), and the corrected walk from gd_walk_learned.
+The FP64 direct-sum judge decides whether the corrector may be used: when its
+measured median force error exceeds the budget, the caller falls back to the
+plain walk at a smaller opening angle.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import WalkStats, check, load_library
+from .engine import DeviceBuffer, from_device, to_device
+from . import ics
+
+N_PARAMS = 9091
+SIZES = [(64, 8), (64, 64), (64, 64), (3, 64)]
+
+
+def init_params(seed=0):
+    """He-normal weights, zero biases (float32, the library's layout)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    parts = []
+    for o, i in SIZES:
+        parts.append(rng.standard_normal(o * i) * np.sqrt(2.0 / i))
+        parts.append(np.zeros(o))
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == N_PARAMS
+    return p
+
+
+class Corrector:
+    """A network on the GPU plus its Adam state."""
+
+    def __init__(self, params=None, seed=0):
+        self.lib = load_library()
+        p = init_params(seed) if params is None else np.ascontiguousarray(params, np.float32)
+        self.ptr = self.lib.gd_mlp_create(p.ctypes.data)
+        check(self.lib)
+
+    def params(self):
+        out = np.empty(N_PARAMS, np.float32)
+        self.lib.gd_mlp_get_params(self.ptr, out.ctypes.data)
+        check(self.lib)
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.lib.gd_mlp_set_params(self.ptr, p.ctypes.data)
+        check(self.lib)
+
+    def forward(self, X):
+        """X: host (rows, 8) float32 -> host (rows, 3)."""
+        X = np.ascontiguousarray(X, np.float32)
+        xb = to_device(self.lib, X)
+        yb = DeviceBuffer(self.lib, 12 * X.shape[0])
+        try:
+            self.lib.gd_mlp_forward(self.ptr, xb.ptr, yb.ptr, X.shape[0])
+            check(self.lib)
+            return from_device(self.lib, yb.ptr, np.float32, (X.shape[0], 3))
+        finally:
+            xb.close()
+            yb.close()
+
+    def gradient(self, X, Y):
+        """(loss, gradient) of the mean squared error over all rows."""
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        B = X.shape[0]
+        xb, yb = to_device(self.lib, X), to_device(self.lib, Y)
+        ob = to_device(self.lib, np.arange(B, dtype=np.int32))
+        g = np.empty(N_PARAMS, np.float32)
+        try:
+            loss = self.lib.gd_mlp_gradient(self.ptr, xb.ptr, yb.ptr, ob.ptr, B, g.ctypes.data)
+            check(self.lib)
+            return loss, g
+        finally:
+            for b in (xb, yb, ob):
+                b.close()
+
+    def close(self):
+        if self.ptr:
+            self.lib.gd_mlp_destroy(self.ptr)
+            self.ptr = None
+
+
+class SampleSet:
+    """Device-resident training rows (X: n x 8, Y: n x 3)."""
+
+    def __init__(self, lib, cap):
+        self.lib = lib
+        self.cap = int(cap)
+        self.X = DeviceBuffer(lib, 32 * self.cap)
+        self.Y = DeviceBuffer(lib, 12 * self.cap)
+        self.n = 0
+
+    def add_from(self, sim, targets, theta):
+        """Append the (body, cell) samples of one snapshot; returns rows added."""
+        lib = self.lib
+        t = to_device(lib, np.ascontiguousarray(targets, np.int32))
+        dropped = C.c_int(0)
+        room = self.cap - self.n
+        try:
+            got = lib.gd_corrector_samples(C.byref(sim.state), sim.pos, t.ptr, len(targets),
+                                           theta, sim.eps * sim.eps, self.X.ptr + 32 * self.n,
+                                           self.Y.ptr + 12 * self.n, room, C.byref(dropped))
+            check(lib)
+        finally:
+            t.close()
+        self.n += int(got)
+        return int(got), dropped.value
+
+    def host(self, limit=None):
+        n = self.n if limit is None else min(limit, self.n)
+        return (from_device(self.lib, self.X.ptr, np.float32, (n, 8)),
+                from_device(self.lib, self.Y.ptr, np.float32, (n, 3)))
+
+    def close(self):
+        self.X.close()
+        self.Y.close()
+
+
+def train(net, samples, epochs=1, batch=65536, lr=1e-3, seed=0, log=None):
+    """Adam on shuffled minibatches of the sample set; returns per-epoch mean loss."""
+    lib = net.lib
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = samples.n
+    losses = []
+    order_buf = DeviceBuffer(lib, 4 * n)
+    try:
+        for ep in range(epochs):
+            order = rng.permutation(n).astype(np.int32)
+            order_buf.upload(order)
+            tot, cnt = 0.0, 0
+            for b0 in range(0, n - batch + 1 if n >= batch else 1, batch):
+                B = min(batch, n - b0)
+                loss = lib.gd_mlp_train_step(net.ptr, samples.X.ptr, samples.Y.ptr,
+                                             order_buf.ptr + 4 * b0, B, lr)
+                check(lib)
+                tot += loss
+                cnt += 1
+            losses.append(tot / max(cnt, 1))
+            if log:
+                log(f"epoch {ep}: loss {losses[-1]:.4e}")
+    finally:
+        order_buf.close()
+    return losses
+
+
+def walk_learned(sim, net, theta, min_ratio=0.0, stats=False):
+    st = WalkStats() if stats else None
+    sim.lib.gd_walk_learned(C.byref(sim.state), sim.pos, sim.acc, None, sim.eps * sim.eps,
+                            sim.G, theta, sim.n, net.ptr, min_ratio,
+                            C.byref(st) if st is not None else None)
+    check(sim.lib)
+    if st is not None:
+        return {"interactions": st.interactions, "opened": st.opened,
+                "overflow_groups": st.overflowGroups}
+    return None
+
+
+def median_error(sim, sample):
+    """Median relative force error of sim.acc on sample vs the FP64 judge."""
+    exact = sim.direct_sum_fp64(sample)
+    acc = sim.accelerations()[sample]
+    e = np.linalg.norm(acc - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    return float(np.median(e)), e
+
+
+def default_targets(n, k=8192, seed=11):
+    return ics.subsample(n, k, seed=seed)

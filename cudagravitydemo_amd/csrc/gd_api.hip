@@ -237,6 +237,7 @@ void sumDestroyer(treeBuilder* sum) {
     GD_HIP(hipFree(b->scanTiles));
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
+    if (b->wquad) GD_HIP(hipFree(b->wquad));
     memset(b, 0, sizeof(*b));
     free(b);
 }
@@ -341,6 +342,36 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
     tree->sortedVelVals = tv;
     /* 5. topology and centre of mass */
     gd_build_octree(b, tree->mortonOut, *pos, n, tree->nodes, tree->maxNodes);
+}
+
+void gd_set_quadrupole(treeState* tree, int enable) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_set_quadrupole: uninitialised tree state");
+        return;
+    }
+    gd_builder* b = gd_builder_of(tree->builder);
+    b->quad = enable ? 1 : 0;
+    if (b->quad && b->quadNodes < tree->maxNodes) {
+        if (b->wquad) GD_HIP(hipFree(b->wquad));
+        GD_HIP(hipMalloc(&b->wquad, 2 * sizeof(float4) * (size_t)tree->maxNodes));
+        b->quadNodes = tree->maxNodes;
+    }
+}
+
+void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* quad, int count) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_tree_walk_arrays: uninitialised tree state");
+        return;
+    }
+    gd_builder* b = gd_builder_of(tree->builder);
+    if (count < 0 || count > b->maxNodes) {
+        GD_FAIL("gd_tree_walk_arrays: count %d outside [0, %d]", count, b->maxNodes);
+        return;
+    }
+    if (com) GD_HIP(hipMemcpy(com, b->wcom, sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
+    if (link) GD_HIP(hipMemcpy(link, b->wlink, sizeof(int2) * (size_t)count, hipMemcpyDeviceToHost));
+    if (quad && b->wquad)
+        GD_HIP(hipMemcpy(quad, b->wquad, 2 * sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
 }
 
 void gd_tree_levels(treeState* tree) {

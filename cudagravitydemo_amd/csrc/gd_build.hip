@@ -330,11 +330,54 @@ __device__ __forceinline__ gd_mass_result gd_node_mass(const node* __restrict__ 
     return r;
 }
 
+/* Second moments about the centre of mass, S = sum m (x - c)(x - c)^T, as
+ * two float4: (Sxx, Syy, Szz, Sxy), (Sxz, Syz, trace, 0).  Leaves sum over
+ * their bodies; internal nodes shift each child's moments by the parallel-axis
+ * term m_c (c_c - c)(c_c - c)^T. */
+__device__ __forceinline__ void gd_node_moments(const float4* __restrict__ pos,
+                                                const float4* __restrict__ wcom,
+                                                const float4* __restrict__ wquad, int first,
+                                                int count, const int* child, bool leaf, float4 c,
+                                                float4* q0, float4* q1) {
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    if (leaf) {
+        for (int b = 0; b < count; ++b) {
+            const float4 p = pos[first + b];
+            const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+            sxx = __builtin_fmaf(p.w * dx, dx, sxx);
+            syy = __builtin_fmaf(p.w * dy, dy, syy);
+            szz = __builtin_fmaf(p.w * dz, dz, szz);
+            sxy = __builtin_fmaf(p.w * dx, dy, sxy);
+            sxz = __builtin_fmaf(p.w * dx, dz, sxz);
+            syz = __builtin_fmaf(p.w * dy, dz, syz);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (child[k] == -1) continue;
+            const float4 cc = wcom[child[k]];
+            const float4 a = wquad[2 * child[k]];
+            const float4 b = wquad[2 * child[k] + 1];
+            const float dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
+            sxx += __builtin_fmaf(cc.w * dx, dx, a.x);
+            syy += __builtin_fmaf(cc.w * dy, dy, a.y);
+            szz += __builtin_fmaf(cc.w * dz, dz, a.z);
+            sxy += __builtin_fmaf(cc.w * dx, dy, a.w);
+            sxz += __builtin_fmaf(cc.w * dx, dz, b.x);
+            syz += __builtin_fmaf(cc.w * dy, dz, b.y);
+        }
+    }
+    *q0 = make_float4(sxx, syy, szz, sxy);
+    *q1 = make_float4(sxz, syz, sxx + syy + szz, 0.f);
+}
+
+template <bool QUAD>
 __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
                                                     const float4* __restrict__ pos,
                                                     gd_tree_info* info, int level,
                                                     float4* __restrict__ wcom,
-                                                    int2* __restrict__ wlink) {
+                                                    int2* __restrict__ wlink,
+                                                    float4* __restrict__ wquad) {
     if (info->overflow) return;
     const int cnt = info->levelCount[level];
     const int off = info->levelOffsets[level];
@@ -355,6 +398,12 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
         wcom[idx] = m.m;
         wlink[idx] = m.leaf ? make_int2(-(first + 1), (count << 5) | level)
                             : make_int2(m.firstChild, (m.nChild << 5) | level);
+        if (QUAD) {
+            float4 q0, q1;
+            gd_node_moments(pos, wcom, wquad, first, count, child, m.leaf, m.m, &q0, &q1);
+            wquad[2 * idx] = q0;
+            wquad[2 * idx + 1] = q1;
+        }
     }
     if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
 }
@@ -384,7 +433,12 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
         }
         int blocks = gd_div_up(bound, 256);
         if (blocks > 1024) blocks = 1024;
-        k_build_mass<<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink);
+        if (b->quad)
+            k_build_mass<true><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
+                                                b->wquad);
+        else
+            k_build_mass<false><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom,
+                                                 b->wlink, nullptr);
         GD_LAUNCHED();
     }
 }

@@ -112,6 +112,10 @@ typedef struct {
     int2* wlink;                     /* .x first child (>= 0) or -(first body + 1) for a leaf,
                                         .y = (children or bodies) << 5 | level */
     int maxNodes;
+    /* second moments about the COM, 2 float4 per node, when quad != 0 */
+    float4* wquad;
+    int quad;
+    int quadNodes;
 } gd_builder;
 
 static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }

@@ -24,6 +24,7 @@
  *    re-walked by the same code with its stack in global memory.
  */
 #include "gd_internal.h"
+#include "gd_mlp.h"
 
 #include <math.h>
 
@@ -144,7 +145,7 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-/* Evaluate list[0..cnt) for one body; returns the partial sums. */
+/* Monopole list[0..cnt) for one body; returns the partial sums. */
 __device__ __forceinline__ void gd_eval_list(const float4* __restrict__ list, int cnt, float px,
                                              float py, float pz, float eps2, float& sx, float& sy,
                                              float& sz, float& sp) {
@@ -179,15 +180,131 @@ __device__ __forceinline__ void gd_eval_list(const float4* __restrict__ list, in
     sx = ax; sy = ay; sz = az; sp = ph;
 }
 
-template <bool GLOBAL_STACK>
+/*
+ * Quadrupole cell list for one body.  With d = c - x (toward the cell),
+ * r^2 = |d|^2 + eps^2 and S the second moments about c:
+ *   a/G = m d / r^3 + [ (15/2) (d.S d) / r^2 - (3/2) tr S ] d / r^5 - 3 S d / r^5
+ *   phi/(-G) = m / r + (3 d.S d - r^2 tr S) / (2 r^5)
+ */
+__device__ __forceinline__ void gd_eval_quad(const float4* __restrict__ cm,
+                                             const float4* __restrict__ q0,
+                                             const float4* __restrict__ q1, int cnt, float px,
+                                             float py, float pz, float eps2, float& sx, float& sy,
+                                             float& sz, float& sp) {
+    float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+    for (int e = 0; e < cnt; ++e) {
+        const float4 c = cm[e];
+        const float4 a = q0[e];   /* Sxx Syy Szz Sxy */
+        const float4 b = q1[e];   /* Sxz Syz tr - */
+        const float dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float mi = c.w * inv;
+        const float inv5 = inv * inv2 * inv2;
+        const float sdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.w, dy, b.x * dz));
+        const float sdy = __builtin_fmaf(a.w, dx, __builtin_fmaf(a.y, dy, b.y * dz));
+        const float sdz = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, a.z * dz));
+        const float q = __builtin_fmaf(dx, sdx, __builtin_fmaf(dy, sdy, dz * sdz));
+        const float cd = __builtin_fmaf(mi, inv2, inv5 * __builtin_fmaf(7.5f * q, inv2, -1.5f * b.z));
+        const float cs = -3.f * inv5;
+        ax = __builtin_fmaf(cd, dx, __builtin_fmaf(cs, sdx, ax));
+        ay = __builtin_fmaf(cd, dy, __builtin_fmaf(cs, sdy, ay));
+        az = __builtin_fmaf(cd, dz, __builtin_fmaf(cs, sdz, az));
+        ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(3.f, q, -r2 * b.z), mi);
+    }
+    sx = ax; sy = ay; sz = az; sp = ph;
+}
+
+#define GD_FW_QLIST 64            /* quadrupole / learned cell-list entries per wavefront */
+
+/*
+ * Learned correction of the cells in a list, evaluated once per group at the
+ * group's centre g: for each cell, 8 features (unit vector to the COM, s/r,
+ * S u / (m s^2), tr S / (m s^2)) go through the network on the matrix cores
+ * (16 cells per pass, one cell per column), and the output, scaled by
+ * m s^2 / r^4, is the correction of the cell's monopole.  Cells with
+ * s/r <= minRatio are left uncorrected.  Returns the summed correction (without
+ * the factor G) in every lane.
+ */
+__device__ __forceinline__ void gd_mlp_correct(const float4* __restrict__ qc,
+                                               const float4* __restrict__ q0,
+                                               const float4* __restrict__ q1, int cnt, float gx,
+                                               float gy, float gz,
+                                               const gd_bf16x4* __restrict__ W,
+                                               const float* __restrict__ Bv, float minRatio,
+                                               float& cx, float& cy, float& cz) {
+    const unsigned l = gd_lane();
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int c0 = 0; c0 < cnt; c0 += 16) {
+        const int cell = c0 + (int)(l % 16);
+        bool ok = cell < cnt;
+        float4 c = make_float4(gx + 1.f, gy, gz, 1.f);
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 b = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (ok) {
+            const float4 cc = qc[cell];
+            ok = cc.w > 0.f;   /* a massless cell needs no correction */
+            if (ok) {
+                c = cc;
+                a = q0[cell];
+                b = q1[cell];
+            }
+        }
+        const float dx = c.x - gx, dy = c.y - gy, dz = c.z - gz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
+        const float size = b.w;
+        const float ratio = size * rinv;
+        const float k = c.w * size * size;
+        const float kinv = 1.f / k;
+        float f0, f1, f2, f3;
+        const unsigned sel = l / 16;
+        if (sel == 0) {
+            f0 = ux; f1 = uy; f2 = uz; f3 = ratio;
+        } else if (sel == 1) {
+            f0 = __builtin_fmaf(a.x, ux, __builtin_fmaf(a.w, uy, b.x * uz)) * kinv;
+            f1 = __builtin_fmaf(a.w, ux, __builtin_fmaf(a.y, uy, b.y * uz)) * kinv;
+            f2 = __builtin_fmaf(b.x, ux, __builtin_fmaf(b.y, uy, a.z * uz)) * kinv;
+            f3 = b.z * kinv;
+        } else {
+            f0 = f1 = f2 = f3 = 0.f;
+        }
+        const gd_f32x4 o = gd_mlp_infer16(W, Bv, gd_pack4(f0, f1, f2, f3));
+        if (l < 16 && ok && ratio > minRatio) {
+            const float r4inv = (rinv * rinv) * (rinv * rinv);
+            const float scale = k * r4inv;
+            sx = __builtin_fmaf(scale, o[0], sx);
+            sy = __builtin_fmaf(scale, o[1], sy);
+            sz = __builtin_fmaf(scale, o[2], sz);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off);
+        sy += __shfl_xor(sy, off);
+        sz += __shfl_xor(sz, off);
+    }
+    cx = sx; cy = sy; cz = sz;
+}
+
+/* MODE: 0 monopole, 1 quadrupole cells, 2 monopole cells + learned correction */
+template <bool GLOBAL_STACK, int MODE>
 __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_fast(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info,
-    float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta, int n,
-    unsigned long long* __restrict__ stats, int* __restrict__ overflowList,
-    int* __restrict__ overflowCount, int* __restrict__ bigStacks) {
-    __shared__ int sStack[GLOBAL_STACK ? 1 : GD_FW_WAVES][GLOBAL_STACK ? 1 : GD_FW_STACK];
-    __shared__ float4 sList[GLOBAL_STACK ? 1 : GD_FW_WAVES][GD_FW_LIST];
+    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const gd_tree_info* __restrict__ info, float3* __restrict__ acc, float* __restrict__ phi,
+    float eps2, float G, float theta, int n, unsigned long long* __restrict__ stats,
+    int* __restrict__ overflowList, int* __restrict__ overflowCount, int* __restrict__ bigStacks,
+    const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio) {
+    constexpr bool QUAD = MODE != 0;
+    constexpr int W = GLOBAL_STACK ? 1 : GD_FW_WAVES;
+    constexpr int QW = QUAD ? W : 1;
+    constexpr int QN = QUAD ? GD_FW_QLIST : 1;
+    __shared__ int sStack[W][GLOBAL_STACK ? 1 : GD_FW_STACK];
+    __shared__ float4 sList[W][GD_FW_LIST];
+    __shared__ float4 sQc[QW][QN], sQ0[QW][QN], sQ1[QW][QN];
     const unsigned lane = gd_lane();
     const int wave = GLOBAL_STACK ? 0 : (int)(threadIdx.x >> 6);
     long long g;
@@ -203,6 +320,9 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
         cap = GD_FW_STACK;
     }
     float4* list = sList[wave];
+    float4* qc = sQc[QUAD ? wave : 0];
+    float4* q0 = sQ0[QUAD ? wave : 0];
+    float4* q1 = sQ1[QUAD ? wave : 0];
     if (g * 64 >= n) return;   /* wave-uniform */
     const long long i = g * 64 + lane;
     const bool valid = i < n;
@@ -220,11 +340,12 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
+    const float gcx = 0.5f * (lx + hx), gcy = 0.5f * (ly + hy), gcz = 0.5f * (lz + hz);
 
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
     unsigned long long nInter = 0, nOpen = 0;
-    int listCount = 0;
+    int listCount = 0, qCount = 0;
     int top = 1;
     if (lane == 0) stack[0] = 0;
     if (GLOBAL_STACK) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
@@ -237,25 +358,27 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
         int idx = 0;
         float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
         int2 lk = make_int2(0, 0);
-        bool accept = false, open = false;
+        bool accept = false, open = false, leaf = false;
+        float size = 0.f;
         if (have) {
             idx = stack[base + lane];
             c = wcom[idx];
             lk = wlink[idx];
             const int level = lk.y & 31;
-            const float size = ldexpf(ext, -level);
+            size = ldexpf(ext, -level);
             const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
             const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
             const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
             const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const bool leaf = lk.x < 0;
+            leaf = lk.x < 0;
             accept = leaf || (size * size < th2 * d2);
             open = !accept;
         }
         /* all of this wave's LDS reads of the popped entries are done before pushes */
         __builtin_amdgcn_wave_barrier();
         top = base;
-        const unsigned long long accBal = __ballot(accept);
+        const bool toList = QUAD ? (accept && leaf) : accept;
+        const unsigned long long accBal = __ballot(toList);
         const int nacc = __popcll(accBal);
         if (listCount + nacc > GD_FW_LIST) {
             float sx, sy, sz, sp;
@@ -268,8 +391,41 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
             listCount = 0;
             __builtin_amdgcn_wave_barrier();
         }
-        if (accept) list[listCount + __popcll(accBal & lt)] = c;
+        if (toList) list[listCount + __popcll(accBal & lt)] = c;
         listCount += nacc;
+        if (QUAD) {
+            const bool toQ = accept && !leaf;
+            const unsigned long long qBal = __ballot(toQ);
+            const int nq = __popcll(qBal);
+            if (qCount + nq > GD_FW_QLIST) {
+                float sx, sy, sz, sp;
+                if (MODE == 2) {
+                    float cx, cy, cz;
+                    gd_eval_list(qc, qCount, px, py, pz, eps2, sx, sy, sz, sp);
+                    gd_mlp_correct(qc, q0, q1, qCount, gcx, gcy, gcz, mlpW, mlpB, minRatio, cx,
+                                   cy, cz);
+                    sx += cx; sy += cy; sz += cz;
+                } else {
+                    gd_eval_quad(qc, q0, q1, qCount, px, py, pz, eps2, sx, sy, sz, sp);
+                }
+                gd_kahan_add(tot.x, tot.cx, sx);
+                gd_kahan_add(tot.y, tot.cy, sy);
+                gd_kahan_add(tot.z, tot.cz, sz);
+                gd_kahan_add(phs, phc, sp);
+                nInter += (unsigned long long)qCount;
+                qCount = 0;
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (toQ) {
+                const int at = qCount + __popcll(qBal & lt);
+                qc[at] = c;
+                q0[at] = wquad[2 * idx];
+                float4 b1 = wquad[2 * idx + 1];
+                if (MODE == 2) b1.w = size;
+                q1[at] = b1;
+            }
+            qCount += nq;
+        }
         /* push children of opened nodes */
         const int nk = open ? (lk.y >> 5) : 0;
         int incl = nk;
@@ -314,6 +470,22 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
         gd_kahan_add(tot.z, tot.cz, sz);
         gd_kahan_add(phs, phc, sp);
         nInter += (unsigned long long)listCount;
+    }
+    if (QUAD && qCount > 0) {
+        float sx, sy, sz, sp;
+        if (MODE == 2) {
+            float cx, cy, cz;
+            gd_eval_list(qc, qCount, px, py, pz, eps2, sx, sy, sz, sp);
+            gd_mlp_correct(qc, q0, q1, qCount, gcx, gcy, gcz, mlpW, mlpB, minRatio, cx, cy, cz);
+            sx += cx; sy += cy; sz += cz;
+        } else {
+            gd_eval_quad(qc, q0, q1, qCount, px, py, pz, eps2, sx, sy, sz, sp);
+        }
+        gd_kahan_add(tot.x, tot.cx, sx);
+        gd_kahan_add(tot.y, tot.cy, sy);
+        gd_kahan_add(tot.z, tot.cz, sz);
+        gd_kahan_add(phs, phc, sp);
+        nInter += (unsigned long long)qCount;
     }
     if (valid) {
         acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
@@ -362,17 +534,24 @@ __global__ void k_walk_reset(unsigned long long* stats, int* overflowCount) {
     if (threadIdx.x == 0) *overflowCount = 0;
 }
 
-extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi,
-                        float eps2, float G, float theta, int starCount, gd_walk_stats* stats) {
+static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, float* phi,
+                         float eps2, float G, float theta, int starCount, gd_walk_stats* stats,
+                         int mode, const gd_mlp* net, float minRatio) {
     if (!tree || !tree->builder) {
         GD_FAIL("gd_walk: tree not initialised");
         return;
     }
     if (starCount <= 0) return;
     gd_builder* b = gd_builder_of(tree->builder);
+    if (mode != 0 && !(b->quad && b->wquad)) {
+        GD_FAIL("gd_walk: this walk needs the second moments (gd_set_quadrupole before the build)");
+        return;
+    }
     const int groups = gd_div_up(starCount, 64);
     gd_walk_ensure(groups);
     if (gd_has_error()) return;
+    const gd_bf16x4* W = net ? gd_mlp_fwd_tiles(net) : nullptr;
+    const float* Bv = net ? gd_mlp_bias(net) : nullptr;
     k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.overflowCount);
     GD_LAUNCHED();
     if (g_walkTiming) {
@@ -382,9 +561,16 @@ extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, f
         }
         GD_HIP(hipEventRecord(g_walkEv[0], nullptr));
     }
-    k_walk_fast<false><<<gd_div_up(groups, GD_FW_WAVES), 64 * GD_FW_WAVES>>>(
-        pos, b->wcom, b->wlink, b->dinfo, acc, phi, eps2, G, theta, starCount, g_walk.stats,
-        g_walk.overflowList, g_walk.overflowCount, nullptr);
+    const int blocks = gd_div_up(groups, GD_FW_WAVES);
+#define GD_WALK_ARGS(LIST, STACKS)                                                             \
+    pos, b->wcom, b->wlink, b->wquad, b->dinfo, acc, phi, eps2, G, theta, starCount,           \
+        g_walk.stats, LIST, g_walk.overflowCount, STACKS, W, Bv, minRatio
+    if (mode == 0)
+        k_walk_fast<false, 0><<<blocks, 64 * GD_FW_WAVES>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+    else if (mode == 1)
+        k_walk_fast<false, 1><<<blocks, 64 * GD_FW_WAVES>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+    else
+        k_walk_fast<false, 2><<<blocks, 64 * GD_FW_WAVES>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
     GD_LAUNCHED();
     if (g_walkTiming) {
         GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
@@ -403,12 +589,16 @@ extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, f
         }
         for (int s = 0; s < over; s += g_walk.bigSlots) {
             const int m = (over - s) < g_walk.bigSlots ? (over - s) : g_walk.bigSlots;
-            k_walk_fast<true><<<m, 64>>>(pos, b->wcom, b->wlink, b->dinfo, acc, phi, eps2, G,
-                                        theta, starCount, g_walk.stats, g_walk.overflowList + s,
-                                        g_walk.overflowCount, g_walk.bigStacks);
+            if (mode == 0)
+                k_walk_fast<true, 0><<<m, 64>>>(GD_WALK_ARGS(g_walk.overflowList + s, g_walk.bigStacks));
+            else if (mode == 1)
+                k_walk_fast<true, 1><<<m, 64>>>(GD_WALK_ARGS(g_walk.overflowList + s, g_walk.bigStacks));
+            else
+                k_walk_fast<true, 2><<<m, 64>>>(GD_WALK_ARGS(g_walk.overflowList + s, g_walk.bigStacks));
             GD_LAUNCHED();
         }
     }
+#undef GD_WALK_ARGS
     if (over > 0 || stats) {
         unsigned long long h[4];
         GD_HIP(hipMemcpy(h, g_walk.stats, sizeof(h), hipMemcpyDeviceToHost));
@@ -418,6 +608,23 @@ extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, f
         stats->opened = h[1];
         stats->overflowGroups = (unsigned long long)over;
     }
+}
+
+extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi,
+                        float eps2, float G, float theta, int starCount, gd_walk_stats* stats) {
+    const gd_builder* b = tree && tree->builder ? gd_builder_of(tree->builder) : nullptr;
+    const int mode = (b && b->quad && b->wquad) ? 1 : 0;
+    gd_walk_impl(tree, pos, acc, phi, eps2, G, theta, starCount, stats, mode, nullptr, 0.f);
+}
+
+extern "C" void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc,
+                                float* phi, float eps2, float G, float theta, int starCount,
+                                const gd_mlp* net, float minRatio, gd_walk_stats* stats) {
+    if (!net) {
+        GD_FAIL("gd_walk_learned: no network");
+        return;
+    }
+    gd_walk_impl(tree, pos, acc, phi, eps2, G, theta, starCount, stats, 2, net, minRatio);
 }
 
 extern "C" void gd_walk_timing(int enable) { g_walkTiming = enable ? 1 : 0; }

@@ -127,6 +127,11 @@ class Simulation:
                                      lo, spare)
         check(self.lib)
 
+    def set_quadrupole(self, enable):
+        """Compute second moments in later builds; the walk then uses them."""
+        self.lib.gd_set_quadrupole(C.byref(self.state), 1 if enable else 0)
+        check(self.lib)
+
     def walk(self, theta, potential=False, stats=False):
         st = WalkStats() if stats else None
         self.lib.gd_walk(C.byref(self.state), self.pos, self.acc,
@@ -200,6 +205,17 @@ class Simulation:
         if count is None:
             count = self.tree_summary()[0]
         return from_device(self.lib, self.state.nodes, np.uint8, (count, 80))
+
+    def walk_arrays(self, count):
+        """Compact walk records of the first `count` nodes: (COM+mass (count,4),
+        link (count,2), second moments (count,8) or None)."""
+        com = np.empty((count, 4), np.float32)
+        link = np.empty((count, 2), np.int32)
+        quad = np.empty((count, 8), np.float32)
+        self.lib.gd_tree_walk_arrays(C.byref(self.state), com.ctypes.data, link.ctypes.data,
+                                     quad.ctypes.data, count)
+        check(self.lib)
+        return com, link, quad
 
     def levels(self):
         """Host copy of levelOffsets / levelCount / deepestLevel."""

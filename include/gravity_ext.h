@@ -53,6 +53,17 @@ void gd_tree_summary(const treeState* tree, int* totalNodes, int* overflow, int*
 void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
                          float3** spareLo);
 
+/* Makes every later build also compute each node's second moments about its
+ * centre of mass (S = sum m (x-c)(x-c)^T, used by the quadrupole and learned
+ * walks); enable = 0 stops it.  Allocates 32 B per node of capacity. */
+void gd_set_quadrupole(treeState* tree, int enable);
+
+/* Copies the compact walk records of nodes [0, count) to host memory:
+ * com (float4: centre of mass, mass), link (int2: first child or -(first
+ * body + 1); count << 5 | level) and, when not NULL and moments are on, quad
+ * (2 float4: Sxx Syy Szz Sxy, Sxz Syz trace 0).  Synchronises. */
+void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* quad, int count);
+
 /* Copies the device level tables into tree->levelOffsets / levelCount /
  * deepestLevel (synchronises). */
 void gd_tree_levels(treeState* tree);
@@ -82,6 +93,49 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
  * last timed kernel's duration in ms (synchronises), or -1 if none. */
 void gd_walk_timing(int enable);
 float gd_walk_kernel_ms(void);
+
+/* ---- learned far-field corrector ------------------------------------------
+ * Network 8 -> 64 -> 64 -> 64 -> 3 with GELU, BF16 weights/activations and
+ * FP32 accumulation on the matrix cores.  Parameters (FP32, row-major
+ * [out][in]): W1[64][8] @0, b1 @512, W2[64][64] @576, b2 @4672,
+ * W3[64][64] @4736, b3 @8832, W4[3][64] @8896, b4 @9088 (9091 values).
+ * Inputs per (point, cell): unit vector to the cell's COM (3), size / r,
+ * S u / (m s^2) (3), tr S / (m s^2); output: the correction of the cell's
+ * monopole acceleration in units of G m s^2 / r^4.  X (rows x 8), Y (rows x 3)
+ * and order are device arrays. */
+#define GD_MLP_PARAMS 9091
+typedef struct gd_mlp gd_mlp;
+gd_mlp* gd_mlp_create(const float* hostParams);          /* NULL: all zero */
+void gd_mlp_destroy(gd_mlp* net);
+void gd_mlp_get_params(const gd_mlp* net, float* hostParams);
+void gd_mlp_set_params(gd_mlp* net, const float* hostParams);
+/* Y = net(X) for `rows` feature rows. */
+void gd_mlp_forward(const gd_mlp* net, const float* X, float* Y, long long rows);
+/* Gradient of mean_b |net(X[order[b]]) - Y[order[b]]|^2 over b < B into
+ * hostGrad (may be NULL); returns the loss.  Synchronises. */
+float gd_mlp_gradient(gd_mlp* net, const float* X, const float* Y, const int* order, long long B,
+                      float* hostGrad);
+/* The same gradient followed by one Adam step (beta 0.9 / 0.999, eps 1e-8,
+ * bias-corrected) on the FP32 parameters; returns the loss. */
+float gd_mlp_train_step(gd_mlp* net, const float* X, const float* Y, const int* order,
+                        long long B, float lr);
+void gd_mlp_adam_state(const gd_mlp* net, float* hostM, float* hostV);
+/* Training samples from the current tree (built with second moments): for
+ * each target body (device indices, sorted order), every internal cell a
+ * per-body traversal at opening angle theta accepts gives one row of X and
+ * the exact scaled correction in Y (FP64 sum over the cell's bodies minus the
+ * softened monopole).  Returns the number of rows written (at most cap);
+ * *droppedTargets counts targets whose traversal outgrew its stack. */
+long long gd_corrector_samples(const treeState* tree, const float4* pos, const int* targets,
+                               int nt, float theta, float eps2, float* X, float* Y,
+                               long long cap, int* droppedTargets);
+/* gd_walk with monopole cells plus the network's correction of every
+ * accepted cell whose size / r (from the group's centre) exceeds minRatio,
+ * evaluated once per (64-body group, cell) on the matrix cores.  Needs a tree
+ * built with second moments. */
+void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc, float* phi,
+                     float eps2, float G, float theta, int starCount, const gd_mlp* net,
+                     float minRatio, gd_walk_stats* stats);
 
 /* ---- hi/lo positions -------------------------------------------------------
  * Kick-drift with positions held as hi (float4 .xyz) + lo (float3):

@@ -370,3 +370,41 @@ def test_large_build_properties():
         ref = ob.build_tree(pos, vel)
         for f in ob.NODE_FIELDS:
             assert np.array_equal(nodes[f], ref["nodes"][f]), f
+
+
+def test_quadrupole_moments_and_walk():
+    """Second moments against FP64 sums over each node's bodies; the
+    quadrupole walk is more accurate than the monopole walk at equal theta."""
+    pos, vel = ics.exponential_disk(16384, seed=77)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.set_quadrupole(True)
+        sim.build_tree_async()
+        total = sim.tree_summary()[0]
+        com, link, quad = sim.walk_arrays(total)
+        nodes = sim.nodes_raw(total).view(ob.NODE_DTYPE)
+        ps = sim.positions().astype(np.float64)
+        for i in range(0, total, 97):
+            a, c = int(nodes[i]["first"]), int(nodes[i]["count"])
+            m = ps[a:a + c, 3]
+            d = ps[a:a + c, :3] - com[i, :3].astype(np.float64)
+            S = (m[:, None, None] * d[:, :, None] * d[:, None, :]).sum(0)
+            want = np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2], np.trace(S)])
+            got = np.r_[quad[i, 0:4], quad[i, 4:7]].astype(np.float64)
+            scale = m.sum() * (np.abs(d).max() ** 2 + 1e-12)
+            assert np.abs(got - want).max() <= 1e-4 * scale, i
+        sample = ics.subsample(16384, 2048, seed=5)
+        exact = sim.direct_sum_fp64(sample)
+        errs = {}
+        for q in (False, True):
+            sim.set_quadrupole(q)
+            sim.build_tree_async()
+            sim.walk(0.7)
+            errs[q] = float(np.median(rel_err(sim.accelerations()[sample], exact)))
+        print(f"theta 0.7 median error: monopole {errs[False]:.3e} quadrupole {errs[True]:.3e}",
+              flush=True)
+        assert errs[True] < 0.5 * errs[False]
+        # theta = 0: only leaves are accepted, the quadrupole walk is the direct sum
+        sim.set_quadrupole(True)
+        sim.build_tree_async()
+        sim.walk(0.0)
+        assert rel_err(sim.accelerations()[sample], exact).max() < 1e-4
