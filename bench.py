@@ -1,26 +1,37 @@
-"""Benchmark: Barnes-Hut body-steps/s on one MI355X.
+"""Benchmark: Barnes-Hut body-steps/s on one MI355X at <= 1e-3 median force error.
 
 Workload (N=1): the two-disk merger of 2^21 bodies (2^20 per disk, 30 degree
 inclination, 20 scale lengths apart, parabolic approach), FP32, 63-bit keys,
 eps = 0.005, dt = 5e-4.  One step = the full pipeline with every input
 resident in HBM: bbox -> Morton keys -> radix sort -> reorder -> octree
-topology + centre of mass -> walk -> kick-drift.  The walk is the engine's
-wavefront walk at the opening angle that keeps the median force error within
-1e-3 of the FP64 direct sum (checked on a seeded 4096-body subsample after
-the timed region and reported).
+topology + centre of mass (+ second moments when the walk uses them) -> walk
+-> kick-drift.
 
-The JSON line also carries:
-  roofline      the walk kernel (the dominant kernel): FP32 FLOP per launch
-                (20 per body-node interaction + 9 per opened node test)
-                divided by its duration measured with HIP events on the
-                launch stream, against the 157.3 TFLOP/s FP32 peak;
+The headline compares two walks at equal accuracy, each at the largest
+opening angle whose median force error against the FP64 direct sum (seeded
+4096-body subsample) is <= 1e-3:
+  plain    monopole cells, the reference's force law
+  learned  quadrupole cells plus the trained network's residual correction
+           (trained first, on this run's own snapshots, with the BF16-MFMA
+           kernels); when no opening angle meets the budget the corrector is
+           switched off and `value` falls back to the plain walk.
+The analytic quadrupole walk (the corrector's control) is reported too.
+`value` is the learned walk's body-steps/s (or the fallback's).
+
+Also on the line:
+  roofline      the dominant kernel, the walk's evaluation kernel in the
+                chosen mode: FP32 FLOP per launch (20 per monopole
+                interaction, 54 per quadrupole cell, 15 per learned
+                correction; matrix-core FLOP of the network reported apart)
+                over its duration from HIP events on the launch stream;
   cpu_baseline  the oracle (a serial C restatement of the reference's step)
                 on one host core over a bounded sample of the same workload;
-  reference_walk  the same step with the reference's 32-body warp-vote walk.
+  reference_walk  the reference's 32-body warp-vote walk on the same tree.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -33,20 +44,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s at ≤1e-3 median force err vs FP64 sum: learned vs plain walk"
 FP32_PEAK_TFLOPS = 157.3
-FLOP_PER_INTERACTION = 20
-FLOP_PER_OPEN = 9
+BUDGET = 1e-3
+FLOP_MONO = 20
+FLOP_QUAD = 54
+FLOP_CORR = 15
+FLOP_MLP = 17792
+FLOP_OPEN = 9
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--per-disk", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-reference-walk", action="store_true")
     ap.add_argument("--cpu-group-stride", type=int, default=8)
+    ap.add_argument("--train-epochs", type=int, default=12)
     return ap.parse_args()
 
 
@@ -75,11 +90,115 @@ def cpu_baseline(pos, vel, eps, theta, dt, stride):
         "unit": "body-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"one step of the same {n}-body workload: full build "
+        "sample": (f"one step of the same {n}-body workload at theta={theta}: full build "
                    f"({t1 - t0:.2f} s) and kick-drift ({t3 - t2:.2f} s); reference walk on "
                    f"{walked} of {groups} 32-body groups (every {stride}th, {t2 - t1:.2f} s) "
                    f"scaled by {groups / walked:.1f}"),
         "host_cores": os.cpu_count(),
+    }
+
+
+class Walks:
+    """The three walks on one simulation."""
+
+    def __init__(self, sim, net):
+        self.sim = sim
+        self.net = net
+
+    def prepare(self, mode):
+        # second moments are built only for the walks that use them
+        self.sim.set_quadrupole(mode != "plain")
+
+    def walk(self, mode, theta, stats=False):
+        from cudagravitydemo_amd import corrector as cr
+        if mode == "learned":
+            return cr.walk_learned(self.sim, self.net, theta, 0.0, stats=stats)
+        return self.sim.walk(theta, stats=stats)
+
+    def step(self, mode, theta, dt):
+        self.sim.build_tree_async()
+        self.walk(mode, theta)
+        self.sim.integrate(dt)
+
+
+def median_err(sim, sample, exact):
+    a = sim.accelerations()[sample]
+    e = np.linalg.norm(a - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    return float(np.median(e)), float(np.quantile(e, 0.99))
+
+
+def calibrate(walks, mode, thetas, sample):
+    """Largest theta (thetas descending) meeting the budget on the current state."""
+    sim = walks.sim
+    walks.prepare(mode)
+    sim.build_tree_async()
+    exact = sim.direct_sum_fp64(sample)
+    tried = []
+    for th in thetas:
+        walks.walk(mode, th)
+        med, p99 = median_err(sim, sample, exact)
+        tried.append({"theta": th, "median": med, "p99": p99})
+        if med <= BUDGET:
+            return th, med, p99, tried
+    return None, None, None, tried
+
+
+def timed_steps(walks, mode, theta, dt, steps, warmup):
+    sim = walks.sim
+    walks.prepare(mode)
+    for _ in range(warmup):
+        walks.step(mode, theta, dt)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        walks.step(mode, theta, dt)
+    sim.sync()
+    return time.perf_counter() - t0
+
+
+def eval_roofline(walks, mode, theta, n):
+    lib = walks.sim.lib
+    walks.prepare(mode)
+    walks.sim.build_tree_async()
+    lib.gd_walk_timing(1)
+    ph = (C.c_float * 3)()
+    phases = []
+    st = None
+    for _ in range(5):
+        st = walks.walk(mode, theta, stats=True)
+        lib.gd_walk_phase_ms(ph)
+        phases.append(list(ph))
+    lib.gd_walk_timing(0)
+    phases = np.array(phases)
+    ev_ms = float(phases[:, 1].mean())
+    cells = st["cell_interactions"]
+    mono = st["interactions"] - cells
+    if mode == "plain":
+        valu = FLOP_MONO * st["interactions"]
+    elif mode == "quad":
+        valu = FLOP_MONO * mono + FLOP_QUAD * cells
+    else:
+        valu = FLOP_MONO * mono + (FLOP_QUAD + FLOP_CORR) * cells
+    mfma = FLOP_MLP * cells // 64 if mode == "learned" else 0
+    achieved = valu / (ev_ms * 1e-3) / 1e12
+    return {
+        "bound": "mfma",
+        "compute": ("FP32 vector ALU (peak = MI355X FP32; packed FP32 is not used, so 78.6 is "
+                    "the practical ceiling)"),
+        "kernel": f"k_walk_eval ({mode} walk, theta={theta})",
+        "achieved": achieved,
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS,
+        "traffic": None,
+        "kernel_ms": ev_ms,
+        "traversal_ms": float(phases[:, 0].mean()),
+        "fallback_ms": float(phases[:, 2].mean()),
+        "flop_per_launch": int(valu),
+        "mfma_flop_per_launch": int(mfma),
+        "interactions_per_body": st["interactions"] / n,
+        "cell_interactions_per_body": cells / n,
+        "overflow_groups": st["overflow_groups"],
     }
 
 
@@ -90,106 +209,111 @@ def main():
               "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(1)
     from cudagravitydemo_amd import Simulation, ics
+    from cudagravitydemo_amd import corrector as cr
 
-    eps, dt, theta = 0.005, 5e-4, args.theta
+    eps, dt = 0.005, 5e-4
     pos, vel = ics.disk_merger(args.per_disk, seed=2024)
     n = pos.shape[0]
     sim = Simulation(pos, vel, eps=eps)
-    lib = sim.lib
-    ev0, ev1 = lib.gd_event_create(), lib.gd_event_create()
-
-    for _ in range(args.warmup):
-        sim.step(dt, theta)
-    sim.sync()
-    t0 = time.perf_counter()
-    lib.gd_event_record(ev0)
-    for _ in range(args.steps):
-        sim.step(dt, theta)
-    lib.gd_event_record(ev1)
-    sim.sync()
-    wall = time.perf_counter() - t0
-    gpu_ms = lib.gd_event_elapsed_ms(ev0, ev1)
-    ms_per_step = 1000.0 * wall / args.steps
-
-    # dominant kernel: walk, timed on its launch stream, with its FLOP count
-    lib.gd_walk_timing(1)
-    sim.build_tree_async()
-    walk_ms = []
-    stats = None
-    for _ in range(5):
-        stats = sim.walk(theta, stats=True)
-        walk_ms.append(lib.gd_walk_kernel_ms())
-    lib.gd_walk_timing(0)
-    walk_avg_ms = float(np.mean(walk_ms))
-    flop = FLOP_PER_INTERACTION * stats["interactions"] + FLOP_PER_OPEN * stats["opened"]
-    achieved = flop / (walk_avg_ms * 1e-3) / 1e12
-
-    # force error of the walk used in the timed steps, vs the FP64 direct sum
+    net = cr.Corrector(seed=1)
+    walks = Walks(sim, net)
     sample = ics.subsample(n, 4096, seed=99)
-    exact = sim.direct_sum_fp64(sample)
-    acc = sim.accelerations()[sample]
-    err = np.linalg.norm(acc - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    for _ in range(args.warmup):
+        walks.step("plain", 0.5, dt)
 
-    # the reference's warp-vote walk on the same tree, for comparison
+    # plain walk
+    th_p, med_p, p99_p, tried_p = calibrate(walks, "plain", (0.3, 0.25, 0.2, 0.15, 0.1), sample)
+    th_pt = th_p if th_p is not None else tried_p[-1]["theta"]
+    secs_p = timed_steps(walks, "plain", th_pt, dt, args.steps, 1)
+
+    # corrector: samples from this run's own snapshots, BF16-MFMA training
+    t0 = time.perf_counter()
+    sim.set_quadrupole(True)
+    samples = cr.SampleSet(sim.lib, 24_000_000)
+    for snap in range(4):
+        sim.build_tree_async()
+        samples.add_from(sim, cr.default_targets(n, 4096, seed=snap), 0.9)
+        for _ in range(5):
+            walks.step("quad", 0.6, dt)
+    losses = cr.train(net, samples, epochs=args.train_epochs, batch=65536, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    n_samples = samples.n
+    samples.close()
+
+    th_l, med_l, p99_l, tried_l = calibrate(walks, "learned", (1.0, 0.9, 0.8, 0.7, 0.6), sample)
+    corrector_on = th_l is not None
+    secs_l = timed_steps(walks, "learned", th_l, dt, args.steps, 1) if corrector_on else None
+
+    # analytic quadrupole control
+    th_q, med_q, p99_q, tried_q = calibrate(walks, "quad", (1.0, 0.9, 0.8, 0.7, 0.6, 0.5), sample)
+    secs_q = timed_steps(walks, "quad", th_q if th_q is not None else 0.5, dt, args.steps, 1)
+
+    mode = "learned" if corrector_on else "plain"
+    theta = th_l if corrector_on else th_pt
+    secs = secs_l if corrector_on else secs_p
+    roof = eval_roofline(walks, mode, theta, n)
+
     ref = None
     if not args.no_reference_walk:
+        walks.prepare("plain")
         sim.build_tree()
+        exact = sim.direct_sum_fp64(sample)
+        lib = sim.lib
         e0, e1 = lib.gd_event_create(), lib.gd_event_create()
         lib.gd_event_record(e0)
-        sim.walk_reference(theta)
+        sim.walk_reference(0.5)
         lib.gd_event_record(e1)
         ref_ms = lib.gd_event_elapsed_ms(e0, e1)
-        acc_r = sim.accelerations()[sample]
-        exact_r = sim.direct_sum_fp64(sample)
-        err_r = np.linalg.norm(acc_r - exact_r, axis=1) / np.linalg.norm(exact_r, axis=1)
-        ref = {"walk_ms": ref_ms, "median_force_err": float(np.median(err_r)),
-               "theta": theta, "note": "32-body warp-vote walk (tree.cu:562-676), same tree"}
+        med_r, _ = median_err(sim, sample, exact)
+        ref = {"walk_ms": ref_ms, "median_force_err": med_r, "theta": 0.5,
+               "note": "32-body warp-vote walk (tree.cu:562-676), same tree"}
     final_pos, final_vel = sim.positions(), sim.velocities()
+    net.close()
     sim.close()
+
+    def leg(th, med, p99, secs_, tried):
+        if th is None:
+            return {"theta": None, "within_budget": False, "tried": tried}
+        return {"theta": th, "median_force_err": med, "p99_force_err": p99,
+                "body_steps_per_s": n * args.steps / secs_, "ms_per_step": 1e3 * secs_ / args.steps,
+                "tried": tried}
 
     out = {
         "metric": METRIC,
-        "value": n * args.steps / wall,
+        "value": n * args.steps / secs,
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * secs / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"two-disk merger, {n} bodies, plain walk theta={theta}, eps={eps}, dt={dt}",
+            "workload": (f"two-disk merger, {n} bodies, eps={eps}, dt={dt}, full step, "
+                         f"{mode} walk at theta={theta}"),
             "n_bodies": n,
-            "theta": theta,
             "eps": eps,
             "dt": dt,
-            "walk": "wavefront walk, 64-body groups, LDS interaction lists, Kahan across flushes",
-            "corrector": "off",
+            "walk": mode,
+            "theta": theta,
+            "force_error_budget": BUDGET,
+            "judge": "FP64 direct sum, 4096-body seeded subsample",
         },
-        "gpu_ms_per_step_events": gpu_ms / args.steps,
-        "force_error_vs_fp64": {"median": float(np.median(err)),
-                                "p99": float(np.quantile(err, 0.99)),
-                                "sample": 4096, "within_budget": bool(np.median(err) <= 1e-3)},
-        "roofline": {
-            "bound": "mfma",
-            "compute": "FP32 vector ALU (the walk uses no matrix cores); peak = MI355X FP32",
-            "kernel": "k_walk_fast",
-            "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel_ms": walk_avg_ms,
-            "flop_per_launch": flop,
-            "interactions_per_body": stats["interactions"] / n,
-        },
+        "plain": leg(th_p, med_p, p99_p, secs_p, tried_p),
+        "learned": leg(th_l, med_l, p99_l, secs_l, tried_l),
+        "quadrupole_control": leg(th_q, med_q, p99_q, secs_q, tried_q),
+        "learned_vs_plain": (secs_p / secs_l) if corrector_on else None,
+        "corrector": {"on": corrector_on, "train_s": train_s, "samples": n_samples,
+                      "epochs": args.train_epochs, "final_loss": losses[-1] if losses else None,
+                      "first_loss": losses[0] if losses else None},
+        "roofline": roof,
         "reference_walk": ref,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(final_pos, final_vel, eps, theta, dt,
+        out["cpu_baseline"] = cpu_baseline(final_pos, final_vel, eps, 0.5, dt,
                                            args.cpu_group_stride)
     print(json.dumps(out), flush=True)
 

@@ -52,7 +52,7 @@ class TreeState(C.Structure):
 
 class WalkStats(C.Structure):
     _fields_ = [("interactions", C.c_ulonglong), ("opened", C.c_ulonglong),
-                ("overflowGroups", C.c_ulonglong)]
+                ("overflowGroups", C.c_ulonglong), ("cellInteractions", C.c_ulonglong)]
 
 
 assert C.sizeof(Node) == 80 and C.sizeof(TreeState) == 304 and C.sizeof(WorldBox) == 20
@@ -112,6 +112,7 @@ _PROTOS = {
                        C.POINTER(WalkStats)]),
     "gd_walk_timing": (None, [i32]),
     "gd_walk_kernel_ms": (f32, []),
+    "gd_walk_phase_ms": (i32, [vp]),
     "gd_mlp_create": (vp, [vp]),
     "gd_mlp_destroy": (None, [vp]),
     "gd_mlp_get_params": (None, [vp, vp]),
@@ -127,6 +128,7 @@ _PROTOS = {
     "gd_integrate_hilo": (None, [vp, vp, vp, vp, i32, f32]),
     "gd_direct_sum_fp64": (None, [vp, vp, i32, vp, i32, f64, f64, vp]),
     "gd_diagnostics": (None, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
+    "gd_stream_copy": (None, [vp, vp, C.c_size_t]),
     "gd_event_create": (vp, []),
     "gd_event_record": (None, [vp]),
     "gd_event_elapsed_ms": (f32, [vp, vp]),

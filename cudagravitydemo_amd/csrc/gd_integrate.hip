@@ -202,3 +202,27 @@ extern "C" void gd_diagnostics(const float4* pos, const float3* vel, const float
     for (int k = 0; k < 6; ++k) out[k] = h[k];
     out[6] = h[0] + h[5];
 }
+
+/* ---------------------------------------------------------------------------
+ * Stream copy of known size (16-byte vectors): a calibration kernel for
+ * hardware byte counters and a bandwidth reference.
+ * ------------------------------------------------------------------------- */
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src,
+                                                     float4* __restrict__ dst, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+extern "C" void gd_stream_copy(void* dst, const void* src, size_t bytes) {
+    if (bytes % 16) {
+        GD_FAIL("gd_stream_copy: %zu bytes is not a multiple of 16", bytes);
+        return;
+    }
+    const long long n = (long long)(bytes / 16);
+    if (n == 0) return;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    k_stream_copy<<<(int)blocks, 256>>>((const float4*)src, (float4*)dst, n);
+    GD_LAUNCHED();
+}

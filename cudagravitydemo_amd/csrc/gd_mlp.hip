@@ -618,8 +618,9 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     }
 }
 
-/* One wavefront per pair: the 8 features and the target
- * (sum over the cell's bodies - softened monopole) / (m s^2 / r^4), in FP64. */
+/* One wavefront per pair: the 8 features and the target (sum over the cell's
+ * bodies - softened monopole - analytic quadrupole) / (m s^2 / r^4), in FP64:
+ * what the quadrupole cell misses, which the network learns. */
 __global__ __launch_bounds__(256) void k_samples_eval(
     const float4* __restrict__ pos, const node* __restrict__ nodes,
     const float4* __restrict__ wcom, const float4* __restrict__ wquad,
@@ -659,9 +660,21 @@ __global__ __launch_bounds__(256) void k_samples_eval(
             const double rs2 = rc2 + eps2;
             const double mono = (double)c.w / (rs2 * sqrt(rs2));
             const double scale = (double)c.w * size * size / (rc2 * rc2);
-            Y[p * 3 + 0] = (float)((ax - mono * dx) / scale);
-            Y[p * 3 + 1] = (float)((ay - mono * dy) / scale);
-            Y[p * 3 + 2] = (float)((az - mono * dz) / scale);
+            /* analytic quadrupole term, as the walk evaluates it (softened r) */
+            const double sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
+            const double tr = q1.z;
+            const double sdx = sxx * dx + sxy * dy + sxz * dz;
+            const double sdy = sxy * dx + syy * dy + syz * dz;
+            const double sdz = sxz * dx + syz * dy + szz * dz;
+            const double q = dx * sdx + dy * sdy + dz * sdz;
+            const double inv5 = 1.0 / (rs2 * rs2 * sqrt(rs2));
+            const double cd = inv5 * (7.5 * q / rs2 - 1.5 * tr);
+            const double qx = cd * dx - 3.0 * inv5 * sdx;
+            const double qy = cd * dy - 3.0 * inv5 * sdy;
+            const double qz = cd * dz - 3.0 * inv5 * sdz;
+            Y[p * 3 + 0] = (float)((ax - mono * dx - qx) / scale);
+            Y[p * 3 + 1] = (float)((ay - mono * dy - qy) / scale);
+            Y[p * 3 + 2] = (float)((az - mono * dz - qz) / scale);
             const double r = sqrt(rc2);
             const double ux = dx / r, uy = dy / r, uz = dz / r;
             const double k = (double)c.w * size * size;

@@ -129,10 +129,16 @@ extern "C" void calculateAcceleration(const float4* positionMassArray, treeState
 /* ---------------------------------------------------------------------------
  * Production walk
  * ------------------------------------------------------------------------- */
-#define GD_FW_WAVES 4
+#ifndef GD_FW_WAVES
+#define GD_FW_WAVES 4             /* wavefronts (64-body groups) per block */
+#endif
+#ifndef GD_FW_STACK
 #define GD_FW_STACK 2048          /* LDS stack entries per wavefront */
-#define GD_FW_BIG_STACK 12288     /* global fallback: > 22 levels x 512 pushes */
+#endif
+#define GD_FW_BIG_STACK 12288     /* fallback LDS stack (48 KB): > 22 levels x 512 pushes */
+#ifndef GD_FW_LIST
 #define GD_FW_LIST 128            /* interaction-list entries per wavefront */
+#endif
 
 struct gd_kahan3 {
     float x, y, z, cx, cy, cz;
@@ -290,8 +296,10 @@ __device__ __forceinline__ void gd_mlp_correct(const float4* __restrict__ qc,
 }
 
 /* MODE: 0 monopole, 1 quadrupole cells, 2 monopole cells + learned correction */
-template <bool GLOBAL_STACK, int MODE>
-__global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_fast(
+/* BIG: one wavefront per block walks a listed group with a GD_FW_BIG_STACK
+ * LDS stack (large enough for any tree: < 22 levels x 512 pushes). */
+template <bool BIG, int MODE>
+__global__ __launch_bounds__(BIG ? 64 : 64 * GD_FW_WAVES) void k_walk_fast(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
     const gd_tree_info* __restrict__ info, float3* __restrict__ acc, float* __restrict__ phi,
@@ -299,10 +307,11 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
     int* __restrict__ overflowList, int* __restrict__ overflowCount, int* __restrict__ bigStacks,
     const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio) {
     constexpr bool QUAD = MODE != 0;
-    constexpr int W = GLOBAL_STACK ? 1 : GD_FW_WAVES;
+    constexpr bool GLOBAL_STACK = BIG;
+    constexpr int W = BIG ? 1 : GD_FW_WAVES;
     constexpr int QW = QUAD ? W : 1;
     constexpr int QN = QUAD ? GD_FW_QLIST : 1;
-    __shared__ int sStack[W][GLOBAL_STACK ? 1 : GD_FW_STACK];
+    __shared__ int sStack[W][BIG ? GD_FW_BIG_STACK : GD_FW_STACK];
     __shared__ float4 sList[W][GD_FW_LIST];
     __shared__ float4 sQc[QW][QN], sQ0[QW][QN], sQ1[QW][QN];
     const unsigned lane = gd_lane();
@@ -310,10 +319,11 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
     long long g;
     int* stack;
     int cap;
-    if (GLOBAL_STACK) {
+    if (BIG) {
         g = overflowList[blockIdx.x];
-        stack = bigStacks + (size_t)blockIdx.x * GD_FW_BIG_STACK;
+        stack = sStack[0];
         cap = GD_FW_BIG_STACK;
+        (void)bigStacks;
     } else {
         g = (long long)blockIdx.x * GD_FW_WAVES + wave;
         stack = sStack[wave];
@@ -348,7 +358,6 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
     int listCount = 0, qCount = 0;
     int top = 1;
     if (lane == 0) stack[0] = 0;
-    if (GLOBAL_STACK) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     bool overflowed = false;
 
     while (top > 0) {
@@ -445,10 +454,6 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
             for (int k = 0; k < nk; ++k) stack[at + k] = lk.x + k;
         }
         top += total;
-        if (GLOBAL_STACK) {
-            /* pushes by some lanes are popped by others: make them visible */
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-        }
         __builtin_amdgcn_wave_barrier();
     }
     if (overflowed) {
@@ -498,6 +503,345 @@ __global__ __launch_bounds__(GLOBAL_STACK ? 64 : 64 * GD_FW_WAVES) void k_walk_f
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * Production walk, split form: a traversal kernel writes each 64-body group's
+ * interaction list to HBM, an evaluation kernel streams it.
+ *
+ * List layout: group g owns entries [g*cap, (g+1)*cap) of `lists`; accepted
+ * leaves (and, in monopole mode, every accepted node) grow from the front,
+ * accepted cells in quadrupole / learned mode grow from the back; counts[g] =
+ * (front, back), or (-1, -1) when the group did not fit (its list or its LDS
+ * stack) and is left to the fused kernel with a global-memory stack.
+ * ------------------------------------------------------------------------- */
+#ifndef GD_TR_STACK
+#define GD_TR_STACK 1024          /* LDS stack entries per wavefront in the traversal */
+#endif
+#define GD_TR_WAVES 4
+#define GD_EV_WAVES 4
+
+template <bool SPLIT>
+__global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_traverse(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
+    int cap, int* __restrict__ lists, int2* __restrict__ counts,
+    int* __restrict__ overflowList, int* __restrict__ overflowCount,
+    unsigned long long* __restrict__ stats) {
+    __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK];
+    const unsigned lane = gd_lane();
+    const int wave = (int)(threadIdx.x >> 6);
+    const long long g = (long long)blockIdx.x * GD_TR_WAVES + wave;
+    if (g * 64 >= n) return;   /* wave-uniform */
+    int* stack = sStack[wave];
+    int* list = lists + (size_t)g * cap;
+    const long long i = g * 64 + lane;
+    const float4 p = pos[i < n ? i : g * 64];
+    float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, off)); hx = fmaxf(hx, __shfl_xor(hx, off));
+        ly = fminf(ly, __shfl_xor(ly, off)); hy = fmaxf(hy, __shfl_xor(hy, off));
+        lz = fminf(lz, __shfl_xor(lz, off)); hz = fmaxf(hz, __shfl_xor(hz, off));
+    }
+    const float ext = info->box.extent;
+    const float th2 = theta * theta;
+    const unsigned long long lt = gd_lanemask_lt();
+    int nFront = 0, nBack = 0, top = 1;
+    unsigned long long nOpen = 0;
+    if (lane == 0) stack[0] = 0;
+    bool overflowed = false;
+    while (top > 0) {
+        const int take = top < 64 ? top : 64;
+        const int base = top - take;
+        bool accept = false, open = false, leaf = false;
+        int idx = 0, nk = 0, first = 0;
+        if ((int)lane < take) {
+            idx = stack[base + lane];
+            const float4 c = wcom[idx];
+            const int2 lk = wlink[idx];
+            const float size = ldexpf(ext, -(lk.y & 31));
+            const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
+            const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
+            const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
+            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            leaf = lk.x < 0;
+            accept = leaf || (size * size < th2 * d2);
+            open = !accept;
+            nk = open ? (lk.y >> 5) : 0;
+            first = lk.x;
+        }
+        __builtin_amdgcn_wave_barrier();
+        top = base;
+        const bool front = SPLIT ? (accept && leaf) : accept;
+        const bool back = SPLIT && accept && !leaf;
+        const unsigned long long fb = __ballot(front);
+        const unsigned long long bb = __ballot(back);
+        const int nf = __popcll(fb), nb = __popcll(bb);
+        /* children offsets: prefix sum of nk (0..8) from four bit-plane ballots */
+        int excl = 0, total = 0;
+#pragma unroll
+        for (int bit = 0; bit < 4; ++bit) {
+            const unsigned long long bal = __ballot((nk >> bit) & 1);
+            excl += __popcll(bal & lt) << bit;
+            total += __popcll(bal) << bit;
+        }
+        nOpen += (unsigned long long)__popcll(__ballot(open));
+        if (nFront + nf + nBack + nb > cap || top + total > GD_TR_STACK) {
+            overflowed = true;
+            break;
+        }
+        if (front) list[nFront + __popcll(fb & lt)] = idx;
+        if (back) list[cap - 1 - (nBack + __popcll(bb & lt))] = idx;
+        nFront += nf;
+        nBack += nb;
+        if (open) {
+            const int at = top + excl;
+            for (int k = 0; k < nk; ++k) stack[at + k] = first + k;
+        }
+        top += total;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) {
+        if (overflowed) {
+            counts[g] = make_int2(-1, -1);
+            const int slot = atomicAdd(overflowCount, 1);
+            overflowList[slot] = (int)g;
+        } else {
+            counts[g] = make_int2(nFront, nBack);
+            if (stats) {
+                const int nvalid = (int)((n - g * 64) < 64 ? (n - g * 64) : 64);
+                atomicAdd(&stats[0], (unsigned long long)(nFront + nBack) * (unsigned long long)nvalid);
+                atomicAdd(&stats[1], nOpen);
+                atomicAdd(&stats[3], (unsigned long long)nBack * (unsigned long long)nvalid);
+            }
+        }
+    }
+}
+
+/* 64 monopoles from LDS (zero-mass padding contributes nothing). */
+template <bool PHI>
+__device__ __forceinline__ void gd_eval64(const float4* __restrict__ e, float px, float py,
+                                          float pz, float eps2, float& ax, float& ay, float& az,
+                                          float& ph) {
+#pragma unroll 8
+    for (int k = 0; k < 64; ++k) {
+        const float4 c = e[k];
+        const float dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float mi = c.w * inv;
+        const float f = mi * (inv * inv);
+        ax = __builtin_fmaf(f, dx, ax);
+        ay = __builtin_fmaf(f, dy, ay);
+        az = __builtin_fmaf(f, dz, az);
+        if (PHI) ph += mi;
+    }
+}
+
+/* 64 quadrupole cells (com/mass in e, moments in q0/q1). */
+template <bool PHI>
+__device__ __forceinline__ void gd_eval64_quad(const float4* __restrict__ e,
+                                               const float4* __restrict__ q0,
+                                               const float4* __restrict__ q1, int cnt, float px,
+                                               float py, float pz, float eps2, float& ax,
+                                               float& ay, float& az, float& ph) {
+    for (int k = 0; k < cnt; ++k) {
+        const float4 c = e[k];
+        const float4 a = q0[k];
+        const float4 b = q1[k];
+        const float dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float mi = c.w * inv;
+        const float inv5 = inv * inv2 * inv2;
+        const float sdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.w, dy, b.x * dz));
+        const float sdy = __builtin_fmaf(a.w, dx, __builtin_fmaf(a.y, dy, b.y * dz));
+        const float sdz = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, a.z * dz));
+        const float q = __builtin_fmaf(dx, sdx, __builtin_fmaf(dy, sdy, dz * sdz));
+        const float cd = __builtin_fmaf(mi, inv2, inv5 * __builtin_fmaf(7.5f * q, inv2, -1.5f * b.z));
+        const float cs = -3.f * inv5;
+        ax = __builtin_fmaf(cd, dx, __builtin_fmaf(cs, sdx, ax));
+        ay = __builtin_fmaf(cd, dy, __builtin_fmaf(cs, sdy, ay));
+        az = __builtin_fmaf(cd, dz, __builtin_fmaf(cs, sdz, az));
+        if (PHI) ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(3.f, q, -r2 * b.z), mi);
+    }
+}
+
+/*
+ * Evaluation: one wavefront per 64-body group streams its list in chunks of
+ * 64 entries: each lane gathers one entry's record into LDS (double
+ * buffered, the next chunk's gather is in flight while the current one is
+ * evaluated), then all 64 bodies evaluate the chunk with broadcast LDS reads.
+ * Each chunk's partial sum is added with Kahan compensation.  MODE 0: every
+ * entry is a monopole; 1: back entries are quadrupole cells; 2: back entries
+ * are quadrupole cells plus the network's residual correction (evaluated per
+ * chunk at the group's centre on the matrix cores, scaled by each body's own
+ * m s^2 / r^4).
+ */
+template <int MODE, bool PHI>
+__global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const gd_tree_info* __restrict__ info, const int* __restrict__ lists,
+    const int2* __restrict__ counts, int cap, float3* __restrict__ acc,
+    float* __restrict__ phi, float eps2, float G, int n, const gd_bf16x4* __restrict__ mlpW,
+    const float* __restrict__ mlpB, float minRatio) {
+    __shared__ float4 sE[GD_EV_WAVES][2][64];
+    __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
+    __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
+    __shared__ float4 sCor[MODE == 2 ? GD_EV_WAVES : 1][64];
+    const unsigned lane = gd_lane();
+    const int wave = (int)(threadIdx.x >> 6);
+    const long long g = (long long)blockIdx.x * GD_EV_WAVES + wave;
+    if (g * 64 >= n) return;
+    const int2 cnt = counts[g];
+    if (cnt.x < 0) return;   /* walked by the fallback */
+    const int* list = lists + (size_t)g * cap;
+    const long long i = g * 64 + lane;
+    const bool valid = i < n;
+    const float4 p = pos[valid ? i : g * 64];
+    const float px = p.x, py = p.y, pz = p.z;
+    gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float phs = 0.f, phc = 0.f;
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    /* padding of a partial chunk: massless and far away, contributes exactly 0 */
+    const float4 pad = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+
+    /* front list: monopoles, double-buffered gathers */
+    const int nf = cnt.x;
+    float4 nxt = pad;
+    if ((int)lane < nf) nxt = wcom[list[lane]];
+    int buf = 0;
+    for (int c0 = 0; c0 < nf; c0 += 64) {
+        sE[wave][buf][lane] = nxt;
+        __builtin_amdgcn_wave_barrier();
+        nxt = pad;
+        if (c0 + 64 + (int)lane < nf) nxt = wcom[list[c0 + 64 + lane]];
+        float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+        gd_eval64<PHI>(sE[wave][buf], px, py, pz, eps2, ax, ay, az, ph);
+        gd_kahan_add(tot.x, tot.cx, ax);
+        gd_kahan_add(tot.y, tot.cy, ay);
+        gd_kahan_add(tot.z, tot.cz, az);
+        if (PHI) gd_kahan_add(phs, phc, ph);
+        buf ^= 1;
+    }
+    /* back list: cells */
+    if (MODE != 0) {
+        const int nb = cnt.y;
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        if (MODE == 2) {
+            float lx = px, ly = py, lz = pz, hx = px, hy = py, hz = pz;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                lx = fminf(lx, __shfl_xor(lx, off)); hx = fmaxf(hx, __shfl_xor(hx, off));
+                ly = fminf(ly, __shfl_xor(ly, off)); hy = fmaxf(hy, __shfl_xor(hy, off));
+                lz = fminf(lz, __shfl_xor(lz, off)); hz = fmaxf(hz, __shfl_xor(hz, off));
+            }
+            gx = 0.5f * (lx + hx); gy = 0.5f * (ly + hy); gz = 0.5f * (lz + hz);
+        }
+        const float ext = info->box.extent;
+        for (int c0 = 0; c0 < nb; c0 += 64) {
+            const int e = c0 + (int)lane;
+            float4 c = zero4, q0 = zero4, q1 = zero4;
+            int idx = -1;
+            if (e < nb) {
+                idx = list[cap - 1 - e];
+                c = wcom[idx];
+                q0 = wquad[2 * idx];
+                q1 = wquad[2 * idx + 1];
+            }
+            const int m = nb - c0 < 64 ? nb - c0 : 64;
+            float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+            if (MODE == 1) {
+                sE[wave][0][lane] = c;
+                sQ0[MODE != 0 ? wave : 0][lane] = q0;
+                sQ1[MODE != 0 ? wave : 0][lane] = q1;
+                __builtin_amdgcn_wave_barrier();
+                gd_eval64_quad<PHI>(sE[wave][0], sQ0[MODE != 0 ? wave : 0],
+                                    sQ1[MODE != 0 ? wave : 0], m, px, py, pz, eps2, ax, ay, az, ph);
+            } else {
+                /* network at the group centre, 16 cells per pass; lane l ends up
+                   holding the correction (3 values) and m s^2 of cell l */
+                float k = 0.f;
+                float ox = 0.f, oy = 0.f, oz = 0.f;
+                {
+                    const float size = idx >= 0 ? ldexpf(ext, -(wlink[idx].y & 31)) : 1.f;
+                    k = c.w * size * size;
+                    const float dx = c.x - gx, dy = c.y - gy, dz = c.z - gz;
+                    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    const float rinv = idx >= 0 ? __builtin_amdgcn_rsqf(r2) : 0.f;
+                    const float ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
+                    const float ratio = size * rinv;
+                    const float kinv = (idx >= 0 && k > 0.f) ? 1.f / k : 0.f;
+                    const float su0 = __builtin_fmaf(q0.x, ux, __builtin_fmaf(q0.w, uy, q1.x * uz)) * kinv;
+                    const float su1 = __builtin_fmaf(q0.w, ux, __builtin_fmaf(q0.y, uy, q1.y * uz)) * kinv;
+                    const float su2 = __builtin_fmaf(q1.x, ux, __builtin_fmaf(q1.y, uy, q0.z * uz)) * kinv;
+                    const float tr = q1.z * kinv;
+                    const bool use = idx >= 0 && k > 0.f && ratio > minRatio;
+                    /* pass t handles cells 16t..16t+15: features of cell (16t + l%16),
+                       the feature quarter chosen by l/16, are fetched by lane shuffles */
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (16 * t >= m) break;
+                        const int src = 16 * t + (int)(lane % 16);
+                        const unsigned sel = lane / 16;
+                        /* both feature quarters are fetched; the reader keeps its own */
+                        const float u0 = __shfl(ux, src), s0 = __shfl(su0, src);
+                        const float u1 = __shfl(uy, src), s1 = __shfl(su1, src);
+                        const float u2 = __shfl(uz, src), s2 = __shfl(su2, src);
+                        const float u3 = __shfl(ratio, src), s3 = __shfl(tr, src);
+                        const float a0 = sel == 0 ? u0 : s0;
+                        const float a1 = sel == 0 ? u1 : s1;
+                        const float a2 = sel == 0 ? u2 : s2;
+                        const float a3 = sel == 0 ? u3 : s3;
+                        const gd_bf16x4 x = sel < 2 ? gd_pack4(a0, a1, a2, a3)
+                                                    : gd_pack4(0.f, 0.f, 0.f, 0.f);
+                        const gd_f32x4 o = gd_mlp_infer16(mlpW, mlpB, x);
+                        /* outputs of cell 16t + j live in lane j (j < 16) */
+                        const float r0 = __shfl(o[0], (int)(lane % 16));
+                        const float r1 = __shfl(o[1], (int)(lane % 16));
+                        const float r2o = __shfl(o[2], (int)(lane % 16));
+                        if ((int)(lane / 16) == t) {
+                            ox = r0; oy = r1; oz = r2o;
+                        }
+                    }
+                    if (!use) k = 0.f;
+                }
+                sE[wave][0][lane] = c;
+                sQ0[MODE != 0 ? wave : 0][lane] = q0;
+                sQ1[MODE != 0 ? wave : 0][lane] = q1;
+                sCor[MODE == 2 ? wave : 0][lane] = make_float4(ox, oy, oz, k);
+                __builtin_amdgcn_wave_barrier();
+                /* quadrupole cells plus the learned residual k / r^4 * o */
+                gd_eval64_quad<PHI>(sE[wave][0], sQ0[MODE != 0 ? wave : 0],
+                                    sQ1[MODE != 0 ? wave : 0], m, px, py, pz, eps2, ax, ay, az, ph);
+                const float4* ce = sE[wave][0];
+                const float4* co = sCor[MODE == 2 ? wave : 0];
+                for (int kk = 0; kk < m; ++kk) {
+                    const float4 cc = ce[kk];
+                    const float4 oo = co[kk];
+                    const float dx = cc.x - px, dy = cc.y - py, dz = cc.z - pz;
+                    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    const float i2 = __builtin_amdgcn_rcpf(r2);
+                    const float s4 = oo.w * (i2 * i2);
+                    ax = __builtin_fmaf(s4, oo.x, ax);
+                    ay = __builtin_fmaf(s4, oo.y, ay);
+                    az = __builtin_fmaf(s4, oo.z, az);
+                }
+            }
+            gd_kahan_add(tot.x, tot.cx, ax);
+            gd_kahan_add(tot.y, tot.cy, ay);
+            gd_kahan_add(tot.z, tot.cz, az);
+            if (PHI) gd_kahan_add(phs, phc, ph);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (valid) {
+        acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
+        if (PHI) phi[i] = -G * phs;
+    }
+}
+
 /* Walk scratch: stats[4], overflow count, overflow list, big stacks. */
 struct gd_walk_scratch {
     unsigned long long* stats;
@@ -506,13 +850,17 @@ struct gd_walk_scratch {
     int* bigStacks;
     int listCap;
     int bigSlots;
+    int* lists;            /* groups x entriesPerGroup interaction lists */
+    int2* counts;          /* per group (front, back) */
+    long long listBytes;
+    int entriesPerGroup;
 };
 
-static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, 0, 0};
+static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, 0, 16384};
 
 /* optional HIP-event timing of the walk kernel (null stream) */
 static int g_walkTiming = 0;
-static hipEvent_t g_walkEv[2] = {nullptr, nullptr};
+static hipEvent_t g_walkEv[4] = {nullptr, nullptr, nullptr, nullptr};
 static int g_walkTimed = 0;
 
 static void gd_walk_ensure(int groups) {
@@ -524,8 +872,16 @@ static void gd_walk_ensure(int groups) {
     }
     if (groups > g_walk.listCap) {
         if (g_walk.overflowList) GD_HIP(hipFree(g_walk.overflowList));
+        if (g_walk.counts) GD_HIP(hipFree(g_walk.counts));
         GD_HIP(hipMalloc(&g_walk.overflowList, sizeof(int) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.counts, sizeof(int2) * (size_t)groups));
         g_walk.listCap = groups;
+    }
+    const long long need = (long long)groups * g_walk.entriesPerGroup * (long long)sizeof(int);
+    if (need > g_walk.listBytes) {
+        if (g_walk.lists) GD_HIP(hipFree(g_walk.lists));
+        GD_HIP(hipMalloc(&g_walk.lists, (size_t)need));
+        g_walk.listBytes = need;
     }
 }
 
@@ -555,50 +911,61 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.overflowCount);
     GD_LAUNCHED();
     if (g_walkTiming) {
-        if (!g_walkEv[0]) {
-            GD_HIP(hipEventCreate(&g_walkEv[0]));
-            GD_HIP(hipEventCreate(&g_walkEv[1]));
-        }
+        if (!g_walkEv[0])
+            for (int e = 0; e < 4; ++e) GD_HIP(hipEventCreate(&g_walkEv[e]));
         GD_HIP(hipEventRecord(g_walkEv[0], nullptr));
     }
-    const int blocks = gd_div_up(groups, GD_FW_WAVES);
 #define GD_WALK_ARGS(LIST, STACKS)                                                             \
     pos, b->wcom, b->wlink, b->wquad, b->dinfo, acc, phi, eps2, G, theta, starCount,           \
         g_walk.stats, LIST, g_walk.overflowCount, STACKS, W, Bv, minRatio
+    const int cap = g_walk.entriesPerGroup;
+    const int tblocks = gd_div_up(groups, GD_TR_WAVES);
     if (mode == 0)
-        k_walk_fast<false, 0><<<blocks, 64 * GD_FW_WAVES>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
-    else if (mode == 1)
-        k_walk_fast<false, 1><<<blocks, 64 * GD_FW_WAVES>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+        k_walk_traverse<false><<<tblocks, 64 * GD_TR_WAVES>>>(
+            pos, b->wcom, b->wlink, b->dinfo, theta, starCount, cap, g_walk.lists, g_walk.counts,
+            g_walk.overflowList, g_walk.overflowCount, g_walk.stats);
     else
-        k_walk_fast<false, 2><<<blocks, 64 * GD_FW_WAVES>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+        k_walk_traverse<true><<<tblocks, 64 * GD_TR_WAVES>>>(
+            pos, b->wcom, b->wlink, b->dinfo, theta, starCount, cap, g_walk.lists, g_walk.counts,
+            g_walk.overflowList, g_walk.overflowCount, g_walk.stats);
     GD_LAUNCHED();
-    if (g_walkTiming) {
-        GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
-        g_walkTimed = 1;
+    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
+    const int eblocks = gd_div_up(groups, GD_EV_WAVES);
+#define GD_EVAL_ARGS                                                                           \
+    pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.lists, g_walk.counts, cap, acc, phi,    \
+        eps2, G, starCount, W, Bv, minRatio
+    if (mode == 0) {
+        if (phi) k_walk_eval<0, true><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+        else k_walk_eval<0, false><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+    } else if (mode == 1) {
+        if (phi) k_walk_eval<1, true><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+        else k_walk_eval<1, false><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+    } else {
+        if (phi) k_walk_eval<2, true><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+        else k_walk_eval<2, false><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
     }
-    /* fallback for groups whose stack outgrew LDS */
+    GD_LAUNCHED();
+    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
+#undef GD_EVAL_ARGS
+    /* groups whose list or stack did not fit: the fused walk with a global stack */
     int over = 0;
     GD_HIP(hipMemcpy(&over, g_walk.overflowCount, sizeof(int), hipMemcpyDeviceToHost));
     if (over > 0) {
-        const int chunk = 4096;
-        if (g_walk.bigSlots < (over < chunk ? over : chunk)) {
-            if (g_walk.bigStacks) GD_HIP(hipFree(g_walk.bigStacks));
-            g_walk.bigSlots = over < chunk ? over : chunk;
-            GD_HIP(hipMalloc(&g_walk.bigStacks,
-                             sizeof(int) * (size_t)g_walk.bigSlots * GD_FW_BIG_STACK));
-        }
-        for (int s = 0; s < over; s += g_walk.bigSlots) {
-            const int m = (over - s) < g_walk.bigSlots ? (over - s) : g_walk.bigSlots;
-            if (mode == 0)
-                k_walk_fast<true, 0><<<m, 64>>>(GD_WALK_ARGS(g_walk.overflowList + s, g_walk.bigStacks));
-            else if (mode == 1)
-                k_walk_fast<true, 1><<<m, 64>>>(GD_WALK_ARGS(g_walk.overflowList + s, g_walk.bigStacks));
-            else
-                k_walk_fast<true, 2><<<m, 64>>>(GD_WALK_ARGS(g_walk.overflowList + s, g_walk.bigStacks));
-            GD_LAUNCHED();
-        }
+        if (mode == 0)
+            k_walk_fast<true, 0><<<over, 64>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+        else if (mode == 1)
+            k_walk_fast<true, 1><<<over, 64>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+        else
+            k_walk_fast<true, 2><<<over, 64>>>(GD_WALK_ARGS(g_walk.overflowList, nullptr));
+        GD_LAUNCHED();
     }
 #undef GD_WALK_ARGS
+    if (g_walkTiming) {
+        GD_HIP(hipEventRecord(g_walkEv[3], nullptr));
+        g_walkTimed = 1;
+    }
+    /* more than 1 group in 256 overflowed its list: give later walks room */
+    if (over > groups / 256 && g_walk.entriesPerGroup < 65536) g_walk.entriesPerGroup *= 2;
     if (over > 0 || stats) {
         unsigned long long h[4];
         GD_HIP(hipMemcpy(h, g_walk.stats, sizeof(h), hipMemcpyDeviceToHost));
@@ -607,6 +974,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         stats->interactions = h[0];
         stats->opened = h[1];
         stats->overflowGroups = (unsigned long long)over;
+        stats->cellInteractions = h[3];
     }
 }
 
@@ -630,9 +998,14 @@ extern "C" void gd_walk_learned(const treeState* tree, const float4* pos, float3
 extern "C" void gd_walk_timing(int enable) { g_walkTiming = enable ? 1 : 0; }
 
 extern "C" float gd_walk_kernel_ms(void) {
-    if (!g_walkTimed) return -1.0f;
-    float ms = 0.f;
-    GD_HIP(hipEventSynchronize(g_walkEv[1]));
-    GD_HIP(hipEventElapsedTime(&ms, g_walkEv[0], g_walkEv[1]));
-    return ms;
+    float ms[3];
+    if (gd_walk_phase_ms(ms) != 0) return -1.0f;
+    return ms[1];
+}
+
+extern "C" int gd_walk_phase_ms(float* ms) {
+    if (!g_walkTimed) return -1;
+    GD_HIP(hipEventSynchronize(g_walkEv[3]));
+    for (int e = 0; e < 3; ++e) GD_HIP(hipEventElapsedTime(&ms[e], g_walkEv[e], g_walkEv[e + 1]));
+    return 0;
 }

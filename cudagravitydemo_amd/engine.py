@@ -140,7 +140,8 @@ class Simulation:
         check(self.lib)
         if st is not None:
             return {"interactions": st.interactions, "opened": st.opened,
-                    "overflow_groups": st.overflowGroups}
+                    "overflow_groups": st.overflowGroups,
+                    "cell_interactions": st.cellInteractions}
         return None
 
     def step(self, dt, theta):

@@ -1,0 +1,66 @@
+"""Profiling driver: a few engine steps of the two-disk merger, for rocprofv3.
+
+Runs the same workload as bench.py (2^21 bodies by default) for --steps
+steps with the chosen walk, then (with --copy-mib) one stream copy of a known
+size, so that hardware byte counters of the step's kernels can be scaled by
+what the same counters read for the copy.  Prints one line per phase.
+
+Examples (from the repository root):
+  rocprofv3 --kernel-trace --stats -d OUT -o run -- python3 cudagravitydemo_amd/profile_step.py
+  rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 cudagravitydemo_amd/profile_step.py --steps 2 --copy-mib 256
+"""
+import argparse
+import os
+import sys
+import time
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from cudagravitydemo_amd import Simulation, ics  # noqa: E402
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--per-disk", type=int, default=1 << 20)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--walk", choices=["fast", "quad", "reference"], default="fast")
+    ap.add_argument("--copy-mib", type=int, default=0)
+    a = ap.parse_args()
+    pos, vel = ics.disk_merger(a.per_disk, seed=2024)
+    n = pos.shape[0]
+    eps, dt = 0.005, 5e-4
+    with Simulation(pos, vel, eps=eps) as sim:
+        if a.walk == "quad":
+            sim.set_quadrupole(True)
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            if a.walk == "reference":
+                sim.step_reference(dt, a.theta)
+            else:
+                sim.step(dt, a.theta)
+        sim.sync()
+        t1 = time.perf_counter()
+        print(f"profile_step: {n} bodies, {a.steps} steps, walk={a.walk}, theta={a.theta}: "
+              f"{1e3 * (t1 - t0) / a.steps:.3f} ms/step", flush=True)
+        if a.copy_mib:
+            lib = sim.lib
+            nbytes = a.copy_mib << 20
+            src = lib.gd_malloc(nbytes)
+            dst = lib.gd_malloc(nbytes)
+            lib.gd_memset(src, 1, nbytes)
+            lib.gd_synchronize()
+            e0, e1 = lib.gd_event_create(), lib.gd_event_create()
+            lib.gd_event_record(e0)
+            lib.gd_stream_copy(dst, src, nbytes)
+            lib.gd_event_record(e1)
+            ms = lib.gd_event_elapsed_ms(e0, e1)
+            print(f"profile_step: stream copy {a.copy_mib} MiB in {ms:.3f} ms = "
+                  f"{2 * nbytes / ms / 1e6:.1f} GB/s (read + write)", flush=True)
+            lib.gd_free(src)
+            lib.gd_free(dst)
+
+
+if __name__ == "__main__":
+    main()

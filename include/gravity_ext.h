@@ -70,9 +70,11 @@ void gd_tree_levels(treeState* tree);
 
 /* ---- production walk ------------------------------------------------------ */
 typedef struct {
-    unsigned long long interactions;   /* accepted body-node interactions */
-    unsigned long long opened;         /* node tests that opened the node */
-    unsigned long long overflowGroups; /* groups re-walked by the fallback */
+    unsigned long long interactions;     /* accepted body-node interactions */
+    unsigned long long opened;           /* node tests that opened the node */
+    unsigned long long overflowGroups;   /* groups walked by the fallback */
+    unsigned long long cellInteractions; /* of the interactions, those with internal
+                                            cells evaluated as quadrupoles (+ network) */
 } gd_walk_stats;
 
 /*
@@ -88,11 +90,16 @@ typedef struct {
 void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, float eps2,
              float G, float theta, int starCount, gd_walk_stats* stats);
 
-/* Time the wavefront-walk kernel of every later gd_walk call with HIP events
- * on the null stream (enable = 1) or stop (0).  gd_walk_kernel_ms returns the
- * last timed kernel's duration in ms (synchronises), or -1 if none. */
+/* Time every later gd_walk / gd_walk_learned call with HIP events on the
+ * null stream (enable = 1) or stop (0).  The walk is a traversal kernel that
+ * writes each group's interaction list, an evaluation kernel (the dominant
+ * one) and, for groups whose list did not fit, a fallback walk.
+ * gd_walk_kernel_ms returns the last evaluation kernel's duration in ms;
+ * gd_walk_phase_ms fills ms[0..2] with traversal, evaluation and fallback
+ * times and returns 0 (or -1 when nothing was timed).  Both synchronise. */
 void gd_walk_timing(int enable);
 float gd_walk_kernel_ms(void);
+int gd_walk_phase_ms(float* ms);
 
 /* ---- learned far-field corrector ------------------------------------------
  * Network 8 -> 64 -> 64 -> 64 -> 3 with GELU, BF16 weights/activations and
@@ -101,8 +108,8 @@ float gd_walk_kernel_ms(void);
  * W3[64][64] @4736, b3 @8832, W4[3][64] @8896, b4 @9088 (9091 values).
  * Inputs per (point, cell): unit vector to the cell's COM (3), size / r,
  * S u / (m s^2) (3), tr S / (m s^2); output: the correction of the cell's
- * monopole acceleration in units of G m s^2 / r^4.  X (rows x 8), Y (rows x 3)
- * and order are device arrays. */
+ * acceleration beyond its monopole and quadrupole terms, in units of
+ * G m s^2 / r^4.  X (rows x 8), Y (rows x 3) and order are device arrays. */
 #define GD_MLP_PARAMS 9091
 typedef struct gd_mlp gd_mlp;
 gd_mlp* gd_mlp_create(const float* hostParams);          /* NULL: all zero */
@@ -123,16 +130,17 @@ void gd_mlp_adam_state(const gd_mlp* net, float* hostM, float* hostV);
 /* Training samples from the current tree (built with second moments): for
  * each target body (device indices, sorted order), every internal cell a
  * per-body traversal at opening angle theta accepts gives one row of X and
- * the exact scaled correction in Y (FP64 sum over the cell's bodies minus the
- * softened monopole).  Returns the number of rows written (at most cap);
- * *droppedTargets counts targets whose traversal outgrew its stack. */
+ * the exact scaled residual in Y (FP64 sum over the cell's bodies minus the
+ * softened monopole and quadrupole terms).  Returns the number of rows
+ * written (at most cap); *droppedTargets counts targets whose traversal
+ * outgrew its stack. */
 long long gd_corrector_samples(const treeState* tree, const float4* pos, const int* targets,
                                int nt, float theta, float eps2, float* X, float* Y,
                                long long cap, int* droppedTargets);
-/* gd_walk with monopole cells plus the network's correction of every
- * accepted cell whose size / r (from the group's centre) exceeds minRatio,
- * evaluated once per (64-body group, cell) on the matrix cores.  Needs a tree
- * built with second moments. */
+/* gd_walk with quadrupole cells plus the network's residual correction of
+ * every accepted cell whose size / r (from the group's centre) exceeds
+ * minRatio, evaluated once per (64-body group, cell) on the matrix cores and
+ * scaled per body by m s^2 / r^4.  Needs a tree built with second moments. */
 void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc, float* phi,
                      float eps2, float G, float theta, int starCount, const gd_mlp* net,
                      float minRatio, gd_walk_stats* stats);
@@ -157,6 +165,11 @@ void gd_direct_sum_fp64(const float4* pos, const float3* posLo, int starCount,
  * FP64 on the device; out[6] = total energy. */
 void gd_diagnostics(const float4* pos, const float3* vel, const float* phi, int starCount,
                     double* out);
+
+/* Copies `bytes` (a multiple of 16) from src to dst with a plain streaming
+ * kernel (k_stream_copy): a known-size reference for bandwidth and for
+ * calibrating hardware byte counters. */
+void gd_stream_copy(void* dst, const void* src, size_t bytes);
 
 /* ---- timing ----------------------------------------------------------------
  * HIP events on the null stream, the stream every entry point launches on. */

@@ -145,13 +145,20 @@ def target(x, bodies, com, mass, size, eps2, G=1.0):
     return (exact - mono) / scale
 
 
-def quadrupole_correction(x, com, mass, size, S, G=1.0):
-    """The analytic quadrupole term in the same scaled units as target()."""
+def quadrupole_correction(x, com, mass, size, S, G=1.0, eps2=0.0):
+    """The analytic quadrupole term (softened r as the walk evaluates it) in
+    the same scaled units as target()."""
     d = np.asarray(com, np.float64) - np.asarray(x, np.float64)
     r2 = d @ d
-    r = np.sqrt(r2)
+    rs2 = r2 + eps2
     S = np.asarray(S, np.float64)
     sd = S @ d
     q = d @ sd
-    a = G * ((7.5 * q / r2 - 1.5 * np.trace(S)) * d - 3.0 * sd) / r ** 5
+    a = G * ((7.5 * q / rs2 - 1.5 * np.trace(S)) * d - 3.0 * sd) / rs2 ** 2.5
     return a / (G * mass / r2 * (size * size / r2))
+
+
+def residual_target(x, bodies, com, mass, size, S, eps2, G=1.0):
+    """What the network learns: target() minus the analytic quadrupole term."""
+    return (target(x, bodies, com, mass, size, eps2, G)
+            - quadrupole_correction(x, com, mass, size, S, G, eps2))

@@ -97,39 +97,47 @@ def test_samples_match_direct_sums():
         total = sim.tree_summary()[0]
         com, link, quad = sim.walk_arrays(total)
         from oracle.binding import NODE_DTYPE
-        nodes = sim.nodes_raw(total).view(NODE_DTYPE)
+        nodes = sim.nodes_raw(total).view(NODE_DTYPE).reshape(-1)
         ps = sim.positions()
         ext = float(np.max(ps[:, :3].max(0) - ps[:, :3].min(0)))
         found = 0
         rng = np.random.default_rng(0)
+        cand = np.flatnonzero(link[:, 0] >= 0)
         for row in rng.choice(n, 40, replace=False):
-            # find the (target, cell) of this row: match features against every candidate
+            # find the (target, cell) of this row among all internal cells: a node and its
+            # only child share the centre of mass, so match on every feature
+            best = None
             for t in targets:
                 x = ps[t, :3].astype(np.float64)
-                cand = np.flatnonzero(link[:, 0] >= 0)
                 d = com[cand, :3] - x
-                r = np.linalg.norm(d, axis=1)
-                u = d / r[:, None]
-                hit = cand[np.all(np.abs(u - X[row, :3]) < 1e-6, axis=1)]
-                if hit.size:
-                    idx = int(hit[0])
+                u = d / np.linalg.norm(d, axis=1)[:, None]
+                for idx in cand[np.all(np.abs(u - X[row, :3]) < 1e-6, axis=1)]:
                     size = ext * 2.0 ** -int(nodes[idx]["level"])
                     S = np.array([[quad[idx, 0], quad[idx, 3], quad[idx, 4]],
                                   [quad[idx, 3], quad[idx, 1], quad[idx, 5]],
                                   [quad[idx, 4], quad[idx, 5], quad[idx, 2]]])
                     f = mlp.features(x, com[idx, :3], com[idx, 3], size, S)
-                    a, c = int(nodes[idx]["first"]), int(nodes[idx]["count"])
-                    y = mlp.target(x, ps[a:a + c], com[idx, :3], com[idx, 3], size, 0.01 ** 2)
-                    assert np.abs(f - X[row]).max() < 1e-4 * max(1, np.abs(f).max())
-                    assert np.abs(y - Y[row]).max() < 1e-3 * max(1.0, np.abs(y).max())
-                    found += 1
-                    break
+                    dev = np.abs(f - X[row]).max()
+                    if best is None or dev < best[0]:
+                        best = (dev, x, idx, size, f)
+            assert best is not None
+            dev, x, idx, size, f = best
+            assert dev < 1e-4 * max(1, np.abs(f).max())
+            a, c = int(nodes[idx]["first"]), int(nodes[idx]["count"])
+            S = np.array([[quad[idx, 0], quad[idx, 3], quad[idx, 4]],
+                          [quad[idx, 3], quad[idx, 1], quad[idx, 5]],
+                          [quad[idx, 4], quad[idx, 5], quad[idx, 2]]])
+            y = mlp.residual_target(x, ps[a:a + c], com[idx, :3], com[idx, 3], size, S,
+                                    0.01 ** 2)
+            assert np.abs(y - Y[row]).max() < 1e-3 * max(1.0, np.abs(y).max())
+            found += 1
         assert found >= 30
 
 
 def test_trained_corrector_gains_over_plain_walk():
-    """Train on snapshots of a disk, then compare at theta 0.8: the corrected
-    walk's median error vs FP64 is below the plain walk's."""
+    """Train on snapshots of a disk, then compare at theta 0.8 against FP64:
+    the learned walk (quadrupole + learned residual) beats the monopole walk
+    and does not lose to the quadrupole walk it refines."""
     pos, vel = ics.exponential_disk(1 << 16, seed=21)
     n = pos.shape[0]
     eps = 0.01
@@ -153,10 +161,14 @@ def test_trained_corrector_gains_over_plain_walk():
             sim.walk(0.8)
             plain, _ = cr.median_error(sim, sample)
             sim.set_quadrupole(True)
+            sim.walk(0.8)
+            quad, _ = cr.median_error(sim, sample)
             cr.walk_learned(sim, net, 0.8)
             learned, _ = cr.median_error(sim, sample)
-            print(f"theta 0.8 median error: plain {plain:.3e} learned {learned:.3e}", flush=True)
-            assert learned < 0.7 * plain
+            print(f"theta 0.8 median error: plain {plain:.3e} quadrupole {quad:.3e} "
+                  f"learned {learned:.3e}", flush=True)
+            assert learned < 0.3 * plain
+            assert learned < 1.05 * quad
         finally:
             net.close()
             s.close()

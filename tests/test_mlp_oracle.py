@@ -79,6 +79,8 @@ def test_target_approaches_quadrupole_far_away():
         qc = mlp.quadrupole_correction(x, com, m, size, S)
         # the remainder is the octupole and higher: O(s/r) relative to the quadrupole
         assert np.linalg.norm(t - qc) <= tol * np.linalg.norm(qc), dist
+    r = mlp.residual_target(x, bodies, com, m, size, S, 0.0)
+    assert np.linalg.norm(r) <= 0.03 * np.linalg.norm(t)
     f = mlp.features(x, com, m, size, S)
     assert f.shape == (8,) and abs(np.linalg.norm(f[:3]) - 1) < 1e-12
     assert abs(f[3] - size / np.linalg.norm(com - x)) < 1e-12
