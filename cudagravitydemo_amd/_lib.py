@@ -111,6 +111,7 @@ _PROTOS = {
     "gd_walk": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32,
                        C.POINTER(WalkStats)]),
     "gd_walk_timing": (None, [i32]),
+    "gd_walk_set_list_capacity": (i32, [i32]),
     "gd_walk_kernel_ms": (f32, []),
     "gd_walk_phase_ms": (i32, [vp]),
     "gd_mlp_create": (vp, [vp]),

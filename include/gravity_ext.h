@@ -72,7 +72,7 @@ void gd_tree_levels(treeState* tree);
 typedef struct {
     unsigned long long interactions;     /* accepted body-node interactions */
     unsigned long long opened;           /* node tests that opened the node */
-    unsigned long long overflowGroups;   /* groups walked by the fallback */
+    unsigned long long overflowGroups;   /* groups walked body by body */
     unsigned long long cellInteractions; /* of the interactions, those with internal
                                             cells evaluated as quadrupoles (+ network) */
 } gd_walk_stats;
@@ -81,8 +81,12 @@ typedef struct {
  * Wavefront walk: one 64-body group per wavefront, the group's bounding box
  * tested against 64 nodes at a time (size^2 < theta^2 d^2, d the distance
  * from the box to the node's centre of mass, size = bbox cube extent / 2^level),
- * accepted nodes gathered in an LDS interaction list and evaluated by all 64
- * bodies; per-body sums are FP32, compensated (Kahan) across list flushes.
+ * accepted nodes written to the group's interaction list in device memory
+ * and evaluated by all 64 bodies; per-body sums are FP32, compensated
+ * (Kahan) across 64-entry chunks.  A group whose list does not fit its
+ * capacity is walked body by body instead (the same acceptance test against
+ * the body's own position).  With the second moments built
+ * (gd_set_quadrupole), accepted internal cells are evaluated as quadrupoles.
  * Needs a tree built by buildTree or gd_build_tree_async.  stats may be
  * NULL; when given, counters are accumulated on the device and copied out
  * (synchronises).  phi (may be NULL) receives the softened potential.
@@ -90,13 +94,19 @@ typedef struct {
 void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, float eps2,
              float G, float theta, int starCount, gd_walk_stats* stats);
 
+/* Interaction-list capacity per 64-body group (entries; 4 bytes each, for
+ * every group).  0 restores the default: 16384, doubled (up to 65536) after
+ * a walk in which more than one group in 256 did not fit.  A value set here
+ * is kept as it is.  Returns the previous capacity. */
+int gd_walk_set_list_capacity(int entries);
+
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
  * null stream (enable = 1) or stop (0).  The walk is a traversal kernel that
  * writes each group's interaction list, an evaluation kernel (the dominant
- * one) and, for groups whose list did not fit, a fallback walk.
+ * one) and, for groups whose list did not fit, the per-body walk.
  * gd_walk_kernel_ms returns the last evaluation kernel's duration in ms;
- * gd_walk_phase_ms fills ms[0..2] with traversal, evaluation and fallback
- * times and returns 0 (or -1 when nothing was timed).  Both synchronise. */
+ * gd_walk_phase_ms fills ms[0..2] with traversal, evaluation and per-body
+ * walk times and returns 0 (or -1 when nothing was timed).  Both synchronise. */
 void gd_walk_timing(int enable);
 float gd_walk_kernel_ms(void);
 int gd_walk_phase_ms(float* ms);

@@ -1,0 +1,162 @@
+"""Production walk (list walk and per-body walk) against its own interaction
+sets summed in FP64.
+
+tests/walk_check.py restates the acceptance test in float32 with the
+kernels' operation order, so the CPU finds the very nodes each GPU walk
+accepts; their monopole / quadrupole / learned terms are summed in float64.
+Bars: 1e-5 relative (vector norm) for the monopole and quadrupole walks; in
+learned mode the network's share is held to 1 % (the BF16 network itself is
+pinned against the oracle in test_gpu_mlp.py) and the rest to 1e-5.  The
+per-body walk runs when a group's list does not fit its capacity; the tests
+force it with gd_walk_set_list_capacity.
+"""
+import numpy as np
+import pytest
+
+from cudagravitydemo_amd import Simulation, ics, load_library
+from cudagravitydemo_amd import corrector as cr
+from oracle import mlp
+from tests.walk_check import accepted, walk_fp64, world_extent
+
+pytestmark = pytest.mark.gpu
+
+N = 16384 - 37          # ragged last group
+THETA = 0.5
+EPS = 0.01
+
+
+def _net_params():
+    p = mlp.init_params(21).astype(np.float64)
+    layers = mlp.unpack(p)
+    W4, b4 = layers[3]
+    layers[3] = (W4 * 1e-2, b4)
+    return mlp.pack(layers).astype(np.float32)
+
+
+def _group_box(ps, i, n):
+    g = i // 64
+    blk = ps[64 * g:min(64 * g + 64, n), :3]
+    return blk.min(0), blk.max(0)
+
+
+def _run(sim, mode, cap, params):
+    lib = load_library()
+    lib.gd_walk_set_list_capacity(cap)
+    try:
+        if mode == 2:
+            net = cr.Corrector(params)
+            try:
+                st = cr.walk_learned(sim, net, THETA, stats=True)
+            finally:
+                net.close()
+        else:
+            st = sim.walk(THETA, stats=True)
+    finally:
+        lib.gd_walk_set_list_capacity(0)
+    return st
+
+
+def _sample(n):
+    rng = np.random.default_rng(8)
+    s = np.r_[0, 1, 63, 64, 4095, n - 1, n - 20, rng.choice(n, 60, replace=False)]
+    return np.unique(s)
+
+
+def _check(sim, mode, per_body_of, params):
+    ps = sim.positions()
+    acc = sim.accelerations()
+    total = sim.tree_summary()[0]
+    com, link, quad = sim.walk_arrays(total)
+    lo, hi = sim.bbox()
+    ext = world_extent(lo, hi)
+    net = (lambda X: mlp.forward(params, X, bf16_ops=True)[0]) if mode == 2 else None
+    worst = 0.0
+    for i in _sample(sim.n):
+        x = ps[i, :3]
+        box = (x, x) if per_body_of(i) else _group_box(ps, i, sim.n)
+        want, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
+                            sim.eps * sim.eps, mode, net)
+        want = sim.G * want
+        tol = 1e-5
+        if mode == 2:
+            base, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
+                                sim.eps * sim.eps, 1)
+            tol += 1e-2 * np.linalg.norm(want - sim.G * base) / np.linalg.norm(want)
+        err = np.linalg.norm(acc[i] - want) / np.linalg.norm(want)
+        worst = max(worst, err / tol)
+        assert err < tol, (i, err, tol, per_body_of(i))
+    print(f"mode {mode}: worst error / tolerance {worst:.3f}", flush=True)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("per_body", [False, True], ids=["lists", "per_body"])
+def test_walk_matches_interaction_set_fp64(mode, per_body):
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode != 0)
+        sim.build_tree_async()
+        st = _run(sim, mode, 1 if per_body else 0, params)
+        groups = (sim.n + 63) // 64
+        assert st["overflow_groups"] == (groups if per_body else 0)
+        _check(sim, mode, lambda i: per_body, params)
+        if mode == 2:
+            # the network's share is well above the 1e-5 bar: the check has teeth
+            a2 = sim.accelerations()
+            sim.walk(THETA)
+            a1 = sim.accelerations()
+            d = np.linalg.norm(a2 - a1, axis=1) / np.linalg.norm(a1, axis=1)
+            assert np.median(d) > 1e-4
+
+
+def test_walk_counts_match_interaction_sets():
+    """Interaction and cell counts of the list walk equal the CPU sets'."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(True)
+        sim.build_tree_async()
+        st = sim.walk(THETA, stats=True)
+        ps = sim.positions()
+        total = sim.tree_summary()[0]
+        com, link, quad = sim.walk_arrays(total)
+        lo, hi = sim.bbox()
+        ext = world_extent(lo, hi)
+        inter = cells = 0
+        for g in range((sim.n + 63) // 64):
+            blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
+            lv, cl = accepted(com, link, ext, blk.min(0), blk.max(0), THETA)
+            inter += (lv.size + cl.size) * blk.shape[0]
+            cells += cl.size * blk.shape[0]
+        assert st["overflow_groups"] == 0
+        assert st["interactions"] == inter
+        assert st["cell_interactions"] == cells
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mixed_lists_and_per_body(mode):
+    """Capacity at the median group list: about half the groups are walked
+    body by body, the rest from their lists, in one walk."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode != 0)
+        sim.build_tree_async()
+        ps = sim.positions()
+        total = sim.tree_summary()[0]
+        com, link, _ = sim.walk_arrays(total)
+        lo, hi = sim.bbox()
+        ext = world_extent(lo, hi)
+        groups = (sim.n + 63) // 64
+        sizes = np.empty(groups, np.int64)
+        for g in range(groups):
+            blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
+            lv, cl = accepted(com, link, ext, blk.min(0), blk.max(0), THETA)
+            sizes[g] = lv.size + cl.size
+        cap = int(np.median(sizes))
+        st = _run(sim, mode, cap, params)
+        over = sizes > cap
+        # a group can also leave the list walk when its traversal stack is full
+        assert st["overflow_groups"] >= over.sum()
+        assert 0 < over.sum() < groups
+        if st["overflow_groups"] == over.sum():
+            _check(sim, mode, lambda i: bool(over[i // 64]), params)

@@ -1,0 +1,119 @@
+"""Pins the CPU restatement of the production walk's interaction sets
+(tests/walk_check.py) on trees built by the oracle (CPU only).
+
+Checks from what the method fixes: at theta = 0 the set is every leaf and its
+monopole sum is the direct sum; for any theta and box the accepted nodes
+partition the bodies, every accepted cell passes the opening test and every
+ancestor of an accepted node fails it; the quadrupole sum of a cell matches
+the brute-force field of its bodies to O((s/r)^3) while the monopole is off
+at O((s/r)^2).
+"""
+import numpy as np
+import pytest
+
+from cudagravitydemo_amd import ics
+from oracle import binding as ob
+from oracle import mlp
+from tests.walk_check import accepted, features, monopole, quadrupole, world_extent
+
+
+@pytest.fixture(scope="module")
+def tree():
+    pos, vel = ics.plummer(3000, seed=4)
+    t = ob.build_tree(pos, vel)
+    nodes = t["nodes"]
+    total = nodes.shape[0]
+    com = nodes["mass"].astype(np.float32)
+    link = np.zeros((total, 2), np.int32)
+    for j in range(total):
+        ch = nodes["child"][j]
+        ch = np.sort(ch[ch >= 0])
+        if ch.size == 0:
+            link[j, 0] = -(nodes["first"][j] + 1)
+        else:
+            # children of a node are consecutive in the level-wise layout
+            assert np.array_equal(ch, np.arange(ch[0], ch[0] + ch.size))
+            link[j, 0] = ch[0]
+        link[j, 1] = (ch.size << 5) | nodes["level"][j]
+    ext = world_extent(t["bmin"], t["bmax"])
+    return dict(t=t, nodes=nodes, com=com, link=link, ext=ext)
+
+
+def _boxes(ps):
+    rng = np.random.default_rng(3)
+    out = []
+    for i in rng.choice(ps.shape[0], 6, replace=False):
+        out.append((ps[i, :3], ps[i, :3]))
+    for g in rng.choice(ps.shape[0] // 64, 6, replace=False):
+        blk = ps[64 * g:64 * g + 64, :3]
+        out.append((blk.min(0), blk.max(0)))
+    return out
+
+
+def test_theta_zero_is_every_leaf_and_the_direct_sum(tree):
+    t, com, link = tree["t"], tree["com"], tree["link"]
+    ps = t["pos"]
+    leaf_nodes = np.flatnonzero(link[:, 0] < 0)
+    for lo, hi in _boxes(ps)[:3]:
+        lv, cl = accepted(com, link, tree["ext"], lo, hi, 0.0)
+        assert cl.size == 0 and np.array_equal(lv, leaf_nodes)
+    x = ps[17, :3].astype(np.float64)
+    a = monopole(x, com, leaf_nodes, 1e-4)
+    want = ob.direct_sum_fp64(ps, np.array([17], np.int32), 1e-4)[0]
+    # leaf COMs are float32 (x m) / m: equal to the body position to FP32 rounding
+    assert np.linalg.norm(a - want) <= 1e-6 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.7, 1.2])
+def test_accepted_nodes_partition_bodies_and_pass_the_test(tree, theta):
+    t, nodes, com, link, ext = tree["t"], tree["nodes"], tree["com"], tree["link"], tree["ext"]
+    ps = t["pos"]
+    n = ps.shape[0]
+    for lo, hi in _boxes(ps):
+        lv, cl = accepted(com, link, ext, lo, hi, theta)
+        acc = np.concatenate([lv, cl])
+        first, cnt = nodes["first"][acc], nodes["count"][acc]
+        cover = np.zeros(n, np.int64)
+        for a, k in zip(first, cnt):
+            cover[a:a + k] += 1
+        assert (cover == 1).all()
+
+        def crit(j):
+            c = com[j, :3].astype(np.float64)
+            d = np.maximum(np.maximum(lo - c, c - hi), 0.0)
+            size = np.ldexp(float(ext), -int(nodes["level"][j]))
+            return size * size, theta * theta * (d @ d)
+
+        for j in cl:
+            s2, rhs = crit(j)
+            assert s2 <= rhs * (1 + 1e-5)
+        for j in acc:
+            p = int(nodes["parent"][j])
+            if p >= 0 and j != 0:
+                s2, rhs = crit(p)
+                assert s2 >= rhs * (1 - 1e-5), (j, p)
+
+
+def test_quadrupole_term_against_brute_force():
+    rng = np.random.default_rng(2)
+    b = np.c_[rng.random((300, 3)) * [1.0, 0.6, 0.3], rng.random(300) + 0.5]
+    m = b[:, 3].sum()
+    c = (b[:, :3] * b[:, 3:]).sum(0) / m
+    d = b[:, :3] - c
+    S = (b[:, 3, None, None] * d[:, :, None] * d[:, None, :]).sum(0)
+    com = np.r_[c, m][None].astype(np.float64)
+    quad = np.array([[S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2], np.trace(S), 0]])
+    for dist in (10.0, 20.0):
+        x = c + dist * np.array([0.48, -0.6, 0.64])
+        r = b[:, :3] - x
+        exact = (b[:, 3:4] * r / ((r * r).sum(1) ** 1.5)[:, None]).sum(0)
+        q = quadrupole(x, com, quad, np.array([0]), 0.0)
+        mono = monopole(x, com, np.array([0]), 0.0)
+        eq = np.linalg.norm(q - exact) / np.linalg.norm(exact)
+        em = np.linalg.norm(mono - exact) / np.linalg.norm(exact)
+        assert eq < 2.0 / dist ** 3 and em > 0.05 / dist ** 2 and eq < 0.2 * em
+    # the network's features are the oracle's
+    link = np.array([[5, (3 << 5) | 2]], np.int32)
+    X, k = features(x, com, quad, np.array([0]), 4.0, link)
+    want = mlp.features(x, c, m, 1.0, S)
+    assert np.allclose(X[0], want, rtol=1e-12, atol=1e-15) and abs(k[0] - m) < 1e-12

@@ -1,0 +1,114 @@
+"""CPU restatement of the production walk's interaction sets (test helper).
+
+The production walk accepts node j for a body box [lo, hi] (the 64-body
+group's bounding box, or the body itself for the per-body walk) when j is a
+leaf or size_j^2 < theta^2 d^2, with d the distance from the box to j's
+centre of mass and size_j = extent / 2^level_j.  The test is restated here in
+float32 with the kernels' operation order (no fused multiply-add), so the
+accepted set is the kernels' own; the forces of that set are then summed in
+float64.  A GPU walk that matches this sum to FP32 rounding has the right
+traversal, the right node records and the right evaluation formulas.
+
+Node records are the walk arrays: com (x, y, z, m), link (first child, or
+-(first body + 1) for a leaf; count << 5 | level), second moments
+(Sxx, Syy, Szz, Sxy, Sxz, Syz, tr S, -) about the COM.
+"""
+import numpy as np
+
+f32 = np.float32
+
+
+def world_extent(lo, hi):
+    w, h, d = (f32(hi[k]) - f32(lo[k]) for k in range(3))
+    return f32(max(w, max(h, d)))
+
+
+def accepted(com, link, extent, lo, hi, theta):
+    """Accepted node indices for the box [lo, hi]: (leaves, cells), sorted."""
+    lo = np.asarray(lo, f32)
+    hi = np.asarray(hi, f32)
+    th2 = f32(theta) * f32(theta)
+    frontier = np.array([0], np.int64)
+    leaves, cells = [], []
+    while frontier.size:
+        c = com[frontier, :3].astype(f32)
+        dd = np.maximum(np.maximum(lo - c, c - hi), f32(0))
+        d2 = (dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2]
+        lk = link[frontier]
+        level = lk[:, 1] & 31
+        size = np.ldexp(f32(extent), -level).astype(f32)
+        leaf = lk[:, 0] < 0
+        acc = leaf | (size * size < th2 * d2)
+        leaves.append(frontier[acc & leaf])
+        cells.append(frontier[acc & ~leaf])
+        op = ~acc
+        first = lk[op, 0].astype(np.int64)
+        cnt = (lk[op, 1] >> 5).astype(np.int64)
+        if first.size:
+            frontier = np.concatenate([np.arange(a, a + k) for a, k in zip(first, cnt)])
+        else:
+            frontier = np.zeros(0, np.int64)
+    return np.sort(np.concatenate(leaves)), np.sort(np.concatenate(cells))
+
+
+def monopole(x, com, idx, eps2):
+    d = com[idx, :3].astype(np.float64) - x
+    r2 = (d * d).sum(1) + eps2
+    return (com[idx, 3:4].astype(np.float64) * d / (r2 * np.sqrt(r2))[:, None]).sum(0)
+
+
+def quadrupole(x, com, quad, idx, eps2):
+    """Monopole plus quadrupole of cells idx at point x (softened r)."""
+    d = com[idx, :3].astype(np.float64) - x
+    m = com[idx, 3].astype(np.float64)
+    q = quad[idx].astype(np.float64)
+    S = np.empty((idx.size, 3, 3))
+    S[:, 0, 0], S[:, 1, 1], S[:, 2, 2] = q[:, 0], q[:, 1], q[:, 2]
+    S[:, 0, 1] = S[:, 1, 0] = q[:, 3]
+    S[:, 0, 2] = S[:, 2, 0] = q[:, 4]
+    S[:, 1, 2] = S[:, 2, 1] = q[:, 5]
+    tr = q[:, 6]
+    r2 = (d * d).sum(1) + eps2
+    sd = np.einsum("kij,kj->ki", S, d)
+    qq = (d * sd).sum(1)
+    inv5 = r2 ** -2.5
+    cd = m / (r2 * np.sqrt(r2)) + inv5 * (7.5 * qq / r2 - 1.5 * tr)
+    return (cd[:, None] * d - 3.0 * inv5[:, None] * sd).sum(0)
+
+
+def features(x, com, quad, idx, extent, link):
+    """The network's 8 inputs for cells idx seen from point x, and k = m s^2."""
+    d = com[idx, :3].astype(np.float64) - x
+    r = np.sqrt((d * d).sum(1))
+    u = d / r[:, None]
+    size = np.ldexp(np.float64(extent), -(link[idx, 1] & 31))
+    m = com[idx, 3].astype(np.float64)
+    k = m * size * size
+    q = quad[idx].astype(np.float64)
+    su = np.stack([q[:, 0] * u[:, 0] + q[:, 3] * u[:, 1] + q[:, 4] * u[:, 2],
+                   q[:, 3] * u[:, 0] + q[:, 1] * u[:, 1] + q[:, 5] * u[:, 2],
+                   q[:, 4] * u[:, 0] + q[:, 5] * u[:, 1] + q[:, 2] * u[:, 2]], 1) / k[:, None]
+    X = np.concatenate([u, (size / r)[:, None], su, (q[:, 6] / k)[:, None]], 1)
+    return X, k
+
+
+def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_ratio=0.0):
+    """Acceleration (without G) at point x for the interaction set of `box`.
+    mode 0: all accepted nodes as monopoles; 1: cells as quadrupoles;
+    2: as 1 plus the network's residual, evaluated at the box centre, scaled by
+    m s^2 / r^4 with r from x (net: a callable (rows, 8) -> (rows, 3))."""
+    lo, hi = box
+    leaves, cells = accepted(com, link, extent, lo, hi, theta)
+    x = np.asarray(x, np.float64)
+    if mode == 0:
+        return monopole(x, com, np.concatenate([leaves, cells]), eps2), leaves.size + cells.size
+    a = monopole(x, com, leaves, eps2) + quadrupole(x, com, quad, cells, eps2)
+    if mode == 2 and cells.size:
+        ctr = 0.5 * (np.asarray(lo, np.float64) + np.asarray(hi, np.float64))
+        X, k = features(ctr, com, quad, cells, extent, link)
+        ok = (k > 0) & (X[:, 3] > min_ratio)
+        o = net(X[ok])
+        d = com[cells[ok], :3].astype(np.float64) - x
+        r2 = (d * d).sum(1)
+        a = a + ((k[ok] / (r2 * r2))[:, None] * o).sum(0)
+    return a, leaves.size + cells.size
