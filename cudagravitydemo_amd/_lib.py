@@ -52,7 +52,8 @@ class TreeState(C.Structure):
 
 class WalkStats(C.Structure):
     _fields_ = [("interactions", C.c_ulonglong), ("opened", C.c_ulonglong),
-                ("overflowGroups", C.c_ulonglong), ("cellInteractions", C.c_ulonglong)]
+                ("overflowGroups", C.c_ulonglong), ("cellInteractions", C.c_ulonglong),
+                ("bodyGroups", C.c_ulonglong)]
 
 
 assert C.sizeof(Node) == 80 and C.sizeof(TreeState) == 304 and C.sizeof(WorldBox) == 20
@@ -111,7 +112,7 @@ _PROTOS = {
     "gd_walk": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32,
                        C.POINTER(WalkStats)]),
     "gd_walk_timing": (None, [i32]),
-    "gd_walk_set_list_capacity": (i32, [i32]),
+    "gd_walk_set_list_capacity": (i32, [i32, i32]),
     "gd_walk_kernel_ms": (f32, []),
     "gd_walk_phase_ms": (i32, [vp]),
     "gd_mlp_create": (vp, [vp]),

@@ -157,7 +157,8 @@ def walk_learned(sim, net, theta, min_ratio=0.0, stats=False):
     check(sim.lib)
     if st is not None:
         return {"interactions": st.interactions, "opened": st.opened,
-                "overflow_groups": st.overflowGroups, "cell_interactions": st.cellInteractions}
+                "overflow_groups": st.overflowGroups, "cell_interactions": st.cellInteractions,
+                "body_groups": st.bodyGroups}
     return None
 
 

@@ -151,30 +151,39 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
  * stack) and is left to the per-body walk.
  * ------------------------------------------------------------------------- */
 #ifndef GD_TR_STACK
-#define GD_TR_STACK 1024          /* LDS stack entries per wavefront in the traversal */
+#define GD_TR_STACK 2048          /* LDS stack entries per wavefront, first pass */
 #endif
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
+#define GD_TR2_STACK 12288        /* second pass: one wavefront per block, > 22 x 512 */
 
-/* Traversal of one group by one wavefront: fills list (front and back) and
- * returns false when the list (cap entries) or the stack (stackCap) is too
- * small.  Adds the group's counts to stats. */
-template <bool SPLIT>
-__device__ __forceinline__ bool gd_traverse_group(
-    const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
-    long long g, int* stack, int stackCap, int* __restrict__ list, int cap, int* nFrontOut,
-    int* nBackOut, unsigned long long* __restrict__ stats) {
-    const unsigned lane = gd_lane();
-    const long long i = g * 64 + lane;
+/* Bounding box of group g's bodies, in every lane. */
+__device__ __forceinline__ void gd_group_box(const float4* __restrict__ pos, int n, long long g,
+                                             float& lx, float& ly, float& lz, float& hx,
+                                             float& hy, float& hz) {
+    const long long i = g * 64 + gd_lane();
     const float4 p = pos[i < n ? i : g * 64];
-    float lx = p.x, ly = p.y, lz = p.z, hx = p.x, hy = p.y, hz = p.z;
+    lx = hx = p.x; ly = hy = p.y; lz = hz = p.z;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         lx = fminf(lx, __shfl_xor(lx, off)); hx = fmaxf(hx, __shfl_xor(hx, off));
         ly = fminf(ly, __shfl_xor(ly, off)); hy = fmaxf(hy, __shfl_xor(hy, off));
         lz = fminf(lz, __shfl_xor(lz, off)); hz = fmaxf(hz, __shfl_xor(hz, off));
     }
+}
+
+/* Traversal by one wavefront for the box [l, h]: writes the accepted nodes to
+ * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
+ * SPLIT) unless list is NULL (counting only), and returns false as soon as
+ * more than limit (<= cap) entries are accepted or the stack (stackCap
+ * entries) would overflow.  *opened counts the opened nodes. */
+template <bool SPLIT>
+__device__ __forceinline__ bool gd_traverse_box(
+    const float4* __restrict__ wcom, const int2* __restrict__ wlink,
+    const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
+    float hy, float hz, int* stack, int stackCap, int* __restrict__ list, int cap, int limit,
+    int* nFrontOut, int* nBackOut, unsigned long long* opened) {
+    const unsigned lane = gd_lane();
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
@@ -217,9 +226,11 @@ __device__ __forceinline__ bool gd_traverse_group(
             total += __popcll(bal) << bit;
         }
         nOpen += (unsigned long long)__popcll(__ballot(open));
-        if (nFront + nf + nBack + nb > cap || top + total > stackCap) return false;
-        if (front) list[nFront + __popcll(fb & lt)] = idx;
-        if (back) list[cap - 1 - (nBack + __popcll(bb & lt))] = idx;
+        if (nFront + nf + nBack + nb > limit || top + total > stackCap) return false;
+        if (list) {
+            if (front) list[nFront + __popcll(fb & lt)] = idx;
+            if (back) list[cap - 1 - (nBack + __popcll(bb & lt))] = idx;
+        }
         nFront += nf;
         nBack += nb;
         if (open) {
@@ -231,40 +242,90 @@ __device__ __forceinline__ bool gd_traverse_group(
     }
     *nFrontOut = nFront;
     *nBackOut = nBack;
-    if (stats && lane == 0) {
-        const int nvalid = (int)((n - g * 64) < 64 ? (n - g * 64) : 64);
-        atomicAdd(&stats[0], (unsigned long long)(nFront + nBack) * (unsigned long long)nvalid);
-        atomicAdd(&stats[1], nOpen);
-        atomicAdd(&stats[3], (unsigned long long)nBack * (unsigned long long)nvalid);
-    }
+    *opened = nOpen;
     return true;
 }
 
-/* Traversal kernel: every group, one wavefront each, list capacity `cap`,
- * LDS stack GD_TR_STACK.  Groups that do not fit go to the overflow list. */
-template <bool SPLIT>
-__global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_traverse(
+/*
+ * Traversal kernel, one wavefront per group.  Slot s walks group
+ * groupList[s] (or group s when groupList is NULL) into lists[s*cap ...] with
+ * an LDS stack of STACK entries and sets counts[s]; a group that does not fit
+ * (cap entries, or slotCap[s] when given) gets counts[s] = (-1, -1) and goes
+ * to overflowList.  maxList receives the longest list written.
+ */
+template <bool SPLIT, int WAVES, int STACK>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
-    int cap, int* __restrict__ lists, int2* __restrict__ counts,
-    int* __restrict__ overflowList, int* __restrict__ overflowCount,
+    const int* __restrict__ groupList, int slots, int cap, const int* __restrict__ slotCap,
+    int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ overflowList,
+    int* __restrict__ overflowCount, int* __restrict__ maxList,
     unsigned long long* __restrict__ stats) {
-    __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK];
-    const int wave = (int)(threadIdx.x >> 6);
-    const long long g = (long long)blockIdx.x * GD_TR_WAVES + wave;
-    if (g * 64 >= n) return;   /* wave-uniform */
+    __shared__ int sStack[WAVES][STACK];
+    const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const long long s = (long long)blockIdx.x * WAVES + wave;
+    if (s >= slots) return;   /* wave-uniform */
+    const long long g = groupList ? groupList[s] : s;
+    float lx, ly, lz, hx, hy, hz;
+    gd_group_box(pos, n, g, lx, ly, lz, hx, hy, hz);
     int nf = 0, nb = 0;
-    const bool ok = gd_traverse_group<SPLIT>(pos, wcom, wlink, info, theta, n, g, sStack[wave],
-                                             GD_TR_STACK, lists + (size_t)g * cap, cap, &nf, &nb,
-                                             stats);
+    unsigned long long nOpen = 0;
+    const int limit = slotCap ? slotCap[s] : cap;
+    int* list = lists + (size_t)s * cap;
+    const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, lx, ly, lz, hx, hy, hz,
+                                           sStack[wave], STACK, list, cap, limit, &nf, &nb,
+                                           &nOpen);
     if (gd_lane() == 0) {
         if (ok) {
-            counts[g] = make_int2(nf, nb);
+            counts[s] = make_int2(nf, nb);
+            atomicMax(maxList, nf + nb);
+            if (stats) {
+                const int nvalid = (int)((n - g * 64) < 64 ? (n - g * 64) : 64);
+                atomicAdd(&stats[0], (unsigned long long)(nf + nb) * (unsigned long long)nvalid);
+                atomicAdd(&stats[1], nOpen);
+                atomicAdd(&stats[3], (unsigned long long)nb * (unsigned long long)nvalid);
+            }
         } else {
-            counts[g] = make_int2(-1, -1);
-            const int slot = atomicAdd(overflowCount, 1);
-            overflowList[slot] = (int)g;
+            counts[s] = make_int2(-1, -1);
+            const int at = atomicAdd(overflowCount, 1);
+            overflowList[at] = (int)g;
         }
+    }
+}
+
+/*
+ * Estimate for the groups the first pass could not list: the list length of
+ * one of the group's bodies (the middle one in Morton order), counted by a
+ * point traversal.  The second pass gives up on the group -- leaving it to
+ * the per-body walk -- once its list passes GD_BODY_RATIO times that: a
+ * compact group's list is 1.5-3x its bodies' own, one that much longer is
+ * spread out (its box spans distant clusters).  Such a group is cheaper body
+ * by body, and a long list walked by a single wavefront would also hold up
+ * the end of the walk.
+ */
+#define GD_BODY_RATIO 8
+template <bool SPLIT>
+__global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
+    const int* __restrict__ groupList, int slots, int cap2, int* __restrict__ slotCap) {
+    __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK];
+    const int wave = (int)(threadIdx.x >> 6);
+    const long long s = (long long)blockIdx.x * GD_TR_WAVES + wave;
+    if (s >= slots) return;   /* wave-uniform */
+    const long long g = groupList[s];
+    const long long last = (n - g * 64) < 64 ? n - 1 : g * 64 + 63;
+    const long long mid = g * 64 + (last - g * 64) / 2;
+    const float4 p = pos[mid];
+    int nf = 0, nb = 0;
+    unsigned long long nOpen = 0;
+    const int bound = cap2 / GD_BODY_RATIO;
+    const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, p.x, p.y, p.z, p.x, p.y,
+                                           p.z, sStack[wave], GD_TR_STACK, nullptr, bound,
+                                           bound, &nf, &nb, &nOpen);
+    if (gd_lane() == 0) {
+        const long long lim = ok ? (long long)GD_BODY_RATIO * (nf + nb) : cap2;
+        slotCap[s] = (int)(lim < cap2 ? lim : cap2);
     }
 }
 
@@ -489,28 +550,30 @@ __device__ __forceinline__ void gd_eval_group(
     }
 }
 
-/* Evaluation kernel: one wavefront per group with a list (counts.x >= 0). */
+/* Evaluation kernel: slot s (one wavefront) evaluates the list of group
+ * groupList[s] (or s) when it has one (counts[s].x >= 0). */
 template <int MODE, bool PHI>
 __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const gd_tree_info* __restrict__ info, const int* __restrict__ lists,
-    const int2* __restrict__ counts, int cap, float3* __restrict__ acc,
-    float* __restrict__ phi, float eps2, float G, int n, const gd_bf16x4* __restrict__ mlpW,
-    const float* __restrict__ mlpB, float minRatio) {
+    const gd_tree_info* __restrict__ info, const int* __restrict__ groupList, int slots,
+    const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
+    float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, int n,
+    const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio) {
     __shared__ float4 sE[GD_EV_WAVES][2 * 64];
     __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sCor[MODE == 2 ? GD_EV_WAVES : 1][64];
     const unsigned lane = gd_lane();
     const int wave = (int)(threadIdx.x >> 6);
-    const long long g = (long long)blockIdx.x * GD_EV_WAVES + wave;
-    if (g * 64 >= n) return;
-    const int2 cnt = counts[g];
-    if (cnt.x < 0) return;   /* left to the per-body walk */
+    const long long s = (long long)blockIdx.x * GD_EV_WAVES + wave;
+    if (s >= slots) return;
+    const int2 cnt = counts[s];
+    if (cnt.x < 0) return;   /* left to the next pass */
+    const long long g = groupList ? groupList[s] : s;
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, info, lists + (size_t)g * cap, cap, cnt,
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, info, lists + (size_t)s * cap, cap, cnt,
                              g, n, eps2, mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
                              sCor[MODE == 2 ? wave : 0], tot, phs, phc);
@@ -665,26 +728,40 @@ __global__ __launch_bounds__(64) void k_walk_body(
     }
 }
 
-/* Walk scratch: stats[4] (interactions, opened, stack-overflow flag, cell
- * interactions), overflow count and list, the groups' lists and counts. */
+/*
+ * Walk scratch.  counters[0..3]: interactions, opened, stack-overflow flag,
+ * cell interactions (64-bit); ints[0..2]: first- and second-pass overflow
+ * counts, longest first-pass list.  First pass: entriesPerGroup entries per
+ * group.  Second pass (groups that did not fit): GD_LIST2 entries per slot,
+ * at most GD_POOL_BYTES of pool, in batches.  Third: the per-body walk.
+ */
 struct gd_walk_scratch {
     unsigned long long* stats;
-    int* overflowCount;
-    int* overflowList;
-    int listCap;           /* groups the overflow list and counts hold */
-    int* lists;            /* groups x entriesPerGroup interaction lists */
-    int2* counts;          /* per group (front, back) */
+    int* ints;
+    int* overflow1;        /* groups entries */
+    int* overflow2;        /* groups entries */
+    int2* counts;          /* groups entries */
+    int2* counts2;         /* pool slots */
+    int* slotCap;          /* pool slots: second-pass list limit per group */
+    int groupCap;          /* groups the arrays above hold */
+    int* lists;
     long long listBytes;
+    int* pool;
+    long long poolEntries;
+    int countSlots;        /* entries of counts2 */
     int entriesPerGroup;
-    int fixedCap;          /* capacity set by gd_walk_set_list_capacity: no growth */
+    int fixedCap;          /* capacity set by gd_walk_set_list_capacity: kept as is */
+    int entries2;          /* second-pass list capacity */
 };
 
 #define GD_LIST_DEFAULT 16384
 #define GD_LIST_MAX 65536
-static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0,
-                                 GD_LIST_DEFAULT, 0};
+#define GD_LIST2 262144
+#define GD_POOL_BYTES (2ll << 30)
+static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                 0, nullptr, 0, nullptr, 0, 0, GD_LIST_DEFAULT, 0, GD_LIST2};
 
-/* optional HIP-event timing of the walk kernel (null stream) */
+/* optional HIP-event timing of the walk's phases (null stream) */
 static int g_walkTiming = 0;
 static hipEvent_t g_walkEv[4] = {nullptr, nullptr, nullptr, nullptr};
 static int g_walkTimed = 0;
@@ -692,16 +769,16 @@ static int g_walkTimed = 0;
 static void gd_walk_ensure(int groups) {
     if (!g_walk.stats) {
         GD_HIP(hipMalloc(&g_walk.stats, 4 * sizeof(unsigned long long)));
-        GD_HIP(hipMalloc(&g_walk.overflowCount, sizeof(int)));
-        GD_HIP(hipMemset(g_walk.stats, 0, 4 * sizeof(unsigned long long)));
-        GD_HIP(hipMemset(g_walk.overflowCount, 0, sizeof(int)));
+        GD_HIP(hipMalloc(&g_walk.ints, 4 * sizeof(int)));
     }
-    if (groups > g_walk.listCap) {
-        if (g_walk.overflowList) GD_HIP(hipFree(g_walk.overflowList));
+    if (groups > g_walk.groupCap) {
+        if (g_walk.overflow1) GD_HIP(hipFree(g_walk.overflow1));
+        if (g_walk.overflow2) GD_HIP(hipFree(g_walk.overflow2));
         if (g_walk.counts) GD_HIP(hipFree(g_walk.counts));
-        GD_HIP(hipMalloc(&g_walk.overflowList, sizeof(int) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.overflow1, sizeof(int) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.overflow2, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.counts, sizeof(int2) * (size_t)groups));
-        g_walk.listCap = groups;
+        g_walk.groupCap = groups;
     }
     const long long need = (long long)groups * g_walk.entriesPerGroup * (long long)sizeof(int);
     if (need > g_walk.listBytes) {
@@ -711,9 +788,93 @@ static void gd_walk_ensure(int groups) {
     }
 }
 
-__global__ void k_walk_reset(unsigned long long* stats, int* overflowCount) {
+static void gd_walk_ensure_pool(int slots, int entries) {
+    const long long need = (long long)slots * entries;
+    if (need > g_walk.poolEntries) {
+        if (g_walk.pool) GD_HIP(hipFree(g_walk.pool));
+        GD_HIP(hipMalloc(&g_walk.pool, (size_t)need * sizeof(int)));
+        g_walk.poolEntries = need;
+    }
+    if (slots > g_walk.countSlots) {
+        if (g_walk.counts2) GD_HIP(hipFree(g_walk.counts2));
+        if (g_walk.slotCap) GD_HIP(hipFree(g_walk.slotCap));
+        GD_HIP(hipMalloc(&g_walk.counts2, (size_t)slots * sizeof(int2)));
+        GD_HIP(hipMalloc(&g_walk.slotCap, (size_t)slots * sizeof(int)));
+        g_walk.countSlots = slots;
+    }
+}
+
+__global__ void k_walk_reset(unsigned long long* stats, int* ints) {
     if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) *overflowCount = 0;
+    if (threadIdx.x < 4) ints[threadIdx.x] = 0;
+}
+
+template <int MODE, bool PHI>
+static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
+                           const int* groupList, const int* lists, const int2* counts, int cap,
+                           float3* acc, float* phi, float eps2, float G, int n,
+                           const gd_bf16x4* W, const float* Bv, float minRatio) {
+    k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
+        pos, b->wcom, b->wlink, b->wquad, b->dinfo, groupList, slots, lists, counts, cap, acc,
+        phi, eps2, G, n, W, Bv, minRatio);
+}
+
+template <int MODE, bool PHI>
+static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, float* phi,
+                           float eps2, float G, float theta, int n, const gd_bf16x4* W,
+                           const float* Bv, float minRatio, int* overOut) {
+    constexpr bool SPLIT = MODE != 0;
+    const int groups = gd_div_up(n, 64);
+    const int cap = g_walk.entriesPerGroup;
+    int* ints = g_walk.ints;
+    /* pass 1: every group */
+    k_walk_traverse<SPLIT, GD_TR_WAVES, GD_TR_STACK><<<gd_div_up(groups, GD_TR_WAVES),
+                                                       64 * GD_TR_WAVES>>>(
+        pos, b->wcom, b->wlink, b->dinfo, theta, n, nullptr, groups, cap, nullptr, g_walk.lists,
+        g_walk.counts, g_walk.overflow1, ints + 0, ints + 2, g_walk.stats);
+    GD_LAUNCHED();
+    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
+    gd_launch_eval<MODE, PHI>(groups, b, pos, nullptr, g_walk.lists, g_walk.counts, cap, acc,
+                              phi, eps2, G, n, W, Bv, minRatio);
+    GD_LAUNCHED();
+    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
+    int h[4];
+    GD_HIP(hipMemcpy(h, ints, sizeof(h), hipMemcpyDeviceToHost));
+    const int over1 = h[0];
+    overOut[0] = over1;
+    overOut[1] = h[2];
+    if (over1 <= 0) return;
+    /* pass 2: the groups that did not fit, with long lists and a deep stack */
+    const int cap2 = g_walk.entries2;
+    const long long fit = GD_POOL_BYTES / ((long long)cap2 * sizeof(int));
+    const int slots = over1 < fit ? over1 : (int)fit;
+    gd_walk_ensure_pool(slots, cap2);
+    if (gd_has_error()) return;
+    for (int b0 = 0; b0 < over1; b0 += slots) {
+        const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
+        k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, cap2,
+            g_walk.slotCap);
+        GD_LAUNCHED();
+        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, cap2,
+            g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.overflow2, ints + 1, ints + 3,
+            g_walk.stats);
+        GD_LAUNCHED();
+        gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool,
+                                  g_walk.counts2, cap2, acc, phi, eps2, G, n, W, Bv,
+                                  minRatio);
+        GD_LAUNCHED();
+    }
+    int over2 = 0;
+    GD_HIP(hipMemcpy(&over2, ints + 1, sizeof(int), hipMemcpyDeviceToHost));
+    overOut[2] = over2;
+    if (over2 <= 0) return;
+    /* pass 3: body by body */
+    k_walk_body<MODE, PHI><<<64u * (unsigned)over2, 64>>>(
+        pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.overflow2, acc, phi, eps2, G, theta,
+        n, W, Bv, minRatio, g_walk.stats);
+    GD_LAUNCHED();
 }
 
 static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, float* phi,
@@ -734,79 +895,46 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     if (gd_has_error()) return;
     const gd_bf16x4* W = net ? gd_mlp_fwd_tiles(net) : nullptr;
     const float* Bv = net ? gd_mlp_bias(net) : nullptr;
-    k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.overflowCount);
+    k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.ints);
     GD_LAUNCHED();
     if (g_walkTiming) {
         if (!g_walkEv[0])
             for (int e = 0; e < 4; ++e) GD_HIP(hipEventCreate(&g_walkEv[e]));
         GD_HIP(hipEventRecord(g_walkEv[0], nullptr));
     }
-    const int cap = g_walk.entriesPerGroup;
-    const int tblocks = gd_div_up(groups, GD_TR_WAVES);
-    if (mode == 0)
-        k_walk_traverse<false><<<tblocks, 64 * GD_TR_WAVES>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, starCount, cap, g_walk.lists, g_walk.counts,
-            g_walk.overflowList, g_walk.overflowCount, g_walk.stats);
-    else
-        k_walk_traverse<true><<<tblocks, 64 * GD_TR_WAVES>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, starCount, cap, g_walk.lists, g_walk.counts,
-            g_walk.overflowList, g_walk.overflowCount, g_walk.stats);
-    GD_LAUNCHED();
-    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
-    const int eblocks = gd_div_up(groups, GD_EV_WAVES);
-#define GD_EVAL_ARGS                                                                           \
-    pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.lists, g_walk.counts, cap, acc, phi,    \
-        eps2, G, starCount, W, Bv, minRatio
+    int over[3] = {0, 0, 0};
+#define GD_PASSES(M, P)                                                                        \
+    gd_walk_passes<M, P>(b, pos, acc, phi, eps2, G, theta, starCount, W, Bv, minRatio, over)
     if (mode == 0) {
-        if (phi) k_walk_eval<0, true><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
-        else k_walk_eval<0, false><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+        if (phi) GD_PASSES(0, true); else GD_PASSES(0, false);
     } else if (mode == 1) {
-        if (phi) k_walk_eval<1, true><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
-        else k_walk_eval<1, false><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+        if (phi) GD_PASSES(1, true); else GD_PASSES(1, false);
     } else {
-        if (phi) k_walk_eval<2, true><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
-        else k_walk_eval<2, false><<<eblocks, 64 * GD_EV_WAVES>>>(GD_EVAL_ARGS);
+        if (phi) GD_PASSES(2, true); else GD_PASSES(2, false);
     }
-    GD_LAUNCHED();
-    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
-#undef GD_EVAL_ARGS
-    /* groups whose list or stack did not fit: walked body by body */
-    int over = 0;
-    GD_HIP(hipMemcpy(&over, g_walk.overflowCount, sizeof(int), hipMemcpyDeviceToHost));
-    if (over > 0) {
-#define GD_BODY_ARGS                                                                           \
-    pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.overflowList, acc, phi, eps2, G, theta, \
-        starCount, W, Bv, minRatio, g_walk.stats
-        const unsigned blocks = 64u * (unsigned)over;
-        if (mode == 0) {
-            if (phi) k_walk_body<0, true><<<blocks, 64>>>(GD_BODY_ARGS);
-            else k_walk_body<0, false><<<blocks, 64>>>(GD_BODY_ARGS);
-        } else if (mode == 1) {
-            if (phi) k_walk_body<1, true><<<blocks, 64>>>(GD_BODY_ARGS);
-            else k_walk_body<1, false><<<blocks, 64>>>(GD_BODY_ARGS);
-        } else {
-            if (phi) k_walk_body<2, true><<<blocks, 64>>>(GD_BODY_ARGS);
-            else k_walk_body<2, false><<<blocks, 64>>>(GD_BODY_ARGS);
-        }
-#undef GD_BODY_ARGS
-        GD_LAUNCHED();
-    }
+#undef GD_PASSES
+    if (gd_has_error()) return;
     if (g_walkTiming) {
         GD_HIP(hipEventRecord(g_walkEv[3], nullptr));
         g_walkTimed = 1;
     }
-    /* more than 1 group in 256 overflowed its list: give later walks room */
-    if (!g_walk.fixedCap && over > groups / 256 && g_walk.entriesPerGroup < GD_LIST_MAX)
-        g_walk.entriesPerGroup *= 2;
-    if (over > 0 || stats) {
+    /* list capacity for later walks: grow while more than one group in 256
+       overflows, shrink while the longest list uses under a quarter of it */
+    if (!g_walk.fixedCap) {
+        int& c = g_walk.entriesPerGroup;
+        if (over[0] > groups / 256 && c < GD_LIST_MAX) c *= 2;
+        else if (over[0] == 0 && over[1] < c / 4 && c > GD_LIST_DEFAULT) c /= 2;
+    }
+    if (over[0] > 0 || stats) {
         unsigned long long h[4];
         GD_HIP(hipMemcpy(h, g_walk.stats, sizeof(h), hipMemcpyDeviceToHost));
         if (h[2]) GD_FAIL("gd_walk: a body's traversal stack exceeded %d entries", GD_BODY_STACK);
         if (!stats) return;
         stats->interactions = h[0];
         stats->opened = h[1];
-        stats->overflowGroups = (unsigned long long)over;
+        stats->overflowGroups = (unsigned long long)over[0];
         stats->cellInteractions = h[3];
+        stats->bodyGroups = (unsigned long long)over[2];
     }
 }
 
@@ -827,14 +955,15 @@ extern "C" void gd_walk_learned(const treeState* tree, const float4* pos, float3
     gd_walk_impl(tree, pos, acc, phi, eps2, G, theta, starCount, stats, 2, net, minRatio);
 }
 
-extern "C" int gd_walk_set_list_capacity(int entries) {
+extern "C" int gd_walk_set_list_capacity(int entries, int secondPass) {
     const int prev = g_walk.entriesPerGroup;
-    if (entries < 0 || entries > (1 << 24)) {
-        GD_FAIL("gd_walk_set_list_capacity: %d entries out of range [0, 2^24]", entries);
+    if (entries < 0 || entries > (1 << 24) || secondPass < 0 || secondPass > (1 << 26)) {
+        GD_FAIL("gd_walk_set_list_capacity: capacities %d, %d out of range", entries, secondPass);
         return prev;
     }
     g_walk.entriesPerGroup = entries > 0 ? entries : GD_LIST_DEFAULT;
     g_walk.fixedCap = entries > 0;
+    g_walk.entries2 = secondPass > 0 ? secondPass : GD_LIST2;
     return prev;
 }
 

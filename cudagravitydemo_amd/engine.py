@@ -141,7 +141,8 @@ class Simulation:
         if st is not None:
             return {"interactions": st.interactions, "opened": st.opened,
                     "overflow_groups": st.overflowGroups,
-                    "cell_interactions": st.cellInteractions}
+                    "cell_interactions": st.cellInteractions,
+                    "body_groups": st.bodyGroups}
         return None
 
     def step(self, dt, theta):

@@ -4,9 +4,11 @@ Runs the same workload as bench.py (2^21 bodies by default) for --steps
 steps with the chosen walk, then (with --copy-mib) one stream copy of a known
 size, so that hardware byte counters of the step's kernels can be scaled by
 what the same counters read for the copy.  Prints one line per phase.
+--walk learned uses an untrained network (the kernels' work does not depend
+on the weights); --params loads trained ones (float32 .npy, 9091 values).
 
 Examples (from the repository root):
-  rocprofv3 --kernel-trace --stats -d OUT -o run -- python3 cudagravitydemo_amd/profile_step.py
+  rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- python3 cudagravitydemo_amd/profile_step.py
   rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 cudagravitydemo_amd/profile_step.py --steps 2 --copy-mib 256
 """
 import argparse
@@ -16,28 +18,43 @@ import time
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
+import numpy as np  # noqa: E402
+
 from cudagravitydemo_amd import Simulation, ics  # noqa: E402
+from cudagravitydemo_amd import corrector as cr  # noqa: E402
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--per-disk", type=int, default=1 << 20)
     ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--walk", choices=["fast", "quad", "reference"], default="fast")
+    ap.add_argument("--walk", choices=["plain", "quad", "learned", "reference"], default="plain")
+    ap.add_argument("--params", default=None)
     ap.add_argument("--copy-mib", type=int, default=0)
     a = ap.parse_args()
     pos, vel = ics.disk_merger(a.per_disk, seed=2024)
     n = pos.shape[0]
     eps, dt = 0.005, 5e-4
+    net = None
+    if a.walk == "learned":
+        net = cr.Corrector(np.load(a.params) if a.params else None, seed=1)
     with Simulation(pos, vel, eps=eps) as sim:
-        if a.walk == "quad":
+        if a.walk in ("quad", "learned"):
             sim.set_quadrupole(True)
         sim.sync()
         t0 = time.perf_counter()
-        for _ in range(a.steps):
+        for it in range(a.warmup + a.steps):
+            if it == a.warmup:
+                sim.sync()
+                t0 = time.perf_counter()
             if a.walk == "reference":
                 sim.step_reference(dt, a.theta)
+            elif a.walk == "learned":
+                sim.build_tree_async()
+                cr.walk_learned(sim, net, a.theta)
+                sim.integrate(dt)
             else:
                 sim.step(dt, a.theta)
         sim.sync()
@@ -60,6 +77,8 @@ def main():
                   f"{2 * nbytes / ms / 1e6:.1f} GB/s (read + write)", flush=True)
             lib.gd_free(src)
             lib.gd_free(dst)
+    if net is not None:
+        net.close()
 
 
 if __name__ == "__main__":

@@ -72,9 +72,10 @@ void gd_tree_levels(treeState* tree);
 typedef struct {
     unsigned long long interactions;     /* accepted body-node interactions */
     unsigned long long opened;           /* node tests that opened the node */
-    unsigned long long overflowGroups;   /* groups walked body by body */
+    unsigned long long overflowGroups;   /* groups whose first-pass list did not fit */
     unsigned long long cellInteractions; /* of the interactions, those with internal
                                             cells evaluated as quadrupoles (+ network) */
+    unsigned long long bodyGroups;       /* of those, groups walked body by body */
 } gd_walk_stats;
 
 /*
@@ -84,8 +85,9 @@ typedef struct {
  * accepted nodes written to the group's interaction list in device memory
  * and evaluated by all 64 bodies; per-body sums are FP32, compensated
  * (Kahan) across 64-entry chunks.  A group whose list does not fit its
- * capacity is walked body by body instead (the same acceptance test against
- * the body's own position).  With the second moments built
+ * capacity is walked again with a larger list (second pass); one that still
+ * does not fit is walked body by body (the same acceptance test against the
+ * body's own position, no list).  With the second moments built
  * (gd_set_quadrupole), accepted internal cells are evaluated as quadrupoles.
  * Needs a tree built by buildTree or gd_build_tree_async.  stats may be
  * NULL; when given, counters are accumulated on the device and copied out
@@ -94,11 +96,14 @@ typedef struct {
 void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, float eps2,
              float G, float theta, int starCount, gd_walk_stats* stats);
 
-/* Interaction-list capacity per 64-body group (entries; 4 bytes each, for
- * every group).  0 restores the default: 16384, doubled (up to 65536) after
- * a walk in which more than one group in 256 did not fit.  A value set here
- * is kept as it is.  Returns the previous capacity. */
-int gd_walk_set_list_capacity(int entries);
+/* Interaction-list capacities (entries of 4 bytes).  entries: per 64-body
+ * group in the first pass (allocated for every group); 0 restores the
+ * default, 16384, doubled (up to 65536) after a walk in which more than one
+ * group in 256 did not fit and halved back while the longest list uses under
+ * a quarter of it; a value set here is kept as it is.  secondPass: per group
+ * in the second pass (0: 262144; at most 2 GiB of lists at a time).
+ * Returns the previous first-pass capacity. */
+int gd_walk_set_list_capacity(int entries, int secondPass);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
  * null stream (enable = 1) or stop (0).  The walk is a traversal kernel that

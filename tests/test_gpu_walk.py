@@ -7,8 +7,9 @@ accepts; their monopole / quadrupole / learned terms are summed in float64.
 Bars: 1e-5 relative (vector norm) for the monopole and quadrupole walks; in
 learned mode the network's share is held to 1 % (the BF16 network itself is
 pinned against the oracle in test_gpu_mlp.py) and the rest to 1e-5.  The
-per-body walk runs when a group's list does not fit its capacity; the tests
-force it with gd_walk_set_list_capacity.
+second pass (a longer list) runs for groups whose first-pass list does not
+fit, the per-body walk for groups that do not fit the second pass either; the
+tests force them with gd_walk_set_list_capacity.
 """
 import numpy as np
 import pytest
@@ -39,9 +40,9 @@ def _group_box(ps, i, n):
     return blk.min(0), blk.max(0)
 
 
-def _run(sim, mode, cap, params):
+def _run(sim, mode, cap, params, cap2=0):
     lib = load_library()
-    lib.gd_walk_set_list_capacity(cap)
+    lib.gd_walk_set_list_capacity(cap, cap2)
     try:
         if mode == 2:
             net = cr.Corrector(params)
@@ -52,17 +53,18 @@ def _run(sim, mode, cap, params):
         else:
             st = sim.walk(THETA, stats=True)
     finally:
-        lib.gd_walk_set_list_capacity(0)
+        lib.gd_walk_set_list_capacity(0, 0)
     return st
 
 
-def _sample(n):
+def _sample(n, extra=()):
     rng = np.random.default_rng(8)
-    s = np.r_[0, 1, 63, 64, 4095, n - 1, n - 20, rng.choice(n, 60, replace=False)]
+    s = np.r_[0, 1, 63, 64, 4095, n - 1, n - 20, rng.choice(n, 60, replace=False),
+              np.asarray(extra, np.int64)]
     return np.unique(s)
 
 
-def _check(sim, mode, per_body_of, params):
+def _check(sim, mode, per_body_of, params, extra=()):
     ps = sim.positions()
     acc = sim.accelerations()
     total = sim.tree_summary()[0]
@@ -71,7 +73,7 @@ def _check(sim, mode, per_body_of, params):
     ext = world_extent(lo, hi)
     net = (lambda X: mlp.forward(params, X, bf16_ops=True)[0]) if mode == 2 else None
     worst = 0.0
-    for i in _sample(sim.n):
+    for i in _sample(sim.n, extra):
         x = ps[i, :3]
         box = (x, x) if per_body_of(i) else _group_box(ps, i, sim.n)
         want, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
@@ -88,18 +90,63 @@ def _check(sim, mode, per_body_of, params):
     print(f"mode {mode}: worst error / tolerance {worst:.3f}", flush=True)
 
 
+def _tree(sim):
+    ps = sim.positions()
+    total = sim.tree_summary()[0]
+    com, link, _ = sim.walk_arrays(total)
+    lo, hi = sim.bbox()
+    return ps, com, link, world_extent(lo, hi)
+
+
+def _group_sizes(sim):
+    ps, com, link, ext = _tree(sim)
+    groups = (sim.n + 63) // 64
+    sizes = np.empty(groups, np.int64)
+    for g in range(groups):
+        blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
+        lv, cl = accepted(com, link, ext, blk.min(0), blk.max(0), THETA)
+        sizes[g] = lv.size + cl.size
+    return sizes
+
+
+def _predict(sim, sizes, cap1, cap2):
+    """Routing of the walk's passes, from the exact list lengths: a group leaves
+    the first pass when its list exceeds cap1; the second pass gives it up when
+    its list exceeds 8x the middle body's own list (or cap2)."""
+    ps, com, link, ext = _tree(sim)
+    over = set(int(g) for g in np.flatnonzero(sizes > cap1))
+    body = set()
+    for g in over:
+        last = min(64 * g + 63, sim.n - 1)
+        x = ps[64 * g + (last - 64 * g) // 2, :3]
+        lv, cl = accepted(com, link, ext, x, x, THETA)
+        lb = lv.size + cl.size
+        lim = 8 * lb if lb <= cap2 // 8 else cap2
+        if sizes[g] > min(lim, cap2):
+            body.add(g)
+    return over, body
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
-@pytest.mark.parametrize("per_body", [False, True], ids=["lists", "per_body"])
-def test_walk_matches_interaction_set_fp64(mode, per_body):
+@pytest.mark.parametrize("path", ["lists", "second_pass", "per_body"])
+def test_walk_matches_interaction_set_fp64(mode, path):
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
+    caps = {"lists": (0, 0), "second_pass": (1, 0), "per_body": (1, 1)}[path]
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(mode != 0)
         sim.build_tree_async()
-        st = _run(sim, mode, 1 if per_body else 0, params)
+        sizes = _group_sizes(sim)
+        over, body = _predict(sim, sizes, caps[0] or 16384, caps[1] or 262144)
         groups = (sim.n + 63) // 64
-        assert st["overflow_groups"] == (groups if per_body else 0)
-        _check(sim, mode, lambda i: per_body, params)
+        if path == "per_body":
+            assert len(body) == groups
+        if path == "second_pass":
+            assert len(over) == groups and len(body) < groups // 16
+        st = _run(sim, mode, caps[0], params, caps[1])
+        assert st["overflow_groups"] == len(over)
+        assert st["body_groups"] == len(body)
+        _check(sim, mode, lambda i: (i // 64) in body, params)
         if mode == 2:
             # the network's share is well above the 1e-5 bar: the check has teeth
             a2 = sim.accelerations()
@@ -116,11 +163,7 @@ def test_walk_counts_match_interaction_sets():
         sim.set_quadrupole(True)
         sim.build_tree_async()
         st = sim.walk(THETA, stats=True)
-        ps = sim.positions()
-        total = sim.tree_summary()[0]
-        com, link, quad = sim.walk_arrays(total)
-        lo, hi = sim.bbox()
-        ext = world_extent(lo, hi)
+        ps, com, link, ext = _tree(sim)
         inter = cells = 0
         for g in range((sim.n + 63) // 64):
             blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
@@ -134,29 +177,43 @@ def test_walk_counts_match_interaction_sets():
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_mixed_lists_and_per_body(mode):
-    """Capacity at the median group list: about half the groups are walked
-    body by body, the rest from their lists, in one walk."""
+    """Capacity at the median group list and no second pass: about half the
+    groups are walked body by body, the rest from their lists, in one walk."""
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(mode != 0)
         sim.build_tree_async()
-        ps = sim.positions()
-        total = sim.tree_summary()[0]
-        com, link, _ = sim.walk_arrays(total)
-        lo, hi = sim.bbox()
-        ext = world_extent(lo, hi)
-        groups = (sim.n + 63) // 64
-        sizes = np.empty(groups, np.int64)
-        for g in range(groups):
-            blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
-            lv, cl = accepted(com, link, ext, blk.min(0), blk.max(0), THETA)
-            sizes[g] = lv.size + cl.size
+        sizes = _group_sizes(sim)
         cap = int(np.median(sizes))
-        st = _run(sim, mode, cap, params)
-        over = sizes > cap
+        over, body = _predict(sim, sizes, cap, 1)
+        groups = (sim.n + 63) // 64
+        assert 0 < len(over) < groups and body == over
+        st = _run(sim, mode, cap, params, 1)
         # a group can also leave the list walk when its traversal stack is full
-        assert st["overflow_groups"] >= over.sum()
-        assert 0 < over.sum() < groups
-        if st["overflow_groups"] == over.sum():
-            _check(sim, mode, lambda i: bool(over[i // 64]), params)
+        assert st["overflow_groups"] >= len(over)
+        assert st["body_groups"] == st["overflow_groups"]
+        if st["overflow_groups"] == len(over):
+            _check(sim, mode, lambda i: (i // 64) in body, params)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_second_pass_and_per_body_together(mode):
+    """Capacity at the median list: the compact groups that do not fit go to
+    the second pass, the spread ones (list > 8x their middle body's) body by
+    body -- both routes in one walk."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode != 0)
+        sim.build_tree_async()
+        sizes = _group_sizes(sim)
+        cap = int(np.median(sizes))
+        over, body = _predict(sim, sizes, cap, 262144)
+        assert len(body) >= 1 and len(over - body) >= 1
+        st = _run(sim, mode, cap, params, 0)
+        assert st["overflow_groups"] >= len(over)
+        if st["overflow_groups"] == len(over):
+            assert st["body_groups"] == len(body)
+            spread = [64 * g + k for g in body for k in (0, 31, 63) if 64 * g + k < sim.n]
+            _check(sim, mode, lambda i: (i // 64) in body, params, extra=spread)
