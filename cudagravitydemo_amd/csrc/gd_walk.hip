@@ -155,7 +155,7 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 #endif
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
-#define GD_TR2_STACK 12288        /* second pass: one wavefront per block, > 22 x 512 */
+#define GD_TR2_STACK 12288        /* second pass, one wavefront per block (48 KB of LDS) */
 
 /* Bounding box of group g's bodies, in every lane. */
 __device__ __forceinline__ void gd_group_box(const float4* __restrict__ pos, int n, long long g,
@@ -172,11 +172,51 @@ __device__ __forceinline__ void gd_group_box(const float4* __restrict__ pos, int
     }
 }
 
+/* One stack entry tested against the box [l, h]. */
+struct gd_probe {
+    int idx, nk, first;
+    bool accept, open, leaf;
+};
+
+__device__ __forceinline__ gd_probe gd_test_node(float4 c, int2 lk, int idx, float ext,
+                                                 float th2, float lx, float ly, float lz,
+                                                 float hx, float hy, float hz) {
+    gd_probe r;
+    const float size = ldexpf(ext, -(lk.y & 31));
+    const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
+    const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
+    const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
+    const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    r.idx = idx;
+    r.leaf = lk.x < 0;
+    r.accept = r.leaf || (size * size < th2 * d2);
+    r.open = !r.accept;
+    r.nk = r.open ? (lk.y >> 5) : 0;
+    r.first = lk.x;
+    return r;
+}
+
+/* Exclusive prefix and total over the wavefront of v in [0, 15]
+ * (four bit-plane ballots). */
+__device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int& excl,
+                                               int& total) {
+    excl = 0;
+    total = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        const unsigned long long bal = __ballot((v >> bit) & 1);
+        excl += __popcll(bal & lt) << bit;
+        total += __popcll(bal) << bit;
+    }
+}
+
 /* Traversal by one wavefront for the box [l, h]: writes the accepted nodes to
  * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
  * SPLIT) unless list is NULL (counting only), and returns false as soon as
  * more than limit (<= cap) entries are accepted or the stack (stackCap
- * entries) would overflow.  *opened counts the opened nodes. */
+ * entries) would overflow.  *opened counts the opened nodes.  Each step pops
+ * up to 128 entries, two per lane, so that two node loads per lane are in
+ * flight; entries are handled in stack order (the upper 64 first). */
 template <bool SPLIT>
 __device__ __forceinline__ bool gd_traverse_box(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
@@ -191,53 +231,53 @@ __device__ __forceinline__ bool gd_traverse_box(
     unsigned long long nOpen = 0;
     if (lane == 0) stack[0] = 0;
     while (top > 0) {
-        const int take = top < 64 ? top : 64;
+        const int take = top < 128 ? top : 128;
         const int base = top - take;
-        bool accept = false, open = false, leaf = false;
-        int idx = 0, nk = 0, first = 0;
-        if ((int)lane < take) {
-            idx = stack[base + lane];
-            const float4 c = wcom[idx];
-            const int2 lk = wlink[idx];
-            const float size = ldexpf(ext, -(lk.y & 31));
-            const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
-            const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
-            const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
-            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            leaf = lk.x < 0;
-            accept = leaf || (size * size < th2 * d2);
-            open = !accept;
-            nk = open ? (lk.y >> 5) : 0;
-            first = lk.x;
-        }
+        /* A: entries top-64 .. top-1, B: the 64 below them */
+        const int ea = take - 64 + (int)lane;   /* offset from base */
+        const int eb = take - 128 + (int)lane;
+        const bool hasA = ea >= 0, hasB = eb >= 0;
+        const int ia = hasA ? stack[base + ea] : 0;
+        const int ib = hasB ? stack[base + eb] : 0;
+        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;
+        int2 la = make_int2(0, 0), lb = la;
+        if (hasA) { ca = wcom[ia]; la = wlink[ia]; }
+        if (hasB) { cb = wcom[ib]; lb = wlink[ib]; }
+        gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz);
+        gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz);
+        if (!hasA) A.accept = A.open = false, A.nk = 0;
+        if (!hasB) B.accept = B.open = false, B.nk = 0;
         __builtin_amdgcn_wave_barrier();
         top = base;
-        const bool front = SPLIT ? (accept && leaf) : accept;
-        const bool back = SPLIT && accept && !leaf;
-        const unsigned long long fb = __ballot(front);
-        const unsigned long long bb = __ballot(back);
-        const int nf = __popcll(fb), nb = __popcll(bb);
-        /* children offsets: prefix sum of nk (0..8) from four bit-plane ballots */
-        int excl = 0, total = 0;
-#pragma unroll
-        for (int bit = 0; bit < 4; ++bit) {
-            const unsigned long long bal = __ballot((nk >> bit) & 1);
-            excl += __popcll(bal & lt) << bit;
-            total += __popcll(bal) << bit;
-        }
-        nOpen += (unsigned long long)__popcll(__ballot(open));
-        if (nFront + nf + nBack + nb > limit || top + total > stackCap) return false;
+        const bool fA = SPLIT ? (A.accept && A.leaf) : A.accept;
+        const bool fB = SPLIT ? (B.accept && B.leaf) : B.accept;
+        const bool kA = SPLIT && A.accept && !A.leaf;
+        const bool kB = SPLIT && B.accept && !B.leaf;
+        const unsigned long long fba = __ballot(fA), fbb = __ballot(fB);
+        const unsigned long long bba = __ballot(kA), bbb = __ballot(kB);
+        const int nfa = __popcll(fba), nfb = __popcll(fbb);
+        const int nba = __popcll(bba), nbb = __popcll(bbb);
+        int exA, totA, exB, totB;
+        gd_wave_scan16(A.nk, lt, exA, totA);
+        gd_wave_scan16(B.nk, lt, exB, totB);
+        nOpen += (unsigned long long)(__popcll(__ballot(A.open)) + __popcll(__ballot(B.open)));
+        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + totA + totB > stackCap)
+            return false;
         if (list) {
-            if (front) list[nFront + __popcll(fb & lt)] = idx;
-            if (back) list[cap - 1 - (nBack + __popcll(bb & lt))] = idx;
+            if (fA) list[nFront + __popcll(fba & lt)] = A.idx;
+            if (fB) list[nFront + nfa + __popcll(fbb & lt)] = B.idx;
+            if (kA) list[cap - 1 - (nBack + __popcll(bba & lt))] = A.idx;
+            if (kB) list[cap - 1 - (nBack + nba + __popcll(bbb & lt))] = B.idx;
         }
-        nFront += nf;
-        nBack += nb;
-        if (open) {
-            const int at = top + excl;
-            for (int k = 0; k < nk; ++k) stack[at + k] = first + k;
-        }
-        top += total;
+        nFront += nfa + nfb;
+        nBack += nba + nbb;
+        /* children of the upper entries go on top: the walk stays depth-first
+           and the stack shallow */
+        if (B.open)
+            for (int k = 0; k < B.nk; ++k) stack[top + exB + k] = B.first + k;
+        if (A.open)
+            for (int k = 0; k < A.nk; ++k) stack[top + totB + exA + k] = A.first + k;
+        top += totA + totB;
         __builtin_amdgcn_wave_barrier();
     }
     *nFrontOut = nFront;
