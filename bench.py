@@ -106,13 +106,13 @@ class Walks:
         self.net = net
 
     def prepare(self, mode):
-        # second moments are built only for the walks that use them
-        self.sim.set_quadrupole(mode != "plain")
+        # moments are built only for the walks that use them
+        self.sim.set_quadrupole({"plain": 0, "quad": 1, "learned": 2}[mode])
 
     def walk(self, mode, theta, stats=False):
         from cudagravitydemo_amd import corrector as cr
         if mode == "learned":
-            return cr.walk_learned(self.sim, self.net, theta, 0.0, stats=stats)
+            return cr.walk_learned(self.sim, self.net, theta, stats=stats)
         return self.sim.walk(theta, stats=stats)
 
     def step(self, mode, theta, dt):
@@ -228,7 +228,7 @@ def main():
 
     # corrector: samples from this run's own snapshots, BF16-MFMA training
     t0 = time.perf_counter()
-    sim.set_quadrupole(True)
+    sim.set_quadrupole(2)
     samples = cr.SampleSet(sim.lib, 24_000_000)
     for snap in range(4):
         sim.build_tree_async()

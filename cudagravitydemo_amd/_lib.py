@@ -109,6 +109,7 @@ _PROTOS = {
     "gd_tree_levels": (None, [C.POINTER(TreeState)]),
     "gd_set_quadrupole": (None, [C.POINTER(TreeState), i32]),
     "gd_tree_walk_arrays": (None, [C.POINTER(TreeState), vp, vp, vp, i32]),
+    "gd_tree_third_moments": (None, [C.POINTER(TreeState), vp, i32]),
     "gd_walk": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32,
                        C.POINTER(WalkStats)]),
     "gd_walk_timing": (None, [i32]),

@@ -20,15 +20,25 @@ SIZES = [(64, 8), (64, 64), (64, 64), (3, 64)]
 
 
 def init_params(seed=0):
-    """He-normal weights, zero biases (float32, the library's layout)."""
+    """He-normal hidden weights, zero biases and a zero output layer (the
+    untrained network predicts no correction), float32 in the library's
+    layout."""
     rng = np.random.Generator(np.random.PCG64(seed))
     parts = []
-    for o, i in SIZES:
-        parts.append(rng.standard_normal(o * i) * np.sqrt(2.0 / i))
+    for li, (o, i) in enumerate(SIZES):
+        w = rng.standard_normal(o * i) * np.sqrt(2.0 / i)
+        parts.append(w if li < len(SIZES) - 1 else np.zeros(o * i))
         parts.append(np.zeros(o))
     p = np.concatenate(parts).astype(np.float32)
     assert p.size == N_PARAMS
     return p
+
+
+def scale_output(p, factor):
+    """Parameters whose outputs are `factor` times those of p (last layer scaled)."""
+    q = np.array(p, np.float32, copy=True)
+    q[-3 - 3 * 64:] *= np.float32(factor)
+    return q
 
 
 class Corrector:
@@ -122,13 +132,27 @@ class SampleSet:
         self.Y.close()
 
 
-def train(net, samples, epochs=1, batch=65536, lr=1e-3, seed=0, log=None):
-    """Adam on shuffled minibatches of the sample set; returns per-epoch mean loss."""
+def train(net, samples, epochs=1, batch=8192, lr=2e-3, seed=0, log=None):
+    """Adam on shuffled minibatches of the sample set, cosine-decayed learning
+    rate (lr down to lr / 20).  The targets are trained at unit RMS (divided
+    by their RMS over the set) and the factor is folded back into the output
+    layer afterwards.  Returns per-epoch mean loss (in the unit-RMS scale)."""
     lib = net.lib
     rng = np.random.Generator(np.random.PCG64(seed))
     n = samples.n
+    if n == 0:
+        return []
+    Y = from_device(lib, samples.Y.ptr, np.float32, (n, 3))
+    rms = float(np.sqrt(np.mean(np.sum(Y.astype(np.float64) ** 2, axis=1))))
+    rms = rms if rms > 0 else 1.0
+    ys = DeviceBuffer(lib, 12 * n)
+    ys.upload(np.ascontiguousarray(Y / np.float32(rms)))
+    net.set_params(scale_output(net.params(), 1.0 / rms))
     losses = []
     order_buf = DeviceBuffer(lib, 4 * n)
+    steps_per_epoch = max(1, n // batch)
+    total = epochs * steps_per_epoch
+    step = 0
     try:
         for ep in range(epochs):
             order = rng.permutation(n).astype(np.int32)
@@ -136,20 +160,29 @@ def train(net, samples, epochs=1, batch=65536, lr=1e-3, seed=0, log=None):
             tot, cnt = 0.0, 0
             for b0 in range(0, n - batch + 1 if n >= batch else 1, batch):
                 B = min(batch, n - b0)
-                loss = lib.gd_mlp_train_step(net.ptr, samples.X.ptr, samples.Y.ptr,
-                                             order_buf.ptr + 4 * b0, B, lr)
+                f = 0.5 * (1.0 + np.cos(np.pi * step / max(total, 1)))
+                rate = lr * (0.05 + 0.95 * f)
+                loss = lib.gd_mlp_train_step(net.ptr, samples.X.ptr, ys.ptr,
+                                             order_buf.ptr + 4 * b0, B, rate)
                 check(lib)
                 tot += loss
                 cnt += 1
+                step += 1
             losses.append(tot / max(cnt, 1))
             if log:
                 log(f"epoch {ep}: loss {losses[-1]:.4e}")
     finally:
         order_buf.close()
+        ys.close()
+        net.set_params(scale_output(net.params(), rms))
     return losses
 
 
-def walk_learned(sim, net, theta, min_ratio=0.0, stats=False):
+def walk_learned(sim, net, theta, min_ratio=0.5, stats=False):
+    """The learned walk on sim's current tree (built with gd_set_quadrupole
+    level 2).  Cells with size / distance <= min_ratio (from the group's
+    centre) keep the analytic quadrupole alone: their residual is small and
+    the network is not evaluated for them."""
     st = WalkStats() if stats else None
     sim.lib.gd_walk_learned(C.byref(sim.state), sim.pos, sim.acc, None, sim.eps * sim.eps,
                             sim.G, theta, sim.n, net.ptr, min_ratio,

@@ -238,6 +238,7 @@ void sumDestroyer(treeBuilder* sum) {
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
     if (b->wquad) GD_HIP(hipFree(b->wquad));
+    if (b->woct) GD_HIP(hipFree(b->woct));
     memset(b, 0, sizeof(*b));
     free(b);
 }
@@ -349,13 +350,39 @@ void gd_set_quadrupole(treeState* tree, int enable) {
         GD_FAIL("gd_set_quadrupole: uninitialised tree state");
         return;
     }
+    if (enable < 0 || enable > 2) {
+        GD_FAIL("gd_set_quadrupole: enable must be 0, 1 or 2 (got %d)", enable);
+        return;
+    }
     gd_builder* b = gd_builder_of(tree->builder);
-    b->quad = enable ? 1 : 0;
+    b->quad = enable;
     if (b->quad && b->quadNodes < tree->maxNodes) {
         if (b->wquad) GD_HIP(hipFree(b->wquad));
         GD_HIP(hipMalloc(&b->wquad, 2 * sizeof(float4) * (size_t)tree->maxNodes));
         b->quadNodes = tree->maxNodes;
     }
+    if (b->quad == 2 && b->octNodes < tree->maxNodes) {
+        if (b->woct) GD_HIP(hipFree(b->woct));
+        GD_HIP(hipMalloc(&b->woct, 3 * sizeof(float4) * (size_t)tree->maxNodes));
+        b->octNodes = tree->maxNodes;
+    }
+}
+
+void gd_tree_third_moments(const treeState* tree, void* oct, int count) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_tree_third_moments: uninitialised tree state");
+        return;
+    }
+    gd_builder* b = gd_builder_of(tree->builder);
+    if (count < 0 || count > b->maxNodes) {
+        GD_FAIL("gd_tree_third_moments: count %d outside [0, %d]", count, b->maxNodes);
+        return;
+    }
+    if (b->quad != 2 || !b->woct) {
+        GD_FAIL("gd_tree_third_moments: third moments are off (gd_set_quadrupole(tree, 2))");
+        return;
+    }
+    GD_HIP(hipMemcpy(oct, b->woct, 3 * sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
 }
 
 void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* quad, int count) {

@@ -371,13 +371,71 @@ __device__ __forceinline__ void gd_node_moments(const float4* __restrict__ pos,
     *q1 = make_float4(sxz, syz, sxx + syy + szz, 0.f);
 }
 
-template <bool QUAD>
+/* Third moments O_ijk = sum m d_i d_j d_k about the COM c (10 components:
+ * xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz).  Leaves sum over their bodies;
+ * internal nodes shift each child's moments to c (d = c_child - c):
+ * O += O_c + (S_c,ij d_k + S_c,ik d_j + S_c,jk d_i) + m_c d_i d_j d_k. */
+__device__ __forceinline__ void gd_node_third(const float4* __restrict__ pos,
+                                              const float4* __restrict__ wcom,
+                                              const float4* __restrict__ wquad,
+                                              const float4* __restrict__ woct, int first,
+                                              int count, const int* child, bool leaf, float4 c,
+                                              float4* o0, float4* o1, float4* o2) {
+    float xxx = 0.f, yyy = 0.f, zzz = 0.f, xxy = 0.f, xxz = 0.f;
+    float xyy = 0.f, yyz = 0.f, xzz = 0.f, yzz = 0.f, xyz = 0.f;
+    if (leaf) {
+        for (int b = 0; b < count; ++b) {
+            const float4 p = pos[first + b];
+            const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+            const float mx = p.w * dx, my = p.w * dy, mz = p.w * dz;
+            xxx = __builtin_fmaf(mx * dx, dx, xxx);
+            yyy = __builtin_fmaf(my * dy, dy, yyy);
+            zzz = __builtin_fmaf(mz * dz, dz, zzz);
+            xxy = __builtin_fmaf(mx * dx, dy, xxy);
+            xxz = __builtin_fmaf(mx * dx, dz, xxz);
+            xyy = __builtin_fmaf(mx * dy, dy, xyy);
+            yyz = __builtin_fmaf(my * dy, dz, yyz);
+            xzz = __builtin_fmaf(mx * dz, dz, xzz);
+            yzz = __builtin_fmaf(my * dz, dz, yzz);
+            xyz = __builtin_fmaf(mx * dy, dz, xyz);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (child[k] == -1) continue;
+            const int j = child[k];
+            const float4 cc = wcom[j];
+            const float4 a = wquad[2 * j], b = wquad[2 * j + 1];
+            const float4 p = woct[3 * j], q = woct[3 * j + 1], r = woct[3 * j + 2];
+            const float sxx = a.x, syy = a.y, szz = a.z, sxy = a.w, sxz = b.x, syz = b.y;
+            const float dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
+            const float mx = cc.w * dx, my = cc.w * dy, mz = cc.w * dz;
+            xxx += p.x + 3.f * sxx * dx + mx * dx * dx;
+            yyy += p.y + 3.f * syy * dy + my * dy * dy;
+            zzz += p.z + 3.f * szz * dz + mz * dz * dz;
+            xxy += p.w + sxx * dy + 2.f * sxy * dx + mx * dx * dy;
+            xxz += q.x + sxx * dz + 2.f * sxz * dx + mx * dx * dz;
+            xyy += q.y + syy * dx + 2.f * sxy * dy + mx * dy * dy;
+            yyz += q.z + syy * dz + 2.f * syz * dy + my * dy * dz;
+            xzz += q.w + szz * dx + 2.f * sxz * dz + mx * dz * dz;
+            yzz += r.x + szz * dy + 2.f * syz * dz + my * dz * dz;
+            xyz += r.y + sxy * dz + sxz * dy + syz * dx + mx * dy * dz;
+        }
+    }
+    *o0 = make_float4(xxx, yyy, zzz, xxy);
+    *o1 = make_float4(xxz, xyy, yyz, xzz);
+    *o2 = make_float4(yzz, xyz, 0.f, 0.f);
+}
+
+/* MOM: 0 centre of mass only, 1 also second moments, 2 also third moments */
+template <int MOM>
 __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
                                                     const float4* __restrict__ pos,
                                                     gd_tree_info* info, int level,
                                                     float4* __restrict__ wcom,
                                                     int2* __restrict__ wlink,
-                                                    float4* __restrict__ wquad) {
+                                                    float4* __restrict__ wquad,
+                                                    float4* __restrict__ woct) {
     if (info->overflow) return;
     const int cnt = info->levelCount[level];
     const int off = info->levelOffsets[level];
@@ -398,11 +456,19 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
         wcom[idx] = m.m;
         wlink[idx] = m.leaf ? make_int2(-(first + 1), (count << 5) | level)
                             : make_int2(m.firstChild, (m.nChild << 5) | level);
-        if (QUAD) {
+        if (MOM >= 1) {
             float4 q0, q1;
             gd_node_moments(pos, wcom, wquad, first, count, child, m.leaf, m.m, &q0, &q1);
             wquad[2 * idx] = q0;
             wquad[2 * idx + 1] = q1;
+        }
+        if (MOM >= 2) {
+            float4 o0, o1, o2;
+            gd_node_third(pos, wcom, wquad, woct, first, count, child, m.leaf, m.m, &o0, &o1,
+                          &o2);
+            woct[3 * idx] = o0;
+            woct[3 * idx + 1] = o1;
+            woct[3 * idx + 2] = o2;
         }
     }
     if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
@@ -433,12 +499,15 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
         }
         int blocks = gd_div_up(bound, 256);
         if (blocks > 1024) blocks = 1024;
-        if (b->quad)
-            k_build_mass<true><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
-                                                b->wquad);
+        if (b->quad == 2)
+            k_build_mass<2><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
+                                             b->wquad, b->woct);
+        else if (b->quad == 1)
+            k_build_mass<1><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
+                                             b->wquad, nullptr);
         else
-            k_build_mass<false><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom,
-                                                 b->wlink, nullptr);
+            k_build_mass<0><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
+                                             nullptr, nullptr);
         GD_LAUNCHED();
     }
 }

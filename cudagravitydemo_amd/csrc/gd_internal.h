@@ -116,6 +116,9 @@ typedef struct {
     float4* wquad;
     int quad;
     int quadNodes;
+    /* third moments about the COM, 3 float4 per node, when quad == 2 */
+    float4* woct;
+    int octNodes;
 } gd_builder;
 
 static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }

@@ -36,9 +36,8 @@ const float* gd_mlp_bias(const gd_mlp* net);
  * BF16 helpers
  * ------------------------------------------------------------------------- */
 __device__ __forceinline__ unsigned short gd_f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);   /* round to nearest even */
-    return (unsigned short)(u >> 16);
+    /* hardware conversion, round to nearest even */
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 __device__ __forceinline__ float gd_bf2f(unsigned short h) {
     return __uint_as_float((uint32_t)h << 16);
@@ -55,22 +54,48 @@ __device__ __forceinline__ gd_f32x4 gd_mfma(gd_bf16x4 a, gd_bf16x4 b, gd_f32x4 c
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+/* 16x16x32 product from two pairs of 16x16x16 operands: K index k of the
+ * 32-wide block is taken as (lane quarter q, slot j) -> j < 4 ? 4q + j :
+ * 16 + 4q + j - 4, so the A operand is the concatenation of the two 16x16x16
+ * weight tiles (k tiles 2kb, 2kb+1) a lane already holds, and the B operand
+ * the concatenation of the two activation tiles.  Same sum, half the
+ * instructions. */
+typedef __bf16 gd_bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ gd_bf16x8 gd_cat8(gd_bf16x4 a, gd_bf16x4 b) {
+    const gd_bf16x4 v0 = a, v1 = b;
+    typedef short gd_s16x8 __attribute__((ext_vector_type(8)));
+    const gd_s16x8 r = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
+    return __builtin_bit_cast(gd_bf16x8, r);
+}
+__device__ __forceinline__ gd_f32x4 gd_mfma32(gd_bf16x4 a0, gd_bf16x4 a1, gd_bf16x4 b0,
+                                              gd_bf16x4 b1, gd_f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(gd_cat8(a0, a1), gd_cat8(b0, b1), c, 0, 0,
+                                                   0);
+}
+
 #define GD_GELU_C 0.7978845608028654f
 #define GD_GELU_A 0.044715f
 
+/* GELU, tanh form: 0.5 z (1 + tanh u), u = c (z + a z^3).  With
+ * 1 + tanh u = 2 - 2 / (1 + e^{2u}) this is z - z / (1 + e^{2u}): one
+ * exponential and one reciprocal (both hardware approximations, ~1 ulp),
+ * exact limits z (u -> +inf) and 0 (u -> -inf). */
 __device__ __forceinline__ float gd_gelu(float z) {
-    const float u = GD_GELU_C * __builtin_fmaf(GD_GELU_A * z * z, z, z);
-    const float t = 1.f - 2.f / (__expf(2.f * u) + 1.f);
-    return 0.5f * z * (1.f + t);
+    const float u2 = (2.f * GD_GELU_C * 1.4426950408889634f) * __builtin_fmaf(GD_GELU_A * z * z, z, z);
+    const float r = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u2));
+    return __builtin_fmaf(-z, r, z);
 }
+/* d/dz: with s = 1 / (1 + e^{2u}) (so tanh u = 1 - 2s):
+ * (1 - s) + z * 2 s (1 - s) * c (1 + 3 a z^2). */
 __device__ __forceinline__ float gd_gelu_grad(float z) {
-    const float u = GD_GELU_C * __builtin_fmaf(GD_GELU_A * z * z, z, z);
-    const float t = 1.f - 2.f / (__expf(2.f * u) + 1.f);
-    return 0.5f * (1.f + t) +
-           0.5f * z * (1.f - t * t) * GD_GELU_C * __builtin_fmaf(3.f * GD_GELU_A, z * z, 1.f);
+    const float u2 = (2.f * GD_GELU_C * 1.4426950408889634f) * __builtin_fmaf(GD_GELU_A * z * z, z, z);
+    const float s = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u2));
+    const float du = GD_GELU_C * __builtin_fmaf(3.f * GD_GELU_A, z * z, 1.f);
+    return (1.f - s) + z * (2.f * s * (1.f - s)) * du;
 }
 
-/* Inference only: the same arithmetic as gd_mlp_forward16, no activations kept. */
+/* Inference only (no activations kept): layer 1 on 16x16x16 tiles, the
+ * 64-wide layers on 16x16x32 (see gd_mfma32). */
 __device__ __forceinline__ gd_f32x4 gd_mlp_infer16(const gd_bf16x4* __restrict__ fwd,
                                                    const float* __restrict__ bias, gd_bf16x4 x) {
     const unsigned l = gd_lane();
@@ -90,9 +115,9 @@ __device__ __forceinline__ gd_f32x4 gd_mlp_infer16(const gd_bf16x4* __restrict__
         gd_bf16x4 nxt[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            gd_f32x4 z = zero;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) z = gd_mfma(W[(t * 4 + k) * 64 + l], in[k], z);
+            gd_f32x4 z = gd_mfma32(W[(t * 4 + 0) * 64 + l], W[(t * 4 + 1) * 64 + l], in[0], in[1],
+                                   zero);
+            z = gd_mfma32(W[(t * 4 + 2) * 64 + l], W[(t * 4 + 3) * 64 + l], in[2], in[3], z);
 #pragma unroll
             for (int j = 0; j < 4; ++j) z[j] += bias[64 * L + 16 * t + r0 + j];
             nxt[t] = gd_pack4(gd_gelu(z[0]), gd_gelu(z[1]), gd_gelu(z[2]), gd_gelu(z[3]));
@@ -100,10 +125,9 @@ __device__ __forceinline__ gd_f32x4 gd_mlp_infer16(const gd_bf16x4* __restrict__
 #pragma unroll
         for (int t = 0; t < 4; ++t) in[t] = nxt[t];
     }
-    gd_f32x4 o = zero;
     const gd_bf16x4* W4 = fwd + 36 * 64;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) o = gd_mfma(W4[k * 64 + l], in[k], o);
+    gd_f32x4 o = gd_mfma32(W4[0 * 64 + l], W4[1 * 64 + l], in[0], in[1], zero);
+    o = gd_mfma32(W4[2 * 64 + l], W4[3 * 64 + l], in[2], in[3], o);
 #pragma unroll
     for (int j = 0; j < 4; ++j) o[j] += bias[192 + r0 + j];
     return o;

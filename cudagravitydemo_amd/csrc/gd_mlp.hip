@@ -22,6 +22,7 @@
  */
 #include "gd_internal.h"
 #include "gd_mlp.h"
+#include "gd_traverse.h"
 
 #include <math.h>
 #include <string.h>
@@ -549,21 +550,27 @@ const float* gd_mlp_bias(const gd_mlp* net) { return net->bias; }
 #define GD_SL_WAVES 4
 #define GD_SL_STACK 2048
 
-/* One wavefront per target body: a per-point traversal (a cell is accepted
- * when size^2 < theta^2 r^2, r the distance to its COM); every accepted
- * internal cell becomes a (target, cell) pair. */
+/* One wavefront per target body: the traversal the learned walk makes for
+ * the body's 64-body group (its bounding box against each node, the same
+ * acceptance test); every accepted internal cell becomes a (target, cell)
+ * pair, and centre[t] receives the group's centre, where the walk evaluates
+ * the network. */
 __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
-    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const float4* __restrict__ pos, int n, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info,
     const int* __restrict__ targets, int nt, float theta, int2* __restrict__ pairs,
-    long long cap, unsigned long long* __restrict__ count, int* __restrict__ dropped) {
+    long long cap, unsigned long long* __restrict__ count, int* __restrict__ dropped,
+    float4* __restrict__ centre) {
     __shared__ int sStack[GD_SL_WAVES][GD_SL_STACK];
     const unsigned lane = gd_lane();
     const int wave = threadIdx.x / 64;
     const int t = blockIdx.x * GD_SL_WAVES + wave;
     if (t >= nt) return;
     int* stack = sStack[wave];
-    const float4 x = pos[targets[t]];
+    float lx, ly, lz, hx, hy, hz;
+    gd_group_box(pos, n, targets[t] / 64, lx, ly, lz, hx, hy, hz);
+    if (lane == 0)
+        centre[t] = make_float4(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz), 0.f);
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
@@ -572,61 +579,47 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     while (top > 0) {
         const int take = top < 64 ? top : 64;
         const int base = top - take;
-        bool accept = false, open = false;
-        int idx = 0;
-        int2 lk = make_int2(0, 0);
+        gd_probe pr = {0, 0, 0, false, false, false};
         if ((int)lane < take) {
-            idx = stack[base + lane];
-            const float4 c = wcom[idx];
-            lk = wlink[idx];
-            const float size = ldexpf(ext, -(lk.y & 31));
-            const float dx = c.x - x.x, dy = c.y - x.y, dz = c.z - x.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const bool leaf = lk.x < 0;
-            accept = !leaf && size * size < th2 * d2;
-            open = !leaf && !accept;
+            const int idx = stack[base + lane];
+            pr = gd_test_node(wcom[idx], wlink[idx], idx, ext, th2, lx, ly, lz, hx, hy, hz);
         }
         __builtin_amdgcn_wave_barrier();
         top = base;
-        const unsigned long long bal = __ballot(accept);
+        const bool cell = pr.accept && !pr.leaf;
+        const unsigned long long bal = __ballot(cell);
         const int nacc = __popcll(bal);
         unsigned long long slot0 = 0;
         if (lane == 0 && nacc) slot0 = atomicAdd(count, (unsigned long long)nacc);
         slot0 = __shfl(slot0, 0);
-        if (accept) {
+        if (cell) {
             const unsigned long long slot = slot0 + (unsigned long long)__popcll(bal & lt);
-            if ((long long)slot < cap) pairs[slot] = make_int2(t, idx);
+            if ((long long)slot < cap) pairs[slot] = make_int2(t, pr.idx);
         }
-        const int nk = open ? (lk.y >> 5) : 0;
-        int incl = nk;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int o = __shfl_up(incl, off);
-            if ((int)lane >= off) incl += o;
-        }
-        const int total = __shfl(incl, 63);
+        int excl, total;
+        gd_wave_scan16(pr.nk, lt, excl, total);
         if (top + total > GD_SL_STACK) {
             if (lane == 0) atomicAdd(dropped, 1);
             break;
         }
-        if (open) {
-            const int at = top + incl - nk;
-            for (int k = 0; k < nk; ++k) stack[at + k] = lk.x + k;
-        }
+        if (pr.open)
+            for (int k = 0; k < pr.nk; ++k) stack[top + excl + k] = pr.first + k;
         top += total;
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-/* One wavefront per pair: the 8 features and the target (sum over the cell's
- * bodies - softened monopole - analytic quadrupole) / (m s^2 / r^4), in FP64:
- * what the quadrupole cell misses, which the network learns. */
+/* One wavefront per pair: the 8 features (the walk's: u, s/r, O:uu/(m s^3),
+ * u.S u/(m s^2)) and the target (sum over the cell's bodies - softened
+ * monopole - analytic quadrupole) / (m s^2 / r^4), in FP64: what the
+ * quadrupole cell misses, which the network learns. */
 __global__ __launch_bounds__(256) void k_samples_eval(
     const float4* __restrict__ pos, const node* __restrict__ nodes,
     const float4* __restrict__ wcom, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ targets,
-    const int2* __restrict__ pairs, long long npairs, double eps2, float* __restrict__ X,
-    float* __restrict__ Y) {
+    const float4* __restrict__ centre, const int2* __restrict__ pairs, long long npairs,
+    double eps2, float* __restrict__ X, float* __restrict__ Y) {
     const unsigned lane = gd_lane();
     const long long waves = (long long)gridDim.x * 4;
     for (long long p = (long long)blockIdx.x * 4 + threadIdx.x / 64; p < npairs; p += waves) {
@@ -675,17 +668,28 @@ __global__ __launch_bounds__(256) void k_samples_eval(
             Y[p * 3 + 0] = (float)((ax - mono * dx - qx) / scale);
             Y[p * 3 + 1] = (float)((ay - mono * dy - qy) / scale);
             Y[p * 3 + 2] = (float)((az - mono * dz - qz) / scale);
-            const double r = sqrt(rc2);
-            const double ux = dx / r, uy = dy / r, uz = dz / r;
+            /* features from the group's centre, as the walk computes them */
+            const float4 g = centre[pr.x];
+            const double gx = (double)c.x - g.x, gy = (double)c.y - g.y, gz = (double)c.z - g.z;
+            const double r = sqrt(gx * gx + gy * gy + gz * gz);
+            const double ux = gx / r, uy = gy / r, uz = gz / r;
             const double k = (double)c.w * size * size;
+            const float4 o0 = woct[3 * idx], o1 = woct[3 * idx + 1], o2 = woct[3 * idx + 2];
+            const double xx = ux * ux, yy = uy * uy, zz = uz * uz;
+            const double xy2 = 2.0 * ux * uy, xz2 = 2.0 * ux * uz, yz2 = 2.0 * uy * uz;
+            const double k3 = k * size;
             X[p * 8 + 0] = (float)ux;
             X[p * 8 + 1] = (float)uy;
             X[p * 8 + 2] = (float)uz;
             X[p * 8 + 3] = (float)(size / r);
-            X[p * 8 + 4] = (float)((q0.x * ux + q0.w * uy + q1.x * uz) / k);
-            X[p * 8 + 5] = (float)((q0.w * ux + q0.y * uy + q1.y * uz) / k);
-            X[p * 8 + 6] = (float)((q1.x * ux + q1.y * uy + q0.z * uz) / k);
-            X[p * 8 + 7] = (float)(q1.z / k);
+            X[p * 8 + 4] = (float)((o0.x * xx + o1.y * yy + o1.w * zz + o0.w * xy2 + o1.x * xz2 +
+                                    o2.y * yz2) / k3);
+            X[p * 8 + 5] = (float)((o0.w * xx + o0.y * yy + o2.x * zz + o1.y * xy2 + o2.y * xz2 +
+                                    o1.z * yz2) / k3);
+            X[p * 8 + 6] = (float)((o1.x * xx + o1.z * yy + o0.z * zz + o2.y * xy2 + o1.w * xz2 +
+                                    o2.x * yz2) / k3);
+            X[p * 8 + 7] = (float)((sxx * xx + syy * yy + szz * zz + sxy * xy2 + sxz * xz2 +
+                                    syz * yz2) / k);
         }
     }
 }
@@ -698,23 +702,27 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         return 0;
     }
     gd_builder* b = gd_builder_of(tree->builder);
-    if (!(b->quad && b->wquad)) {
-        GD_FAIL("gd_corrector_samples: build the tree with second moments first");
+    if (!(b->quad == 2 && b->wquad && b->woct)) {
+        GD_FAIL("gd_corrector_samples: build the tree with second and third moments first "
+                "(gd_set_quadrupole(tree, 2))");
         return 0;
     }
     if (nt <= 0 || cap <= 0) return 0;
     int2* pairs = nullptr;
     unsigned long long* count = nullptr;
     int* dropped = nullptr;
+    float4* centre = nullptr;
     GD_HIP(hipMalloc(&pairs, sizeof(int2) * (size_t)cap));
     GD_HIP(hipMalloc(&count, sizeof(unsigned long long)));
     GD_HIP(hipMalloc(&dropped, sizeof(int)));
+    GD_HIP(hipMalloc(&centre, sizeof(float4) * (size_t)nt));
     GD_HIP(hipMemset(count, 0, sizeof(unsigned long long)));
     GD_HIP(hipMemset(dropped, 0, sizeof(int)));
     long long n = 0;
     if (!gd_has_error()) {
         k_samples_list<<<gd_div_up(nt, GD_SL_WAVES), 64 * GD_SL_WAVES>>>(
-            pos, b->wcom, b->wlink, b->dinfo, targets, nt, theta, pairs, cap, count, dropped);
+            pos, tree->starCount, b->wcom, b->wlink, b->dinfo, targets, nt, theta, pairs, cap,
+            count, dropped, centre);
         GD_LAUNCHED();
         unsigned long long total = 0;
         int drop = 0;
@@ -725,12 +733,14 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         if (n > 0) {
             long long blocks = (n + 3) / 4;
             if (blocks > 65536) blocks = 65536;
-            k_samples_eval<<<(int)blocks, 256>>>(pos, tree->nodes, b->wcom, b->wquad, b->dinfo,
-                                                 targets, pairs, n, (double)eps2, X, Y);
+            k_samples_eval<<<(int)blocks, 256>>>(pos, tree->nodes, b->wcom, b->wquad, b->woct,
+                                                 b->dinfo, targets, centre, pairs, n,
+                                                 (double)eps2, X, Y);
             GD_LAUNCHED();
         }
         GD_HIP(hipDeviceSynchronize());
     }
+    GD_HIP(hipFree(centre));
     GD_HIP(hipFree(pairs));
     GD_HIP(hipFree(count));
     GD_HIP(hipFree(dropped));

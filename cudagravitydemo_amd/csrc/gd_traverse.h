@@ -1,0 +1,140 @@
+/*
+ * gd_traverse.h -- the production walk's group traversal (one wavefront per
+ * 64-body group), shared by the walk and by the corrector's sample
+ * generator, which must see the very interaction lists the walk evaluates.
+ */
+#ifndef GD_TRAVERSE_H
+#define GD_TRAVERSE_H
+
+#include "gd_internal.h"
+
+/* Bounding box of group g's bodies, in every lane. */
+__device__ __forceinline__ void gd_group_box(const float4* __restrict__ pos, int n, long long g,
+                                             float& lx, float& ly, float& lz, float& hx,
+                                             float& hy, float& hz) {
+    const long long i = g * 64 + gd_lane();
+    const float4 p = pos[i < n ? i : g * 64];
+    lx = hx = p.x; ly = hy = p.y; lz = hz = p.z;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, off)); hx = fmaxf(hx, __shfl_xor(hx, off));
+        ly = fminf(ly, __shfl_xor(ly, off)); hy = fmaxf(hy, __shfl_xor(hy, off));
+        lz = fminf(lz, __shfl_xor(lz, off)); hz = fmaxf(hz, __shfl_xor(hz, off));
+    }
+}
+
+/* One stack entry tested against the box [l, h]. */
+struct gd_probe {
+    int idx, nk, first;
+    bool accept, open, leaf;
+};
+
+__device__ __forceinline__ gd_probe gd_test_node(float4 c, int2 lk, int idx, float ext,
+                                                 float th2, float lx, float ly, float lz,
+                                                 float hx, float hy, float hz) {
+    gd_probe r;
+    const float size = ldexpf(ext, -(lk.y & 31));
+    const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
+    const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
+    const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
+    const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    r.idx = idx;
+    r.leaf = lk.x < 0;
+    r.accept = r.leaf || (size * size < th2 * d2);
+    r.open = !r.accept;
+    r.nk = r.open ? (lk.y >> 5) : 0;
+    r.first = lk.x;
+    return r;
+}
+
+/* Exclusive prefix and total over the wavefront of v in [0, 15]
+ * (four bit-plane ballots). */
+__device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int& excl,
+                                               int& total) {
+    excl = 0;
+    total = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        const unsigned long long bal = __ballot((v >> bit) & 1);
+        excl += __popcll(bal & lt) << bit;
+        total += __popcll(bal) << bit;
+    }
+}
+
+/* Traversal by one wavefront for the box [l, h]: writes the accepted nodes to
+ * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
+ * SPLIT) unless list is NULL (counting only), and returns false as soon as
+ * more than limit (<= cap) entries are accepted or the stack (stackCap
+ * entries) would overflow.  *opened counts the opened nodes.  Each step pops
+ * up to 128 entries, two per lane, so that two node loads per lane are in
+ * flight; entries are handled in stack order (the upper 64 first). */
+template <bool SPLIT>
+__device__ __forceinline__ bool gd_traverse_box(
+    const float4* __restrict__ wcom, const int2* __restrict__ wlink,
+    const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
+    float hy, float hz, int* stack, int stackCap, int* __restrict__ list, int cap, int limit,
+    int* nFrontOut, int* nBackOut, unsigned long long* opened) {
+    const unsigned lane = gd_lane();
+    const float ext = info->box.extent;
+    const float th2 = theta * theta;
+    const unsigned long long lt = gd_lanemask_lt();
+    int nFront = 0, nBack = 0, top = 1;
+    unsigned long long nOpen = 0;
+    if (lane == 0) stack[0] = 0;
+    while (top > 0) {
+        const int take = top < 128 ? top : 128;
+        const int base = top - take;
+        /* A: entries top-64 .. top-1, B: the 64 below them */
+        const int ea = take - 64 + (int)lane;   /* offset from base */
+        const int eb = take - 128 + (int)lane;
+        const bool hasA = ea >= 0, hasB = eb >= 0;
+        const int ia = hasA ? stack[base + ea] : 0;
+        const int ib = hasB ? stack[base + eb] : 0;
+        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;
+        int2 la = make_int2(0, 0), lb = la;
+        if (hasA) { ca = wcom[ia]; la = wlink[ia]; }
+        if (hasB) { cb = wcom[ib]; lb = wlink[ib]; }
+        gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz);
+        gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz);
+        if (!hasA) A.accept = A.open = false, A.nk = 0;
+        if (!hasB) B.accept = B.open = false, B.nk = 0;
+        __builtin_amdgcn_wave_barrier();
+        top = base;
+        const bool fA = SPLIT ? (A.accept && A.leaf) : A.accept;
+        const bool fB = SPLIT ? (B.accept && B.leaf) : B.accept;
+        const bool kA = SPLIT && A.accept && !A.leaf;
+        const bool kB = SPLIT && B.accept && !B.leaf;
+        const unsigned long long fba = __ballot(fA), fbb = __ballot(fB);
+        const unsigned long long bba = __ballot(kA), bbb = __ballot(kB);
+        const int nfa = __popcll(fba), nfb = __popcll(fbb);
+        const int nba = __popcll(bba), nbb = __popcll(bbb);
+        int exA, totA, exB, totB;
+        gd_wave_scan16(A.nk, lt, exA, totA);
+        gd_wave_scan16(B.nk, lt, exB, totB);
+        nOpen += (unsigned long long)(__popcll(__ballot(A.open)) + __popcll(__ballot(B.open)));
+        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + totA + totB > stackCap)
+            return false;
+        if (list) {
+            if (fA) list[nFront + __popcll(fba & lt)] = A.idx;
+            if (fB) list[nFront + nfa + __popcll(fbb & lt)] = B.idx;
+            if (kA) list[cap - 1 - (nBack + __popcll(bba & lt))] = A.idx;
+            if (kB) list[cap - 1 - (nBack + nba + __popcll(bbb & lt))] = B.idx;
+        }
+        nFront += nfa + nfb;
+        nBack += nba + nbb;
+        /* children of the upper entries go on top: the walk stays depth-first
+           and the stack shallow */
+        if (B.open)
+            for (int k = 0; k < B.nk; ++k) stack[top + exB + k] = B.first + k;
+        if (A.open)
+            for (int k = 0; k < A.nk; ++k) stack[top + totB + exA + k] = A.first + k;
+        top += totA + totB;
+        __builtin_amdgcn_wave_barrier();
+    }
+    *nFrontOut = nFront;
+    *nBackOut = nBack;
+    *opened = nOpen;
+    return true;
+}
+
+#endif

@@ -26,6 +26,7 @@
  */
 #include "gd_internal.h"
 #include "gd_mlp.h"
+#include "gd_traverse.h"
 
 #include <math.h>
 
@@ -156,135 +157,6 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
 #define GD_TR2_STACK 12288        /* second pass, one wavefront per block (48 KB of LDS) */
-
-/* Bounding box of group g's bodies, in every lane. */
-__device__ __forceinline__ void gd_group_box(const float4* __restrict__ pos, int n, long long g,
-                                             float& lx, float& ly, float& lz, float& hx,
-                                             float& hy, float& hz) {
-    const long long i = g * 64 + gd_lane();
-    const float4 p = pos[i < n ? i : g * 64];
-    lx = hx = p.x; ly = hy = p.y; lz = hz = p.z;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        lx = fminf(lx, __shfl_xor(lx, off)); hx = fmaxf(hx, __shfl_xor(hx, off));
-        ly = fminf(ly, __shfl_xor(ly, off)); hy = fmaxf(hy, __shfl_xor(hy, off));
-        lz = fminf(lz, __shfl_xor(lz, off)); hz = fmaxf(hz, __shfl_xor(hz, off));
-    }
-}
-
-/* One stack entry tested against the box [l, h]. */
-struct gd_probe {
-    int idx, nk, first;
-    bool accept, open, leaf;
-};
-
-__device__ __forceinline__ gd_probe gd_test_node(float4 c, int2 lk, int idx, float ext,
-                                                 float th2, float lx, float ly, float lz,
-                                                 float hx, float hy, float hz) {
-    gd_probe r;
-    const float size = ldexpf(ext, -(lk.y & 31));
-    const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
-    const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
-    const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
-    const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-    r.idx = idx;
-    r.leaf = lk.x < 0;
-    r.accept = r.leaf || (size * size < th2 * d2);
-    r.open = !r.accept;
-    r.nk = r.open ? (lk.y >> 5) : 0;
-    r.first = lk.x;
-    return r;
-}
-
-/* Exclusive prefix and total over the wavefront of v in [0, 15]
- * (four bit-plane ballots). */
-__device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int& excl,
-                                               int& total) {
-    excl = 0;
-    total = 0;
-#pragma unroll
-    for (int bit = 0; bit < 4; ++bit) {
-        const unsigned long long bal = __ballot((v >> bit) & 1);
-        excl += __popcll(bal & lt) << bit;
-        total += __popcll(bal) << bit;
-    }
-}
-
-/* Traversal by one wavefront for the box [l, h]: writes the accepted nodes to
- * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
- * SPLIT) unless list is NULL (counting only), and returns false as soon as
- * more than limit (<= cap) entries are accepted or the stack (stackCap
- * entries) would overflow.  *opened counts the opened nodes.  Each step pops
- * up to 128 entries, two per lane, so that two node loads per lane are in
- * flight; entries are handled in stack order (the upper 64 first). */
-template <bool SPLIT>
-__device__ __forceinline__ bool gd_traverse_box(
-    const float4* __restrict__ wcom, const int2* __restrict__ wlink,
-    const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
-    float hy, float hz, int* stack, int stackCap, int* __restrict__ list, int cap, int limit,
-    int* nFrontOut, int* nBackOut, unsigned long long* opened) {
-    const unsigned lane = gd_lane();
-    const float ext = info->box.extent;
-    const float th2 = theta * theta;
-    const unsigned long long lt = gd_lanemask_lt();
-    int nFront = 0, nBack = 0, top = 1;
-    unsigned long long nOpen = 0;
-    if (lane == 0) stack[0] = 0;
-    while (top > 0) {
-        const int take = top < 128 ? top : 128;
-        const int base = top - take;
-        /* A: entries top-64 .. top-1, B: the 64 below them */
-        const int ea = take - 64 + (int)lane;   /* offset from base */
-        const int eb = take - 128 + (int)lane;
-        const bool hasA = ea >= 0, hasB = eb >= 0;
-        const int ia = hasA ? stack[base + ea] : 0;
-        const int ib = hasB ? stack[base + eb] : 0;
-        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;
-        int2 la = make_int2(0, 0), lb = la;
-        if (hasA) { ca = wcom[ia]; la = wlink[ia]; }
-        if (hasB) { cb = wcom[ib]; lb = wlink[ib]; }
-        gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz);
-        gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz);
-        if (!hasA) A.accept = A.open = false, A.nk = 0;
-        if (!hasB) B.accept = B.open = false, B.nk = 0;
-        __builtin_amdgcn_wave_barrier();
-        top = base;
-        const bool fA = SPLIT ? (A.accept && A.leaf) : A.accept;
-        const bool fB = SPLIT ? (B.accept && B.leaf) : B.accept;
-        const bool kA = SPLIT && A.accept && !A.leaf;
-        const bool kB = SPLIT && B.accept && !B.leaf;
-        const unsigned long long fba = __ballot(fA), fbb = __ballot(fB);
-        const unsigned long long bba = __ballot(kA), bbb = __ballot(kB);
-        const int nfa = __popcll(fba), nfb = __popcll(fbb);
-        const int nba = __popcll(bba), nbb = __popcll(bbb);
-        int exA, totA, exB, totB;
-        gd_wave_scan16(A.nk, lt, exA, totA);
-        gd_wave_scan16(B.nk, lt, exB, totB);
-        nOpen += (unsigned long long)(__popcll(__ballot(A.open)) + __popcll(__ballot(B.open)));
-        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + totA + totB > stackCap)
-            return false;
-        if (list) {
-            if (fA) list[nFront + __popcll(fba & lt)] = A.idx;
-            if (fB) list[nFront + nfa + __popcll(fbb & lt)] = B.idx;
-            if (kA) list[cap - 1 - (nBack + __popcll(bba & lt))] = A.idx;
-            if (kB) list[cap - 1 - (nBack + nba + __popcll(bbb & lt))] = B.idx;
-        }
-        nFront += nfa + nfb;
-        nBack += nba + nbb;
-        /* children of the upper entries go on top: the walk stays depth-first
-           and the stack shallow */
-        if (B.open)
-            for (int k = 0; k < B.nk; ++k) stack[top + exB + k] = B.first + k;
-        if (A.open)
-            for (int k = 0; k < A.nk; ++k) stack[top + totB + exA + k] = A.first + k;
-        top += totA + totB;
-        __builtin_amdgcn_wave_barrier();
-    }
-    *nFrontOut = nFront;
-    *nBackOut = nBack;
-    *opened = nOpen;
-    return true;
-}
 
 /*
  * Traversal kernel, one wavefront per group.  Slot s walks group
@@ -420,35 +292,39 @@ __device__ __forceinline__ void gd_eval64_quad(const float4* __restrict__ e,
 }
 
 /*
- * The network on up to 64 cells, one per lane (lanes 0..m-1 active): lane l
- * holds cell l's features (u = unit vector to the COM, ratio = s/r,
- * su = S u / (m s^2), tr = tr S / (m s^2)) and receives the cell's scaled
- * residual (ox, oy, oz).  Pass t evaluates cells 16t..16t+15 on the matrix
- * cores: column j of the 16x16 tile is cell 16t + j, whose feature quarter
- * (u, ratio) or (su, tr) the lanes of rows 0-3 / 4-7 fetch by shuffles.
+ * The network on the m cells of a compacted list: cell c (c < m) is held by
+ * lane map[c] (map in LDS), whose 8 features f[0..7] are fetched by
+ * shuffles.  Pass t evaluates cells 16t..16t+15 on the matrix cores (column j
+ * of the 16x16 input tile is cell 16t + j; the lanes of rows 0-3 / 4-7 fetch
+ * features 0-3 / 4-7).  On return lane c holds cell c's scaled residual
+ * (ox, oy, oz).
  */
 __device__ __forceinline__ void gd_mlp_cells(const gd_bf16x4* __restrict__ W,
-                                             const float* __restrict__ Bv, int m, float ux,
-                                             float uy, float uz, float ratio, float su0,
-                                             float su1, float su2, float tr, float& ox,
+                                             const float* __restrict__ Bv, int m,
+                                             const int* map, const float* f, float& ox,
                                              float& oy, float& oz) {
     const unsigned lane = gd_lane();
     ox = oy = oz = 0.f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         if (16 * t >= m) break;   /* wave-uniform */
-        const int src = 16 * t + (int)(lane % 16);
+        const int c = 16 * t + (int)(lane % 16);
+        const int src = c < m ? map[c] : 0;
         const unsigned sel = lane / 16;
-        const float u0 = __shfl(ux, src), s0 = __shfl(su0, src);
-        const float u1 = __shfl(uy, src), s1 = __shfl(su1, src);
-        const float u2 = __shfl(uz, src), s2 = __shfl(su2, src);
-        const float u3 = __shfl(ratio, src), s3 = __shfl(tr, src);
-        const float a0 = sel == 0 ? u0 : s0;
-        const float a1 = sel == 0 ? u1 : s1;
-        const float a2 = sel == 0 ? u2 : s2;
-        const float a3 = sel == 0 ? u3 : s3;
-        const gd_bf16x4 x = sel < 2 ? gd_pack4(a0, a1, a2, a3) : gd_pack4(0.f, 0.f, 0.f, 0.f);
-        const gd_f32x4 o = gd_mlp_infer16(W, Bv, x);
+        float a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            /* both quarters are fetched; the reader keeps its own */
+            const float lo = __shfl(f[q], src), hi = __shfl(f[4 + q], src);
+            a[q] = sel == 0 ? lo : hi;
+        }
+        const gd_bf16x4 x = sel < 2 ? gd_pack4(a[0], a[1], a[2], a[3])
+                                    : gd_pack4(0.f, 0.f, 0.f, 0.f);
+        /* an offset the compiler cannot see through: the weight and bias reads
+           stay in the pass instead of being hoisted into ~130 registers */
+        int off = 0;
+        asm volatile("" : "+v"(off));
+        const gd_f32x4 o = gd_mlp_infer16(W + off, Bv + off, x);
         /* outputs of cell 16t + j live in lane j (j < 16) */
         const float r0 = __shfl(o[0], (int)(lane % 16));
         const float r1 = __shfl(o[1], (int)(lane % 16));
@@ -459,26 +335,58 @@ __device__ __forceinline__ void gd_mlp_cells(const gd_bf16x4* __restrict__ W,
     }
 }
 
-/* Features of a cell (COM c, moments q0/q1, edge size) seen from point x;
- * k = m s^2 (0 when the cell must not be corrected: massless, or
- * s/r <= minRatio). */
-__device__ __forceinline__ void gd_cell_features(float4 c, float4 q0, float4 q1, float size,
-                                                 float x, float y, float z, bool valid,
-                                                 float minRatio, float& ux, float& uy,
-                                                 float& uz, float& ratio, float& su0,
-                                                 float& su1, float& su2, float& tr, float& k) {
+/* The network for the lanes with need set: compacts them (map in LDS, 64
+ * ints), evaluates, and returns each needing lane its own residual. */
+__device__ __forceinline__ void gd_mlp_lanes(const gd_bf16x4* __restrict__ W,
+                                             const float* __restrict__ Bv, bool need, int* map,
+                                             const float* f, float& ox, float& oy, float& oz) {
+    const unsigned long long bal = __ballot(need);
+    ox = oy = oz = 0.f;
+    if (!bal) return;   /* wave-uniform */
+    const int m = __popcll(bal);
+    const int at = __popcll(bal & gd_lanemask_lt());
+    if (need) map[at] = (int)gd_lane();
+    __builtin_amdgcn_wave_barrier();
+    float cx, cy, cz;
+    gd_mlp_cells(W, Bv, m, map, f, cx, cy, cz);
+    const float rx = __shfl(cx, at), ry = __shfl(cy, at), rz = __shfl(cz, at);
+    if (need) {
+        ox = rx; oy = ry; oz = rz;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+/*
+ * The network's inputs for a cell seen from point x: with d = c - x, r = |d|,
+ * u = d / r, s the cell's edge, m its mass, S and O its second and third
+ * moments about c:
+ *   f = [u (3), s / r, O:uu / (m s^3) (3), u.S u / (m s^2)],
+ * (O:uu)_i = O_ijk u_j u_k.  q0/q1 hold S (Sxx Syy Szz Sxy, Sxz Syz tr -),
+ * o0/o1/o2 hold O (xxx yyy zzz xxy, xxz xyy yyz xzz, yzz xyz - -).
+ * k = m s^2, or 0 when the cell is not corrected (invalid, massless, or
+ * s / r <= minRatio).
+ */
+__device__ __forceinline__ void gd_cell_features(float4 c, float4 q0, float4 q1, float4 o0,
+                                                 float4 o1, float4 o2, float size, float x,
+                                                 float y, float z, bool valid, float minRatio,
+                                                 float* f, float& k) {
     k = c.w * size * size;
     const float dx = c.x - x, dy = c.y - y, dz = c.z - z;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float rinv = valid ? __builtin_amdgcn_rsqf(r2) : 0.f;
-    ux = dx * rinv; uy = dy * rinv; uz = dz * rinv;
-    ratio = size * rinv;
-    const float kinv = (valid && k > 0.f) ? 1.f / k : 0.f;
-    su0 = __builtin_fmaf(q0.x, ux, __builtin_fmaf(q0.w, uy, q1.x * uz)) * kinv;
-    su1 = __builtin_fmaf(q0.w, ux, __builtin_fmaf(q0.y, uy, q1.y * uz)) * kinv;
-    su2 = __builtin_fmaf(q1.x, ux, __builtin_fmaf(q1.y, uy, q0.z * uz)) * kinv;
-    tr = q1.z * kinv;
-    if (!(valid && k > 0.f && ratio > minRatio)) k = 0.f;
+    const float ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
+    const float ratio = size * rinv;
+    const bool use = valid && k > 0.f;
+    const float kinv = use ? __builtin_amdgcn_rcpf(k) : 0.f;
+    const float k3inv = use ? kinv * __builtin_amdgcn_rcpf(size) : 0.f;
+    const float xx = ux * ux, yy = uy * uy, zz = uz * uz;
+    const float xy2 = 2.f * ux * uy, xz2 = 2.f * ux * uz, yz2 = 2.f * uy * uz;
+    f[0] = ux; f[1] = uy; f[2] = uz; f[3] = ratio;
+    f[4] = (o0.x * xx + o1.y * yy + o1.w * zz + o0.w * xy2 + o1.x * xz2 + o2.y * yz2) * k3inv;
+    f[5] = (o0.w * xx + o0.y * yy + o2.x * zz + o1.y * xy2 + o2.y * xz2 + o1.z * yz2) * k3inv;
+    f[6] = (o1.x * xx + o1.z * yy + o0.z * zz + o2.y * xy2 + o1.w * xz2 + o2.x * yz2) * k3inv;
+    f[7] = (q0.x * xx + q0.y * yy + q0.z * zz + q0.w * xy2 + q1.x * xz2 + q1.y * yz2) * kinv;
+    if (!(use && ratio > minRatio)) k = 0.f;
 }
 
 /*
@@ -490,16 +398,17 @@ __device__ __forceinline__ void gd_cell_features(float4 c, float4 q0, float4 q1,
  * 1: back entries are quadrupole cells; 2: back entries are quadrupole cells
  * plus the network's residual correction (evaluated per chunk at the group's
  * centre on the matrix cores, scaled by each body's own m s^2 / r^4).
- * sE holds 2 x 64 float4; sQ0, sQ1, sCor 64 each.
+ * sE holds 2 x 64 float4; sQ0, sQ1, sCor 64 each; sMap 64 ints.
  */
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_group(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ list, int cap, int2 cnt,
     long long g, int n, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sE, float4* sQ0, float4* sQ1,
-    float4* sCor, gd_kahan3& tot, float& phs, float& phc) {
+    float4* sCor, int* sMap, gd_kahan3& tot, float& phs, float& phc) {
     const unsigned lane = gd_lane();
     const long long i = g * 64 + lane;
     const bool valid = i < n;
@@ -559,11 +468,17 @@ __device__ __forceinline__ void gd_eval_group(
         sQ1[lane] = q1;
         if (MODE == 2) {
             /* network at the group centre; lane l gets cell l's residual and m s^2 */
-            const float size = idx >= 0 ? ldexpf(ext, -(wlink[idx].y & 31)) : 1.f;
-            float ux, uy, uz, ratio, su0, su1, su2, tr, k, ox, oy, oz;
-            gd_cell_features(c, q0, q1, size, gx, gy, gz, idx >= 0, minRatio, ux, uy, uz, ratio,
-                             su0, su1, su2, tr, k);
-            gd_mlp_cells(mlpW, mlpB, m, ux, uy, uz, ratio, su0, su1, su2, tr, ox, oy, oz);
+            float4 o0 = zero4, o1 = zero4, o2 = zero4;
+            float size = 1.f;
+            if (idx >= 0) {
+                o0 = woct[3 * idx];
+                o1 = woct[3 * idx + 1];
+                o2 = woct[3 * idx + 2];
+                size = ldexpf(ext, -(wlink[idx].y & 31));
+            }
+            float f[8], k, ox, oy, oz;
+            gd_cell_features(c, q0, q1, o0, o1, o2, size, gx, gy, gz, idx >= 0, minRatio, f, k);
+            gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
             sCor[lane] = make_float4(ox, oy, oz, k);
         }
         __builtin_amdgcn_wave_barrier();
@@ -596,6 +511,7 @@ template <int MODE, bool PHI>
 __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ groupList, int slots,
     const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, int n,
@@ -604,6 +520,18 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sCor[MODE == 2 ? GD_EV_WAVES : 1][64];
+    __shared__ int sMap[MODE == 2 ? GD_EV_WAVES : 1][64];
+    __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
+    __shared__ float sB[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
+    if (MODE == 2) {
+        /* the network's packed weights and biases, once per block */
+        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64 * GD_EV_WAVES)
+            sW[e] = mlpW[e];
+        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64 * GD_EV_WAVES) sB[e] = mlpB[e];
+        __syncthreads();
+        mlpW = sW;
+        mlpB = sB;
+    }
     const unsigned lane = gd_lane();
     const int wave = (int)(threadIdx.x >> 6);
     const long long s = (long long)blockIdx.x * GD_EV_WAVES + wave;
@@ -613,10 +541,11 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const long long g = groupList ? groupList[s] : s;
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, info, lists + (size_t)s * cap, cap, cnt,
-                             g, n, eps2, mlpW, mlpB, minRatio, sE[wave],
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, lists + (size_t)s * cap, cap,
+                             cnt, g, n, eps2, mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
-                             sCor[MODE == 2 ? wave : 0], tot, phs, phc);
+                             sCor[MODE == 2 ? wave : 0], sMap[MODE == 2 ? wave : 0], tot, phs,
+                             phc);
     const long long i = g * 64 + lane;
     if (i < n) {
         acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
@@ -641,11 +570,22 @@ template <int MODE, bool PHI>
 __global__ __launch_bounds__(64) void k_walk_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ groupList,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
     int n, const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB,
     float minRatio, unsigned long long* __restrict__ stats) {
     __shared__ int sStack[GD_BODY_STACK];
+    __shared__ int sMap[MODE == 2 ? 64 : 1];
+    __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
+    __shared__ float sB[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
+    if (MODE == 2) {
+        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64) sW[e] = mlpW[e];
+        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64) sB[e] = mlpB[e];
+        __syncthreads();
+        mlpW = sW;
+        mlpB = sB;
+    }
     const unsigned lane = gd_lane();
     const long long i = (long long)groupList[blockIdx.x >> 6] * 64 + (blockIdx.x & 63);
     if (i >= n) return;   /* block-uniform */
@@ -728,11 +668,16 @@ __global__ __launch_bounds__(64) void k_walk_body(
                 ph = __builtin_fmaf(0.5f * inv5, __builtin_fmaf(3.f, q, -r2 * q1.z), mi);
             }
             if (MODE == 2 && cb) {
-                const int m = 64 - __builtin_clzll(cb);
-                float ux, uy, uz, ratio, su0, su1, su2, tr, k, ox, oy, oz;
-                gd_cell_features(c, q0, q1, size, p.x, p.y, p.z, cell, minRatio, ux, uy, uz,
-                                 ratio, su0, su1, su2, tr, k);
-                gd_mlp_cells(mlpW, mlpB, m, ux, uy, uz, ratio, su0, su1, su2, tr, ox, oy, oz);
+                float4 o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0;
+                if (cell) {
+                    o0 = woct[3 * idx];
+                    o1 = woct[3 * idx + 1];
+                    o2 = woct[3 * idx + 2];
+                }
+                float f[8], k, ox, oy, oz;
+                gd_cell_features(c, q0, q1, o0, o1, o2, size, p.x, p.y, p.z, cell, minRatio, f,
+                                 k);
+                gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
                 if (cell) {
                     const float i2 = __builtin_amdgcn_rcpf(d2);
                     const float s4 = k * (i2 * i2);
@@ -855,8 +800,8 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            float3* acc, float* phi, float eps2, float G, int n,
                            const gd_bf16x4* W, const float* Bv, float minRatio) {
     k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->dinfo, groupList, slots, lists, counts, cap, acc,
-        phi, eps2, G, n, W, Bv, minRatio);
+        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, groupList, slots, lists, counts,
+        cap, acc, phi, eps2, G, n, W, Bv, minRatio);
 }
 
 template <int MODE, bool PHI>
@@ -912,8 +857,8 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     if (over2 <= 0) return;
     /* pass 3: body by body */
     k_walk_body<MODE, PHI><<<64u * (unsigned)over2, 64>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.overflow2, acc, phi, eps2, G, theta,
-        n, W, Bv, minRatio, g_walk.stats);
+        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.overflow2, acc, phi, eps2,
+        G, theta, n, W, Bv, minRatio, g_walk.stats);
     GD_LAUNCHED();
 }
 
@@ -928,6 +873,11 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     gd_builder* b = gd_builder_of(tree->builder);
     if (mode != 0 && !(b->quad && b->wquad)) {
         GD_FAIL("gd_walk: this walk needs the second moments (gd_set_quadrupole before the build)");
+        return;
+    }
+    if (mode == 2 && !(b->quad == 2 && b->woct)) {
+        GD_FAIL("gd_walk_learned: the network needs the third moments "
+                "(gd_set_quadrupole(tree, 2) before the build)");
         return;
     }
     const int groups = gd_div_up(starCount, 64);

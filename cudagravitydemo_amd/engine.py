@@ -128,8 +128,10 @@ class Simulation:
         check(self.lib)
 
     def set_quadrupole(self, enable):
-        """Compute second moments in later builds; the walk then uses them."""
-        self.lib.gd_set_quadrupole(C.byref(self.state), 1 if enable else 0)
+        """Compute second moments in later builds (the walk then uses them);
+        enable=2 also computes the third moments (the learned walk's inputs)."""
+        level = 2 if enable == 2 else (1 if enable else 0)
+        self.lib.gd_set_quadrupole(C.byref(self.state), level)
         check(self.lib)
 
     def walk(self, theta, potential=False, stats=False):
@@ -218,6 +220,14 @@ class Simulation:
                                      quad.ctypes.data, count)
         check(self.lib)
         return com, link, quad
+
+    def third_moments(self, count):
+        """Third moments (count, 12) of the first `count` nodes: Oxxx Oyyy Ozzz
+        Oxxy Oxxz Oxyy Oyyz Oxzz Oyzz Oxyz 0 0."""
+        out = np.empty((count, 12), np.float32)
+        self.lib.gd_tree_third_moments(C.byref(self.state), out.ctypes.data, count)
+        check(self.lib)
+        return out
 
     def levels(self):
         """Host copy of levelOffsets / levelCount / deepestLevel."""

@@ -42,7 +42,7 @@ def main():
         net = cr.Corrector(np.load(a.params) if a.params else None, seed=1)
     with Simulation(pos, vel, eps=eps) as sim:
         if a.walk in ("quad", "learned"):
-            sim.set_quadrupole(True)
+            sim.set_quadrupole(2 if a.walk == "learned" else 1)
         sim.sync()
         t0 = time.perf_counter()
         for it in range(a.warmup + a.steps):

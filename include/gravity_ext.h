@@ -53,10 +53,17 @@ void gd_tree_summary(const treeState* tree, int* totalNodes, int* overflow, int*
 void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
                          float3** spareLo);
 
-/* Makes every later build also compute each node's second moments about its
- * centre of mass (S = sum m (x-c)(x-c)^T, used by the quadrupole and learned
- * walks); enable = 0 stops it.  Allocates 32 B per node of capacity. */
+/* Makes every later build also compute each node's moments about its centre
+ * of mass: enable = 1 the second moments S = sum m (x-c)(x-c)^T (used by the
+ * quadrupole walk; 32 B per node of capacity), 2 also the third moments
+ * O = sum m (x-c)(x-c)(x-c) (the learned walk's inputs; 48 B more per
+ * node); 0 stops both. */
 void gd_set_quadrupole(treeState* tree, int enable);
+
+/* Copies the third moments of nodes [0, count) to host memory, 3 float4 per
+ * node: Oxxx Oyyy Ozzz Oxxy, Oxxz Oxyy Oyyz Oxzz, Oyzz Oxyz 0 0
+ * (gd_set_quadrupole(tree, 2) before the build).  Synchronises. */
+void gd_tree_third_moments(const treeState* tree, void* oct, int count);
 
 /* Copies the compact walk records of nodes [0, count) to host memory:
  * com (float4: centre of mass, mass), link (int2: first child or -(first
@@ -121,8 +128,10 @@ int gd_walk_phase_ms(float* ms);
  * FP32 accumulation on the matrix cores.  Parameters (FP32, row-major
  * [out][in]): W1[64][8] @0, b1 @512, W2[64][64] @576, b2 @4672,
  * W3[64][64] @4736, b3 @8832, W4[3][64] @8896, b4 @9088 (9091 values).
- * Inputs per (point, cell): unit vector to the cell's COM (3), size / r,
- * S u / (m s^2) (3), tr S / (m s^2); output: the correction of the cell's
+ * Inputs per (point, cell), with u the unit vector from the point to the
+ * cell's COM, r the distance, s the cell's edge, m its mass, S and O its
+ * second and third moments about the COM: u (3), s / r, (O_ijk u_j u_k) /
+ * (m s^3) (3), u.S u / (m s^2).  Output: the correction of the cell's
  * acceleration beyond its monopole and quadrupole terms, in units of
  * G m s^2 / r^4.  X (rows x 8), Y (rows x 3) and order are device arrays. */
 #define GD_MLP_PARAMS 9091
@@ -142,7 +151,8 @@ float gd_mlp_gradient(gd_mlp* net, const float* X, const float* Y, const int* or
 float gd_mlp_train_step(gd_mlp* net, const float* X, const float* Y, const int* order,
                         long long B, float lr);
 void gd_mlp_adam_state(const gd_mlp* net, float* hostM, float* hostV);
-/* Training samples from the current tree (built with second moments): for
+/* Training samples from the current tree (built with second and third
+ * moments, gd_set_quadrupole(tree, 2)): for
  * each target body (device indices, sorted order), every internal cell a
  * per-body traversal at opening angle theta accepts gives one row of X and
  * the exact scaled residual in Y (FP64 sum over the cell's bodies minus the
@@ -155,7 +165,8 @@ long long gd_corrector_samples(const treeState* tree, const float4* pos, const i
 /* gd_walk with quadrupole cells plus the network's residual correction of
  * every accepted cell whose size / r (from the group's centre) exceeds
  * minRatio, evaluated once per (64-body group, cell) on the matrix cores and
- * scaled per body by m s^2 / r^4.  Needs a tree built with second moments. */
+ * scaled per body by m s^2 / r^4.  Needs a tree built with second and third
+ * moments (gd_set_quadrupole(tree, 2)). */
 void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc, float* phi,
                      float eps2, float G, float theta, int starCount, const gd_mlp* net,
                      float minRatio, gd_walk_stats* stats);

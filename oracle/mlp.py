@@ -119,16 +119,41 @@ def adam(params, m, v, grad, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
 
 # ---- features and targets of one (point, cell) interaction ------------------
 
-def features(x, com, mass, size, S):
+def third_tensor(o):
+    """The symmetric 3x3x3 tensor of the 10 third moments in the library's
+    order: xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz."""
+    xxx, yyy, zzz, xxy, xxz, xyy, yyz, xzz, yzz, xyz = (float(v) for v in o[:10])
+    T = np.empty((3, 3, 3))
+    val = {(0, 0, 0): xxx, (1, 1, 1): yyy, (2, 2, 2): zzz, (0, 0, 1): xxy, (0, 0, 2): xxz,
+           (0, 1, 1): xyy, (1, 1, 2): yyz, (0, 2, 2): xzz, (1, 2, 2): yzz, (0, 1, 2): xyz}
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                T[i, j, k] = val[tuple(sorted((i, j, k)))]
+    return T
+
+
+def moments(bodies, com):
+    """Second and third moments (3x3, 3x3x3) of bodies (x, y, z, m) about com."""
+    b = np.asarray(bodies, np.float64)
+    d = b[:, :3] - np.asarray(com, np.float64)
+    m = b[:, 3]
+    S = np.einsum("k,ki,kj->ij", m, d, d)
+    O = np.einsum("k,ki,kj,kl->ijl", m, d, d, d)
+    return S, O
+
+
+def features(x, com, mass, size, S, O):
     """x: (3,) target point; com, mass, size: cell COM, mass, edge length;
-    S: (3,3) second moments about the COM.  Returns the 8 inputs:
-    d/r (unit vector to the COM), s/r, S d/r / (m s^2) (3), tr S / (m s^2)."""
+    S (3,3), O (3,3,3): second and third moments about the COM.  Returns the 8
+    inputs: u = (com - x)/r (3), s/r, O_ijk u_j u_k / (m s^3) (3),
+    u.S u / (m s^2)."""
     d = np.asarray(com, np.float64) - np.asarray(x, np.float64)
     r = np.linalg.norm(d)
     u = d / r
     k = mass * size * size
-    sd = np.asarray(S, np.float64) @ u / k
-    return np.r_[u, size / r, sd, np.trace(S) / k]
+    ouu = np.einsum("ijk,j,k->i", np.asarray(O, np.float64), u, u) / (k * size)
+    return np.r_[u, size / r, ouu, u @ np.asarray(S, np.float64) @ u / k]
 
 
 def target(x, bodies, com, mass, size, eps2, G=1.0):

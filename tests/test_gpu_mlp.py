@@ -83,7 +83,7 @@ def test_adam_step_matches_closed_form():
 def test_samples_match_direct_sums():
     pos, vel = ics.exponential_disk(8192, seed=13)
     with Simulation(pos, vel, eps=0.01) as sim:
-        sim.set_quadrupole(True)
+        sim.set_quadrupole(2)
         sim.build_tree_async()
         targets = ics.subsample(8192, 64, seed=2)
         s = cr.SampleSet(sim.lib, 200000)
@@ -96,6 +96,7 @@ def test_samples_match_direct_sums():
         # recompute a few rows from the tree with the oracle's definitions
         total = sim.tree_summary()[0]
         com, link, quad = sim.walk_arrays(total)
+        oct_ = sim.third_moments(total)
         from oracle.binding import NODE_DTYPE
         nodes = sim.nodes_raw(total).view(NODE_DTYPE).reshape(-1)
         ps = sim.positions()
@@ -104,73 +105,82 @@ def test_samples_match_direct_sums():
         rng = np.random.default_rng(0)
         cand = np.flatnonzero(link[:, 0] >= 0)
         for row in rng.choice(n, 40, replace=False):
-            # find the (target, cell) of this row among all internal cells: a node and its
-            # only child share the centre of mass, so match on every feature
-            best = None
+            # find the (target, cell) of this row among all internal cells; targets in
+            # one 64-body group share the features, and a node and its only child share
+            # the centre of mass, so keep every candidate that matches all features
+            cands = []
             for t in targets:
                 x = ps[t, :3].astype(np.float64)
-                d = com[cand, :3] - x
+                # features are taken at the centre of the target's 64-body group
+                blk = ps[64 * (t // 64):64 * (t // 64) + 64, :3]
+                g = (np.float32(0.5) * (blk.min(0) + blk.max(0))).astype(np.float64)
+                d = com[cand, :3] - g
                 u = d / np.linalg.norm(d, axis=1)[:, None]
                 for idx in cand[np.all(np.abs(u - X[row, :3]) < 1e-6, axis=1)]:
                     size = ext * 2.0 ** -int(nodes[idx]["level"])
                     S = np.array([[quad[idx, 0], quad[idx, 3], quad[idx, 4]],
                                   [quad[idx, 3], quad[idx, 1], quad[idx, 5]],
                                   [quad[idx, 4], quad[idx, 5], quad[idx, 2]]])
-                    f = mlp.features(x, com[idx, :3], com[idx, 3], size, S)
-                    dev = np.abs(f - X[row]).max()
-                    if best is None or dev < best[0]:
-                        best = (dev, x, idx, size, f)
-            assert best is not None
-            dev, x, idx, size, f = best
-            assert dev < 1e-4 * max(1, np.abs(f).max())
-            a, c = int(nodes[idx]["first"]), int(nodes[idx]["count"])
-            S = np.array([[quad[idx, 0], quad[idx, 3], quad[idx, 4]],
-                          [quad[idx, 3], quad[idx, 1], quad[idx, 5]],
-                          [quad[idx, 4], quad[idx, 5], quad[idx, 2]]])
-            y = mlp.residual_target(x, ps[a:a + c], com[idx, :3], com[idx, 3], size, S,
-                                    0.01 ** 2)
-            assert np.abs(y - Y[row]).max() < 1e-3 * max(1.0, np.abs(y).max())
+                    O = mlp.third_tensor(oct_[idx])
+                    f = mlp.features(g, com[idx, :3], com[idx, 3], size, S, O)
+                    if np.abs(f - X[row]).max() < 1e-4 * max(1, np.abs(f).max()):
+                        cands.append((x, idx, size, S))
+            assert cands, row
+            ok = False
+            for x, idx, size, S in cands:
+                a, c = int(nodes[idx]["first"]), int(nodes[idx]["count"])
+                y = mlp.residual_target(x, ps[a:a + c], com[idx, :3], com[idx, 3], size, S,
+                                        0.01 ** 2)
+                ok = ok or np.abs(y - Y[row]).max() < 1e-3 * max(1.0, np.abs(y).max())
+            assert ok, row
             found += 1
         assert found >= 30
 
 
-@pytest.mark.xfail(reason="with quadrupole-only inputs the network explains ~10% of the "
-                   "residual and does not yet beat the analytic quadrupole walk", strict=False)
-def test_trained_corrector_gains_over_plain_walk():
+def test_trained_corrector_gains_over_quadrupole_walk():
     """Train on snapshots of a disk, then compare at theta 0.8 against FP64:
-    the learned walk (quadrupole + learned residual) beats the monopole walk
-    and does not lose to the quadrupole walk it refines."""
+    the learned walk (quadrupole + learned residual) beats the monopole walk by
+    far and the quadrupole walk it refines, in the median and in the tail."""
     pos, vel = ics.exponential_disk(1 << 16, seed=21)
     n = pos.shape[0]
     eps = 0.01
     with Simulation(pos, vel, eps=eps) as sim:
-        sim.set_quadrupole(True)
-        s = cr.SampleSet(sim.lib, 3_000_000)
+        sim.set_quadrupole(2)
+        s = cr.SampleSet(sim.lib, 4_000_000)
         net = cr.Corrector(seed=1)
         try:
             for snap in range(3):
                 sim.build_tree_async()
-                got, _ = s.add_from(sim, cr.default_targets(n, 2048, seed=snap), 0.8)
+                got, _ = s.add_from(sim, cr.default_targets(n, 2048, seed=snap), 0.9)
                 print(f"snapshot {snap}: {got} samples", flush=True)
                 for _ in range(20):
                     sim.step(1e-3, 0.5)
-            losses = cr.train(net, s, epochs=12, batch=65536, lr=1e-3)
+            losses = cr.train(net, s, epochs=6)
             print("losses", ["%.3e" % v for v in losses], flush=True)
-            assert losses[-1] < 0.5 * losses[0]
+            assert losses[-1] < 0.9 * losses[0]
             sim.build_tree_async()
-            sample = ics.subsample(n, 1024, seed=77)
-            sim.set_quadrupole(False)   # moments of this tree stay; the walk is monopole
+            sample = ics.subsample(n, 2048, seed=77)
+            exact = sim.direct_sum_fp64(sample)
+
+            def err():
+                a = sim.accelerations()[sample]
+                e = np.linalg.norm(a - exact, axis=1) / np.linalg.norm(exact, axis=1)
+                return float(np.median(e)), float(np.quantile(e, 0.99))
+
+            sim.set_quadrupole(0)   # moments of this tree stay; the walk is monopole
             sim.walk(0.8)
-            plain, _ = cr.median_error(sim, sample)
-            sim.set_quadrupole(True)
+            plain = err()
+            sim.set_quadrupole(2)
             sim.walk(0.8)
-            quad, _ = cr.median_error(sim, sample)
+            quad = err()
             cr.walk_learned(sim, net, 0.8)
-            learned, _ = cr.median_error(sim, sample)
-            print(f"theta 0.8 median error: plain {plain:.3e} quadrupole {quad:.3e} "
-                  f"learned {learned:.3e}", flush=True)
-            assert learned < 0.3 * plain
-            assert learned < 1.05 * quad
+            learned = err()
+            print(f"theta 0.8 median / p99 error: plain {plain[0]:.3e} / {plain[1]:.3e}, "
+                  f"quadrupole {quad[0]:.3e} / {quad[1]:.3e}, "
+                  f"learned {learned[0]:.3e} / {learned[1]:.3e}", flush=True)
+            assert learned[0] < 0.3 * plain[0]
+            assert learned[0] < 0.95 * quad[0]
+            assert learned[1] < 0.9 * quad[1]
         finally:
             net.close()
             s.close()

@@ -14,6 +14,7 @@ from cudagravitydemo_amd import Simulation, ics, load_library
 from cudagravitydemo_amd.engine import DeviceBuffer, from_device, to_device
 from cudagravitydemo_amd._lib import WorldBox, check
 from oracle import binding as ob
+from oracle import mlp
 from tests.octree_check import check_invariants
 
 pytestmark = pytest.mark.gpu
@@ -413,6 +414,19 @@ def test_quadrupole_moments_and_walk():
         print(f"theta 0.7 median error: monopole {errs[False]:.3e} quadrupole {errs[True]:.3e}",
               flush=True)
         assert errs[True] < 0.5 * errs[False]
+        # third moments against FP64 sums over each node's bodies
+        sim.set_quadrupole(2)
+        sim.build_tree_async()
+        oct_ = sim.third_moments(total)
+        com, link, quad = sim.walk_arrays(total)
+        for i in range(0, total, 97):
+            a, c = int(nodes[i]["first"]), int(nodes[i]["count"])
+            _, O = mlp.moments(ps[a:a + c], com[i, :3].astype(np.float64))
+            want = [O[0, 0, 0], O[1, 1, 1], O[2, 2, 2], O[0, 0, 1], O[0, 0, 2], O[0, 1, 1],
+                    O[1, 1, 2], O[0, 2, 2], O[1, 2, 2], O[0, 1, 2]]
+            d = ps[a:a + c, :3] - com[i, :3].astype(np.float64)
+            scale = ps[a:a + c, 3].sum() * (np.abs(d).max() ** 3 + 1e-18)
+            assert np.abs(oct_[i, :10] - want).max() <= 1e-4 * scale, i
         # theta = 0: only leaves are accepted, the quadrupole walk is the direct sum
         sim.set_quadrupole(True)
         sim.build_tree_async()

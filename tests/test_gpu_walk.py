@@ -24,6 +24,7 @@ pytestmark = pytest.mark.gpu
 N = 16384 - 37          # ragged last group
 THETA = 0.5
 EPS = 0.01
+MIN_RATIO = 0.3         # learned walk: cells with s/r below this keep the quadrupole alone
 
 
 def _net_params():
@@ -47,7 +48,7 @@ def _run(sim, mode, cap, params, cap2=0):
         if mode == 2:
             net = cr.Corrector(params)
             try:
-                st = cr.walk_learned(sim, net, THETA, stats=True)
+                st = cr.walk_learned(sim, net, THETA, MIN_RATIO, stats=True)
             finally:
                 net.close()
         else:
@@ -69,6 +70,7 @@ def _check(sim, mode, per_body_of, params, extra=()):
     acc = sim.accelerations()
     total = sim.tree_summary()[0]
     com, link, quad = sim.walk_arrays(total)
+    oct_ = sim.third_moments(total) if mode == 2 else None
     lo, hi = sim.bbox()
     ext = world_extent(lo, hi)
     net = (lambda X: mlp.forward(params, X, bf16_ops=True)[0]) if mode == 2 else None
@@ -77,7 +79,7 @@ def _check(sim, mode, per_body_of, params, extra=()):
         x = ps[i, :3]
         box = (x, x) if per_body_of(i) else _group_box(ps, i, sim.n)
         want, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
-                            sim.eps * sim.eps, mode, net)
+                            sim.eps * sim.eps, mode, net, MIN_RATIO, oct_=oct_)
         want = sim.G * want
         tol = 1e-5
         if mode == 2:
@@ -134,7 +136,7 @@ def test_walk_matches_interaction_set_fp64(mode, path):
     params = _net_params()
     caps = {"lists": (0, 0), "second_pass": (1, 0), "per_body": (1, 1)}[path]
     with Simulation(pos, vel, eps=EPS) as sim:
-        sim.set_quadrupole(mode != 0)
+        sim.set_quadrupole(mode)
         sim.build_tree_async()
         sizes = _group_sizes(sim)
         over, body = _predict(sim, sizes, caps[0] or 16384, caps[1] or 262144)
@@ -182,7 +184,7 @@ def test_mixed_lists_and_per_body(mode):
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
     with Simulation(pos, vel, eps=EPS) as sim:
-        sim.set_quadrupole(mode != 0)
+        sim.set_quadrupole(mode)
         sim.build_tree_async()
         sizes = _group_sizes(sim)
         cap = int(np.median(sizes))
@@ -205,7 +207,7 @@ def test_second_pass_and_per_body_together(mode):
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
     with Simulation(pos, vel, eps=EPS) as sim:
-        sim.set_quadrupole(mode != 0)
+        sim.set_quadrupole(mode)
         sim.build_tree_async()
         sizes = _group_sizes(sim)
         cap = int(np.median(sizes))

@@ -81,6 +81,56 @@ def test_target_approaches_quadrupole_far_away():
         assert np.linalg.norm(t - qc) <= tol * np.linalg.norm(qc), dist
     r = mlp.residual_target(x, bodies, com, m, size, S, 0.0)
     assert np.linalg.norm(r) <= 0.03 * np.linalg.norm(t)
-    f = mlp.features(x, com, m, size, S)
+    S2, O = mlp.moments(bodies, com)
+    assert np.allclose(S2, S, rtol=1e-12, atol=1e-15)
+    f = mlp.features(x, com, m, size, S, O)
     assert f.shape == (8,) and abs(np.linalg.norm(f[:3]) - 1) < 1e-12
     assert abs(f[3] - size / np.linalg.norm(com - x)) < 1e-12
+
+
+def test_third_moments_shift_and_octupole_term():
+    """The third moments are symmetric and shift by the parallel-axis rule;
+    the octupole field they define (built from O:uu and friends by the
+    textbook expansion) closes most of what the quadrupole leaves."""
+    rng = np.random.default_rng(11)
+    b = np.c_[rng.random((400, 3)) * [1.0, 0.4, 0.7] + [[0.3, 0.0, 0.0]] * (rng.random((400, 1)) < 0.3),
+              rng.random(400) + 0.2]
+    m = b[:, 3].sum()
+    c = (b[:, :3] * b[:, 3:]).sum(0) / m
+    S, O = mlp.moments(b, c)
+    assert np.allclose(O, O.transpose(1, 0, 2)) and np.allclose(O, O.transpose(0, 2, 1))
+    # tensor from its 10 components
+    v10 = [O[0, 0, 0], O[1, 1, 1], O[2, 2, 2], O[0, 0, 1], O[0, 0, 2], O[0, 1, 1], O[1, 1, 2],
+           O[0, 2, 2], O[1, 2, 2], O[0, 1, 2]]
+    assert np.allclose(mlp.third_tensor(v10), O)
+    # parallel-axis: moments of two halves combined equal the whole's
+    h = b[:200], b[200:]
+    tot = np.zeros((3, 3, 3))
+    for part in h:
+        mc = part[:, 3].sum()
+        cc = (part[:, :3] * part[:, 3:]).sum(0) / mc
+        Sc, Oc = mlp.moments(part, cc)
+        dd = cc - c
+        tot += (Oc + np.einsum("ij,k->ijk", Sc, dd) + np.einsum("ik,j->ijk", Sc, dd)
+                + np.einsum("jk,i->ijk", Sc, dd) + mc * np.einsum("i,j,k->ijk", dd, dd, dd))
+    assert np.allclose(tot, O, rtol=1e-10, atol=1e-12)
+    # octupole field, from the third-order term of 1/|R - e| (R = x - com,
+    # u = -R/r): a = [7.5 O:uu - 1.5 T - 17.5 (O:uuu) u + 7.5 (T.u) u] / r^5,
+    # T_i = O_ijj.  The residual beyond the quadrupole is this term plus
+    # O((s/r)^4) relative corrections.
+    left = {}
+    for dist in (6.0, 12.0):
+        x = c + dist * np.array([0.36, 0.48, 0.8])
+        res = mlp.residual_target(x, b, c, m, 1.0, S, 0.0)
+        d = c - x
+        r = np.linalg.norm(d)
+        u = d / r
+        ouu = np.einsum("ijk,j,k->i", O, u, u)
+        ouuu = ouu @ u
+        tr_u = np.einsum("ijj->i", O) @ u   # O_ijj u_i
+        trv = np.einsum("ijj->i", O)        # O_ijj
+        oct = (7.5 * ouu - 1.5 * trv - 17.5 * ouuu * u + 7.5 * tr_u * u) / r ** 5
+        scale = m / r ** 2 * (1.0 / r ** 2)
+        left[dist] = np.linalg.norm(res - oct / scale) / np.linalg.norm(res)
+    # 95 % of the residual is the octupole, and what is left is O(s/r) of it
+    assert left[6.0] < 0.08 and left[12.0] < 0.6 * left[6.0], left

@@ -113,7 +113,10 @@ def test_quadrupole_term_against_brute_force():
         em = np.linalg.norm(mono - exact) / np.linalg.norm(exact)
         assert eq < 2.0 / dist ** 3 and em > 0.05 / dist ** 2 and eq < 0.2 * em
     # the network's features are the oracle's
+    _, O = mlp.moments(b, c)
+    oct_ = np.array([[O[0, 0, 0], O[1, 1, 1], O[2, 2, 2], O[0, 0, 1], O[0, 0, 2], O[0, 1, 1],
+                      O[1, 1, 2], O[0, 2, 2], O[1, 2, 2], O[0, 1, 2], 0, 0]])
     link = np.array([[5, (3 << 5) | 2]], np.int32)
-    X, k = features(x, com, quad, np.array([0]), 4.0, link)
-    want = mlp.features(x, c, m, 1.0, S)
+    X, k = features(x, com, quad, oct_, np.array([0]), 4.0, link)
+    want = mlp.features(x, c, m, 1.0, S, O)
     assert np.allclose(X[0], want, rtol=1e-12, atol=1e-15) and abs(k[0] - m) < 1e-12
