@@ -53,7 +53,7 @@ class TreeState(C.Structure):
 class WalkStats(C.Structure):
     _fields_ = [("interactions", C.c_ulonglong), ("opened", C.c_ulonglong),
                 ("overflowGroups", C.c_ulonglong), ("cellInteractions", C.c_ulonglong),
-                ("bodyGroups", C.c_ulonglong)]
+                ("bodyGroups", C.c_ulonglong), ("networkEvals", C.c_ulonglong)]
 
 
 assert C.sizeof(Node) == 80 and C.sizeof(TreeState) == 304 and C.sizeof(WorldBox) == 20

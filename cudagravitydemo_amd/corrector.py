@@ -191,7 +191,7 @@ def walk_learned(sim, net, theta, min_ratio=0.5, stats=False):
     if st is not None:
         return {"interactions": st.interactions, "opened": st.opened,
                 "overflow_groups": st.overflowGroups, "cell_interactions": st.cellInteractions,
-                "body_groups": st.bodyGroups}
+                "body_groups": st.bodyGroups, "network_evals": st.networkEvals}
     return None
 
 

@@ -129,6 +129,7 @@ extern "C" void calculateAcceleration(const float4* positionMassArray, treeState
 }
 
 #define GD_BODY_STACK 12288       /* per-body walk LDS stack (48 KB): > 22 levels x 512 pushes */
+#define GD_BODY_WAVES 4           /* per-body walk: bodies (wavefronts) per block, first try */
 
 struct gd_kahan3 {
     float x, y, z, cx, cy, cz;
@@ -157,39 +158,65 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
 #define GD_TR2_STACK 12288        /* second pass, one wavefront per block (48 KB of LDS) */
+#define GD_LIST_MIN 16384         /* first-pass room of any group (entries) */
+#define GD_SPREAD (-2)            /* sizes[] mark: the group is walked body by body */
 
 /*
  * Traversal kernel, one wavefront per group.  Slot s walks group
- * groupList[s] (or group s when groupList is NULL) into lists[s*cap ...] with
- * an LDS stack of STACK entries and sets counts[s]; a group that does not fit
- * (cap entries, or slotCap[s] when given) gets counts[s] = (-1, -1) and goes
- * to overflowList.  maxList receives the longest list written.
+ * groupList[s] (or group s when groupList is NULL).  Its list is
+ * lists[offsets[s] .. + caps[s]) when offsets is given (the first pass's
+ * packed layout), else lists[s*cap .. + cap); it gives up once the list
+ * passes slotCap[s] (when given) or its capacity, or the LDS stack of STACK
+ * entries overflows: counts[s] = (-1, -1) and the group goes to
+ * overflowList.  Otherwise counts[s] = (front, back) and, when sizes is
+ * given, sizes[g] records the list length for the next walk's layout.  In
+ * the second pass (groupList given) a group given up on is marked spread
+ * (sizes -2); the first pass, when spreadList is given, sends a marked group
+ * straight to the per-body walk (spreadList) without traversing it.
  */
 template <bool SPLIT, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
-    const int* __restrict__ groupList, int slots, int cap, const int* __restrict__ slotCap,
-    int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ overflowList,
-    int* __restrict__ overflowCount, int* __restrict__ maxList,
+    const int* __restrict__ groupList, int slots, const long long* __restrict__ offsets,
+    const int* __restrict__ caps, int cap, const int* __restrict__ slotCap,
+    int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ sizes,
+    int* __restrict__ overflowList, int* __restrict__ overflowCount, int* __restrict__ maxList,
+    int* __restrict__ spreadList, int* __restrict__ spreadCount,
     unsigned long long* __restrict__ stats) {
     __shared__ int sStack[WAVES][STACK];
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
     const long long s = (long long)blockIdx.x * WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
     const long long g = groupList ? groupList[s] : s;
+    if (spreadList && sizes[g] == GD_SPREAD) {
+        if (gd_lane() == 0) {
+            counts[s] = make_int2(-1, -1);
+            spreadList[atomicAdd(spreadCount, 1)] = (int)g;
+        }
+        return;
+    }
     float lx, ly, lz, hx, hy, hz;
     gd_group_box(pos, n, g, lx, ly, lz, hx, hy, hz);
     int nf = 0, nb = 0;
     unsigned long long nOpen = 0;
-    const int limit = slotCap ? slotCap[s] : cap;
-    int* list = lists + (size_t)s * cap;
+    int* list;
+    int size;
+    if (offsets) {
+        list = lists + offsets[s];
+        size = caps[s];
+    } else {
+        list = lists + (size_t)s * cap;
+        size = cap;
+    }
+    const int limit = slotCap ? min(slotCap[s], size) : size;
     const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, lx, ly, lz, hx, hy, hz,
-                                           sStack[wave], STACK, list, cap, limit, &nf, &nb,
+                                           sStack[wave], STACK, list, size, limit, &nf, &nb,
                                            &nOpen);
     if (gd_lane() == 0) {
         if (ok) {
             counts[s] = make_int2(nf, nb);
+            if (sizes) sizes[g] = nf + nb;
             atomicMax(maxList, nf + nb);
             if (stats) {
                 const int nvalid = (int)((n - g * 64) < 64 ? (n - g * 64) : 64);
@@ -199,6 +226,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
             }
         } else {
             counts[s] = make_int2(-1, -1);
+            if (sizes) sizes[g] = groupList ? GD_SPREAD : -1;
             const int at = atomicAdd(overflowCount, 1);
             overflowList[at] = (int)g;
         }
@@ -206,21 +234,83 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
 }
 
 /*
+ * First-pass list layout: group g gets room for one and a half times its
+ * last list length, at least GD_LIST_MIN entries, rounded up to 64 entries;
+ * a group marked spread (sizes -2, see k_walk_traverse) gets none -- or
+ * every group gets exactly `fixed` entries when fixed > 0.  The rooms are
+ * packed back to back by an exclusive scan (one 1024-thread block).  The
+ * margin keeps groups whose list grows between steps (a Morton run that
+ * shifts onto a neighbouring cluster) in the first pass, where their work
+ * hides among the others'; a long list walked alone in the second pass would
+ * hold up the end of the walk.  With `reset`, spread marks are cleared
+ * (the group is tried again as a list).  Groups that would end
+ * beyond `capacity` entries get no room (they go to the second pass);
+ * *total receives the unclamped sum, for growing the pool.
+ */
+__device__ __forceinline__ long long gd_list_room(int sz, int fixed) {
+    if (fixed > 0) return fixed;
+    if (sz == GD_SPREAD) return 0;
+    long long want = sz >= 0 ? sz + sz / 2 : GD_LIST_MIN;
+    want = want > GD_LIST_MIN ? want : GD_LIST_MIN;
+    return (want + 63) & ~63ll;
+}
+
+__global__ __launch_bounds__(1024) void k_walk_layout(int* __restrict__ sizes, int groups,
+                                                      int fixed, int reset,
+                                                      long long capacity,
+                                                      long long* __restrict__ offsets,
+                                                      int* __restrict__ caps,
+                                                      unsigned long long* __restrict__ total) {
+    __shared__ long long sWave[16];
+    const int per = (groups + 1023) / 1024;
+    const int g0 = threadIdx.x * per;
+    const int g1 = min(g0 + per, groups);
+    long long mine = 0;
+    for (int g = g0; g < g1; ++g) {
+        if (reset && sizes[g] == GD_SPREAD) sizes[g] = -1;
+        mine += gd_list_room(sizes[g], fixed);
+    }
+    /* block exclusive scan of mine */
+    long long inc = mine;
+    const unsigned lane = gd_lane();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long o = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += o;
+    }
+    if (lane == 63) sWave[threadIdx.x / 64] = inc;
+    __syncthreads();
+    long long base = 0;
+    for (int w = 0; w < (int)(threadIdx.x / 64); ++w) base += sWave[w];
+    long long at = base + inc - mine;
+    for (int g = g0; g < g1; ++g) {
+        const long long want = gd_list_room(sizes[g], fixed);
+        offsets[g] = at;
+        caps[g] = at + want <= capacity ? (int)want : 0;
+        at += want;
+    }
+    if (threadIdx.x == 1023) *total = (unsigned long long)(base + inc);
+}
+
+/*
  * Estimate for the groups the first pass could not list: the list length of
  * one of the group's bodies (the middle one in Morton order), counted by a
  * point traversal.  The second pass gives up on the group -- leaving it to
- * the per-body walk -- once its list passes GD_BODY_RATIO times that: a
- * compact group's list is 1.5-3x its bodies' own, one that much longer is
- * spread out (its box spans distant clusters).  Such a group is cheaper body
- * by body, and a long list walked by a single wavefront would also hold up
- * the end of the walk.
+ * the per-body walk -- once its list passes GD_BODY_RATIO times that, and
+ * at least `floor` entries: a compact group's list is 1.5-3x its bodies'
+ * own, one that much longer is spread out (its box spans distant clusters).
+ * Such a group is cheaper body by body -- the per-body walk does ~25x less
+ * work per second than the list evaluation, so short lists stay lists -- and
+ * a long list walked by a single wavefront would also hold up the end of the
+ * walk.
  */
 #define GD_BODY_RATIO 8
 template <bool SPLIT>
 __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
-    const int* __restrict__ groupList, int slots, int cap2, int* __restrict__ slotCap) {
+    const int* __restrict__ groupList, int slots, int cap2, int floor,
+    int* __restrict__ slotCap) {
     __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK];
     const int wave = (int)(threadIdx.x >> 6);
     const long long s = (long long)blockIdx.x * GD_TR_WAVES + wave;
@@ -236,7 +326,8 @@ __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
                                            p.z, sStack[wave], GD_TR_STACK, nullptr, bound,
                                            bound, &nf, &nb, &nOpen);
     if (gd_lane() == 0) {
-        const long long lim = ok ? (long long)GD_BODY_RATIO * (nf + nb) : cap2;
+        long long lim = ok ? (long long)GD_BODY_RATIO * (nf + nb) : cap2;
+        lim = lim > floor ? lim : floor;
         slotCap[s] = (int)(lim < cap2 ? lim : cap2);
     }
 }
@@ -408,7 +499,8 @@ __device__ __forceinline__ void gd_eval_group(
     const gd_tree_info* __restrict__ info, const int* __restrict__ list, int cap, int2 cnt,
     long long g, int n, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sE, float4* sQ0, float4* sQ1,
-    float4* sCor, int* sMap, gd_kahan3& tot, float& phs, float& phc) {
+    float4* sCor, int* sMap, gd_kahan3& tot, float& phs, float& phc,
+    unsigned long long* nnEvals) {
     const unsigned lane = gd_lane();
     const long long i = g * 64 + lane;
     const bool valid = i < n;
@@ -479,6 +571,7 @@ __device__ __forceinline__ void gd_eval_group(
             float f[8], k, ox, oy, oz;
             gd_cell_features(c, q0, q1, o0, o1, o2, size, gx, gy, gz, idx >= 0, minRatio, f, k);
             gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
+            *nnEvals += (unsigned long long)__popcll(__ballot(k > 0.f));
             sCor[lane] = make_float4(ox, oy, oz, k);
         }
         __builtin_amdgcn_wave_barrier();
@@ -513,9 +606,11 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
     const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ groupList, int slots,
-    const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
+    const int* __restrict__ lists, const long long* __restrict__ offsets,
+    const int* __restrict__ caps, const int2* __restrict__ counts, int cap,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, int n,
-    const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio) {
+    const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio,
+    unsigned long long* __restrict__ stats) {
     __shared__ float4 sE[GD_EV_WAVES][2 * 64];
     __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
@@ -541,11 +636,15 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const long long g = groupList ? groupList[s] : s;
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, lists + (size_t)s * cap, cap,
-                             cnt, g, n, eps2, mlpW, mlpB, minRatio, sE[wave],
+    unsigned long long nn = 0;
+    const int* list = offsets ? lists + offsets[s] : lists + (size_t)s * cap;
+    const int size = offsets ? caps[s] : cap;
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, list, size, cnt, g, n, eps2,
+                             mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
                              sCor[MODE == 2 ? wave : 0], sMap[MODE == 2 ? wave : 0], tot, phs,
-                             phc);
+                             phc, &nn);
+    if (MODE == 2 && stats && lane == 0) atomicAdd(&stats[4], nn);
     const long long i = g * 64 + lane;
     if (i < n) {
         acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
@@ -561,41 +660,51 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
  * body itself, each accepted node is evaluated at once by the lane that
  * tested it (monopole; in MODE 1/2 quadrupole for cells, and in MODE 2 the
  * network's residual evaluated at the body), per-lane sums are
- * Kahan-compensated and combined by a fixed butterfly at the end.  The LDS
- * stack holds GD_BODY_STACK entries: a step pops up to 64 nodes and pushes at
- * most 8 children each, and the pushed segments along the stack have strictly
+ * Kahan-compensated and combined by a fixed butterfly at the end.  Slot q
+ * walks body bodyList[q], or, without bodyList, body q % 64 of group
+ * groupList[q / 64]; WAVES wavefronts per block, each with an LDS stack of
+ * STACK entries.  A body whose stack would overflow goes to overflowBodies
+ * (when given; else the host reports an error): with STACK = GD_BODY_STACK
+ * that cannot happen, since a step pops up to 64 nodes and pushes at most 8
+ * children each, and the pushed segments along the stack have strictly
  * increasing minimum level, so the stack never exceeds 22 x 512 entries.
  */
-template <int MODE, bool PHI>
-__global__ __launch_bounds__(64) void k_walk_body(
+template <int MODE, bool PHI, int WAVES, int STACK>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct,
-    const gd_tree_info* __restrict__ info, const int* __restrict__ groupList,
+    const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
+    const int* __restrict__ groupList, const int* __restrict__ bodyList, long long count,
+    int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
     int n, const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB,
     float minRatio, unsigned long long* __restrict__ stats) {
-    __shared__ int sStack[GD_BODY_STACK];
-    __shared__ int sMap[MODE == 2 ? 64 : 1];
+    __shared__ int sStackAll[WAVES][STACK];
+    __shared__ int sMapAll[MODE == 2 ? WAVES : 1][64];
     __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
     __shared__ float sB[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
     if (MODE == 2) {
-        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64) sW[e] = mlpW[e];
-        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64) sB[e] = mlpB[e];
+        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64 * WAVES) sW[e] = mlpW[e];
+        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64 * WAVES) sB[e] = mlpB[e];
         __syncthreads();
         mlpW = sW;
         mlpB = sB;
     }
     const unsigned lane = gd_lane();
-    const long long i = (long long)groupList[blockIdx.x >> 6] * 64 + (blockIdx.x & 63);
-    if (i >= n) return;   /* block-uniform */
+    const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
+    int* sStack = sStackAll[wave];
+    int* sMap = sMapAll[MODE == 2 ? wave : 0];
+    const long long q = (long long)blockIdx.x * WAVES + wave;
+    if (q >= count) return;   /* wave-uniform */
+    const long long i = bodyList ? (long long)bodyList[q] : (long long)groupList[q >> 6] * 64 + (q & 63);
+    if (i >= n) return;   /* wave-uniform */
     const float4 p = pos[i];
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
-    unsigned long long nInter = 0, nCell = 0, nOpen = 0;
+    unsigned long long nInter = 0, nCell = 0, nOpen = 0, nNet = 0;
     int top = 1;
     if (lane == 0) sStack[0] = 0;
     while (top > 0) {
@@ -628,8 +737,11 @@ __global__ __launch_bounds__(64) void k_walk_body(
             total += __popcll(bal) << bit;
         }
         nOpen += (unsigned long long)__popcll(__ballot(open));
-        if (top + total > GD_BODY_STACK) {
-            if (lane == 0) atomicOr(&stats[2], 1ull);   /* reported as an error by the host */
+        if (top + total > STACK) {
+            if (lane == 0) {
+                if (overflowBodies) overflowBodies[atomicAdd(overflowCount, 1)] = (int)i;
+                else atomicOr(&stats[2], 1ull);   /* reported as an error by the host */
+            }
             return;
         }
         if (open)
@@ -678,6 +790,7 @@ __global__ __launch_bounds__(64) void k_walk_body(
                 gd_cell_features(c, q0, q1, o0, o1, o2, size, p.x, p.y, p.z, cell, minRatio, f,
                                  k);
                 gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
+                nNet += (unsigned long long)__popcll(__ballot(k > 0.f));
                 if (cell) {
                     const float i2 = __builtin_amdgcn_rcpf(d2);
                     const float s4 = k * (i2 * i2);
@@ -709,68 +822,104 @@ __global__ __launch_bounds__(64) void k_walk_body(
             atomicAdd(&stats[0], nInter);
             atomicAdd(&stats[1], nOpen);
             atomicAdd(&stats[3], nCell);
+            if (MODE == 2) atomicAdd(&stats[4], nNet);
         }
     }
 }
 
 /*
- * Walk scratch.  counters[0..3]: interactions, opened, stack-overflow flag,
- * cell interactions (64-bit); ints[0..2]: first- and second-pass overflow
- * counts, longest first-pass list.  First pass: entriesPerGroup entries per
- * group.  Second pass (groups that did not fit): GD_LIST2 entries per slot,
- * at most GD_POOL_BYTES of pool, in batches.  Third: the per-body walk.
+ * Walk scratch.  counters[0..4]: interactions, opened, stack-overflow flag,
+ * cell interactions, network evaluations (64-bit); status: overflow counts
+ * of the first and second pass, longest lists, the first pass's requested
+ * list entries.  First pass: packed layout from the previous walk's list
+ * lengths (sizes), in a pool of listEntries.  Second pass (groups that did
+ * not fit): entries2 entries per slot, at most GD_POOL_BYTES of slots at a
+ * time, in batches.  Third: the per-body walk.
  */
-struct gd_walk_scratch {
-    unsigned long long* stats;
-    int* ints;
-    int* overflow1;        /* groups entries */
-    int* overflow2;        /* groups entries */
-    int2* counts;          /* groups entries */
-    int2* counts2;         /* pool slots */
-    int* slotCap;          /* pool slots: second-pass list limit per group */
-    int groupCap;          /* groups the arrays above hold */
-    int* lists;
-    long long listBytes;
-    int* pool;
-    long long poolEntries;
-    int countSlots;        /* entries of counts2 */
-    int entriesPerGroup;
-    int fixedCap;          /* capacity set by gd_walk_set_list_capacity: kept as is */
-    int entries2;          /* second-pass list capacity */
+struct gd_walk_status {
+    int over1, over2, maxList1, maxList2, over3, pad;
+    unsigned long long requested;
 };
 
-#define GD_LIST_DEFAULT 16384
-#define GD_LIST_MAX 65536
+struct gd_walk_scratch {
+    unsigned long long* stats;
+    gd_walk_status* status;
+    int* overflow1;        /* groups entries */
+    int* overflow2;        /* groups entries */
+    int* bodyOver;         /* 64 x groups entries: bodies for the deep-stack per-body walk */
+    int2* counts;          /* groups entries */
+    int* sizes;            /* groups entries: last list length, -1 unknown */
+    long long* offsets;    /* groups entries: first-pass layout */
+    int* caps;             /* groups entries */
+    int groupCap;          /* groups the arrays above hold */
+    int* lists;            /* first-pass pool */
+    long long listEntries;
+    int2* counts2;         /* second-pass slots */
+    int* slotCap;          /* second-pass slots: list limit per group */
+    int countSlots;
+    int* pool;             /* second-pass lists */
+    long long poolEntries;
+    int fixedCap;          /* > 0: first-pass capacity of every group (no packing) */
+    int entries2;          /* second-pass list capacity */
+    long long walks;       /* walks so far (spread marks are cleared every 32) */
+};
+
 #define GD_LIST2 262144
 #define GD_POOL_BYTES (2ll << 30)
-static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                 0, nullptr, 0, nullptr, 0, 0, GD_LIST_DEFAULT, 0, GD_LIST2};
+static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, 0,       nullptr, 0,
+                                 nullptr, nullptr, 0,       nullptr, 0,       0,
+                                 GD_LIST2, 0};
 
 /* optional HIP-event timing of the walk's phases (null stream) */
 static int g_walkTiming = 0;
 static hipEvent_t g_walkEv[4] = {nullptr, nullptr, nullptr, nullptr};
 static int g_walkTimed = 0;
 
+__global__ void k_walk_fill(int* a, int n, int v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 static void gd_walk_ensure(int groups) {
     if (!g_walk.stats) {
-        GD_HIP(hipMalloc(&g_walk.stats, 4 * sizeof(unsigned long long)));
-        GD_HIP(hipMalloc(&g_walk.ints, 4 * sizeof(int)));
+        GD_HIP(hipMalloc(&g_walk.stats, 5 * sizeof(unsigned long long)));
+        GD_HIP(hipMalloc(&g_walk.status, sizeof(gd_walk_status)));
     }
     if (groups > g_walk.groupCap) {
         if (g_walk.overflow1) GD_HIP(hipFree(g_walk.overflow1));
         if (g_walk.overflow2) GD_HIP(hipFree(g_walk.overflow2));
+        if (g_walk.bodyOver) GD_HIP(hipFree(g_walk.bodyOver));
         if (g_walk.counts) GD_HIP(hipFree(g_walk.counts));
+        if (g_walk.sizes) GD_HIP(hipFree(g_walk.sizes));
+        if (g_walk.offsets) GD_HIP(hipFree(g_walk.offsets));
+        if (g_walk.caps) GD_HIP(hipFree(g_walk.caps));
         GD_HIP(hipMalloc(&g_walk.overflow1, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.overflow2, sizeof(int) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.bodyOver, 64 * sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.counts, sizeof(int2) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.sizes, sizeof(int) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.offsets, sizeof(long long) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.caps, sizeof(int) * (size_t)groups));
+        k_walk_fill<<<gd_div_up(groups, 256), 256>>>(g_walk.sizes, groups, -1);
+        GD_LAUNCHED();
         g_walk.groupCap = groups;
     }
-    const long long need = (long long)groups * g_walk.entriesPerGroup * (long long)sizeof(int);
-    if (need > g_walk.listBytes) {
-        if (g_walk.lists) GD_HIP(hipFree(g_walk.lists));
-        GD_HIP(hipMalloc(&g_walk.lists, (size_t)need));
-        g_walk.listBytes = need;
+    if (!g_walk.lists) {
+        /* first guess: the minimum room for every group; grown to what walks ask for */
+        const long long e = (long long)groups * GD_LIST_MIN;
+        GD_HIP(hipMalloc(&g_walk.lists, (size_t)e * sizeof(int)));
+        g_walk.listEntries = e;
     }
+}
+
+/* Grow the first-pass pool to hold `want` entries with room to spare. */
+static void gd_walk_grow_lists(long long want) {
+    if (want <= g_walk.listEntries) return;
+    const long long e = want + want / 4;
+    if (g_walk.lists) GD_HIP(hipFree(g_walk.lists));
+    GD_HIP(hipMalloc(&g_walk.lists, (size_t)e * sizeof(int)));
+    g_walk.listEntries = e;
 }
 
 static void gd_walk_ensure_pool(int slots, int entries) {
@@ -789,76 +938,98 @@ static void gd_walk_ensure_pool(int slots, int entries) {
     }
 }
 
-__global__ void k_walk_reset(unsigned long long* stats, int* ints) {
-    if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
-    if (threadIdx.x < 4) ints[threadIdx.x] = 0;
+__global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st) {
+    if (threadIdx.x < 5) stats[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        st->over1 = st->over2 = st->maxList1 = st->maxList2 = st->over3 = 0;
+        st->requested = 0ull;
+    }
 }
 
 template <int MODE, bool PHI>
 static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
-                           const int* groupList, const int* lists, const int2* counts, int cap,
-                           float3* acc, float* phi, float eps2, float G, int n,
-                           const gd_bf16x4* W, const float* Bv, float minRatio) {
+                           const int* groupList, const int* lists, const long long* offsets,
+                           const int* caps, const int2* counts, int cap, float3* acc,
+                           float* phi, float eps2, float G, int n, const gd_bf16x4* W,
+                           const float* Bv, float minRatio) {
     k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, groupList, slots, lists, counts,
-        cap, acc, phi, eps2, G, n, W, Bv, minRatio);
+        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, groupList, slots, lists, offsets,
+        caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.stats);
 }
 
 template <int MODE, bool PHI>
 static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, float* phi,
                            float eps2, float G, float theta, int n, const gd_bf16x4* W,
-                           const float* Bv, float minRatio, int* overOut) {
+                           const float* Bv, float minRatio, int* overOut,
+                           unsigned long long* requested) {
     constexpr bool SPLIT = MODE != 0;
     const int groups = gd_div_up(n, 64);
-    const int cap = g_walk.entriesPerGroup;
-    int* ints = g_walk.ints;
-    /* pass 1: every group */
+    gd_walk_status* st = g_walk.status;
+    /* pass 1: every group, packed lists */
+    const int reset = (g_walk.walks++ % 32) == 0;
+    k_walk_layout<<<1, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, reset,
+                               g_walk.listEntries, g_walk.offsets, g_walk.caps, &st->requested);
+    GD_LAUNCHED();
     k_walk_traverse<SPLIT, GD_TR_WAVES, GD_TR_STACK><<<gd_div_up(groups, GD_TR_WAVES),
                                                        64 * GD_TR_WAVES>>>(
-        pos, b->wcom, b->wlink, b->dinfo, theta, n, nullptr, groups, cap, nullptr, g_walk.lists,
-        g_walk.counts, g_walk.overflow1, ints + 0, ints + 2, g_walk.stats);
+        pos, b->wcom, b->wlink, b->dinfo, theta, n, nullptr, groups, g_walk.offsets,
+        g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,
+        &st->over1, &st->maxList1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2, &st->over2,
+        g_walk.stats);
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
-    gd_launch_eval<MODE, PHI>(groups, b, pos, nullptr, g_walk.lists, g_walk.counts, cap, acc,
-                              phi, eps2, G, n, W, Bv, minRatio);
+    gd_launch_eval<MODE, PHI>(groups, b, pos, nullptr, g_walk.lists, g_walk.offsets,
+                              g_walk.caps, g_walk.counts, 0, acc, phi, eps2, G, n, W, Bv,
+                              minRatio);
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
-    int h[4];
-    GD_HIP(hipMemcpy(h, ints, sizeof(h), hipMemcpyDeviceToHost));
-    const int over1 = h[0];
-    overOut[0] = over1;
-    overOut[1] = h[2];
-    if (over1 <= 0) return;
+    gd_walk_status h;
+    GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    const int over1 = h.over1;
+    overOut[0] = over1 + h.over2;
+    overOut[1] = h.maxList1;
+    *requested = h.requested;
     /* pass 2: the groups that did not fit, with long lists and a deep stack */
     const int cap2 = g_walk.entries2;
     const long long fit = GD_POOL_BYTES / ((long long)cap2 * sizeof(int));
     const int slots = over1 < fit ? over1 : (int)fit;
-    gd_walk_ensure_pool(slots, cap2);
+    if (over1 > 0) gd_walk_ensure_pool(slots, cap2);
     if (gd_has_error()) return;
     for (int b0 = 0; b0 < over1; b0 += slots) {
         const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
         k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, cap2,
-            g_walk.slotCap);
+            cap2 / 4 < GD_LIST_MIN ? cap2 / 4 : GD_LIST_MIN, g_walk.slotCap);
         GD_LAUNCHED();
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, cap2,
-            g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.overflow2, ints + 1, ints + 3,
-            g_walk.stats);
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, nullptr,
+            nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
+            g_walk.overflow2, &st->over2, &st->maxList2, nullptr, nullptr, g_walk.stats);
         GD_LAUNCHED();
-        gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool,
-                                  g_walk.counts2, cap2, acc, phi, eps2, G, n, W, Bv,
+        gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool, nullptr,
+                                  nullptr, g_walk.counts2, cap2, acc, phi, eps2, G, n, W, Bv,
                                   minRatio);
         GD_LAUNCHED();
     }
-    int over2 = 0;
-    GD_HIP(hipMemcpy(&over2, ints + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (over1 > 0) GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    const int over2 = h.over2;
     overOut[2] = over2;
     if (over2 <= 0) return;
-    /* pass 3: body by body */
-    k_walk_body<MODE, PHI><<<64u * (unsigned)over2, 64>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.overflow2, acc, phi, eps2,
-        G, theta, n, W, Bv, minRatio, g_walk.stats);
+    /* pass 3: body by body, 2048-entry stacks; the rare body that needs more
+       is walked again with a stack that fits any tree */
+    const long long bodies = 64ll * over2;
+    k_walk_body<MODE, PHI, GD_BODY_WAVES, GD_TR_STACK>
+        <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES>>>(
+            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.overflow2, nullptr,
+            bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n, W, Bv, minRatio,
+            g_walk.stats);
+    GD_LAUNCHED();
+    GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    overOut[3] = h.over3;
+    if (h.over3 <= 0) return;
+    k_walk_body<MODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64>>>(
+        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, nullptr, g_walk.bodyOver, h.over3,
+        nullptr, nullptr, acc, phi, eps2, G, theta, n, W, Bv, minRatio, g_walk.stats);
     GD_LAUNCHED();
 }
 
@@ -882,19 +1053,22 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     }
     const int groups = gd_div_up(starCount, 64);
     gd_walk_ensure(groups);
+    if (g_walk.fixedCap > 0) gd_walk_grow_lists((long long)groups * g_walk.fixedCap);
     if (gd_has_error()) return;
     const gd_bf16x4* W = net ? gd_mlp_fwd_tiles(net) : nullptr;
     const float* Bv = net ? gd_mlp_bias(net) : nullptr;
-    k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.ints);
+    k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.status);
     GD_LAUNCHED();
     if (g_walkTiming) {
         if (!g_walkEv[0])
             for (int e = 0; e < 4; ++e) GD_HIP(hipEventCreate(&g_walkEv[e]));
         GD_HIP(hipEventRecord(g_walkEv[0], nullptr));
     }
-    int over[3] = {0, 0, 0};
+    int over[4] = {0, 0, 0, 0};
+    unsigned long long requested = 0;
 #define GD_PASSES(M, P)                                                                        \
-    gd_walk_passes<M, P>(b, pos, acc, phi, eps2, G, theta, starCount, W, Bv, minRatio, over)
+    gd_walk_passes<M, P>(b, pos, acc, phi, eps2, G, theta, starCount, W, Bv, minRatio, over,   \
+                         &requested)
     if (mode == 0) {
         if (phi) GD_PASSES(0, true); else GD_PASSES(0, false);
     } else if (mode == 1) {
@@ -908,15 +1082,11 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         GD_HIP(hipEventRecord(g_walkEv[3], nullptr));
         g_walkTimed = 1;
     }
-    /* list capacity for later walks: grow while more than one group in 256
-       overflows, shrink while the longest list uses under a quarter of it */
-    if (!g_walk.fixedCap) {
-        int& c = g_walk.entriesPerGroup;
-        if (over[0] > groups / 256 && c < GD_LIST_MAX) c *= 2;
-        else if (over[0] == 0 && over[1] < c / 4 && c > GD_LIST_DEFAULT) c /= 2;
-    }
+    /* room for what this walk asked for, for the next walk (groups that did
+       not get theirs went to the second pass) */
+    gd_walk_grow_lists((long long)requested);
     if (over[0] > 0 || stats) {
-        unsigned long long h[4];
+        unsigned long long h[5];
         GD_HIP(hipMemcpy(h, g_walk.stats, sizeof(h), hipMemcpyDeviceToHost));
         if (h[2]) GD_FAIL("gd_walk: a body's traversal stack exceeded %d entries", GD_BODY_STACK);
         if (!stats) return;
@@ -925,6 +1095,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         stats->overflowGroups = (unsigned long long)over[0];
         stats->cellInteractions = h[3];
         stats->bodyGroups = (unsigned long long)over[2];
+        stats->networkEvals = h[4];
     }
 }
 
@@ -946,14 +1117,18 @@ extern "C" void gd_walk_learned(const treeState* tree, const float4* pos, float3
 }
 
 extern "C" int gd_walk_set_list_capacity(int entries, int secondPass) {
-    const int prev = g_walk.entriesPerGroup;
+    const int prev = g_walk.fixedCap;
     if (entries < 0 || entries > (1 << 24) || secondPass < 0 || secondPass > (1 << 26)) {
         GD_FAIL("gd_walk_set_list_capacity: capacities %d, %d out of range", entries, secondPass);
         return prev;
     }
-    g_walk.entriesPerGroup = entries > 0 ? entries : GD_LIST_DEFAULT;
-    g_walk.fixedCap = entries > 0;
+    g_walk.fixedCap = entries;
     g_walk.entries2 = secondPass > 0 ? secondPass : GD_LIST2;
+    /* forget the list lengths and spread marks recorded so far */
+    if (g_walk.sizes) {
+        k_walk_fill<<<gd_div_up(g_walk.groupCap, 256), 256>>>(g_walk.sizes, g_walk.groupCap, -1);
+        GD_LAUNCHED();
+    }
     return prev;
 }
 

@@ -144,7 +144,7 @@ class Simulation:
             return {"interactions": st.interactions, "opened": st.opened,
                     "overflow_groups": st.overflowGroups,
                     "cell_interactions": st.cellInteractions,
-                    "body_groups": st.bodyGroups}
+                    "body_groups": st.bodyGroups, "network_evals": st.networkEvals}
         return None
 
     def step(self, dt, theta):

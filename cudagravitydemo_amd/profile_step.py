@@ -32,6 +32,8 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--walk", choices=["plain", "quad", "learned", "reference"], default="plain")
     ap.add_argument("--params", default=None)
+    ap.add_argument("--list-cap", type=int, default=0,
+                    help="fixed first-pass list capacity per group (0: adaptive)")
     ap.add_argument("--copy-mib", type=int, default=0)
     a = ap.parse_args()
     pos, vel = ics.disk_merger(a.per_disk, seed=2024)
@@ -41,6 +43,8 @@ def main():
     if a.walk == "learned":
         net = cr.Corrector(np.load(a.params) if a.params else None, seed=1)
     with Simulation(pos, vel, eps=eps) as sim:
+        if a.list_cap:
+            sim.lib.gd_walk_set_list_capacity(a.list_cap, 0)
         if a.walk in ("quad", "learned"):
             sim.set_quadrupole(2 if a.walk == "learned" else 1)
         sim.sync()

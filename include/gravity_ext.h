@@ -83,6 +83,8 @@ typedef struct {
     unsigned long long cellInteractions; /* of the interactions, those with internal
                                             cells evaluated as quadrupoles (+ network) */
     unsigned long long bodyGroups;       /* of those, groups walked body by body */
+    unsigned long long networkEvals;     /* learned walk: (group or body, cell) pairs
+                                            evaluated by the network */
 } gd_walk_stats;
 
 /*
@@ -104,12 +106,14 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
              float G, float theta, int starCount, gd_walk_stats* stats);
 
 /* Interaction-list capacities (entries of 4 bytes).  entries: per 64-body
- * group in the first pass (allocated for every group); 0 restores the
- * default, 16384, doubled (up to 65536) after a walk in which more than one
- * group in 256 did not fit and halved back while the longest list uses under
- * a quarter of it; a value set here is kept as it is.  secondPass: per group
- * in the second pass (0: 262144; at most 2 GiB of lists at a time).
- * Returns the previous first-pass capacity. */
+ * group in the first pass.  0 (the default) packs the first pass's lists
+ * back to back, each group getting 1.5 times its previous walk's list length
+ * and at least 16384 entries; groups found spread out (walked body by body)
+ * skip the list passes, re-checked every 32 walks.  A value > 0 gives every
+ * group exactly that many entries and no skipping.  Either call forgets the
+ * recorded lengths.  secondPass: per
+ * group in the second pass (0: 262144; at most 2 GiB of lists at a time).
+ * Returns the previous first-pass setting. */
 int gd_walk_set_list_capacity(int entries, int secondPass);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the

@@ -114,7 +114,8 @@ def _group_sizes(sim):
 def _predict(sim, sizes, cap1, cap2):
     """Routing of the walk's passes, from the exact list lengths: a group leaves
     the first pass when its list exceeds cap1; the second pass gives it up when
-    its list exceeds 8x the middle body's own list (or cap2)."""
+    its list exceeds 8x the middle body's own list and min(16384, cap2 / 4)
+    entries, or cap2."""
     ps, com, link, ext = _tree(sim)
     over = set(int(g) for g in np.flatnonzero(sizes > cap1))
     body = set()
@@ -123,7 +124,7 @@ def _predict(sim, sizes, cap1, cap2):
         x = ps[64 * g + (last - 64 * g) // 2, :3]
         lv, cl = accepted(com, link, ext, x, x, THETA)
         lb = lv.size + cl.size
-        lim = 8 * lb if lb <= cap2 // 8 else cap2
+        lim = max(8 * lb, min(16384, cap2 // 4)) if lb <= cap2 // 8 else cap2
         if sizes[g] > min(lim, cap2):
             body.add(g)
     return over, body
@@ -134,12 +135,12 @@ def _predict(sim, sizes, cap1, cap2):
 def test_walk_matches_interaction_set_fp64(mode, path):
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
-    caps = {"lists": (0, 0), "second_pass": (1, 0), "per_body": (1, 1)}[path]
+    caps = {"lists": (16384, 0), "second_pass": (1, 0), "per_body": (1, 1)}[path]
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(mode)
         sim.build_tree_async()
         sizes = _group_sizes(sim)
-        over, body = _predict(sim, sizes, caps[0] or 16384, caps[1] or 262144)
+        over, body = _predict(sim, sizes, caps[0], caps[1] or 262144)
         groups = (sim.n + 63) // 64
         if path == "per_body":
             assert len(body) == groups
@@ -164,7 +165,7 @@ def test_walk_counts_match_interaction_sets():
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(True)
         sim.build_tree_async()
-        st = sim.walk(THETA, stats=True)
+        st = _run(sim, 1, 16384, None)
         ps, com, link, ext = _tree(sim)
         inter = cells = 0
         for g in range((sim.n + 63) // 64):
@@ -201,9 +202,9 @@ def test_mixed_lists_and_per_body(mode):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_second_pass_and_per_body_together(mode):
-    """Capacity at the median list: the compact groups that do not fit go to
-    the second pass, the spread ones (list > 8x their middle body's) body by
-    body -- both routes in one walk."""
+    """Capacity at the median list, second-pass lists of 8192: the compact
+    groups that do not fit go to the second pass, the spread ones (list > 8x
+    their middle body's and > 2048) body by body -- both routes in one walk."""
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
     with Simulation(pos, vel, eps=EPS) as sim:
@@ -211,11 +212,41 @@ def test_second_pass_and_per_body_together(mode):
         sim.build_tree_async()
         sizes = _group_sizes(sim)
         cap = int(np.median(sizes))
-        over, body = _predict(sim, sizes, cap, 262144)
+        over, body = _predict(sim, sizes, cap, 8192)
         assert len(body) >= 1 and len(over - body) >= 1
-        st = _run(sim, mode, cap, params, 0)
+        st = _run(sim, mode, cap, params, 8192)
         assert st["overflow_groups"] >= len(over)
         if st["overflow_groups"] == len(over):
             assert st["body_groups"] == len(body)
             spread = [64 * g + k for g in body for k in (0, 31, 63) if 64 * g + k < sim.n]
             _check(sim, mode, lambda i: (i // 64) in body, params, extra=spread)
+
+
+def test_packed_lists_follow_the_walk():
+    """Default layout: lists packed from the previous walk's lengths.  The
+    first walk (no record: 16384 entries each) and the ones after give the
+    same accelerations as a walk with fixed room for every list, and after the
+    first walk no compact group leaves the first pass."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    lib = load_library()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(1)
+        sim.build_tree_async()
+        want = None
+        lib.gd_walk_set_list_capacity(16384, 0)
+        try:
+            sim.walk(0.3)
+            want = sim.accelerations()
+        finally:
+            lib.gd_walk_set_list_capacity(0, 0)
+        for k in range(3):
+            st = sim.walk(0.3, stats=True)
+            got = sim.accelerations()
+            spread = st["body_groups"]
+            if spread == 0:
+                assert np.array_equal(got, want), k
+            else:
+                d = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+                assert np.median(d) < 1e-6
+            if k > 0:
+                assert st["overflow_groups"] == spread, (k, st)
