@@ -150,4 +150,22 @@ __device__ __forceinline__ unsigned long long gd_lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+/* The dispatcher hands workgroups to the 8 XCDs round-robin (block b runs on
+ * XCD b % 8, each XCD with its own L2).  This bijection on [0, nblocks) deals
+ * logical blocks to the XCDs in chunks of GD_XCD_CHUNK consecutive ones
+ * (chunk c to XCD c % 8), so neighbouring work (groups adjacent in Morton
+ * order, which read mostly the same tree nodes) shares an L2 while every XCD
+ * still sees work from everywhere (whole contiguous ranges per XCD leave the
+ * XCDs with the dense regions last to finish).  Blocks past the last whole
+ * round of chunks keep their index. */
+#define GD_XCDS 8
+#define GD_XCD_CHUNK 16
+__device__ __forceinline__ unsigned gd_xcd_block() {
+    const unsigned b = blockIdx.x;
+    const unsigned full = gridDim.x / (GD_XCDS * GD_XCD_CHUNK) * (GD_XCDS * GD_XCD_CHUNK);
+    if (b >= full) return b;
+    const unsigned x = b % GD_XCDS, i = b / GD_XCDS;
+    return ((i / GD_XCD_CHUNK) * GD_XCDS + x) * GD_XCD_CHUNK + i % GD_XCD_CHUNK;
+}
+
 #endif

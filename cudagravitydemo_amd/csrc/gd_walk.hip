@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     unsigned long long* __restrict__ stats) {
     __shared__ int sStack[WAVES][STACK];
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
-    const long long s = (long long)blockIdx.x * WAVES + wave;
+    const long long s = (long long)gd_xcd_block() * WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
     const long long g = groupList ? groupList[s] : s;
     if (spreadList && sizes[g] == GD_SPREAD) {
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     }
     const unsigned lane = gd_lane();
     const int wave = (int)(threadIdx.x >> 6);
-    const long long s = (long long)blockIdx.x * GD_EV_WAVES + wave;
+    const long long s = (long long)gd_xcd_block() * GD_EV_WAVES + wave;
     if (s >= slots) return;
     const int2 cnt = counts[s];
     if (cnt.x < 0) return;   /* left to the next pass */
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
     int* sStack = sStackAll[wave];
     int* sMap = sMapAll[MODE == 2 ? wave : 0];
-    const long long q = (long long)blockIdx.x * WAVES + wave;
+    const long long q = (long long)gd_xcd_block() * WAVES + wave;
     if (q >= count) return;   /* wave-uniform */
     const long long i = bodyList ? (long long)bodyList[q] : (long long)groupList[q >> 6] * 64 + (q & 63);
     if (i >= n) return;   /* wave-uniform */
