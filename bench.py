@@ -44,6 +44,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s at ≤1e-3 median force err vs FP64 sum: learned vs plain walk"
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of the evaluation kernel, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
+# (walk, theta) of the workload they were measured on.
+MEASURED_TRAFFIC = {
+    ("learned", 0.75): {"bytes": None,
+                        "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
+}
 BUDGET = 1e-3
 FLOP_MONO = 20
 FLOP_QUAD = 54
@@ -61,7 +68,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-reference-walk", action="store_true")
     ap.add_argument("--cpu-group-stride", type=int, default=8)
-    ap.add_argument("--train-epochs", type=int, default=12)
+    ap.add_argument("--train-epochs", type=int, default=6)
     return ap.parse_args()
 
 
@@ -179,7 +186,7 @@ def eval_roofline(walks, mode, theta, n):
         valu = FLOP_MONO * mono + FLOP_QUAD * cells
     else:
         valu = FLOP_MONO * mono + (FLOP_QUAD + FLOP_CORR) * cells
-    mfma = FLOP_MLP * cells // 64 if mode == "learned" else 0
+    mfma = FLOP_MLP * st.get("network_evals", 0) if mode == "learned" else 0
     achieved = valu / (ev_ms * 1e-3) / 1e12
     return {
         "bound": "mfma",
@@ -190,7 +197,9 @@ def eval_roofline(walks, mode, theta, n):
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": None,
+        "traffic": (MEASURED_TRAFFIC.get((mode, theta)) or {}).get("bytes"),
+        "traffic_unit": "bytes per launch (HBM read + write)",
+        "traffic_source": (MEASURED_TRAFFIC.get((mode, theta)) or {}).get("profile"),
         "kernel_ms": ev_ms,
         "traversal_ms": float(phases[:, 0].mean()),
         "fallback_ms": float(phases[:, 2].mean()),
@@ -235,17 +244,18 @@ def main():
         samples.add_from(sim, cr.default_targets(n, 4096, seed=snap), 0.9)
         for _ in range(5):
             walks.step("quad", 0.6, dt)
-    losses = cr.train(net, samples, epochs=args.train_epochs, batch=65536, lr=1e-3)
+    losses = cr.train(net, samples, epochs=args.train_epochs)
     train_s = time.perf_counter() - t0
     n_samples = samples.n
     samples.close()
 
-    th_l, med_l, p99_l, tried_l = calibrate(walks, "learned", (1.0, 0.9, 0.8, 0.7, 0.6), sample)
+    thetas = (1.0, 0.9, 0.85, 0.8, 0.75, 0.7, 0.65, 0.6, 0.5)
+    th_l, med_l, p99_l, tried_l = calibrate(walks, "learned", thetas, sample)
     corrector_on = th_l is not None
     secs_l = timed_steps(walks, "learned", th_l, dt, args.steps, 1) if corrector_on else None
 
     # analytic quadrupole control
-    th_q, med_q, p99_q, tried_q = calibrate(walks, "quad", (1.0, 0.9, 0.8, 0.7, 0.6, 0.5), sample)
+    th_q, med_q, p99_q, tried_q = calibrate(walks, "quad", thetas, sample)
     secs_q = timed_steps(walks, "quad", th_q if th_q is not None else 0.5, dt, args.steps, 1)
 
     mode = "learned" if corrector_on else "plain"

@@ -134,3 +134,25 @@ def test_third_moments_shift_and_octupole_term():
         left[dist] = np.linalg.norm(res - oct / scale) / np.linalg.norm(res)
     # 95 % of the residual is the octupole, and what is left is O(s/r) of it
     assert left[6.0] < 0.08 and left[12.0] < 0.6 * left[6.0], left
+
+
+def test_features_equal_sums_over_bodies():
+    """The moment features, restated as sums over the cell's bodies instead of
+    tensor contractions: (O:uu)_i = sum m (d.u)^2 d_i, u.S u = sum m (d.u)^2,
+    with d = body - com."""
+    rng = np.random.default_rng(4)
+    b = np.c_[rng.random((150, 3)) * [0.8, 0.5, 0.3], rng.random(150) + 0.1]
+    m = b[:, 3].sum()
+    c = (b[:, :3] * b[:, 3:]).sum(0) / m
+    S, O = mlp.moments(b, c)
+    size = 0.9
+    x = c + np.array([3.0, -2.0, 1.5])
+    f = mlp.features(x, c, m, size, S, O)
+    u = (c - x) / np.linalg.norm(c - x)
+    d = b[:, :3] - c
+    du = d @ u
+    ouu = (b[:, 3, None] * (du ** 2)[:, None] * d).sum(0)
+    usu = (b[:, 3] * du ** 2).sum()
+    assert np.allclose(f[:3], u, rtol=1e-14, atol=1e-15)
+    assert np.allclose(f[4:7] * m * size ** 3, ouu, rtol=1e-12, atol=1e-15)
+    assert abs(f[7] * m * size ** 2 - usu) <= 1e-12 * abs(usu)
