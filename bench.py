@@ -48,7 +48,7 @@ FP32_PEAK_TFLOPS = 157.3
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
 # (walk, theta) of the workload they were measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75): {"bytes": None,
+    ("learned", 0.75): {"bytes": 1285711385,
                         "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
 }
 BUDGET = 1e-3
