@@ -65,7 +65,8 @@ __device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int
  * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
  * SPLIT) unless list is NULL (counting only), and returns false as soon as
  * more than limit (<= cap) entries are accepted or the stack (stackCap
- * entries) would overflow.  *opened counts the opened nodes.  Each step pops
+ * entries) would overflow.  *opened counts the opened nodes; *depth receives
+ * the deepest stack reached (on an overflow, the depth that did not fit).  Each step pops
  * up to 128 entries, two per lane, so that two node loads per lane are in
  * flight; entries are handled in stack order (the upper 64 first). */
 template <bool SPLIT>
@@ -73,12 +74,12 @@ __device__ __forceinline__ bool gd_traverse_box(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
     const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
     float hy, float hz, int* stack, int stackCap, int* __restrict__ list, int cap, int limit,
-    int* nFrontOut, int* nBackOut, unsigned long long* opened) {
+    int* nFrontOut, int* nBackOut, unsigned long long* opened, int* depth) {
     const unsigned lane = gd_lane();
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
-    int nFront = 0, nBack = 0, top = 1;
+    int nFront = 0, nBack = 0, top = 1, deepest = 1;
     unsigned long long nOpen = 0;
     if (lane == 0) stack[0] = 0;
     while (top > 0) {
@@ -112,8 +113,11 @@ __device__ __forceinline__ bool gd_traverse_box(
         gd_wave_scan16(A.nk, lt, exA, totA);
         gd_wave_scan16(B.nk, lt, exB, totB);
         nOpen += (unsigned long long)(__popcll(__ballot(A.open)) + __popcll(__ballot(B.open)));
-        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + totA + totB > stackCap)
+        deepest = max(deepest, top + totA + totB);
+        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + totA + totB > stackCap) {
+            *depth = deepest;
             return false;
+        }
         if (list) {
             if (fA) list[nFront + __popcll(fba & lt)] = A.idx;
             if (fB) list[nFront + nfa + __popcll(fbb & lt)] = B.idx;
@@ -134,6 +138,7 @@ __device__ __forceinline__ bool gd_traverse_box(
     *nFrontOut = nFront;
     *nBackOut = nBack;
     *opened = nOpen;
+    *depth = deepest;
     return true;
 }
 

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const int* __restrict__ caps, int cap, const int* __restrict__ slotCap,
     int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ sizes,
     int* __restrict__ overflowList, int* __restrict__ overflowCount, int* __restrict__ maxList,
-    int* __restrict__ spreadList, int* __restrict__ spreadCount,
+    int* __restrict__ spreadList, int* __restrict__ spreadCount, int* __restrict__ maxDepth,
     unsigned long long* __restrict__ stats) {
     __shared__ int sStack[WAVES][STACK];
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
@@ -210,10 +210,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
         size = cap;
     }
     const int limit = slotCap ? min(slotCap[s], size) : size;
+    int depth = 0;
     const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, lx, ly, lz, hx, hy, hz,
                                            sStack[wave], STACK, list, size, limit, &nf, &nb,
-                                           &nOpen);
+                                           &nOpen, &depth);
     if (gd_lane() == 0) {
+        if (maxDepth) atomicMax(maxDepth, depth);
         if (ok) {
             counts[s] = make_int2(nf, nb);
             if (sizes) sizes[g] = nf + nb;
@@ -319,12 +321,12 @@ __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     const long long last = (n - g * 64) < 64 ? n - 1 : g * 64 + 63;
     const long long mid = g * 64 + (last - g * 64) / 2;
     const float4 p = pos[mid];
-    int nf = 0, nb = 0;
+    int nf = 0, nb = 0, depth = 0;
     unsigned long long nOpen = 0;
     const int bound = cap2 / GD_BODY_RATIO;
     const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, p.x, p.y, p.z, p.x, p.y,
                                            p.z, sStack[wave], GD_TR_STACK, nullptr, bound,
-                                           bound, &nf, &nb, &nOpen);
+                                           bound, &nf, &nb, &nOpen, &depth);
     if (gd_lane() == 0) {
         long long lim = ok ? (long long)GD_BODY_RATIO * (nf + nb) : cap2;
         lim = lim > floor ? lim : floor;
@@ -837,7 +839,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
  * time, in batches.  Third: the per-body walk.
  */
 struct gd_walk_status {
-    int over1, over2, maxList1, maxList2, over3, pad;
+    int over1, over2, maxList1, maxList2, over3, maxDepth1;
     unsigned long long requested;
 };
 
@@ -862,14 +864,15 @@ struct gd_walk_scratch {
     int fixedCap;          /* > 0: first-pass capacity of every group (no packing) */
     int entries2;          /* second-pass list capacity */
     long long walks;       /* walks so far (spread marks are cleared every 32) */
+    int deepStack;         /* first pass with 2 x GD_TR_STACK stack entries */
 };
 
 #define GD_LIST2 262144
-#define GD_POOL_BYTES (2ll << 30)
+#define GD_POOL_BYTES (8ll << 30)
 static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  nullptr, nullptr, nullptr, 0,       nullptr, 0,
                                  nullptr, nullptr, 0,       nullptr, 0,       0,
-                                 GD_LIST2, 0};
+                                 GD_LIST2, 0,       0};
 
 /* optional HIP-event timing of the walk's phases (null stream) */
 static int g_walkTiming = 0;
@@ -941,7 +944,7 @@ static void gd_walk_ensure_pool(int slots, int entries) {
 __global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st) {
     if (threadIdx.x < 5) stats[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
-        st->over1 = st->over2 = st->maxList1 = st->maxList2 = st->over3 = 0;
+        st->over1 = st->over2 = st->maxList1 = st->maxList2 = st->over3 = st->maxDepth1 = 0;
         st->requested = 0ull;
     }
 }
@@ -970,12 +973,21 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     k_walk_layout<<<1, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, reset,
                                g_walk.listEntries, g_walk.offsets, g_walk.caps, &st->requested);
     GD_LAUNCHED();
-    k_walk_traverse<SPLIT, GD_TR_WAVES, GD_TR_STACK><<<gd_div_up(groups, GD_TR_WAVES),
-                                                       64 * GD_TR_WAVES>>>(
-        pos, b->wcom, b->wlink, b->dinfo, theta, n, nullptr, groups, g_walk.offsets,
-        g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,
-        &st->over1, &st->maxList1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2, &st->over2,
-        g_walk.stats);
+    /* The first pass's LDS stack: 2048 entries per wavefront, or 4096 (half
+       the occupancy) while the deepest traversal of the last walk needed more
+       than 7/8 of 2048 -- small opening angles on deep trees; a group that
+       overflows either goes to the second pass, which walks one group per
+       wavefront at a fraction of the first pass's rate. */
+#define GD_PASS1(STK)                                                                          \
+    k_walk_traverse<SPLIT, GD_TR_WAVES, STK><<<gd_div_up(groups, GD_TR_WAVES),                  \
+                                               64 * GD_TR_WAVES>>>(                             \
+        pos, b->wcom, b->wlink, b->dinfo, theta, n, nullptr, groups, g_walk.offsets,           \
+        g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,  \
+        &st->over1, &st->maxList1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2,           \
+        &st->over2, &st->maxDepth1, g_walk.stats)
+    if (g_walk.deepStack) GD_PASS1(2 * GD_TR_STACK);
+    else GD_PASS1(GD_TR_STACK);
+#undef GD_PASS1
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
     gd_launch_eval<MODE, PHI>(groups, b, pos, nullptr, g_walk.lists, g_walk.offsets,
@@ -989,6 +1001,7 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     overOut[0] = over1 + h.over2;
     overOut[1] = h.maxList1;
     *requested = h.requested;
+    g_walk.deepStack = h.maxDepth1 > GD_TR_STACK - GD_TR_STACK / 8;
     /* pass 2: the groups that did not fit, with long lists and a deep stack */
     const int cap2 = g_walk.entries2;
     const long long fit = GD_POOL_BYTES / ((long long)cap2 * sizeof(int));
@@ -1004,7 +1017,8 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, nullptr,
             nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
-            g_walk.overflow2, &st->over2, &st->maxList2, nullptr, nullptr, g_walk.stats);
+            g_walk.overflow2, &st->over2, &st->maxList2, nullptr, nullptr, nullptr,
+            g_walk.stats);
         GD_LAUNCHED();
         gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool, nullptr,
                                   nullptr, g_walk.counts2, cap2, acc, phi, eps2, G, n, W, Bv,
@@ -1018,7 +1032,12 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     /* pass 3: body by body, 2048-entry stacks; the rare body that needs more
        is walked again with a stack that fits any tree */
     const long long bodies = 64ll * over2;
-    k_walk_body<MODE, PHI, GD_BODY_WAVES, GD_TR_STACK>
+    /* In the learned walk the per-body groups take the quadrupole alone: the
+       network is trained and evaluated per 64-body group, and once per body
+       it would cost 64 times as much for ~1 % of the bodies (the spread
+       ones), which the point-based opening test already serves better. */
+    constexpr int BMODE = MODE == 2 ? 1 : MODE;
+    k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_TR_STACK>
         <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES>>>(
             pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.overflow2, nullptr,
             bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n, W, Bv, minRatio,
@@ -1027,7 +1046,7 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
-    k_walk_body<MODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64>>>(
+    k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64>>>(
         pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, nullptr, g_walk.bodyOver, h.over3,
         nullptr, nullptr, acc, phi, eps2, G, theta, n, W, Bv, minRatio, g_walk.stats);
     GD_LAUNCHED();

@@ -27,6 +27,9 @@ from cudagravitydemo_amd import corrector as cr  # noqa: E402
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--per-disk", type=int, default=1 << 20)
+    ap.add_argument("--eps", type=float, default=0.005)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--hilo", action="store_true", help="FP32 hi/lo positions")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--theta", type=float, default=0.5)
@@ -38,11 +41,11 @@ def main():
     a = ap.parse_args()
     pos, vel = ics.disk_merger(a.per_disk, seed=2024)
     n = pos.shape[0]
-    eps, dt = 0.005, 5e-4
+    eps, dt = a.eps, a.dt
     net = None
     if a.walk == "learned":
         net = cr.Corrector(np.load(a.params) if a.params else None, seed=1)
-    with Simulation(pos, vel, eps=eps) as sim:
+    with Simulation(pos, vel, eps=eps, hilo=a.hilo) as sim:
         if a.list_cap:
             sim.lib.gd_walk_set_list_capacity(a.list_cap, 0)
         if a.walk in ("quad", "learned"):

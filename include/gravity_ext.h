@@ -112,7 +112,7 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
  * skip the list passes, re-checked every 32 walks.  A value > 0 gives every
  * group exactly that many entries and no skipping.  Either call forgets the
  * recorded lengths.  secondPass: per
- * group in the second pass (0: 262144; at most 2 GiB of lists at a time).
+ * group in the second pass (0: 262144; at most 8 GiB of lists at a time).
  * Returns the previous first-pass setting. */
 int gd_walk_set_list_capacity(int entries, int secondPass);
 
