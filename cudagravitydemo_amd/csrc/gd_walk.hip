@@ -257,23 +257,11 @@ __device__ __forceinline__ long long gd_list_room(int sz, int fixed) {
     return (want + 63) & ~63ll;
 }
 
-__global__ __launch_bounds__(1024) void k_walk_layout(int* __restrict__ sizes, int groups,
-                                                      int fixed, int reset,
-                                                      long long capacity,
-                                                      long long* __restrict__ offsets,
-                                                      int* __restrict__ caps,
-                                                      unsigned long long* __restrict__ total) {
-    __shared__ long long sWave[16];
-    const int per = (groups + 1023) / 1024;
-    const int g0 = threadIdx.x * per;
-    const int g1 = min(g0 + per, groups);
-    long long mine = 0;
-    for (int g = g0; g < g1; ++g) {
-        if (reset && sizes[g] == GD_SPREAD) sizes[g] = -1;
-        mine += gd_list_room(sizes[g], fixed);
-    }
-    /* block exclusive scan of mine */
-    long long inc = mine;
+/* Block-wide exclusive scan of one 64-bit value per thread (1024 threads);
+ * returns the thread's exclusive prefix, *blockTotal the block's sum. */
+__device__ __forceinline__ long long gd_block_scan1024(long long v, long long* sWave,
+                                                      long long* blockTotal) {
+    long long inc = v;
     const unsigned lane = gd_lane();
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -282,16 +270,59 @@ __global__ __launch_bounds__(1024) void k_walk_layout(int* __restrict__ sizes, i
     }
     if (lane == 63) sWave[threadIdx.x / 64] = inc;
     __syncthreads();
-    long long base = 0;
-    for (int w = 0; w < (int)(threadIdx.x / 64); ++w) base += sWave[w];
-    long long at = base + inc - mine;
-    for (int g = g0; g < g1; ++g) {
-        const long long want = gd_list_room(sizes[g], fixed);
-        offsets[g] = at;
-        caps[g] = at + want <= capacity ? (int)want : 0;
-        at += want;
+    long long base = 0, all = 0;
+    for (int w = 0; w < 16; ++w) {
+        const long long t = sWave[w];
+        if (w < (int)(threadIdx.x / 64)) base += t;
+        all += t;
     }
-    if (threadIdx.x == 1023) *total = (unsigned long long)(base + inc);
+    *blockTotal = all;
+    return base + inc - v;
+}
+
+/* Layout, part 1: each 1024-thread block scans the rooms of its 1024 groups
+ * (one per thread, coalesced) into block-local offsets and its total. */
+__global__ __launch_bounds__(1024) void k_walk_layout(int* __restrict__ sizes, int groups,
+                                                      int fixed, int reset,
+                                                      long long* __restrict__ offsets,
+                                                      long long* __restrict__ blockSums) {
+    __shared__ long long sWave[16];
+    const int g = blockIdx.x * 1024 + threadIdx.x;
+    long long room = 0;
+    if (g < groups) {
+        if (reset && sizes[g] == GD_SPREAD) sizes[g] = -1;
+        room = gd_list_room(sizes[g], fixed);
+    }
+    long long all;
+    const long long at = gd_block_scan1024(room, sWave, &all);
+    if (g < groups) offsets[g] = at;
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = all;
+}
+
+/* Layout, part 2: adds the preceding blocks' totals, gives each group its
+ * room when it ends within `capacity` (else none), and the last block
+ * writes the requested total. */
+__global__ __launch_bounds__(1024) void k_walk_place(const int* __restrict__ sizes, int groups,
+                                                     int fixed, long long capacity,
+                                                     const long long* __restrict__ blockSums,
+                                                     long long* __restrict__ offsets,
+                                                     int* __restrict__ caps,
+                                                     unsigned long long* __restrict__ total) {
+    __shared__ long long sBase;
+    if (threadIdx.x == 0) {
+        long long base = 0;
+        for (unsigned k = 0; k < blockIdx.x; ++k) base += blockSums[k];
+        sBase = base;
+        if (blockIdx.x == gridDim.x - 1)
+            *total = (unsigned long long)(base + blockSums[blockIdx.x]);
+    }
+    __syncthreads();
+    const int g = blockIdx.x * 1024 + threadIdx.x;
+    if (g >= groups) return;
+    const long long at = sBase + offsets[g];
+    const long long room = gd_list_room(sizes[g], fixed);
+    offsets[g] = at;
+    caps[g] = at + room <= capacity ? (int)room : 0;
 }
 
 /*
@@ -853,6 +884,7 @@ struct gd_walk_scratch {
     int* sizes;            /* groups entries: last list length, -1 unknown */
     long long* offsets;    /* groups entries: first-pass layout */
     int* caps;             /* groups entries */
+    long long* blockSums;  /* groups / 1024 entries: layout scan */
     int groupCap;          /* groups the arrays above hold */
     int* lists;            /* first-pass pool */
     long long listEntries;
@@ -870,7 +902,7 @@ struct gd_walk_scratch {
 #define GD_LIST2 262144
 #define GD_POOL_BYTES (8ll << 30)
 static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                 nullptr, nullptr, nullptr, 0,       nullptr, 0,
+                                 nullptr, nullptr, nullptr, nullptr, 0,       nullptr, 0,
                                  nullptr, nullptr, 0,       nullptr, 0,       0,
                                  GD_LIST2, 0,       0};
 
@@ -897,6 +929,7 @@ static void gd_walk_ensure(int groups) {
         if (g_walk.sizes) GD_HIP(hipFree(g_walk.sizes));
         if (g_walk.offsets) GD_HIP(hipFree(g_walk.offsets));
         if (g_walk.caps) GD_HIP(hipFree(g_walk.caps));
+        if (g_walk.blockSums) GD_HIP(hipFree(g_walk.blockSums));
         GD_HIP(hipMalloc(&g_walk.overflow1, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.overflow2, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.bodyOver, 64 * sizeof(int) * (size_t)groups));
@@ -904,6 +937,7 @@ static void gd_walk_ensure(int groups) {
         GD_HIP(hipMalloc(&g_walk.sizes, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.offsets, sizeof(long long) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.caps, sizeof(int) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.blockSums, sizeof(long long) * (size_t)gd_div_up(groups, 1024)));
         k_walk_fill<<<gd_div_up(groups, 256), 256>>>(g_walk.sizes, groups, -1);
         GD_LAUNCHED();
         g_walk.groupCap = groups;
@@ -970,8 +1004,13 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     gd_walk_status* st = g_walk.status;
     /* pass 1: every group, packed lists */
     const int reset = (g_walk.walks++ % 32) == 0;
-    k_walk_layout<<<1, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, reset,
-                               g_walk.listEntries, g_walk.offsets, g_walk.caps, &st->requested);
+    const int lblocks = gd_div_up(groups, 1024);
+    k_walk_layout<<<lblocks, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, reset,
+                                     g_walk.offsets, g_walk.blockSums);
+    GD_LAUNCHED();
+    k_walk_place<<<lblocks, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, g_walk.listEntries,
+                                    g_walk.blockSums, g_walk.offsets, g_walk.caps,
+                                    &st->requested);
     GD_LAUNCHED();
     /* The first pass's LDS stack: 2048 entries per wavefront, or 4096 (half
        the occupancy) while the deepest traversal of the last walk needed more
