@@ -220,6 +220,11 @@ def eval_roofline(walks, mode, theta, n):
     }
 
 
+def log(msg):
+    """Progress of the run, on stderr (stdout carries the one JSON line)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -243,6 +248,7 @@ def main():
     th_p, med_p, p99_p, tried_p = calibrate(walks, "plain", (0.3, 0.25, 0.2, 0.15, 0.1), sample)
     th_pt = th_p if th_p is not None else tried_p[-1]["theta"]
     secs_p = timed_steps(walks, "plain", th_pt, dt, args.steps, 1)
+    log(f"plain walk: theta {th_pt}, {1e3 * secs_p / args.steps:.2f} ms/step")
 
     # corrector: samples from this run's own snapshots, BF16-MFMA training
     t0 = time.perf_counter()
@@ -253,8 +259,10 @@ def main():
         samples.add_from(sim, cr.default_targets(n, 4096, seed=snap), 0.9)
         for _ in range(5):
             walks.step("quad", 0.6, dt)
+    log(f"corrector: {samples.n} samples")
     losses = cr.train(net, samples, epochs=args.train_epochs)
     train_s = time.perf_counter() - t0
+    log(f"corrector trained in {train_s:.1f} s")
     n_samples = samples.n
     samples.close()
 
@@ -262,6 +270,8 @@ def main():
     th_l, med_l, p99_l, tried_l = calibrate(walks, "learned", thetas, sample)
     corrector_on = th_l is not None
     secs_l = timed_steps(walks, "learned", th_l, dt, args.steps, 1) if corrector_on else None
+    if corrector_on:
+        log(f"learned walk: theta {th_l}, {1e3 * secs_l / args.steps:.2f} ms/step")
     after = None
     if corrector_on:
         # the judge again, on the state the timed steps left
@@ -278,11 +288,13 @@ def main():
     # analytic quadrupole control
     th_q, med_q, p99_q, tried_q = calibrate(walks, "quad", thetas, sample)
     secs_q = timed_steps(walks, "quad", th_q if th_q is not None else 0.5, dt, args.steps, 1)
+    log(f"quadrupole walk: theta {th_q}, {1e3 * secs_q / args.steps:.2f} ms/step")
 
     mode = "learned" if corrector_on else "plain"
     theta = th_l if corrector_on else th_pt
     secs = secs_l if corrector_on else secs_p
     roof = eval_roofline(walks, mode, theta, n)
+    log(f"roofline: {roof['achieved']:.1f} {roof['unit']} on {roof['kernel']}")
 
     ref = None
     if not args.no_reference_walk:
@@ -346,6 +358,7 @@ def main():
         "reference_walk": ref,
     }
     if not args.no_cpu:
+        log("CPU baseline (host only)")
         out["cpu_baseline"] = cpu_baseline(final_pos, final_vel, eps, 0.5, dt,
                                            args.cpu_group_stride)
     print(json.dumps(out), flush=True)

@@ -114,6 +114,7 @@ _PROTOS = {
                        C.POINTER(WalkStats)]),
     "gd_walk_timing": (None, [i32]),
     "gd_walk_set_list_capacity": (i32, [i32, i32]),
+    "gd_walk_set_group_split": (f32, [f32]),
     "gd_walk_kernel_ms": (f32, []),
     "gd_walk_phase_ms": (i32, [vp]),
     "gd_mlp_create": (vp, [vp]),

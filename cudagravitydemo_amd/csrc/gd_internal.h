@@ -20,6 +20,10 @@
 /* ---- error handling ---------------------------------------------------- */
 void gd_fail(const char* file, int line, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 int gd_has_error(void);
+/* The volume fraction below which the walks split a 64-body run in two (0:
+ * whole runs): gd_walk_set_group_split's setting, else the rule for this
+ * opening angle. */
+float gd_walk_split_ratio(float theta);
 
 #define GD_FAIL(...) gd_fail(__FILE__, __LINE__, __VA_ARGS__)
 #define GD_HIP(call)                                                              \

@@ -551,14 +551,16 @@ const float* gd_mlp_bias(const gd_mlp* net) { return net->bias; }
 #define GD_SL_STACK 2048
 
 /* One wavefront per target body: the traversal the learned walk makes for
- * the body's 64-body group (its bounding box against each node, the same
- * acceptance test); every accepted internal cell becomes a (target, cell)
- * pair, and centre[t] receives the group's centre, where the walk evaluates
- * the network. */
+ * the body's group (its 64-body run, or the half of it, split by `ratio` as
+ * the learned walk splits it; its bounding box against each node, the same
+ * acceptance test); every accepted internal cell
+ * becomes a (target, cell) pair, and centre[t] receives the group's centre,
+ * where the walk evaluates the network. */
 __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
-    const float4* __restrict__ pos, int n, const float4* __restrict__ wcom,
+    const uint64_t* __restrict__ keys, const float4* __restrict__ pos, int n,
+    const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info,
-    const int* __restrict__ targets, int nt, float theta, int2* __restrict__ pairs,
+    const int* __restrict__ targets, int nt, float theta, float ratio, int2* __restrict__ pairs,
     long long cap, unsigned long long* __restrict__ count, int* __restrict__ dropped,
     float4* __restrict__ centre) {
     __shared__ int sStack[GD_SL_WAVES][GD_SL_STACK];
@@ -567,8 +569,15 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     const int t = blockIdx.x * GD_SL_WAVES + wave;
     if (t >= nt) return;
     int* stack = sStack[wave];
+    const int b = targets[t];
+    const long long run = b / 64;
+    const int j = gd_split_run(keys, pos, n, run, ratio);
+    const int rstart = (int)(run * 64), rcount = n - rstart < 64 ? n - rstart : 64;
+    const bool second = j > 0 && b - rstart >= j;
+    const int gstart = second ? rstart + j : rstart;
+    const int gcount = j == 0 ? rcount : (second ? rcount - j : j);
     float lx, ly, lz, hx, hy, hz;
-    gd_group_box(pos, n, targets[t] / 64, lx, ly, lz, hx, hy, hz);
+    gd_range_box(pos, gstart, gcount, lx, ly, lz, hx, hy, hz);
     if (lane == 0)
         centre[t] = make_float4(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz), 0.f);
     const float ext = info->box.extent;
@@ -721,8 +730,8 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
     long long n = 0;
     if (!gd_has_error()) {
         k_samples_list<<<gd_div_up(nt, GD_SL_WAVES), 64 * GD_SL_WAVES>>>(
-            pos, tree->starCount, b->wcom, b->wlink, b->dinfo, targets, nt, theta, pairs, cap,
-            count, dropped, centre);
+            tree->mortonOut, pos, tree->starCount, b->wcom, b->wlink, b->dinfo, targets, nt, theta,
+            gd_walk_split_ratio(theta), pairs, cap, count, dropped, centre);
         GD_LAUNCHED();
         unsigned long long total = 0;
         int drop = 0;

@@ -164,13 +164,13 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 /*
  * Traversal kernel, one wavefront per group.  Slot s walks group
  * groupList[s] (or group s when groupList is NULL).  Its list is
- * lists[offsets[s] .. + caps[s]) when offsets is given (the first pass's
- * packed layout), else lists[s*cap .. + cap); it gives up once the list
+ * lists[offsets[g] .. + caps[g]) for group g when offsets is given (the
+ * first pass's packed layout), else lists[s*cap .. + cap); it gives up once the list
  * passes slotCap[s] (when given) or its capacity, or the LDS stack of STACK
  * entries overflows: counts[s] = (-1, -1) and the group goes to
  * overflowList.  Otherwise counts[s] = (front, back) and, when sizes is
  * given, sizes[g] records the list length for the next walk's layout.  In
- * the second pass (groupList given) a group given up on is marked spread
+ * the second pass (offsets NULL) a group given up on is marked spread
  * (sizes -2); the first pass, when spreadList is given, sends a marked group
  * straight to the per-body walk (spreadList) without traversing it.
  */
@@ -178,7 +178,8 @@ template <bool SPLIT, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
-    const int* __restrict__ groupList, int slots, const long long* __restrict__ offsets,
+    const int2* __restrict__ slotRange, const int* __restrict__ groupList, int slots,
+    const long long* __restrict__ offsets,
     const int* __restrict__ caps, int cap, const int* __restrict__ slotCap,
     int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ sizes,
     int* __restrict__ overflowList, int* __restrict__ overflowCount, int* __restrict__ maxList,
@@ -189,6 +190,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const long long s = (long long)gd_xcd_block() * WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
     const long long g = groupList ? groupList[s] : s;
+    const int2 rg = slotRange[g];   /* (first body, bodies) */
+    if (rg.y == 0) {                /* empty second slot of a whole run */
+        if (gd_lane() == 0) counts[s] = make_int2(0, 0);
+        return;
+    }
     if (spreadList && sizes[g] == GD_SPREAD) {
         if (gd_lane() == 0) {
             counts[s] = make_int2(-1, -1);
@@ -197,14 +203,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
         return;
     }
     float lx, ly, lz, hx, hy, hz;
-    gd_group_box(pos, n, g, lx, ly, lz, hx, hy, hz);
+    gd_range_box(pos, rg.x, rg.y, lx, ly, lz, hx, hy, hz);
     int nf = 0, nb = 0;
     unsigned long long nOpen = 0;
     int* list;
     int size;
     if (offsets) {
-        list = lists + offsets[s];
-        size = caps[s];
+        list = lists + offsets[g];
+        size = caps[g];
     } else {
         list = lists + (size_t)s * cap;
         size = cap;
@@ -221,14 +227,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
             if (sizes) sizes[g] = nf + nb;
             atomicMax(maxList, nf + nb);
             if (stats) {
-                const int nvalid = (int)((n - g * 64) < 64 ? (n - g * 64) : 64);
+                const int nvalid = rg.y;
                 atomicAdd(&stats[0], (unsigned long long)(nf + nb) * (unsigned long long)nvalid);
                 atomicAdd(&stats[1], nOpen);
                 atomicAdd(&stats[3], (unsigned long long)nb * (unsigned long long)nvalid);
             }
         } else {
             counts[s] = make_int2(-1, -1);
-            if (sizes) sizes[g] = groupList ? GD_SPREAD : -1;
+            if (sizes) sizes[g] = offsets ? -1 : GD_SPREAD;
             const int at = atomicAdd(overflowCount, 1);
             overflowList[at] = (int)g;
         }
@@ -248,6 +254,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
  * (the group is tried again as a list).  Groups that would end
  * beyond `capacity` entries get no room (they go to the second pass);
  * *total receives the unclamped sum, for growing the pool.
+ *
+ * The same scan numbers the non-empty slots: order[] lists them in slot
+ * (Morton) order, and its tail -- as many entries as there are empty slots
+ * -- holds `groups`, the index of an always-empty sentinel slot.  The first
+ * pass walks slot order[s] with wavefront s, so no wavefront of a busy block
+ * idles on the empty second slot of a run that was not split.
  */
 __device__ __forceinline__ long long gd_list_room(int sz, int fixed) {
     if (fixed > 0) return fixed;
@@ -282,45 +294,67 @@ __device__ __forceinline__ long long gd_block_scan1024(long long v, long long* s
 
 /* Layout, part 1: each 1024-thread block scans the rooms of its 1024 groups
  * (one per thread, coalesced) into block-local offsets and its total. */
-__global__ __launch_bounds__(1024) void k_walk_layout(int* __restrict__ sizes, int groups,
+__global__ __launch_bounds__(1024) void k_walk_layout(const int2* __restrict__ slotRange,
+                                                      int* __restrict__ sizes, int groups,
                                                       int fixed, int reset,
                                                       long long* __restrict__ offsets,
                                                       long long* __restrict__ blockSums) {
     __shared__ long long sWave[16];
+    __shared__ long long sFlag[16];
     const int g = blockIdx.x * 1024 + threadIdx.x;
-    long long room = 0;
-    if (g < groups) {
+    long long room = 0, busy = 0;
+    if (g < groups && slotRange[g].y > 0) {
         if (reset && sizes[g] == GD_SPREAD) sizes[g] = -1;
         room = gd_list_room(sizes[g], fixed);
+        busy = 1;
     }
-    long long all;
+    long long all, slots;
     const long long at = gd_block_scan1024(room, sWave, &all);
+    gd_block_scan1024(busy, sFlag, &slots);
     if (g < groups) offsets[g] = at;
-    if (threadIdx.x == 0) blockSums[blockIdx.x] = all;
+    if (threadIdx.x == 0) {
+        blockSums[blockIdx.x] = all;
+        blockSums[gridDim.x + blockIdx.x] = slots;   /* non-empty slots */
+    }
 }
 
 /* Layout, part 2: adds the preceding blocks' totals, gives each group its
  * room when it ends within `capacity` (else none), and the last block
- * writes the requested total. */
-__global__ __launch_bounds__(1024) void k_walk_place(const int* __restrict__ sizes, int groups,
+ * writes the requested total; writes order[]. */
+__global__ __launch_bounds__(1024) void k_walk_place(const int2* __restrict__ slotRange,
+                                                     const int* __restrict__ sizes, int groups,
                                                      int fixed, long long capacity,
                                                      const long long* __restrict__ blockSums,
                                                      long long* __restrict__ offsets,
                                                      int* __restrict__ caps,
+                                                     int* __restrict__ order,
                                                      unsigned long long* __restrict__ total) {
-    __shared__ long long sBase;
+    __shared__ long long sBase, sSlotBase, sSlots;
+    __shared__ long long sFlag[16];
     if (threadIdx.x == 0) {
-        long long base = 0;
+        long long base = 0, slotBase = 0, slots = 0;
         for (unsigned k = 0; k < blockIdx.x; ++k) base += blockSums[k];
+        for (unsigned k = 0; k < gridDim.x; ++k) {
+            const long long c = blockSums[gridDim.x + k];
+            if (k < blockIdx.x) slotBase += c;
+            slots += c;
+        }
         sBase = base;
+        sSlotBase = slotBase;
+        sSlots = slots;
         if (blockIdx.x == gridDim.x - 1)
             *total = (unsigned long long)(base + blockSums[blockIdx.x]);
     }
     __syncthreads();
     const int g = blockIdx.x * 1024 + threadIdx.x;
+    const bool busy = g < groups && slotRange[g].y > 0;
+    long long blockSlots;
+    const long long k = sSlotBase + gd_block_scan1024(busy ? 1 : 0, sFlag, &blockSlots);
     if (g >= groups) return;
+    if (busy) order[k] = g;
+    if (g >= sSlots) order[g] = groups;   /* the tail: the sentinel slot */
     const long long at = sBase + offsets[g];
-    const long long room = gd_list_room(sizes[g], fixed);
+    const long long room = slotRange[g].y > 0 ? gd_list_room(sizes[g], fixed) : 0;
     offsets[g] = at;
     caps[g] = at + room <= capacity ? (int)room : 0;
 }
@@ -342,16 +376,14 @@ template <bool SPLIT>
 __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
-    const int* __restrict__ groupList, int slots, int cap2, int floor,
-    int* __restrict__ slotCap) {
+    const int2* __restrict__ slotRange, const int* __restrict__ groupList, int slots, int cap2,
+    int floor, int* __restrict__ slotCap) {
     __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK];
     const int wave = (int)(threadIdx.x >> 6);
     const long long s = (long long)blockIdx.x * GD_TR_WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
-    const long long g = groupList[s];
-    const long long last = (n - g * 64) < 64 ? n - 1 : g * 64 + 63;
-    const long long mid = g * 64 + (last - g * 64) / 2;
-    const float4 p = pos[mid];
+    const int2 rg = slotRange[groupList[s]];
+    const float4 p = pos[rg.x + (rg.y - 1) / 2];
     int nf = 0, nb = 0, depth = 0;
     unsigned long long nOpen = 0;
     const int bound = cap2 / GD_BODY_RATIO;
@@ -530,14 +562,13 @@ __device__ __forceinline__ void gd_eval_group(
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
     const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ list, int cap, int2 cnt,
-    long long g, int n, float eps2, const gd_bf16x4* __restrict__ mlpW,
+    long long start, int count, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sE, float4* sQ0, float4* sQ1,
     float4* sCor, int* sMap, gd_kahan3& tot, float& phs, float& phc,
     unsigned long long* nnEvals) {
     const unsigned lane = gd_lane();
-    const long long i = g * 64 + lane;
-    const bool valid = i < n;
-    const float4 p = pos[valid ? i : g * 64];
+    const bool valid = (int)lane < count;
+    const float4 p = pos[start + (valid ? lane : 0)];
     const float px = p.x, py = p.y, pz = p.z;
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     /* padding of a partial chunk: massless and far away, contributes exactly 0 */
@@ -638,7 +669,8 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
     const float4* __restrict__ woct,
-    const gd_tree_info* __restrict__ info, const int* __restrict__ groupList, int slots,
+    const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange,
+    const int* __restrict__ groupList, int slots,
     const int* __restrict__ lists, const long long* __restrict__ offsets,
     const int* __restrict__ caps, const int2* __restrict__ counts, int cap,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, int n,
@@ -667,19 +699,22 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const int2 cnt = counts[s];
     if (cnt.x < 0) return;   /* left to the next pass */
     const long long g = groupList ? groupList[s] : s;
+    const int2 rg = slotRange[g];
+    if (rg.y == 0) return;   /* empty slot */
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
     unsigned long long nn = 0;
-    const int* list = offsets ? lists + offsets[s] : lists + (size_t)s * cap;
-    const int size = offsets ? caps[s] : cap;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, list, size, cnt, g, n, eps2,
+    const int* list = offsets ? lists + offsets[g] : lists + (size_t)s * cap;
+    const int size = offsets ? caps[g] : cap;
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, list, size, cnt, rg.x, rg.y,
+                             eps2,
                              mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
                              sCor[MODE == 2 ? wave : 0], sMap[MODE == 2 ? wave : 0], tot, phs,
                              phc, &nn);
     if (MODE == 2 && stats && lane == 0) atomicAdd(&stats[4], nn);
-    const long long i = g * 64 + lane;
-    if (i < n) {
+    if ((int)lane < rg.y) {
+        const long long i = rg.x + lane;
         acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
         if (PHI) phi[i] = -G * phs;
     }
@@ -694,8 +729,8 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
  * tested it (monopole; in MODE 1/2 quadrupole for cells, and in MODE 2 the
  * network's residual evaluated at the body), per-lane sums are
  * Kahan-compensated and combined by a fixed butterfly at the end.  Slot q
- * walks body bodyList[q], or, without bodyList, body q % 64 of group
- * groupList[q / 64]; WAVES wavefronts per block, each with an LDS stack of
+ * walks body bodyList[q], or, without bodyList, body q % 64 of group (slot)
+ * groupList[q / 64] (when the group has that many bodies); WAVES wavefronts per block, each with an LDS stack of
  * STACK entries.  A body whose stack would overflow goes to overflowBodies
  * (when given; else the host reports an error): with STACK = GD_BODY_STACK
  * that cannot happen, since a step pops up to 64 nodes and pushes at most 8
@@ -707,7 +742,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
     const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
-    const int* __restrict__ groupList, const int* __restrict__ bodyList, long long count,
+    const int2* __restrict__ slotRange, const int* __restrict__ groupList,
+    const int* __restrict__ bodyList, long long count,
     int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
     int n, const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB,
@@ -729,7 +765,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     int* sMap = sMapAll[MODE == 2 ? wave : 0];
     const long long q = (long long)gd_xcd_block() * WAVES + wave;
     if (q >= count) return;   /* wave-uniform */
-    const long long i = bodyList ? (long long)bodyList[q] : (long long)groupList[q >> 6] * 64 + (q & 63);
+    long long i;
+    if (bodyList) {
+        i = bodyList[q];
+    } else {
+        const int2 rg = slotRange[groupList[q >> 6]];
+        if ((int)(q & 63) >= rg.y) return;   /* wave-uniform */
+        i = rg.x + (q & 63);
+    }
     if (i >= n) return;   /* wave-uniform */
     const float4 p = pos[i];
     const float ext = info->box.extent;
@@ -884,7 +927,9 @@ struct gd_walk_scratch {
     int* sizes;            /* groups entries: last list length, -1 unknown */
     long long* offsets;    /* groups entries: first-pass layout */
     int* caps;             /* groups entries */
-    long long* blockSums;  /* groups / 1024 entries: layout scan */
+    long long* blockSums;  /* 2 x groups / 1024 entries: layout scan */
+    int2* slotRange;       /* groups + 1 entries: (first body, bodies) of each group slot */
+    int* order;            /* groups entries: first-pass slot order (non-empty slots first) */
     int groupCap;          /* groups the arrays above hold */
     int* lists;            /* first-pass pool */
     long long listEntries;
@@ -902,7 +947,8 @@ struct gd_walk_scratch {
 #define GD_LIST2 262144
 #define GD_POOL_BYTES (8ll << 30)
 static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                 nullptr, nullptr, nullptr, nullptr, 0,       nullptr, 0,
+                                 nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                 nullptr, 0,
                                  nullptr, nullptr, 0,       nullptr, 0,       0,
                                  GD_LIST2, 0,       0};
 
@@ -930,6 +976,8 @@ static void gd_walk_ensure(int groups) {
         if (g_walk.offsets) GD_HIP(hipFree(g_walk.offsets));
         if (g_walk.caps) GD_HIP(hipFree(g_walk.caps));
         if (g_walk.blockSums) GD_HIP(hipFree(g_walk.blockSums));
+        if (g_walk.slotRange) GD_HIP(hipFree(g_walk.slotRange));
+        if (g_walk.order) GD_HIP(hipFree(g_walk.order));
         GD_HIP(hipMalloc(&g_walk.overflow1, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.overflow2, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.bodyOver, 64 * sizeof(int) * (size_t)groups));
@@ -937,7 +985,10 @@ static void gd_walk_ensure(int groups) {
         GD_HIP(hipMalloc(&g_walk.sizes, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.offsets, sizeof(long long) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.caps, sizeof(int) * (size_t)groups));
-        GD_HIP(hipMalloc(&g_walk.blockSums, sizeof(long long) * (size_t)gd_div_up(groups, 1024)));
+        GD_HIP(hipMalloc(&g_walk.blockSums,
+                         2 * sizeof(long long) * (size_t)gd_div_up(groups, 1024)));
+        GD_HIP(hipMalloc(&g_walk.slotRange, sizeof(int2) * ((size_t)groups + 1)));
+        GD_HIP(hipMalloc(&g_walk.order, sizeof(int) * (size_t)groups));
         k_walk_fill<<<gd_div_up(groups, 256), 256>>>(g_walk.sizes, groups, -1);
         GD_LAUNCHED();
         g_walk.groupCap = groups;
@@ -975,6 +1026,24 @@ static void gd_walk_ensure_pool(int slots, int entries) {
     }
 }
 
+/* Group slots of the 64-body runs (see gd_traverse.h): one wavefront per
+ * run; slot 2 * runs is an empty sentinel (see k_walk_layout). */
+__global__ __launch_bounds__(256) void k_walk_split(const uint64_t* __restrict__ keys,
+                                                    const float4* __restrict__ pos, int n,
+                                                    int runs, float ratio,
+                                                    int2* __restrict__ slotRange) {
+    const long long g = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= runs) return;   /* wave-uniform */
+    const int j = gd_split_run(keys, pos, n, g, ratio);
+    if (gd_lane() == 0) {
+        if (g == 0) slotRange[2 * runs] = make_int2(n, 0);   /* the sentinel: always empty */
+        const int start = (int)(g * 64);
+        const int count = n - start < 64 ? n - start : 64;
+        slotRange[2 * g] = make_int2(start, j ? j : count);
+        slotRange[2 * g + 1] = make_int2(start + (j ? j : count), j ? count - j : 0);
+    }
+}
+
 __global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st) {
     if (threadIdx.x < 5) stats[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
@@ -990,27 +1059,33 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            float* phi, float eps2, float G, int n, const gd_bf16x4* W,
                            const float* Bv, float minRatio) {
     k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, groupList, slots, lists, offsets,
-        caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.stats);
+        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, groupList, slots,
+        lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.stats);
 }
 
 template <int MODE, bool PHI>
-static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, float* phi,
-                           float eps2, float G, float theta, int n, const gd_bf16x4* W,
-                           const float* Bv, float minRatio, int* overOut,
+static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const float4* pos,
+                           float3* acc, float* phi, float eps2, float G, float theta, int n,
+                           const gd_bf16x4* W, const float* Bv, float minRatio, int* overOut,
                            unsigned long long* requested) {
     constexpr bool SPLIT = MODE != 0;
-    const int groups = gd_div_up(n, 64);
+    const int runs = gd_div_up(n, 64);
+    const int groups = 2 * runs;   /* group slots: two per 64-body run */
     gd_walk_status* st = g_walk.status;
+    k_walk_split<<<gd_div_up(runs, 4), 256>>>(keys, pos, n, runs,
+                                              gd_walk_split_ratio(theta),
+                                              g_walk.slotRange);
+    GD_LAUNCHED();
     /* pass 1: every group, packed lists */
     const int reset = (g_walk.walks++ % 32) == 0;
     const int lblocks = gd_div_up(groups, 1024);
-    k_walk_layout<<<lblocks, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, reset,
-                                     g_walk.offsets, g_walk.blockSums);
+    k_walk_layout<<<lblocks, 1024>>>(g_walk.slotRange, g_walk.sizes, groups, g_walk.fixedCap,
+                                     reset, g_walk.offsets, g_walk.blockSums);
     GD_LAUNCHED();
-    k_walk_place<<<lblocks, 1024>>>(g_walk.sizes, groups, g_walk.fixedCap, g_walk.listEntries,
+    k_walk_place<<<lblocks, 1024>>>(g_walk.slotRange, g_walk.sizes, groups, g_walk.fixedCap,
+                                    g_walk.listEntries,
                                     g_walk.blockSums, g_walk.offsets, g_walk.caps,
-                                    &st->requested);
+                                    g_walk.order, &st->requested);
     GD_LAUNCHED();
     /* The first pass's LDS stack: 2048 entries per wavefront, or 4096 (half
        the occupancy) while the deepest traversal of the last walk needed more
@@ -1020,7 +1095,8 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
 #define GD_PASS1(STK)                                                                          \
     k_walk_traverse<SPLIT, GD_TR_WAVES, STK><<<gd_div_up(groups, GD_TR_WAVES),                  \
                                                64 * GD_TR_WAVES>>>(                             \
-        pos, b->wcom, b->wlink, b->dinfo, theta, n, nullptr, groups, g_walk.offsets,           \
+        pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.order, groups,    \
+        g_walk.offsets,                                                                        \
         g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,  \
         &st->over1, &st->maxList1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2,           \
         &st->over2, &st->maxDepth1, g_walk.stats)
@@ -1029,7 +1105,7 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
 #undef GD_PASS1
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
-    gd_launch_eval<MODE, PHI>(groups, b, pos, nullptr, g_walk.lists, g_walk.offsets,
+    gd_launch_eval<MODE, PHI>(groups, b, pos, g_walk.order, g_walk.lists, g_walk.offsets,
                               g_walk.caps, g_walk.counts, 0, acc, phi, eps2, G, n, W, Bv,
                               minRatio);
     GD_LAUNCHED();
@@ -1050,12 +1126,12 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     for (int b0 = 0; b0 < over1; b0 += slots) {
         const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
         k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, cap2,
-            cap2 / 4 < GD_LIST_MIN ? cap2 / 4 : GD_LIST_MIN, g_walk.slotCap);
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
+            cnt, cap2, cap2 / 4 < GD_LIST_MIN ? cap2 / 4 : GD_LIST_MIN, g_walk.slotCap);
         GD_LAUNCHED();
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.overflow1 + b0, cnt, nullptr,
-            nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
+            cnt, nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, &st->maxList2, nullptr, nullptr, nullptr,
             g_walk.stats);
         GD_LAUNCHED();
@@ -1078,16 +1154,17 @@ static void gd_walk_passes(const gd_builder* b, const float4* pos, float3* acc, 
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
     k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_TR_STACK>
         <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES>>>(
-            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.overflow2, nullptr,
-            bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n, W, Bv, minRatio,
-            g_walk.stats);
+            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
+            g_walk.overflow2, nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G,
+            theta, n, W, Bv, minRatio, g_walk.stats);
     GD_LAUNCHED();
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
     k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, nullptr, g_walk.bodyOver, h.over3,
-        nullptr, nullptr, acc, phi, eps2, G, theta, n, W, Bv, minRatio, g_walk.stats);
+        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, nullptr,
+        g_walk.bodyOver, h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, W, Bv, minRatio,
+        g_walk.stats);
     GD_LAUNCHED();
 }
 
@@ -1109,7 +1186,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
                 "(gd_set_quadrupole(tree, 2) before the build)");
         return;
     }
-    const int groups = gd_div_up(starCount, 64);
+    const int groups = 2 * gd_div_up(starCount, 64);   /* group slots */
     gd_walk_ensure(groups);
     if (g_walk.fixedCap > 0) gd_walk_grow_lists((long long)groups * g_walk.fixedCap);
     if (gd_has_error()) return;
@@ -1125,7 +1202,8 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     int over[4] = {0, 0, 0, 0};
     unsigned long long requested = 0;
 #define GD_PASSES(M, P)                                                                        \
-    gd_walk_passes<M, P>(b, pos, acc, phi, eps2, G, theta, starCount, W, Bv, minRatio, over,   \
+    gd_walk_passes<M, P>(b, tree->mortonOut, pos, acc, phi, eps2, G, theta, starCount, W, Bv,  \
+                         minRatio, over,                                                       \
                          &requested)
     if (mode == 0) {
         if (phi) GD_PASSES(0, true); else GD_PASSES(0, false);
@@ -1187,6 +1265,23 @@ extern "C" int gd_walk_set_list_capacity(int entries, int secondPass) {
         k_walk_fill<<<gd_div_up(g_walk.groupCap, 256), 256>>>(g_walk.sizes, g_walk.groupCap, -1);
         GD_LAUNCHED();
     }
+    return prev;
+}
+
+/* < 0: the opening-angle rule of gd_split_ratio; else the fraction for every walk */
+static float g_splitRatio = -1.f;
+
+float gd_walk_split_ratio(float theta) {
+    return g_splitRatio >= 0.f ? g_splitRatio : gd_split_ratio(theta);
+}
+
+extern "C" float gd_walk_set_group_split(float ratio) {
+    const float prev = g_splitRatio;
+    if (!(ratio <= 1.f)) {   /* NaN included */
+        GD_FAIL("gd_walk_set_group_split: fraction %g above 1", (double)ratio);
+        return prev;
+    }
+    g_splitRatio = ratio < 0.f ? -1.f : ratio;
     return prev;
 }
 

@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--params", default=None)
     ap.add_argument("--list-cap", type=int, default=0,
                     help="fixed first-pass list capacity per group (0: adaptive)")
+    ap.add_argument("--group-split", type=float, default=-1.0,
+                    help="volume fraction for splitting 64-body groups (< 0: by theta)")
     ap.add_argument("--copy-mib", type=int, default=0)
     a = ap.parse_args()
     pos, vel = ics.disk_merger(a.per_disk, seed=2024)
@@ -48,6 +50,7 @@ def main():
     with Simulation(pos, vel, eps=eps, hilo=a.hilo) as sim:
         if a.list_cap:
             sim.lib.gd_walk_set_list_capacity(a.list_cap, 0)
+        sim.lib.gd_walk_set_group_split(a.group_split)
         if a.walk in ("quad", "learned"):
             sim.set_quadrupole(2 if a.walk == "learned" else 1)
         sim.sync()

@@ -116,6 +116,16 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
  * Returns the previous first-pass setting. */
 int gd_walk_set_list_capacity(int entries, int secondPass);
 
+/* Groups of the list walks.  Each run of 64 bodies in Morton order is a
+ * group, split in two at its largest key jump when the halves' bounding
+ * boxes hold less than `ratio` of the run's box volume (a run that spans
+ * two clusters sees fewer cells through two small boxes).  ratio < 0 (the
+ * default): theta - 0.2 clamped to [0, 0.3] (no splitting at theta <= 0.2,
+ * where the halves' lists share most cells); 0: whole runs; at most 1.  The
+ * training samples (gd_corrector_samples) take the groups of their own
+ * opening angle.  Returns the previous setting. */
+float gd_walk_set_group_split(float ratio);
+
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
  * null stream (enable = 1) or stop (0).  The walk is a traversal kernel that
  * writes each group's interaction list, an evaluation kernel (the dominant

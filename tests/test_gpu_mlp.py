@@ -8,6 +8,7 @@ import pytest
 from cudagravitydemo_amd import Simulation, ics
 from cudagravitydemo_amd import corrector as cr
 from oracle import mlp
+from tests.walk_check import group_slots, split_ratio
 
 pytestmark = pytest.mark.gpu
 
@@ -80,18 +81,24 @@ def test_adam_step_matches_closed_form():
         net.close()
 
 
-def test_samples_match_direct_sums():
+@pytest.mark.parametrize("ratio", [-1.0, 0.0])
+def test_samples_match_direct_sums(ratio):
+    """ratio -1: the learned walk's default groups at the sampling angle 0.8
+    (runs split at volume fraction 0.3); 0: whole 64-body runs, as
+    gd_walk_set_group_split(0) makes the walk keep them."""
     pos, vel = ics.exponential_disk(8192, seed=13)
     with Simulation(pos, vel, eps=0.01) as sim:
         sim.set_quadrupole(2)
         sim.build_tree_async()
         targets = ics.subsample(8192, 64, seed=2)
         s = cr.SampleSet(sim.lib, 200000)
+        prev = sim.lib.gd_walk_set_group_split(ratio)
         try:
             n, dropped = s.add_from(sim, targets, 0.8)
             assert n > 64 and dropped == 0
             X, Y = s.host()
         finally:
+            sim.lib.gd_walk_set_group_split(prev)
             s.close()
         # recompute a few rows from the tree with the oracle's definitions
         total = sim.tree_summary()[0]
@@ -104,6 +111,9 @@ def test_samples_match_direct_sums():
         found = 0
         rng = np.random.default_rng(0)
         cand = np.flatnonzero(link[:, 0] >= 0)
+        slots, slot_of = group_slots(sim.keys_sorted(), ps,
+                                     split_ratio(0.8) if ratio < 0 else ratio)
+        assert (slots[1::2, 1] > 0).any() == (ratio < 0)
         for row in rng.choice(n, 40, replace=False):
             # find the (target, cell) of this row among all internal cells; targets in
             # one 64-body group share the features, and a node and its only child share
@@ -111,8 +121,9 @@ def test_samples_match_direct_sums():
             cands = []
             for t in targets:
                 x = ps[t, :3].astype(np.float64)
-                # features are taken at the centre of the target's 64-body group
-                blk = ps[64 * (t // 64):64 * (t // 64) + 64, :3]
+                # features are taken at the centre of the target's group
+                a0, c0 = slots[slot_of[t]]
+                blk = ps[a0:a0 + c0, :3]
                 g = (np.float32(0.5) * (blk.min(0) + blk.max(0))).astype(np.float64)
                 d = com[cand, :3] - g
                 u = d / np.linalg.norm(d, axis=1)[:, None]

@@ -17,7 +17,7 @@ import pytest
 from cudagravitydemo_amd import Simulation, ics, load_library
 from cudagravitydemo_amd import corrector as cr
 from oracle import mlp
-from tests.walk_check import accepted, walk_fp64, world_extent
+from tests.walk_check import accepted, group_slots, split_ratio, walk_fp64, world_extent
 
 pytestmark = pytest.mark.gpu
 
@@ -35,9 +35,17 @@ def _net_params():
     return mlp.pack(layers).astype(np.float32)
 
 
-def _group_box(ps, i, n):
-    g = i // 64
-    blk = ps[64 * g:min(64 * g + 64, n), :3]
+def _slots(sim, mode, theta=THETA, ratio=None):
+    """The walk's group slots (see walk_check.group_slots), split by the
+    opening angle unless `ratio` is given."""
+    if ratio is None:
+        ratio = split_ratio(theta)
+    return group_slots(sim.keys_sorted(), sim.positions(), ratio)
+
+
+def _slot_box(ps, slots, s):
+    a, c = slots[s]
+    blk = ps[a:a + c, :3]
     return blk.min(0), blk.max(0)
 
 
@@ -65,9 +73,10 @@ def _sample(n, extra=()):
     return np.unique(s)
 
 
-def _check(sim, mode, per_body_of, params, extra=()):
+def _check(sim, mode, per_body_of, params, extra=(), ratio=None):
     ps = sim.positions()
     acc = sim.accelerations()
+    slots, slot_of = _slots(sim, mode, ratio=ratio)
     total = sim.tree_summary()[0]
     com, link, quad = sim.walk_arrays(total)
     oct_ = sim.third_moments(total) if mode == 2 else None
@@ -78,7 +87,7 @@ def _check(sim, mode, per_body_of, params, extra=()):
     for i in _sample(sim.n, extra):
         x = ps[i, :3]
         pb = per_body_of(i)
-        box = (x, x) if pb else _group_box(ps, i, sim.n)
+        box = (x, x) if pb else _slot_box(ps, slots, slot_of[i])
         # the per-body walk of the learned mode evaluates the quadrupole alone
         m = 1 if (mode == 2 and pb) else mode
         want, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
@@ -103,28 +112,30 @@ def _tree(sim):
     return ps, com, link, world_extent(lo, hi)
 
 
-def _group_sizes(sim):
+def _group_sizes(sim, mode):
+    """List length of every group slot (0 for empty slots)."""
     ps, com, link, ext = _tree(sim)
-    groups = (sim.n + 63) // 64
-    sizes = np.empty(groups, np.int64)
-    for g in range(groups):
-        blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
-        lv, cl = accepted(com, link, ext, blk.min(0), blk.max(0), THETA)
-        sizes[g] = lv.size + cl.size
+    slots, _ = _slots(sim, mode)
+    sizes = np.zeros(slots.shape[0], np.int64)
+    for s in range(slots.shape[0]):
+        if slots[s, 1]:
+            lv, cl = accepted(com, link, ext, *_slot_box(ps, slots, s), THETA)
+            sizes[s] = lv.size + cl.size
     return sizes
 
 
-def _predict(sim, sizes, cap1, cap2):
+def _predict(sim, mode, sizes, cap1, cap2):
     """Routing of the walk's passes, from the exact list lengths: a group leaves
     the first pass when its list exceeds cap1; the second pass gives it up when
     its list exceeds 8x the middle body's own list and min(16384, cap2 / 4)
     entries, or cap2."""
     ps, com, link, ext = _tree(sim)
+    slots, _ = _slots(sim, mode)
     over = set(int(g) for g in np.flatnonzero(sizes > cap1))
     body = set()
     for g in over:
-        last = min(64 * g + 63, sim.n - 1)
-        x = ps[64 * g + (last - 64 * g) // 2, :3]
+        a, c = slots[g]
+        x = ps[a + (c - 1) // 2, :3]
         lv, cl = accepted(com, link, ext, x, x, THETA)
         lb = lv.size + cl.size
         lim = max(8 * lb, min(16384, cap2 // 4)) if lb <= cap2 // 8 else cap2
@@ -142,9 +153,10 @@ def test_walk_matches_interaction_set_fp64(mode, path):
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(mode)
         sim.build_tree_async()
-        sizes = _group_sizes(sim)
-        over, body = _predict(sim, sizes, caps[0], caps[1] or 262144)
-        groups = (sim.n + 63) // 64
+        sizes = _group_sizes(sim, mode)
+        over, body = _predict(sim, mode, sizes, caps[0], caps[1] or 262144)
+        slots, slot_of = _slots(sim, mode)
+        groups = int((slots[:, 1] > 0).sum())
         if path == "per_body":
             assert len(body) == groups
         if path == "second_pass":
@@ -152,7 +164,7 @@ def test_walk_matches_interaction_set_fp64(mode, path):
         st = _run(sim, mode, caps[0], params, caps[1])
         assert st["overflow_groups"] == len(over)
         assert st["body_groups"] == len(body)
-        _check(sim, mode, lambda i: (i // 64) in body, params)
+        _check(sim, mode, lambda i: slot_of[i] in body, params)
         if mode == 2 and path != "per_body":
             # the network's share is well above the 1e-5 bar: the check has teeth
             a2 = sim.accelerations()
@@ -160,6 +172,33 @@ def test_walk_matches_interaction_set_fp64(mode, path):
             a1 = sim.accelerations()
             d = np.linalg.norm(a2 - a1, axis=1) / np.linalg.norm(a1, axis=1)
             assert np.median(d) > 1e-4
+
+
+@pytest.mark.parametrize("mode,ratio", [(0, 0.0), (1, 1.0), (2, 0.0)])
+def test_walk_with_set_group_split(mode, ratio):
+    """gd_walk_set_group_split overrides the opening-angle rule: whole runs
+    (monopole and learned walks) and a split of every run whose halves' boxes
+    are smaller -- each against its interaction sets."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    lib = load_library()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        slots, slot_of = _slots(sim, mode, ratio=ratio)
+        split = int((slots[1::2, 1] > 0).sum())
+        if ratio == 0.0:
+            assert split == 0
+        else:
+            assert split > slots.shape[0] // 20
+        prev = lib.gd_walk_set_group_split(ratio)
+        try:
+            st = _run(sim, mode, 16384, params)
+        finally:
+            lib.gd_walk_set_group_split(prev)
+        assert prev < 0
+        assert st["overflow_groups"] == 0
+        _check(sim, mode, lambda i: False, params, ratio=ratio)
 
 
 def test_walk_counts_match_interaction_sets():
@@ -170,12 +209,14 @@ def test_walk_counts_match_interaction_sets():
         sim.build_tree_async()
         st = _run(sim, 1, 16384, None)
         ps, com, link, ext = _tree(sim)
+        slots, _ = _slots(sim, 1)
         inter = cells = 0
-        for g in range((sim.n + 63) // 64):
-            blk = ps[64 * g:min(64 * g + 64, sim.n), :3]
-            lv, cl = accepted(com, link, ext, blk.min(0), blk.max(0), THETA)
-            inter += (lv.size + cl.size) * blk.shape[0]
-            cells += cl.size * blk.shape[0]
+        for sl in range(slots.shape[0]):
+            c = int(slots[sl, 1])
+            if c:
+                lv, cl = accepted(com, link, ext, *_slot_box(ps, slots, sl), THETA)
+                inter += (lv.size + cl.size) * c
+                cells += cl.size * c
         assert st["overflow_groups"] == 0
         assert st["interactions"] == inter
         assert st["cell_interactions"] == cells
@@ -190,39 +231,43 @@ def test_mixed_lists_and_per_body(mode):
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(mode)
         sim.build_tree_async()
-        sizes = _group_sizes(sim)
-        cap = int(np.median(sizes))
-        over, body = _predict(sim, sizes, cap, 1)
-        groups = (sim.n + 63) // 64
+        sizes = _group_sizes(sim, mode)
+        cap = int(np.median(sizes[sizes > 0]))
+        over, body = _predict(sim, mode, sizes, cap, 1)
+        slots, slot_of = _slots(sim, mode)
+        groups = int((slots[:, 1] > 0).sum())
         assert 0 < len(over) < groups and body == over
         st = _run(sim, mode, cap, params, 1)
         # a group can also leave the list walk when its traversal stack is full
         assert st["overflow_groups"] >= len(over)
         assert st["body_groups"] == st["overflow_groups"]
         if st["overflow_groups"] == len(over):
-            _check(sim, mode, lambda i: (i // 64) in body, params)
+            _check(sim, mode, lambda i: slot_of[i] in body, params)
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_second_pass_and_per_body_together(mode):
-    """Capacity at the median list, second-pass lists of 8192: the compact
-    groups that do not fit go to the second pass, the spread ones (list > 8x
-    their middle body's and > 2048) body by body -- both routes in one walk."""
+    """First-pass capacity at the median list, second-pass capacity at the
+    median of the groups that do not fit: those within it go to the second
+    pass, the others body by body -- both routes in one walk."""
     pos, vel = ics.exponential_disk(N, seed=1234)
     params = _net_params()
     with Simulation(pos, vel, eps=EPS) as sim:
         sim.set_quadrupole(mode)
         sim.build_tree_async()
-        sizes = _group_sizes(sim)
-        cap = int(np.median(sizes))
-        over, body = _predict(sim, sizes, cap, 8192)
+        sizes = _group_sizes(sim, mode)
+        cap = int(np.median(sizes[sizes > 0]))
+        cap2 = int(np.median(sizes[sizes > cap]))
+        over, body = _predict(sim, mode, sizes, cap, cap2)
         assert len(body) >= 1 and len(over - body) >= 1
-        st = _run(sim, mode, cap, params, 8192)
+        slots, slot_of = _slots(sim, mode)
+        st = _run(sim, mode, cap, params, cap2)
         assert st["overflow_groups"] >= len(over)
         if st["overflow_groups"] == len(over):
             assert st["body_groups"] == len(body)
-            spread = [64 * g + k for g in body for k in (0, 31, 63) if 64 * g + k < sim.n]
-            _check(sim, mode, lambda i: (i // 64) in body, params, extra=spread)
+            spread = [int(slots[g, 0]) + k for g in body
+                      for k in (0, int(slots[g, 1] - 1) // 2, int(slots[g, 1]) - 1)]
+            _check(sim, mode, lambda i: slot_of[i] in body, params, extra=spread)
 
 
 def test_packed_lists_follow_the_walk():

@@ -14,7 +14,8 @@ import pytest
 from cudagravitydemo_amd import ics
 from oracle import binding as ob
 from oracle import mlp
-from tests.walk_check import accepted, features, monopole, quadrupole, world_extent
+from tests.walk_check import (accepted, features, group_slots, monopole, quadrupole,
+                               split_ratio, world_extent)
 
 
 @pytest.fixture(scope="module")
@@ -120,3 +121,33 @@ def test_quadrupole_term_against_brute_force():
     X, k = features(x, com, quad, oct_, np.array([0]), 4.0, link)
     want = mlp.features(x, c, m, 1.0, S, O)
     assert np.allclose(X[0], want, rtol=1e-12, atol=1e-15) and abs(k[0] - m) < 1e-12
+
+
+def test_group_slots_partition_and_split_straddling_runs():
+    """Slots cover every body once, in order; a run whose two halves sit in
+    distant clusters is split at the jump between them; a compact run is not."""
+    rng = np.random.default_rng(6)
+    n = 64 * 3 + 17
+    pos = np.zeros((n, 4), np.float32)
+    # run 0: compact; run 1: 40 bodies near (0,0,0) then 24 near (9,9,9); run 2, tail: compact
+    pos[:64, :3] = rng.random((64, 3)) * 0.1
+    pos[64:104, :3] = rng.random((40, 3)) * 0.1 + 1.0
+    pos[104:128, :3] = rng.random((24, 3)) * 0.1 + 9.0
+    pos[128:, :3] = rng.random((n - 128, 3)) * 0.1 + 9.5
+    keys = np.arange(n, dtype=np.uint64) * np.uint64(3)
+    keys[104:] += np.uint64(1) << np.uint64(60)       # the largest jump: between 103 and 104
+    slots, slot_of = group_slots(keys, pos, 0.3)
+    assert slots.shape == (8, 2)
+    covered = np.zeros(n, np.int64)
+    for a, c in slots:
+        covered[a:a + c] += 1
+    assert (covered == 1).all()
+    assert list(slots[2]) == [64, 40] and list(slots[3]) == [104, 24]
+    assert slots[1, 1] == 0 and slots[0, 1] == 64
+    assert slot_of[103] == 2 and slot_of[104] == 3 and slot_of[0] == 0
+    assert slots[7, 1] == 0 and slots[6, 1] == n - 192
+    whole, _ = group_slots(keys, pos, 0.0)
+    assert (whole[1::2, 1] == 0).all()
+    # the walk's fraction: none at small opening angles, at most 0.3
+    assert split_ratio(0.2) == 0 and split_ratio(0.1) == 0
+    assert abs(split_ratio(0.4) - 0.2) < 1e-6 and split_ratio(0.7) == np.float32(0.3)

@@ -121,3 +121,54 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
         r2 = (d * d).sum(1)
         a = a + ((k[ok] / (r2 * r2))[:, None] * o).sum(0)
     return a, leaves.size + cells.size
+
+
+def split_ratio(theta):
+    """The walk's volume fraction for splitting runs (float32, as the host)."""
+    r = f32(theta) - f32(0.2)
+    return f32(0) if r < 0 else (f32(0.3) if r > f32(0.3) else r)
+
+
+def group_slots(keys, pos, ratio):
+    """The walk's groups: each run of 64 consecutive bodies (sorted order) is
+    split in two at its largest key jump (the adjacent pair whose keys differ
+    in the highest bit; the first such pair) when the halves' box volumes add
+    up to less than `ratio` of the run's, restated in float32 with the
+    kernel's operation order.  Returns (slots, slot_of_body): slots is an int array
+    (2 per run) of (first body, bodies), bodies 0 for an empty slot."""
+    keys = np.asarray(keys, np.uint64)
+    p = np.asarray(pos[:, :3], f32)
+    n = keys.shape[0]
+    runs = (n + 63) // 64
+    slots = np.zeros((2 * runs, 2), np.int64)
+    slot_of = np.empty(n, np.int64)
+    for g in range(runs):
+        s0 = 64 * g
+        cnt = min(64, n - s0)
+        j = 0
+        if cnt > 1 and ratio > 0:
+            k = keys[s0:s0 + cnt]
+            x = k[:-1] ^ k[1:]
+            hb = np.array([int(v).bit_length() for v in x])
+            if hb.max() > 0:
+                jj = int(np.argmax(hb)) + 1
+                a, b = p[s0:s0 + jj], p[s0 + jj:s0 + cnt]
+                amin, amax, bmin, bmax = a.min(0), a.max(0), b.min(0), b.max(0)
+                da, db = amax - amin, bmax - bmin
+                va = (da[0] * da[1]) * da[2]
+                vb = (db[0] * db[1]) * db[2]
+                lo, hi = np.minimum(amin, bmin), np.maximum(amax, bmax)
+                d = hi - lo
+                v = (d[0] * d[1]) * d[2]
+                if f32(va + vb) < f32(ratio) * v:
+                    j = jj
+        if j:
+            slots[2 * g] = (s0, j)
+            slots[2 * g + 1] = (s0 + j, cnt - j)
+            slot_of[s0:s0 + j] = 2 * g
+            slot_of[s0 + j:s0 + cnt] = 2 * g + 1
+        else:
+            slots[2 * g] = (s0, cnt)
+            slots[2 * g + 1] = (s0 + cnt, 0)
+            slot_of[s0:s0 + cnt] = 2 * g
+    return slots, slot_of
