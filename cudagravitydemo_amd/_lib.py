@@ -125,7 +125,8 @@ _PROTOS = {
     "gd_mlp_gradient": (f32, [vp, vp, vp, vp, C.c_longlong, vp]),
     "gd_mlp_train_step": (f32, [vp, vp, vp, vp, C.c_longlong, f32]),
     "gd_mlp_adam_state": (None, [vp, vp, vp]),
-    "gd_corrector_samples": (C.c_longlong, [C.POINTER(TreeState), vp, vp, i32, f32, f32, vp, vp,
+    "gd_corrector_samples": (C.c_longlong, [C.POINTER(TreeState), vp, vp, i32, f32, f32, f32,
+                                             vp, vp,
                                              C.c_longlong, C.POINTER(i32)]),
     "gd_walk_learned": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32, vp, f32,
                                C.POINTER(WalkStats)]),

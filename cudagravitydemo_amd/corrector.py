@@ -106,15 +106,19 @@ class SampleSet:
         self.Y = DeviceBuffer(lib, 12 * self.cap)
         self.n = 0
 
-    def add_from(self, sim, targets, theta):
-        """Append the (body, cell) samples of one snapshot; returns rows added."""
+    def add_from(self, sim, targets, theta, min_ratio=0.5):
+        """Append the (body, cell) samples of one snapshot: the cells whose
+        size / distance from the group's centre exceeds min_ratio, the ones
+        walk_learned corrects with the same min_ratio; returns (rows added,
+        targets dropped)."""
         lib = self.lib
         t = to_device(lib, np.ascontiguousarray(targets, np.int32))
         dropped = C.c_int(0)
         room = self.cap - self.n
         try:
             got = lib.gd_corrector_samples(C.byref(sim.state), sim.pos, t.ptr, len(targets),
-                                           theta, sim.eps * sim.eps, self.X.ptr + 32 * self.n,
+                                           theta, min_ratio, sim.eps * sim.eps,
+                                           self.X.ptr + 32 * self.n,
                                            self.Y.ptr + 12 * self.n, room, C.byref(dropped))
             check(lib)
         finally:

@@ -553,14 +553,16 @@ const float* gd_mlp_bias(const gd_mlp* net) { return net->bias; }
 /* One wavefront per target body: the traversal the learned walk makes for
  * the body's group (its 64-body run, or the half of it, split by `ratio` as
  * the learned walk splits it; its bounding box against each node, the same
- * acceptance test); every accepted internal cell
- * becomes a (target, cell) pair, and centre[t] receives the group's centre,
- * where the walk evaluates the network. */
+ * acceptance test); every accepted internal cell whose size / distance from
+ * the group's centre exceeds minRatio -- the cells the learned walk corrects
+ * -- becomes a (target, cell) pair, and centre[t] receives the group's
+ * centre, where the walk evaluates the network. */
 __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     const uint64_t* __restrict__ keys, const float4* __restrict__ pos, int n,
     const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info,
-    const int* __restrict__ targets, int nt, float theta, float ratio, int2* __restrict__ pairs,
+    const int* __restrict__ targets, int nt, float theta, float ratio, float minRatio,
+    int2* __restrict__ pairs,
     long long cap, unsigned long long* __restrict__ count, int* __restrict__ dropped,
     float4* __restrict__ centre) {
     __shared__ int sStack[GD_SL_WAVES][GD_SL_STACK];
@@ -578,8 +580,9 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     const int gcount = j == 0 ? rcount : (second ? rcount - j : j);
     float lx, ly, lz, hx, hy, hz;
     gd_range_box(pos, gstart, gcount, lx, ly, lz, hx, hy, hz);
-    if (lane == 0)
-        centre[t] = make_float4(0.5f * (lx + hx), 0.5f * (ly + hy), 0.5f * (lz + hz), 0.f);
+    const float gx = 0.5f * (lx + hx), gy = 0.5f * (ly + hy), gz = 0.5f * (lz + hz);
+    if (lane == 0) centre[t] = make_float4(gx, gy, gz, 0.f);
+    const float mr2 = minRatio > 0.f ? minRatio * minRatio : 0.f;
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
@@ -595,7 +598,13 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
         }
         __builtin_amdgcn_wave_barrier();
         top = base;
-        const bool cell = pr.accept && !pr.leaf;
+        bool cell = pr.accept && !pr.leaf;
+        if (cell) {
+            const float4 c = wcom[pr.idx];
+            const float size = ldexpf(ext, -(wlink[pr.idx].y & 31));
+            const float dx = c.x - gx, dy = c.y - gy, dz = c.z - gz;
+            cell = size * size > mr2 * (dx * dx + dy * dy + dz * dz);
+        }
         const unsigned long long bal = __ballot(cell);
         const int nacc = __popcll(bal);
         unsigned long long slot0 = 0;
@@ -618,10 +627,10 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     }
 }
 
-/* One wavefront per pair: the 8 features (the walk's: u, s/r, O:uu/(m s^3),
- * u.S u/(m s^2)) and the target (sum over the cell's bodies - softened
- * monopole - analytic quadrupole) / (m s^2 / r^4), in FP64: what the
- * quadrupole cell misses, which the network learns. */
+/* One wavefront per pair: the 8 features (the walk's: u, s/r,
+ * (7.5 O:uu - 1.5 T)/(m s^3), u.O:uu/(m s^3)) and the target (sum over the
+ * cell's bodies - softened monopole - analytic quadrupole) / (m s^3 / r^5),
+ * in FP64: what the quadrupole cell misses, which the network learns. */
 __global__ __launch_bounds__(256) void k_samples_eval(
     const float4* __restrict__ pos, const node* __restrict__ nodes,
     const float4* __restrict__ wcom, const float4* __restrict__ wquad,
@@ -661,7 +670,7 @@ __global__ __launch_bounds__(256) void k_samples_eval(
             const double rc2 = dx * dx + dy * dy + dz * dz;
             const double rs2 = rc2 + eps2;
             const double mono = (double)c.w / (rs2 * sqrt(rs2));
-            const double scale = (double)c.w * size * size / (rc2 * rc2);
+            const double scale = (double)c.w * size * size * size / (rc2 * rc2 * sqrt(rc2));
             /* analytic quadrupole term, as the walk evaluates it (softened r) */
             const double sxx = q0.x, syy = q0.y, szz = q0.z, sxy = q0.w, sxz = q1.x, syz = q1.y;
             const double tr = q1.z;
@@ -682,30 +691,36 @@ __global__ __launch_bounds__(256) void k_samples_eval(
             const double gx = (double)c.x - g.x, gy = (double)c.y - g.y, gz = (double)c.z - g.z;
             const double r = sqrt(gx * gx + gy * gy + gz * gz);
             const double ux = gx / r, uy = gy / r, uz = gz / r;
-            const double k = (double)c.w * size * size;
+            const double k3 = (double)c.w * size * size * size;
             const float4 o0 = woct[3 * idx], o1 = woct[3 * idx + 1], o2 = woct[3 * idx + 2];
             const double xx = ux * ux, yy = uy * uy, zz = uz * uz;
             const double xy2 = 2.0 * ux * uy, xz2 = 2.0 * ux * uz, yz2 = 2.0 * uy * uz;
-            const double k3 = k * size;
+            /* O_ijk u_j u_k and the trace vector T_i = O_ijj */
+            const double ox = o0.x * xx + o1.y * yy + o1.w * zz + o0.w * xy2 + o1.x * xz2 +
+                              o2.y * yz2;
+            const double oy = o0.w * xx + o0.y * yy + o2.x * zz + o1.y * xy2 + o2.y * xz2 +
+                              o1.z * yz2;
+            const double oz = o1.x * xx + o1.z * yy + o0.z * zz + o2.y * xy2 + o1.w * xz2 +
+                              o2.x * yz2;
+            const double tx = (double)o0.x + o1.y + o1.w;
+            const double ty = (double)o0.w + o0.y + o2.x;
+            const double tz = (double)o1.x + o1.z + o0.z;
             X[p * 8 + 0] = (float)ux;
             X[p * 8 + 1] = (float)uy;
             X[p * 8 + 2] = (float)uz;
             X[p * 8 + 3] = (float)(size / r);
-            X[p * 8 + 4] = (float)((o0.x * xx + o1.y * yy + o1.w * zz + o0.w * xy2 + o1.x * xz2 +
-                                    o2.y * yz2) / k3);
-            X[p * 8 + 5] = (float)((o0.w * xx + o0.y * yy + o2.x * zz + o1.y * xy2 + o2.y * xz2 +
-                                    o1.z * yz2) / k3);
-            X[p * 8 + 6] = (float)((o1.x * xx + o1.z * yy + o0.z * zz + o2.y * xy2 + o1.w * xz2 +
-                                    o2.x * yz2) / k3);
-            X[p * 8 + 7] = (float)((sxx * xx + syy * yy + szz * zz + sxy * xy2 + sxz * xz2 +
-                                    syz * yz2) / k);
+            X[p * 8 + 4] = (float)((7.5 * ox - 1.5 * tx) / k3);
+            X[p * 8 + 5] = (float)((7.5 * oy - 1.5 * ty) / k3);
+            X[p * 8 + 6] = (float)((7.5 * oz - 1.5 * tz) / k3);
+            X[p * 8 + 7] = (float)((ux * ox + uy * oy + uz * oz) / k3);
         }
     }
 }
 
 extern "C" long long gd_corrector_samples(const treeState* tree, const float4* pos,
-                                          const int* targets, int nt, float theta, float eps2,
-                                          float* X, float* Y, long long cap, int* droppedTargets) {
+                                          const int* targets, int nt, float theta,
+                                          float minRatio, float eps2, float* X, float* Y,
+                                          long long cap, int* droppedTargets) {
     if (!tree || !tree->builder) {
         GD_FAIL("gd_corrector_samples: uninitialised tree state");
         return 0;
@@ -731,7 +746,7 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
     if (!gd_has_error()) {
         k_samples_list<<<gd_div_up(nt, GD_SL_WAVES), 64 * GD_SL_WAVES>>>(
             tree->mortonOut, pos, tree->starCount, b->wcom, b->wlink, b->dinfo, targets, nt, theta,
-            gd_walk_split_ratio(theta), pairs, cap, count, dropped, centre);
+            gd_walk_split_ratio(theta), minRatio, pairs, cap, count, dropped, centre);
         GD_LAUNCHED();
         unsigned long long total = 0;
         int drop = 0;

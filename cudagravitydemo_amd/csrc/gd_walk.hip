@@ -512,36 +512,64 @@ __device__ __forceinline__ void gd_mlp_lanes(const gd_bf16x4* __restrict__ W,
     __builtin_amdgcn_wave_barrier();
 }
 
+/* The network for the lanes with need set, results compacted: returns their
+ * number n (wave-uniform); for i < n, corr[i] = (k * (ox, oy, oz), the
+ * lane's index as int bits) of the i-th such lane in lane order, with k that
+ * lane's scale.  map: 64 ints of LDS; corr: 64 float4 of LDS. */
+__device__ __forceinline__ int gd_mlp_compact(const gd_bf16x4* __restrict__ W,
+                                             const float* __restrict__ Bv, bool need, int* map,
+                                             const float* f, float k, float4* corr) {
+    const unsigned long long bal = __ballot(need);
+    if (!bal) return 0;   /* wave-uniform */
+    const int m = __popcll(bal);
+    const unsigned lane = gd_lane();
+    if (need) map[__popcll(bal & gd_lanemask_lt())] = (int)lane;
+    __builtin_amdgcn_wave_barrier();
+    float cx, cy, cz;
+    gd_mlp_cells(W, Bv, m, map, f, cx, cy, cz);
+    const int src = (int)lane < m ? map[lane] : 0;
+    const float ks = __shfl(k, src);
+    if ((int)lane < m) corr[lane] = make_float4(ks * cx, ks * cy, ks * cz, __int_as_float(src));
+    __builtin_amdgcn_wave_barrier();
+    return m;
+}
+
 /*
  * The network's inputs for a cell seen from point x: with d = c - x, r = |d|,
- * u = d / r, s the cell's edge, m its mass, S and O its second and third
- * moments about c:
- *   f = [u (3), s / r, O:uu / (m s^3) (3), u.S u / (m s^2)],
- * (O:uu)_i = O_ijk u_j u_k.  q0/q1 hold S (Sxx Syy Szz Sxy, Sxz Syz tr -),
- * o0/o1/o2 hold O (xxx yyy zzz xxy, xxz xyy yyz xzz, yzz xyz - -).
- * k = m s^2, or 0 when the cell is not corrected (invalid, massless, or
- * s / r <= minRatio).
+ * u = d / r, s the cell's edge, m its mass, O its third moments about c,
+ * (O:uu)_i = O_ijk u_j u_k and T_i = O_ijj:
+ *   f = [u (3), s / r, (7.5 O:uu - 1.5 T) / (m s^3) (3), u.O:uu / (m s^3)].
+ * The cell's octupole term is (m s^3 / r^5) (v + u (20 a - 5 u.v)) with
+ * v = f[4..6], a = f[7], so the leading part of what the quadrupole misses
+ * is a linear function of the inputs; the network learns it and the
+ * higher orders that correlate with it.  o0/o1/o2 hold O (xxx yyy zzz xxy,
+ * xxz xyy yyz xzz, yzz xyz - -).  k = m s^3, or 0 when the cell is not
+ * corrected (invalid, massless, or s / r <= minRatio).
  */
-__device__ __forceinline__ void gd_cell_features(float4 c, float4 q0, float4 q1, float4 o0,
-                                                 float4 o1, float4 o2, float size, float x,
-                                                 float y, float z, bool valid, float minRatio,
-                                                 float* f, float& k) {
-    k = c.w * size * size;
+__device__ __forceinline__ void gd_cell_features(float4 c, float4 o0, float4 o1, float4 o2,
+                                                 float size, float x, float y, float z,
+                                                 bool valid, float minRatio, float* f,
+                                                 float& k) {
+    k = c.w * size * size * size;
     const float dx = c.x - x, dy = c.y - y, dz = c.z - z;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float rinv = valid ? __builtin_amdgcn_rsqf(r2) : 0.f;
     const float ux = dx * rinv, uy = dy * rinv, uz = dz * rinv;
     const float ratio = size * rinv;
     const bool use = valid && k > 0.f;
-    const float kinv = use ? __builtin_amdgcn_rcpf(k) : 0.f;
-    const float k3inv = use ? kinv * __builtin_amdgcn_rcpf(size) : 0.f;
+    const float k3inv = use ? __builtin_amdgcn_rcpf(k) : 0.f;
     const float xx = ux * ux, yy = uy * uy, zz = uz * uz;
     const float xy2 = 2.f * ux * uy, xz2 = 2.f * ux * uz, yz2 = 2.f * uy * uz;
+    /* O_ijk u_j u_k and the trace vector T_i = O_ijj */
+    const float ox = o0.x * xx + o1.y * yy + o1.w * zz + o0.w * xy2 + o1.x * xz2 + o2.y * yz2;
+    const float oy = o0.w * xx + o0.y * yy + o2.x * zz + o1.y * xy2 + o2.y * xz2 + o1.z * yz2;
+    const float oz = o1.x * xx + o1.z * yy + o0.z * zz + o2.y * xy2 + o1.w * xz2 + o2.x * yz2;
+    const float tx = o0.x + o1.y + o1.w, ty = o0.w + o0.y + o2.x, tz = o1.x + o1.z + o0.z;
     f[0] = ux; f[1] = uy; f[2] = uz; f[3] = ratio;
-    f[4] = (o0.x * xx + o1.y * yy + o1.w * zz + o0.w * xy2 + o1.x * xz2 + o2.y * yz2) * k3inv;
-    f[5] = (o0.w * xx + o0.y * yy + o2.x * zz + o1.y * xy2 + o2.y * xz2 + o1.z * yz2) * k3inv;
-    f[6] = (o1.x * xx + o1.z * yy + o0.z * zz + o2.y * xy2 + o1.w * xz2 + o2.x * yz2) * k3inv;
-    f[7] = (q0.x * xx + q0.y * yy + q0.z * zz + q0.w * xy2 + q1.x * xz2 + q1.y * yz2) * kinv;
+    f[4] = (7.5f * ox - 1.5f * tx) * k3inv;
+    f[5] = (7.5f * oy - 1.5f * ty) * k3inv;
+    f[6] = (7.5f * oz - 1.5f * tz) * k3inv;
+    f[7] = (ux * ox + uy * oy + uz * oz) * k3inv;
     if (!(use && ratio > minRatio)) k = 0.f;
 }
 
@@ -553,8 +581,12 @@ __device__ __forceinline__ void gd_cell_features(float4 c, float4 q0, float4 q1,
  * sum is added with Kahan compensation.  MODE 0: every entry is a monopole;
  * 1: back entries are quadrupole cells; 2: back entries are quadrupole cells
  * plus the network's residual correction (evaluated per chunk at the group's
- * centre on the matrix cores, scaled by each body's own m s^2 / r^4).
- * sE holds 2 x 64 float4; sQ0, sQ1, sCor 64 each; sMap 64 ints.
+ * centre on the matrix cores, scaled by each body's own m s^3 / r^5).
+ * Only the corrected cells (s / r > minRatio, ~5 % of the cell entries at
+ * theta 0.7) reach the per-body loop of the learned term, compacted into the
+ * second half of sE (unused by the cell chunks), each pointing at its
+ * record in the first half.  sE holds 2 x 64 float4; sQ0, sQ1 64 each;
+ * sMap 64 ints.
  */
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_group(
@@ -564,7 +596,7 @@ __device__ __forceinline__ void gd_eval_group(
     const gd_tree_info* __restrict__ info, const int* __restrict__ list, int cap, int2 cnt,
     long long start, int count, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sE, float4* sQ0, float4* sQ1,
-    float4* sCor, int* sMap, gd_kahan3& tot, float& phs, float& phc,
+    int* sMap, gd_kahan3& tot, float& phs, float& phc,
     unsigned long long* nnEvals) {
     const unsigned lane = gd_lane();
     const bool valid = (int)lane < count;
@@ -622,8 +654,9 @@ __device__ __forceinline__ void gd_eval_group(
         sE[lane] = c;
         sQ0[lane] = q0;
         sQ1[lane] = q1;
+        int nc = 0;
         if (MODE == 2) {
-            /* network at the group centre; lane l gets cell l's residual and m s^2 */
+            /* network at the group centre, on the cells with s / r > minRatio */
             float4 o0 = zero4, o1 = zero4, o2 = zero4;
             float size = 1.f;
             if (idx >= 0) {
@@ -632,26 +665,26 @@ __device__ __forceinline__ void gd_eval_group(
                 o2 = woct[3 * idx + 2];
                 size = ldexpf(ext, -(wlink[idx].y & 31));
             }
-            float f[8], k, ox, oy, oz;
-            gd_cell_features(c, q0, q1, o0, o1, o2, size, gx, gy, gz, idx >= 0, minRatio, f, k);
-            gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
-            *nnEvals += (unsigned long long)__popcll(__ballot(k > 0.f));
-            sCor[lane] = make_float4(ox, oy, oz, k);
+            float f[8], k;
+            gd_cell_features(c, o0, o1, o2, size, gx, gy, gz, idx >= 0, minRatio, f, k);
+            nc = gd_mlp_compact(mlpW, mlpB, k > 0.f, sMap, f, k, sE + 64);
+            *nnEvals += (unsigned long long)nc;
         }
         __builtin_amdgcn_wave_barrier();
         gd_eval64_quad<PHI>(sE, sQ0, sQ1, m, px, py, pz, eps2, ax, ay, az, ph);
         if (MODE == 2) {
-            /* learned residual k / r^4 * o of each cell */
-            for (int kk = 0; kk < m; ++kk) {
-                const float4 cc = sE[kk];
-                const float4 oo = sCor[kk];
+            /* learned residual k / r^5 * o of each corrected cell */
+            for (int kk = 0; kk < nc; ++kk) {
+                const float4 oo = sE[64 + kk];
+                const float4 cc = sE[__float_as_int(oo.w)];
                 const float dx = cc.x - px, dy = cc.y - py, dz = cc.z - pz;
                 const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                const float i2 = __builtin_amdgcn_rcpf(r2);
-                const float s4 = oo.w * (i2 * i2);
-                ax = __builtin_fmaf(s4, oo.x, ax);
-                ay = __builtin_fmaf(s4, oo.y, ay);
-                az = __builtin_fmaf(s4, oo.z, az);
+                const float i1 = __builtin_amdgcn_rsqf(r2);
+                const float i2 = i1 * i1;
+                const float s5 = i2 * i2 * i1;
+                ax = __builtin_fmaf(s5, oo.x, ax);
+                ay = __builtin_fmaf(s5, oo.y, ay);
+                az = __builtin_fmaf(s5, oo.z, az);
             }
         }
         gd_kahan_add(tot.x, tot.cx, ax);
@@ -679,7 +712,6 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     __shared__ float4 sE[GD_EV_WAVES][2 * 64];
     __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
-    __shared__ float4 sCor[MODE == 2 ? GD_EV_WAVES : 1][64];
     __shared__ int sMap[MODE == 2 ? GD_EV_WAVES : 1][64];
     __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
     __shared__ float sB[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
@@ -710,7 +742,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
                              eps2,
                              mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
-                             sCor[MODE == 2 ? wave : 0], sMap[MODE == 2 ? wave : 0], tot, phs,
+                             sMap[MODE == 2 ? wave : 0], tot, phs,
                              phc, &nn);
     if (MODE == 2 && stats && lane == 0) atomicAdd(&stats[4], nn);
     if ((int)lane < rg.y) {
@@ -863,16 +895,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
                     o2 = woct[3 * idx + 2];
                 }
                 float f[8], k, ox, oy, oz;
-                gd_cell_features(c, q0, q1, o0, o1, o2, size, p.x, p.y, p.z, cell, minRatio, f,
-                                 k);
+                gd_cell_features(c, o0, o1, o2, size, p.x, p.y, p.z, cell, minRatio, f, k);
                 gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
                 nNet += (unsigned long long)__popcll(__ballot(k > 0.f));
                 if (cell) {
-                    const float i2 = __builtin_amdgcn_rcpf(d2);
-                    const float s4 = k * (i2 * i2);
-                    ax = __builtin_fmaf(s4, ox, ax);
-                    ay = __builtin_fmaf(s4, oy, ay);
-                    az = __builtin_fmaf(s4, oz, az);
+                    const float i1 = __builtin_amdgcn_rsqf(d2);
+                    const float i2 = i1 * i1;
+                    const float s5 = k * (i2 * i2 * i1);
+                    ax = __builtin_fmaf(s5, ox, ax);
+                    ay = __builtin_fmaf(s5, oy, ay);
+                    az = __builtin_fmaf(s5, oz, az);
                 }
             }
         }

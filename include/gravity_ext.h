@@ -143,11 +143,12 @@ int gd_walk_phase_ms(float* ms);
  * [out][in]): W1[64][8] @0, b1 @512, W2[64][64] @576, b2 @4672,
  * W3[64][64] @4736, b3 @8832, W4[3][64] @8896, b4 @9088 (9091 values).
  * Inputs per (point, cell), with u the unit vector from the point to the
- * cell's COM, r the distance, s the cell's edge, m its mass, S and O its
- * second and third moments about the COM: u (3), s / r, (O_ijk u_j u_k) /
- * (m s^3) (3), u.S u / (m s^2).  Output: the correction of the cell's
- * acceleration beyond its monopole and quadrupole terms, in units of
- * G m s^2 / r^4.  X (rows x 8), Y (rows x 3) and order are device arrays. */
+ * cell's COM, r the distance, s the cell's edge, m its mass, O its third
+ * moments about the COM, w_i = O_ijk u_j u_k and T_i = O_ijj: u (3), s / r,
+ * (7.5 w - 1.5 T) / (m s^3) (3), u.w / (m s^3).  Output: the correction of
+ * the cell's acceleration beyond its monopole and quadrupole terms, in units
+ * of G m s^3 / r^5 (the octupole term is a linear function of the inputs in
+ * these units).  X (rows x 8), Y (rows x 3) and order are device arrays. */
 #define GD_MLP_PARAMS 9091
 typedef struct gd_mlp gd_mlp;
 gd_mlp* gd_mlp_create(const float* hostParams);          /* NULL: all zero */
@@ -166,20 +167,23 @@ float gd_mlp_train_step(gd_mlp* net, const float* X, const float* Y, const int* 
                         long long B, float lr);
 void gd_mlp_adam_state(const gd_mlp* net, float* hostM, float* hostV);
 /* Training samples from the current tree (built with second and third
- * moments, gd_set_quadrupole(tree, 2)): for
- * each target body (device indices, sorted order), every internal cell a
- * per-body traversal at opening angle theta accepts gives one row of X and
- * the exact scaled residual in Y (FP64 sum over the cell's bodies minus the
- * softened monopole and quadrupole terms).  Returns the number of rows
- * written (at most cap); *droppedTargets counts targets whose traversal
- * outgrew its stack. */
+ * moments, gd_set_quadrupole(tree, 2)): for each target body (device
+ * indices, sorted order), every internal cell that the traversal of the
+ * body's group (as the list walks form groups, gd_walk_set_group_split) at
+ * opening angle theta accepts and whose size / distance from the group's
+ * centre exceeds minRatio (the cells gd_walk_learned corrects) gives one row
+ * of X (the features at the group's centre) and the exact scaled residual in
+ * Y (FP64 sum over the cell's bodies minus the softened monopole and
+ * quadrupole terms, from the body).  Returns the number of rows written (at
+ * most cap); *droppedTargets counts targets whose traversal outgrew its
+ * stack. */
 long long gd_corrector_samples(const treeState* tree, const float4* pos, const int* targets,
-                               int nt, float theta, float eps2, float* X, float* Y,
-                               long long cap, int* droppedTargets);
+                               int nt, float theta, float minRatio, float eps2, float* X,
+                               float* Y, long long cap, int* droppedTargets);
 /* gd_walk with quadrupole cells plus the network's residual correction of
  * every accepted cell whose size / r (from the group's centre) exceeds
  * minRatio, evaluated once per (64-body group, cell) on the matrix cores and
- * scaled per body by m s^2 / r^4.  Needs a tree built with second and third
+ * scaled per body by m s^3 / r^5.  Needs a tree built with second and third
  * moments (gd_set_quadrupole(tree, 2)). */
 void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc, float* phi,
                      float eps2, float G, float theta, int starCount, const gd_mlp* net,

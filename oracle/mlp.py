@@ -143,21 +143,32 @@ def moments(bodies, com):
     return S, O
 
 
-def features(x, com, mass, size, S, O):
+def features(x, com, mass, size, O):
     """x: (3,) target point; com, mass, size: cell COM, mass, edge length;
-    S (3,3), O (3,3,3): second and third moments about the COM.  Returns the 8
-    inputs: u = (com - x)/r (3), s/r, O_ijk u_j u_k / (m s^3) (3),
-    u.S u / (m s^2)."""
+    O (3,3,3): third moments about the COM.  Returns the 8 inputs:
+    u = (com - x)/r (3), s/r, v = (7.5 w - 1.5 T) / (m s^3) (3) and
+    a = u.w / (m s^3), with w_i = O_ijk u_j u_k and T_i = O_ijj."""
     d = np.asarray(com, np.float64) - np.asarray(x, np.float64)
     r = np.linalg.norm(d)
     u = d / r
-    k = mass * size * size
-    ouu = np.einsum("ijk,j,k->i", np.asarray(O, np.float64), u, u) / (k * size)
-    return np.r_[u, size / r, ouu, u @ np.asarray(S, np.float64) @ u / k]
+    O = np.asarray(O, np.float64)
+    k3 = mass * size ** 3
+    w = np.einsum("ijk,j,k->i", O, u, u)
+    T = np.einsum("ijj->i", O)
+    return np.r_[u, size / r, (7.5 * w - 1.5 * T) / k3, (u @ w) / k3]
+
+
+def octupole(f):
+    """The cell's octupole term in target() units from its features:
+    v + u (20 a - 5 u.v), the textbook term [7.5 w - 1.5 T - 17.5 (u.w) u +
+    7.5 (T.u) u] / r^5 divided by m s^3 / r^5 (T.u = 5 a - u.v / 1.5)."""
+    f = np.asarray(f, np.float64)
+    u, v, a = f[..., 0:3], f[..., 4:7], f[..., 7:8]
+    return v + u * (20.0 * a - 5.0 * (u * v).sum(-1, keepdims=True))
 
 
 def target(x, bodies, com, mass, size, eps2, G=1.0):
-    """Scaled correction: (exact - softened monopole) / (G m / r^2) / (s/r)^2."""
+    """Scaled correction: (exact - softened monopole) / (G m s^3 / r^5)."""
     x = np.asarray(x, np.float64)
     b = np.asarray(bodies, np.float64)
     d = b[:, :3] - x
@@ -166,7 +177,7 @@ def target(x, bodies, com, mass, size, eps2, G=1.0):
     dc = np.asarray(com, np.float64) - x
     rc2 = dc @ dc
     mono = G * mass * dc / (rc2 + eps2) ** 1.5
-    scale = G * mass / rc2 * (size * size / rc2)
+    scale = G * mass * size ** 3 / rc2 ** 2.5
     return (exact - mono) / scale
 
 
@@ -180,7 +191,7 @@ def quadrupole_correction(x, com, mass, size, S, G=1.0, eps2=0.0):
     sd = S @ d
     q = d @ sd
     a = G * ((7.5 * q / rs2 - 1.5 * np.trace(S)) * d - 3.0 * sd) / rs2 ** 2.5
-    return a / (G * mass / r2 * (size * size / r2))
+    return a / (G * mass * size ** 3 / r2 ** 2.5)
 
 
 def residual_target(x, bodies, com, mass, size, S, eps2, G=1.0):

@@ -81,11 +81,12 @@ def test_adam_step_matches_closed_form():
         net.close()
 
 
-@pytest.mark.parametrize("ratio", [-1.0, 0.0])
-def test_samples_match_direct_sums(ratio):
+@pytest.mark.parametrize("ratio,min_ratio", [(-1.0, 0.5), (0.0, 0.0)])
+def test_samples_match_direct_sums(ratio, min_ratio):
     """ratio -1: the learned walk's default groups at the sampling angle 0.8
-    (runs split at volume fraction 0.3); 0: whole 64-body runs, as
-    gd_walk_set_group_split(0) makes the walk keep them."""
+    (runs split at volume fraction 0.3), cells with s/r > 0.5 only; 0: whole
+    64-body runs, as gd_walk_set_group_split(0) makes the walk keep them, and
+    every accepted cell."""
     pos, vel = ics.exponential_disk(8192, seed=13)
     with Simulation(pos, vel, eps=0.01) as sim:
         sim.set_quadrupole(2)
@@ -94,12 +95,15 @@ def test_samples_match_direct_sums(ratio):
         s = cr.SampleSet(sim.lib, 200000)
         prev = sim.lib.gd_walk_set_group_split(ratio)
         try:
-            n, dropped = s.add_from(sim, targets, 0.8)
+            n, dropped = s.add_from(sim, targets, 0.8, min_ratio)
             assert n > 64 and dropped == 0
             X, Y = s.host()
         finally:
             sim.lib.gd_walk_set_group_split(prev)
             s.close()
+        assert (X[:, 3] > min_ratio * (1 - 1e-6)).all()
+        if min_ratio == 0:
+            assert (X[:, 3] < 0.5).mean() > 0.3   # the far cells are there too
         # recompute a few rows from the tree with the oracle's definitions
         total = sim.tree_summary()[0]
         com, link, quad = sim.walk_arrays(total)
@@ -133,7 +137,7 @@ def test_samples_match_direct_sums(ratio):
                                   [quad[idx, 3], quad[idx, 1], quad[idx, 5]],
                                   [quad[idx, 4], quad[idx, 5], quad[idx, 2]]])
                     O = mlp.third_tensor(oct_[idx])
-                    f = mlp.features(g, com[idx, :3], com[idx, 3], size, S, O)
+                    f = mlp.features(g, com[idx, :3], com[idx, 3], size, O)
                     if np.abs(f - X[row]).max() < 1e-4 * max(1, np.abs(f).max()):
                         cands.append((x, idx, size, S))
             assert cands, row

@@ -31,7 +31,7 @@ def _net_params():
     p = mlp.init_params(21).astype(np.float64)
     layers = mlp.unpack(p)
     W4, b4 = layers[3]
-    layers[3] = (W4 * 1e-2, b4)
+    layers[3] = (W4 * 5e-2, b4)
     return mlp.pack(layers).astype(np.float32)
 
 

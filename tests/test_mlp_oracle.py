@@ -83,7 +83,7 @@ def test_target_approaches_quadrupole_far_away():
     assert np.linalg.norm(r) <= 0.03 * np.linalg.norm(t)
     S2, O = mlp.moments(bodies, com)
     assert np.allclose(S2, S, rtol=1e-12, atol=1e-15)
-    f = mlp.features(x, com, m, size, S, O)
+    f = mlp.features(x, com, m, size, O)
     assert f.shape == (8,) and abs(np.linalg.norm(f[:3]) - 1) < 1e-12
     assert abs(f[3] - size / np.linalg.norm(com - x)) < 1e-12
 
@@ -130,16 +130,19 @@ def test_third_moments_shift_and_octupole_term():
         tr_u = np.einsum("ijj->i", O) @ u   # O_ijj u_i
         trv = np.einsum("ijj->i", O)        # O_ijj
         oct = (7.5 * ouu - 1.5 * trv - 17.5 * ouuu * u + 7.5 * tr_u * u) / r ** 5
-        scale = m / r ** 2 * (1.0 / r ** 2)
+        scale = m / r ** 5     # m s^3 / r^5 with s = 1
         left[dist] = np.linalg.norm(res - oct / scale) / np.linalg.norm(res)
+        # the features carry the whole octupole term
+        f = mlp.features(x, c, m, 1.0, O)
+        assert np.allclose(mlp.octupole(f), oct / scale, rtol=1e-10, atol=1e-13)
     # 95 % of the residual is the octupole, and what is left is O(s/r) of it
     assert left[6.0] < 0.08 and left[12.0] < 0.6 * left[6.0], left
 
 
 def test_features_equal_sums_over_bodies():
     """The moment features, restated as sums over the cell's bodies instead of
-    tensor contractions: (O:uu)_i = sum m (d.u)^2 d_i, u.S u = sum m (d.u)^2,
-    with d = body - com."""
+    tensor contractions: (O:uu)_i = sum m (d.u)^2 d_i, T_i = sum m |d|^2 d_i,
+    u.O:uu = sum m (d.u)^3, with d = body - com."""
     rng = np.random.default_rng(4)
     b = np.c_[rng.random((150, 3)) * [0.8, 0.5, 0.3], rng.random(150) + 0.1]
     m = b[:, 3].sum()
@@ -147,12 +150,14 @@ def test_features_equal_sums_over_bodies():
     S, O = mlp.moments(b, c)
     size = 0.9
     x = c + np.array([3.0, -2.0, 1.5])
-    f = mlp.features(x, c, m, size, S, O)
+    f = mlp.features(x, c, m, size, O)
     u = (c - x) / np.linalg.norm(c - x)
     d = b[:, :3] - c
     du = d @ u
     ouu = (b[:, 3, None] * (du ** 2)[:, None] * d).sum(0)
-    usu = (b[:, 3] * du ** 2).sum()
+    tv = (b[:, 3, None] * (d * d).sum(1)[:, None] * d).sum(0)
+    uuu = (b[:, 3] * du ** 3).sum()
     assert np.allclose(f[:3], u, rtol=1e-14, atol=1e-15)
-    assert np.allclose(f[4:7] * m * size ** 3, ouu, rtol=1e-12, atol=1e-15)
-    assert abs(f[7] * m * size ** 2 - usu) <= 1e-12 * abs(usu)
+    assert abs(f[3] - size / np.linalg.norm(c - x)) < 1e-14
+    assert np.allclose(f[4:7] * m * size ** 3, 7.5 * ouu - 1.5 * tv, rtol=1e-12, atol=1e-14)
+    assert abs(f[7] * m * size ** 3 - uuu) <= 1e-12 * abs(uuu)

@@ -118,8 +118,8 @@ def test_quadrupole_term_against_brute_force():
     oct_ = np.array([[O[0, 0, 0], O[1, 1, 1], O[2, 2, 2], O[0, 0, 1], O[0, 0, 2], O[0, 1, 1],
                       O[1, 1, 2], O[0, 2, 2], O[1, 2, 2], O[0, 1, 2], 0, 0]])
     link = np.array([[5, (3 << 5) | 2]], np.int32)
-    X, k = features(x, com, quad, oct_, np.array([0]), 4.0, link)
-    want = mlp.features(x, c, m, 1.0, S, O)
+    X, k = features(x, com, oct_, np.array([0]), 4.0, link)
+    want = mlp.features(x, c, m, 1.0, O)
     assert np.allclose(X[0], want, rtol=1e-12, atol=1e-15) and abs(k[0] - m) < 1e-12
 
 

@@ -76,26 +76,26 @@ def quadrupole(x, com, quad, idx, eps2):
     return (cd[:, None] * d - 3.0 * inv5[:, None] * sd).sum(0)
 
 
-def features(x, com, quad, oct_, idx, extent, link):
-    """The network's 8 inputs for cells idx seen from point x, and k = m s^2:
-    u (3), s/r, O_ijk u_j u_k / (m s^3) (3), u.S u / (m s^2)."""
+def features(x, com, oct_, idx, extent, link):
+    """The network's 8 inputs for cells idx seen from point x, and k = m s^3:
+    u (3), s/r, (7.5 w - 1.5 T) / (m s^3) (3), u.w / (m s^3), with
+    w_i = O_ijk u_j u_k and T_i = O_ijj, from the 10 stored components."""
     d = com[idx, :3].astype(np.float64) - x
     r = np.sqrt((d * d).sum(1))
     u = d / r[:, None]
     size = np.ldexp(np.float64(extent), -(link[idx, 1] & 31))
     m = com[idx, 3].astype(np.float64)
-    k = m * size * size
-    q = quad[idx].astype(np.float64)
+    k = m * size ** 3
     o = oct_[idx].astype(np.float64)
     xxx, yyy, zzz, xxy, xxz, xyy, yyz, xzz, yzz, xyz = (o[:, j] for j in range(10))
     ux, uy, uz = u[:, 0], u[:, 1], u[:, 2]
     xx, yy, zz, xy2, xz2, yz2 = ux * ux, uy * uy, uz * uz, 2 * ux * uy, 2 * ux * uz, 2 * uy * uz
-    ouu = np.stack([xxx * xx + xyy * yy + xzz * zz + xxy * xy2 + xxz * xz2 + xyz * yz2,
-                    xxy * xx + yyy * yy + yzz * zz + xyy * xy2 + xyz * xz2 + yyz * yz2,
-                    xxz * xx + yyz * yy + zzz * zz + xyz * xy2 + xzz * xz2 + yzz * yz2], 1)
-    usu = (q[:, 0] * xx + q[:, 1] * yy + q[:, 2] * zz + q[:, 3] * xy2 + q[:, 4] * xz2
-           + q[:, 5] * yz2)
-    X = np.concatenate([u, (size / r)[:, None], ouu / (k * size)[:, None], (usu / k)[:, None]], 1)
+    w = np.stack([xxx * xx + xyy * yy + xzz * zz + xxy * xy2 + xxz * xz2 + xyz * yz2,
+                  xxy * xx + yyy * yy + yzz * zz + xyy * xy2 + xyz * xz2 + yyz * yz2,
+                  xxz * xx + yyz * yy + zzz * zz + xyz * xy2 + xzz * xz2 + yzz * yz2], 1)
+    T = np.stack([xxx + xyy + xzz, xxy + yyy + yzz, xxz + yyz + zzz], 1)
+    X = np.concatenate([u, (size / r)[:, None], (7.5 * w - 1.5 * T) / k[:, None],
+                        ((u * w).sum(1) / k)[:, None]], 1)
     return X, k
 
 
@@ -104,7 +104,7 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
     """Acceleration (without G) at point x for the interaction set of `box`.
     mode 0: all accepted nodes as monopoles; 1: cells as quadrupoles;
     2: as 1 plus the network's residual, evaluated at the box centre, scaled by
-    m s^2 / r^4 with r from x (net: a callable (rows, 8) -> (rows, 3); oct_:
+    m s^3 / r^5 with r from x (net: a callable (rows, 8) -> (rows, 3); oct_:
     the third moments, (nodes, 10+))."""
     lo, hi = box
     leaves, cells = accepted(com, link, extent, lo, hi, theta)
@@ -114,12 +114,12 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
     a = monopole(x, com, leaves, eps2) + quadrupole(x, com, quad, cells, eps2)
     if mode == 2 and cells.size:
         ctr = 0.5 * (np.asarray(lo, np.float64) + np.asarray(hi, np.float64))
-        X, k = features(ctr, com, quad, oct_, cells, extent, link)
+        X, k = features(ctr, com, oct_, cells, extent, link)
         ok = (k > 0) & (X[:, 3] > min_ratio)
         o = net(X[ok])
         d = com[cells[ok], :3].astype(np.float64) - x
         r2 = (d * d).sum(1)
-        a = a + ((k[ok] / (r2 * r2))[:, None] * o).sum(0)
+        a = a + ((k[ok] / (r2 * r2 * np.sqrt(r2)))[:, None] * o).sum(0)
     return a, leaves.size + cells.size
 
 
