@@ -221,7 +221,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
                                            sStack[wave], STACK, list, size, limit, &nf, &nb,
                                            &nOpen, &depth);
     if (gd_lane() == 0) {
-        if (maxDepth) atomicMax(maxDepth, depth);
+        if (maxDepth) {
+            /* [0]: the deepest stack, [1]: groups deeper than 7/8 of GD_TR_STACK
+               (or overflowing it) */
+            atomicMax(maxDepth, depth);
+            if (depth > GD_TR_STACK - GD_TR_STACK / 8) atomicAdd(maxDepth + 1, 1);
+        }
         if (ok) {
             counts[s] = make_int2(nf, nb);
             if (sizes) sizes[g] = nf + nb;
@@ -945,7 +950,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
  * time, in batches.  Third: the per-body walk.
  */
 struct gd_walk_status {
-    int over1, over2, maxList1, maxList2, over3, maxDepth1;
+    int over1, over2, maxList1, maxList2, over3, maxDepth1, deep1;
     unsigned long long requested;
 };
 
@@ -1080,6 +1085,7 @@ __global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st) {
     if (threadIdx.x < 5) stats[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) {
         st->over1 = st->over2 = st->maxList1 = st->maxList2 = st->over3 = st->maxDepth1 = 0;
+        st->deep1 = 0;
         st->requested = 0ull;
     }
 }
@@ -1119,11 +1125,12 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
                                     g_walk.blockSums, g_walk.offsets, g_walk.caps,
                                     g_walk.order, &st->requested);
     GD_LAUNCHED();
-    /* The first pass's LDS stack: 2048 entries per wavefront, or 4096 (half
-       the occupancy) while the deepest traversal of the last walk needed more
-       than 7/8 of 2048 -- small opening angles on deep trees; a group that
-       overflows either goes to the second pass, which walks one group per
-       wavefront at a fraction of the first pass's rate. */
+    /* The first pass's LDS stack: 2048 entries per wavefront (5 wavefronts
+       per SIMD), or 4096 (2 per SIMD) while more than 1/16 of the runs needed
+       more than 7/8 of 2048 in the last walk -- small opening angles on deep
+       trees.  A group that overflows either goes to the second pass, which
+       walks one group per wavefront at a fraction of the first pass's rate:
+       worth it for a few groups, not for a large share. */
 #define GD_PASS1(STK)                                                                          \
     k_walk_traverse<SPLIT, GD_TR_WAVES, STK><<<gd_div_up(groups, GD_TR_WAVES),                  \
                                                64 * GD_TR_WAVES>>>(                             \
@@ -1148,7 +1155,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     overOut[0] = over1 + h.over2;
     overOut[1] = h.maxList1;
     *requested = h.requested;
-    g_walk.deepStack = h.maxDepth1 > GD_TR_STACK - GD_TR_STACK / 8;
+    g_walk.deepStack = 16ll * h.deep1 > runs;
     /* pass 2: the groups that did not fit, with long lists and a deep stack */
     const int cap2 = g_walk.entries2;
     const long long fit = GD_POOL_BYTES / ((long long)cap2 * sizeof(int));

@@ -39,6 +39,8 @@ def main():
                     help="fixed first-pass list capacity per group (0: adaptive)")
     ap.add_argument("--group-split", type=float, default=-1.0,
                     help="volume fraction for splitting 64-body groups (< 0: by theta)")
+    ap.add_argument("--phases", action="store_true",
+                    help="time the walk's phases with HIP events (last step's walk)")
     ap.add_argument("--copy-mib", type=int, default=0)
     a = ap.parse_args()
     pos, vel = ics.disk_merger(a.per_disk, seed=2024)
@@ -71,6 +73,19 @@ def main():
         t1 = time.perf_counter()
         print(f"profile_step: {n} bodies, {a.steps} steps, walk={a.walk}, theta={a.theta}: "
               f"{1e3 * (t1 - t0) / a.steps:.3f} ms/step", flush=True)
+        if a.phases and a.walk != "reference":
+            import ctypes as C
+            sim.lib.gd_walk_timing(1)
+            sim.build_tree_async()
+            if a.walk == "learned":
+                cr.walk_learned(sim, net, a.theta)
+            else:
+                sim.walk(a.theta)
+            ms = (C.c_float * 3)()
+            sim.lib.gd_walk_phase_ms(C.cast(ms, C.c_void_p))
+            sim.lib.gd_walk_timing(0)
+            print(f"profile_step: walk phases: traversal {ms[0]:.3f} ms, evaluation {ms[1]:.3f} "
+                  f"ms, second pass and per-body walk {ms[2]:.3f} ms", flush=True)
         if a.copy_mib:
             lib = sim.lib
             nbytes = a.copy_mib << 20
