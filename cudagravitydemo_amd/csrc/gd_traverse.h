@@ -134,31 +134,52 @@ __device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int
  * SPLIT) unless list is NULL (counting only), and returns false as soon as
  * more than limit (<= cap) entries are accepted or the stack (stackCap
  * entries) would overflow.  *opened counts the opened nodes; *depth receives
- * the deepest stack reached (on an overflow, the depth that did not fit).  Each step pops
- * up to 128 entries, two per lane, so that two node loads per lane are in
- * flight; entries are handled in stack order (the upper 64 first). */
+ * the deepest stack reached (on an overflow, the depth that did not fit).
+ *
+ * The stack holds ranges of sibling nodes, one per opened node: its first
+ * child and child count (1-8) packed as first << 3 | (count - 1), so a node
+ * index must stay below 2^29 (a larger one fails the traversal; the caller's
+ * later passes walk the group).  A range takes one LDS word where its nodes
+ * would take up to eight, which keeps the stack small enough for eight
+ * wavefronts per SIMD.  Each step pops up to 64 ranges from the top -- as
+ * many as hold at most 128 nodes -- and expands them into pop (128 ints of
+ * LDS); lane l then tests nodes l and 64 + l, so that two node loads per
+ * lane are in flight.  Children of the upper nodes go on top: the walk stays
+ * depth-first and the stack shallow. */
 template <bool SPLIT>
 __device__ __forceinline__ bool gd_traverse_box(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
     const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
-    float hy, float hz, int* stack, int stackCap, int* __restrict__ list, int cap, int limit,
-    int* nFrontOut, int* nBackOut, unsigned long long* opened, int* depth) {
+    float hy, float hz, int* stack, int stackCap, int* pop, int* __restrict__ list, int cap,
+    int limit, int* nFrontOut, int* nBackOut, unsigned long long* opened, int* depth) {
     const unsigned lane = gd_lane();
     const float ext = info->box.extent;
     const float th2 = theta * theta;
     const unsigned long long lt = gd_lanemask_lt();
     int nFront = 0, nBack = 0, top = 1, deepest = 1;
     unsigned long long nOpen = 0;
-    if (lane == 0) stack[0] = 0;
+    if (lane == 0) stack[0] = 0;   /* the root: range (0, 1) */
     while (top > 0) {
-        const int take = top < 128 ? top : 128;
-        const int base = top - take;
-        /* A: entries top-64 .. top-1, B: the 64 below them */
-        const int ea = take - 64 + (int)lane;   /* offset from base */
-        const int eb = take - 128 + (int)lane;
-        const bool hasA = ea >= 0, hasB = eb >= 0;
-        const int ia = hasA ? stack[base + ea] : 0;
-        const int ib = hasB ? stack[base + eb] : 0;
+        /* pop: the top ranges (lane l reads entry top-1-l), as many as hold
+           <= 128 nodes -- at least one, as a range holds at most 8 */
+        const int avail = top < 64 ? top : 64;
+        const bool has = (int)lane < avail;
+        const unsigned rg = has ? (unsigned)stack[top - 1 - (int)lane] : 0u;
+        const int cnt = has ? (int)(rg & 7u) + 1 : 0;
+        int ex, all;
+        gd_wave_scan16(cnt, lt, ex, all);
+        const bool take = has && ex + cnt <= 128;
+        const int nr = __popcll(__ballot(take));
+        const int nn = __shfl(ex + cnt, nr - 1);
+        if (take) {
+            const int f = (int)(rg >> 3);
+            for (int k = 0; k < cnt; ++k) pop[ex + k] = f + k;
+        }
+        __builtin_amdgcn_wave_barrier();
+        top -= nr;
+        const bool hasA = (int)lane < nn, hasB = 64 + (int)lane < nn;
+        const int ia = hasA ? pop[lane] : 0;
+        const int ib = hasB ? pop[64 + lane] : 0;
         float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;
         int2 la = make_int2(0, 0), lb = la;
         if (hasA) { ca = wcom[ia]; la = wlink[ia]; }
@@ -168,21 +189,21 @@ __device__ __forceinline__ bool gd_traverse_box(
         if (!hasA) A.accept = A.open = false, A.nk = 0;
         if (!hasB) B.accept = B.open = false, B.nk = 0;
         __builtin_amdgcn_wave_barrier();
-        top = base;
         const bool fA = SPLIT ? (A.accept && A.leaf) : A.accept;
         const bool fB = SPLIT ? (B.accept && B.leaf) : B.accept;
         const bool kA = SPLIT && A.accept && !A.leaf;
         const bool kB = SPLIT && B.accept && !B.leaf;
         const unsigned long long fba = __ballot(fA), fbb = __ballot(fB);
         const unsigned long long bba = __ballot(kA), bbb = __ballot(kB);
+        const unsigned long long oba = __ballot(A.open), obb = __ballot(B.open);
         const int nfa = __popcll(fba), nfb = __popcll(fbb);
         const int nba = __popcll(bba), nbb = __popcll(bbb);
-        int exA, totA, exB, totB;
-        gd_wave_scan16(A.nk, lt, exA, totA);
-        gd_wave_scan16(B.nk, lt, exB, totB);
-        nOpen += (unsigned long long)(__popcll(__ballot(A.open)) + __popcll(__ballot(B.open)));
-        deepest = max(deepest, top + totA + totB);
-        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + totA + totB > stackCap) {
+        const int noa = __popcll(oba), nob = __popcll(obb);
+        nOpen += (unsigned long long)(noa + nob);
+        deepest = max(deepest, top + noa + nob);
+        const bool wide = (A.open && A.first >= (1 << 29)) || (B.open && B.first >= (1 << 29));
+        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + noa + nob > stackCap ||
+            __ballot(wide)) {
             *depth = deepest;
             return false;
         }
@@ -194,13 +215,13 @@ __device__ __forceinline__ bool gd_traverse_box(
         }
         nFront += nfa + nfb;
         nBack += nba + nbb;
-        /* children of the upper entries go on top: the walk stays depth-first
-           and the stack shallow */
+        /* children of the upper nodes (A) go on top */
         if (B.open)
-            for (int k = 0; k < B.nk; ++k) stack[top + exB + k] = B.first + k;
+            stack[top + __popcll(obb & lt)] = (int)(((unsigned)B.first << 3) | (unsigned)(B.nk - 1));
         if (A.open)
-            for (int k = 0; k < A.nk; ++k) stack[top + totB + exA + k] = A.first + k;
-        top += totA + totB;
+            stack[top + nob + __popcll(oba & lt)] =
+                (int)(((unsigned)A.first << 3) | (unsigned)(A.nk - 1));
+        top += noa + nob;
         __builtin_amdgcn_wave_barrier();
     }
     *nFrontOut = nFront;

@@ -153,11 +153,25 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
  * stack) and is left to the per-body walk.
  * ------------------------------------------------------------------------- */
 #ifndef GD_TR_STACK
-#define GD_TR_STACK 2048          /* LDS stack entries per wavefront, first pass */
+#define GD_TR_STACK 1024          /* LDS stack ranges per wavefront, first pass */
 #endif
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
-#define GD_TR2_STACK 12288        /* second pass, one wavefront per block (48 KB of LDS) */
+#define GD_TR2_STACK 4096         /* second pass, one wavefront per block (16 KB of LDS) */
+#define GD_POP 128                /* LDS pop buffer per traversing wavefront (nodes) */
+#define GD_BODY_STACK1 2048       /* per-body walk, first try (nodes) */
+
+/* Walk statistics: 64 copies of the counters, 128 bytes apart, picked by
+ * block index, so that a launch's per-group atomics spread over 64 cache
+ * lines instead of queueing on one; the host sums them.  [0] interactions,
+ * [1] opened nodes, [2] per-body stack overflow flag, [3] cell interactions,
+ * [4] network evaluations. */
+#define GD_STAT_SLOTS 64
+#define GD_STAT_STRIDE 16
+#define GD_STAT_WORDS (GD_STAT_SLOTS * GD_STAT_STRIDE)
+__device__ __forceinline__ unsigned long long* gd_stat_slot(unsigned long long* stats) {
+    return stats + (blockIdx.x & (GD_STAT_SLOTS - 1)) * GD_STAT_STRIDE;
+}
 #define GD_LIST_MIN 16384         /* first-pass room of any group (entries) */
 #define GD_SPREAD (-2)            /* sizes[] mark: the group is walked body by body */
 
@@ -182,10 +196,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const long long* __restrict__ offsets,
     const int* __restrict__ caps, int cap, const int* __restrict__ slotCap,
     int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ sizes,
-    int* __restrict__ overflowList, int* __restrict__ overflowCount, int* __restrict__ maxList,
-    int* __restrict__ spreadList, int* __restrict__ spreadCount, int* __restrict__ maxDepth,
+    int* __restrict__ overflowList, int* __restrict__ overflowCount,
+    int* __restrict__ spreadList, int* __restrict__ spreadCount, int* __restrict__ deepCount,
     unsigned long long* __restrict__ stats) {
-    __shared__ int sStack[WAVES][STACK];
+    __shared__ int sStack[WAVES][STACK + GD_POP];
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
     const long long s = (long long)gd_xcd_block() * WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
@@ -218,24 +232,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const int limit = slotCap ? min(slotCap[s], size) : size;
     int depth = 0;
     const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, lx, ly, lz, hx, hy, hz,
-                                           sStack[wave], STACK, list, size, limit, &nf, &nb,
+                                           sStack[wave], STACK, sStack[wave] + STACK, list, size,
+                                           limit, &nf, &nb,
                                            &nOpen, &depth);
     if (gd_lane() == 0) {
-        if (maxDepth) {
-            /* [0]: the deepest stack, [1]: groups deeper than 7/8 of GD_TR_STACK
-               (or overflowing it) */
-            atomicMax(maxDepth, depth);
-            if (depth > GD_TR_STACK - GD_TR_STACK / 8) atomicAdd(maxDepth + 1, 1);
-        }
+        /* groups deeper than 7/8 of GD_TR_STACK (or overflowing it) */
+        if (deepCount && depth > GD_TR_STACK - GD_TR_STACK / 8) atomicAdd(deepCount, 1);
         if (ok) {
             counts[s] = make_int2(nf, nb);
             if (sizes) sizes[g] = nf + nb;
-            atomicMax(maxList, nf + nb);
             if (stats) {
                 const int nvalid = rg.y;
-                atomicAdd(&stats[0], (unsigned long long)(nf + nb) * (unsigned long long)nvalid);
-                atomicAdd(&stats[1], nOpen);
-                atomicAdd(&stats[3], (unsigned long long)nb * (unsigned long long)nvalid);
+                unsigned long long* sl = gd_stat_slot(stats);
+                atomicAdd(&sl[0], (unsigned long long)(nf + nb) * (unsigned long long)nvalid);
+                atomicAdd(&sl[1], nOpen);
+                atomicAdd(&sl[3], (unsigned long long)nb * (unsigned long long)nvalid);
             }
         } else {
             counts[s] = make_int2(-1, -1);
@@ -383,7 +394,7 @@ __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList, int slots, int cap2,
     int floor, int* __restrict__ slotCap) {
-    __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK];
+    __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK + GD_POP];
     const int wave = (int)(threadIdx.x >> 6);
     const long long s = (long long)blockIdx.x * GD_TR_WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
@@ -393,7 +404,8 @@ __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     unsigned long long nOpen = 0;
     const int bound = cap2 / GD_BODY_RATIO;
     const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, p.x, p.y, p.z, p.x, p.y,
-                                           p.z, sStack[wave], GD_TR_STACK, nullptr, bound,
+                                           p.z, sStack[wave], GD_TR_STACK,
+                                           sStack[wave] + GD_TR_STACK, nullptr, bound,
                                            bound, &nf, &nb, &nOpen, &depth);
     if (gd_lane() == 0) {
         long long lim = ok ? (long long)GD_BODY_RATIO * (nf + nb) : cap2;
@@ -749,7 +761,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
                              sMap[MODE == 2 ? wave : 0], tot, phs,
                              phc, &nn);
-    if (MODE == 2 && stats && lane == 0) atomicAdd(&stats[4], nn);
+    if (MODE == 2 && stats && lane == 0 && nn) atomicAdd(&gd_stat_slot(stats)[4], nn);
     if ((int)lane < rg.y) {
         const long long i = rg.x + lane;
         acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
@@ -853,7 +865,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
         if (top + total > STACK) {
             if (lane == 0) {
                 if (overflowBodies) overflowBodies[atomicAdd(overflowCount, 1)] = (int)i;
-                else atomicOr(&stats[2], 1ull);   /* reported as an error by the host */
+                else atomicOr(&gd_stat_slot(stats)[2], 1ull);   /* reported by the host */
             }
             return;
         }
@@ -932,10 +944,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
         acc[i] = make_float3(G * sx, G * sy, G * sz);
         if (PHI) phi[i] = -G * sp;
         if (stats) {
-            atomicAdd(&stats[0], nInter);
-            atomicAdd(&stats[1], nOpen);
-            atomicAdd(&stats[3], nCell);
-            if (MODE == 2) atomicAdd(&stats[4], nNet);
+            unsigned long long* sl = gd_stat_slot(stats);
+            atomicAdd(&sl[0], nInter);
+            atomicAdd(&sl[1], nOpen);
+            atomicAdd(&sl[3], nCell);
+            if (MODE == 2) atomicAdd(&sl[4], nNet);
         }
     }
 }
@@ -950,7 +963,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
  * time, in batches.  Third: the per-body walk.
  */
 struct gd_walk_status {
-    int over1, over2, maxList1, maxList2, over3, maxDepth1, deep1;
+    int over1, over2, over3, deep1;
     unsigned long long requested;
 };
 
@@ -979,6 +992,7 @@ struct gd_walk_scratch {
     int entries2;          /* second-pass list capacity */
     long long walks;       /* walks so far (spread marks are cleared every 32) */
     int deepStack;         /* first pass with 2 x GD_TR_STACK stack entries */
+    unsigned long long* kstats;   /* the list passes' statistics, or NULL (not asked for) */
 };
 
 #define GD_LIST2 262144
@@ -987,7 +1001,7 @@ static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nu
                                  nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                                  nullptr, 0,
                                  nullptr, nullptr, 0,       nullptr, 0,       0,
-                                 GD_LIST2, 0,       0};
+                                 GD_LIST2, 0,       0,       nullptr};
 
 /* optional HIP-event timing of the walk's phases (null stream) */
 static int g_walkTiming = 0;
@@ -1001,7 +1015,7 @@ __global__ void k_walk_fill(int* a, int n, int v) {
 
 static void gd_walk_ensure(int groups) {
     if (!g_walk.stats) {
-        GD_HIP(hipMalloc(&g_walk.stats, 5 * sizeof(unsigned long long)));
+        GD_HIP(hipMalloc(&g_walk.stats, GD_STAT_WORDS * sizeof(unsigned long long)));
         GD_HIP(hipMalloc(&g_walk.status, sizeof(gd_walk_status)));
     }
     if (groups > g_walk.groupCap) {
@@ -1082,10 +1096,9 @@ __global__ __launch_bounds__(256) void k_walk_split(const uint64_t* __restrict__
 }
 
 __global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st) {
-    if (threadIdx.x < 5) stats[threadIdx.x] = 0ull;
+    for (int i = threadIdx.x; i < GD_STAT_WORDS; i += blockDim.x) stats[i] = 0ull;
     if (threadIdx.x == 0) {
-        st->over1 = st->over2 = st->maxList1 = st->maxList2 = st->over3 = st->maxDepth1 = 0;
-        st->deep1 = 0;
+        st->over1 = st->over2 = st->over3 = st->deep1 = 0;
         st->requested = 0ull;
     }
 }
@@ -1098,7 +1111,7 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            const float* Bv, float minRatio) {
     k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
         pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, groupList, slots,
-        lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.stats);
+        lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.kstats);
 }
 
 template <int MODE, bool PHI>
@@ -1125,9 +1138,9 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
                                     g_walk.blockSums, g_walk.offsets, g_walk.caps,
                                     g_walk.order, &st->requested);
     GD_LAUNCHED();
-    /* The first pass's LDS stack: 2048 entries per wavefront (5 wavefronts
-       per SIMD), or 4096 (2 per SIMD) while more than 1/16 of the runs needed
-       more than 7/8 of 2048 in the last walk -- small opening angles on deep
+    /* The first pass's LDS stack: 1024 ranges per wavefront (8 wavefronts
+       per SIMD), or 2048 (4 per SIMD) while more than 1/16 of the runs needed
+       more than 7/8 of 1024 in the last walk -- small opening angles on deep
        trees.  A group that overflows either goes to the second pass, which
        walks one group per wavefront at a fraction of the first pass's rate:
        worth it for a few groups, not for a large share. */
@@ -1137,8 +1150,8 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.order, groups,    \
         g_walk.offsets,                                                                        \
         g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,  \
-        &st->over1, &st->maxList1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2,           \
-        &st->over2, &st->maxDepth1, g_walk.stats)
+        &st->over1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2, &st->over2, &st->deep1,   \
+        g_walk.kstats)
     if (g_walk.deepStack) GD_PASS1(2 * GD_TR_STACK);
     else GD_PASS1(GD_TR_STACK);
 #undef GD_PASS1
@@ -1153,7 +1166,6 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
     const int over1 = h.over1;
     overOut[0] = over1 + h.over2;
-    overOut[1] = h.maxList1;
     *requested = h.requested;
     g_walk.deepStack = 16ll * h.deep1 > runs;
     /* pass 2: the groups that did not fit, with long lists and a deep stack */
@@ -1171,8 +1183,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
-            g_walk.overflow2, &st->over2, &st->maxList2, nullptr, nullptr, nullptr,
-            g_walk.stats);
+            g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats);
         GD_LAUNCHED();
         gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool, nullptr,
                                   nullptr, g_walk.counts2, cap2, acc, phi, eps2, G, n, W, Bv,
@@ -1183,7 +1194,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     const int over2 = h.over2;
     overOut[2] = over2;
     if (over2 <= 0) return;
-    /* pass 3: body by body, 2048-entry stacks; the rare body that needs more
+    /* pass 3: body by body, 2048-node stacks; the rare body that needs more
        is walked again with a stack that fits any tree */
     const long long bodies = 64ll * over2;
     /* In the learned walk the per-body groups take the quadrupole alone: the
@@ -1191,7 +1202,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
        it would cost 64 times as much for ~1 % of the bodies (the spread
        ones), which the point-based opening test already serves better. */
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
-    k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_TR_STACK>
+    k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
         <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES>>>(
             pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
             g_walk.overflow2, nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G,
@@ -1231,7 +1242,10 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     if (gd_has_error()) return;
     const gd_bf16x4* W = net ? gd_mlp_fwd_tiles(net) : nullptr;
     const float* Bv = net ? gd_mlp_bias(net) : nullptr;
-    k_walk_reset<<<1, 64>>>(g_walk.stats, g_walk.status);
+    k_walk_reset<<<1, 256>>>(g_walk.stats, g_walk.status);
+    /* the list passes count interactions only when asked to (their per-group
+       atomics are not free); the per-body walk always reports stack overflows */
+    g_walk.kstats = stats ? g_walk.stats : nullptr;
     GD_LAUNCHED();
     if (g_walkTiming) {
         if (!g_walkEv[0])
@@ -1261,8 +1275,10 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
        not get theirs went to the second pass) */
     gd_walk_grow_lists((long long)requested);
     if (over[0] > 0 || stats) {
-        unsigned long long h[5];
-        GD_HIP(hipMemcpy(h, g_walk.stats, sizeof(h), hipMemcpyDeviceToHost));
+        unsigned long long raw[GD_STAT_WORDS], h[5] = {0, 0, 0, 0, 0};
+        GD_HIP(hipMemcpy(raw, g_walk.stats, sizeof(raw), hipMemcpyDeviceToHost));
+        for (int sl = 0; sl < GD_STAT_SLOTS; ++sl)
+            for (int k = 0; k < 5; ++k) h[k] += raw[sl * GD_STAT_STRIDE + k];
         if (h[2]) GD_FAIL("gd_walk: a body's traversal stack exceeded %d entries", GD_BODY_STACK);
         if (!stats) return;
         stats->interactions = h[0];
