@@ -281,7 +281,10 @@ struct gd_mass_result {
     int zero;
 };
 
-__device__ __forceinline__ gd_mass_result gd_node_mass(const node* __restrict__ nodes,
+/* childCom(j): child j's centre of mass (the engine reads the compact walk
+ * array, the reference-API kernel the node records; the values are equal). */
+template <typename ChildCom>
+__device__ __forceinline__ gd_mass_result gd_node_mass(ChildCom childCom,
                                                        const float4* __restrict__ pos,
                                                        int first, int count, const int* child) {
     gd_mass_result r;
@@ -313,7 +316,7 @@ __device__ __forceinline__ gd_mass_result gd_node_mass(const node* __restrict__ 
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            q[c] = child[c] != -1 ? nodes[child[c]].massData : make_float4(0.f, 0.f, 0.f, 0.f);
+            q[c] = child[c] != -1 ? childCom(child[c]) : make_float4(0.f, 0.f, 0.f, 0.f);
             if (child[c] != -1) a.w += q[c].w;
         }
 #pragma unroll
@@ -446,7 +449,8 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
         const int4 w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3];
         const int child[8] = {w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, w3.x};
         const int first = w0.w, count = w1.x;
-        const gd_mass_result m = gd_node_mass(nodes, pos, first, count, child);
+        const gd_mass_result m =
+            gd_node_mass([wcom](int j) { return wcom[j]; }, pos, first, count, child);
         nodes[idx].massData = m.m;
         zeros += m.zero;
         if (level > 0) {
@@ -498,7 +502,7 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
             bound = p8 < bound ? p8 : bound;
         }
         int blocks = gd_div_up(bound, 256);
-        if (blocks > 1024) blocks = 1024;
+        if (blocks > 4096) blocks = 4096;   /* 16 blocks per CU: every wave slot busy */
         if (b->quad == 2)
             k_build_mass<2><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
                                              b->wquad, b->woct);
@@ -572,7 +576,8 @@ __global__ __launch_bounds__(256) void k_tree_mass_api(node* nodes, const float4
     const int idx = levelStart + t;
     int child[8];
     for (int c = 0; c < 8; ++c) child[c] = nodes[idx].child[c];
-    const gd_mass_result m = gd_node_mass(nodes, pos, nodes[idx].firstParticleIndex,
+    const gd_mass_result m = gd_node_mass([nodes](int j) { return nodes[j].massData; }, pos,
+                                          nodes[idx].firstParticleIndex,
                                           nodes[idx].particleCount, child);
     nodes[idx].massData = m.m;
 }
