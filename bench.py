@@ -50,9 +50,9 @@ FP32_PEAK_TFLOPS = 157.3
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
 # (walk, theta, bodies) of the workload they were measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75, 1 << 21): {"bytes": 1285711385,
+    ("learned", 0.75, 1 << 21): {"bytes": 1056543354,
                                  "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.7, 1 << 24): {"bytes": 12463723019,
+    ("learned", 0.7, 1 << 24): {"bytes": 9246210451,
                                 "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
 }
 BUDGET = 1e-3
