@@ -434,14 +434,19 @@ __device__ __forceinline__ void gd_eval64(const float4* __restrict__ e, float px
     }
 }
 
-/* 64 quadrupole cells (com/mass in e, moments in q0/q1). */
+/* 64 quadrupole cells: com/mass in e, second moments in q0/q1 prescaled for
+ * this loop, 7.5 (Sxx, Syy, Szz, Sxy) and (7.5 Sxz, 7.5 Syz, -1.5 tr S, tr S)
+ * (two multiplies per cell instead of per body and cell).  Padding entries
+ * (massless, far away, zero moments) contribute exactly 0, so the loop always
+ * runs 64 entries and needs no per-lane trip count. */
 template <bool PHI>
 __device__ __forceinline__ void gd_eval64_quad(const float4* __restrict__ e,
                                                const float4* __restrict__ q0,
-                                               const float4* __restrict__ q1, int cnt, float px,
+                                               const float4* __restrict__ q1, float px,
                                                float py, float pz, float eps2, float& ax,
                                                float& ay, float& az, float& ph) {
-    for (int k = 0; k < cnt; ++k) {
+#pragma unroll 2
+    for (int k = 0; k < 64; ++k) {
         const float4 c = e[k];
         const float4 a = q0[k];
         const float4 b = q1[k];
@@ -451,16 +456,17 @@ __device__ __forceinline__ void gd_eval64_quad(const float4* __restrict__ e,
         const float inv2 = inv * inv;
         const float mi = c.w * inv;
         const float inv5 = inv * inv2 * inv2;
+        /* with S' = 7.5 S: sd = S' d, q = d.S' d */
         const float sdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.w, dy, b.x * dz));
         const float sdy = __builtin_fmaf(a.w, dx, __builtin_fmaf(a.y, dy, b.y * dz));
         const float sdz = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, a.z * dz));
         const float q = __builtin_fmaf(dx, sdx, __builtin_fmaf(dy, sdy, dz * sdz));
-        const float cd = __builtin_fmaf(mi, inv2, inv5 * __builtin_fmaf(7.5f * q, inv2, -1.5f * b.z));
-        const float cs = -3.f * inv5;
+        const float cd = __builtin_fmaf(mi, inv2, inv5 * __builtin_fmaf(q, inv2, b.z));
+        const float cs = -0.4f * inv5;
         ax = __builtin_fmaf(cd, dx, __builtin_fmaf(cs, sdx, ax));
         ay = __builtin_fmaf(cd, dy, __builtin_fmaf(cs, sdy, ay));
         az = __builtin_fmaf(cd, dz, __builtin_fmaf(cs, sdz, az));
-        if (PHI) ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(3.f, q, -r2 * b.z), mi);
+        if (PHI) ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(0.4f, q, -r2 * b.w), mi);
     }
 }
 
@@ -658,7 +664,7 @@ __device__ __forceinline__ void gd_eval_group(
     const float ext = info->box.extent;
     for (int c0 = 0; c0 < nb; c0 += 64) {
         const int e = c0 + (int)lane;
-        float4 c = zero4, q0 = zero4, q1 = zero4;
+        float4 c = pad, q0 = zero4, q1 = zero4;
         int idx = -1;
         if (e < nb) {
             idx = list[cap - 1 - e];
@@ -666,11 +672,10 @@ __device__ __forceinline__ void gd_eval_group(
             q0 = wquad[2 * idx];
             q1 = wquad[2 * idx + 1];
         }
-        const int m = nb - c0 < 64 ? nb - c0 : 64;
         float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
         sE[lane] = c;
-        sQ0[lane] = q0;
-        sQ1[lane] = q1;
+        sQ0[lane] = make_float4(7.5f * q0.x, 7.5f * q0.y, 7.5f * q0.z, 7.5f * q0.w);
+        sQ1[lane] = make_float4(7.5f * q1.x, 7.5f * q1.y, -1.5f * q1.z, q1.z);
         int nc = 0;
         if (MODE == 2) {
             /* network at the group centre, on the cells with s / r > minRatio */
@@ -688,7 +693,7 @@ __device__ __forceinline__ void gd_eval_group(
             *nnEvals += (unsigned long long)nc;
         }
         __builtin_amdgcn_wave_barrier();
-        gd_eval64_quad<PHI>(sE, sQ0, sQ1, m, px, py, pz, eps2, ax, ay, az, ph);
+        gd_eval64_quad<PHI>(sE, sQ0, sQ1, px, py, pz, eps2, ax, ay, az, ph);
         if (MODE == 2) {
             /* learned residual k / r^5 * o of each corrected cell */
             for (int kk = 0; kk < nc; ++kk) {
