@@ -114,3 +114,29 @@ def test_public_headers_compile_as_c_and_cxx(tmp_path):
                     str(tmp_path / "t.o")], check=True)
     subprocess.run(["g++", "-x", "c++", "-std=c++17", "-Wall", "-Werror", "-I", inc, "-c",
                     str(src), "-o", str(tmp_path / "t2.o")], check=True)
+
+
+def test_walk_settings_are_checked_on_the_host():
+    """gd_walk_set_group_split and gd_walk_set_list_capacity return the previous
+    setting and reject values out of range without touching the GPU."""
+    lib = _lib.load_library()
+    lib.gd_set_exit_on_error(0)
+    try:
+        assert lib.gd_walk_set_group_split(0.3) == -1.0          # default: the theta rule
+        assert abs(lib.gd_walk_set_group_split(0.0) - 0.3) < 1e-7
+        assert lib.gd_walk_set_group_split(-5.0) == 0.0          # any negative: the rule again
+        assert lib.gd_walk_set_group_split(1.5) == -1.0
+        assert b"above 1" in lib.gd_last_error()
+        lib.gd_clear_error()
+        lib.gd_walk_set_group_split(float("nan"))
+        assert b"above 1" in lib.gd_last_error()
+        lib.gd_clear_error()
+        assert lib.gd_walk_set_group_split(-1.0) == -1.0         # the rejected values left no trace
+        assert lib.gd_walk_set_list_capacity(-1, 0) == 0
+        assert b"out of range" in lib.gd_last_error()
+        lib.gd_clear_error()
+        assert lib.gd_walk_set_list_capacity(0, 1 << 27) == 0
+        assert b"out of range" in lib.gd_last_error()
+    finally:
+        lib.gd_clear_error()
+        lib.gd_set_exit_on_error(1)
