@@ -3,30 +3,32 @@
  * wavefront per group), shared by the walk and by the corrector's sample
  * generator, which must see the very interaction lists the walk evaluates.
  *
- * Groups.  Bodies are taken in runs of 64 consecutive bodies in Morton order.
- * A run that straddles a high-level cell boundary has a long, thin box whose
- * interaction list is much longer than its halves' would be, so each run may
- * be split in two at its largest key jump (the adjacent pair whose keys
- * differ in the highest bit): it is split when the two halves' box volumes
- * add up to less than a fraction r of the run's.  Run g owns slots 2g and
- * 2g+1; slot 2g+1 is empty when the run stays whole.  The halves' lists
- * share the far cells, so the gain shrinks with the opening angle: measured
- * on the 2^21-body merger, the best r is 0 at theta = 0.2, 0.1 at 0.3, 0.2
- * at 0.4-0.5 and 0.3 from 0.6 on (gd_split_ratio), where ~21% of the runs
- * split and the total list length drops by ~9-11%.  The learned walk splits
- * the same way: its network is evaluated once per group, on more groups, but
- * on shorter lists (16.8M-body merger at theta 0.7: 55.4 -> 50.9 ms/step).
+ * Groups.  Bodies are taken in runs of 64 consecutive bodies in Morton order,
+ * and each run is split in two at its largest key jump (the first adjacent
+ * pair whose keys differ in the highest bit): each half has its own, smaller
+ * bounding box and its own interaction list, and the first-pass evaluation
+ * streams the two lists side by side in one wavefront, so a run costs the
+ * longer of its halves' lists.  Run g owns slots 2g and 2g+1; slot 2g+1 is
+ * empty when the run is not split (all keys equal, one body, or a volume
+ * rule that keeps it whole).  On the 2^21-body merger the evaluation work is
+ * 0.57 of whole runs' at theta 0.7 (0.85 at theta 0.2), for about the same
+ * traversal work (x1.6 at theta 0.2).  A volume rule (split only when the
+ * halves' boxes hold less than a fraction r <= 1 of the run's box volume)
+ * remains available through gd_walk_set_group_split.
  */
 #ifndef GD_TRAVERSE_H
 #define GD_TRAVERSE_H
 
 #include "gd_internal.h"
 
-/* The volume fraction below which a run is split, for opening angle theta
- * (0: never). */
+/* The default split rule: every run with two distinct keys (a fraction
+ * above 1 means "always"; see gd_split_run).  The opening angle does not
+ * change it: splitting pays at every angle once the halves share a
+ * wavefront. */
+#define GD_SPLIT_ALWAYS 2.f
 __host__ __device__ inline float gd_split_ratio(float theta) {
-    const float r = theta - 0.2f;
-    return r < 0.f ? 0.f : (r > 0.3f ? 0.3f : r);
+    (void)theta;
+    return GD_SPLIT_ALWAYS;
 }
 
 /* Bounding box of bodies [start, start + count), count in 1..64, in every lane. */
@@ -79,6 +81,7 @@ __device__ __forceinline__ int gd_split_run(const uint64_t* __restrict__ keys,
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
     if (mx == 0 || !(ratio > 0.f)) return 0;   /* all keys equal (or one body), or no splits */
     const int j = __ffsll((long long)__ballot(hb == mx));   /* lowest such lane + 1 */
+    if (ratio > 1.f) return j;                               /* every run is split */
     const float4 p = pos[i];
     float ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
     gd_masked_box(valid && l < j, p.x, p.y, p.z, ax, ay, az, bx, by, bz);

@@ -181,8 +181,9 @@ __device__ __forceinline__ unsigned long long* gd_stat_slot(unsigned long long* 
  * lists[offsets[g] .. + caps[g]) for group g when offsets is given (the
  * first pass's packed layout), else lists[s*cap .. + cap); it gives up once the list
  * passes slotCap[s] (when given) or its capacity, or the LDS stack of STACK
- * entries overflows: counts[s] = (-1, -1) and the group goes to
- * overflowList.  Otherwise counts[s] = (front, back) and, when sizes is
+ * entries overflows: counts = (-1, -1) and the group goes to
+ * overflowList.  Otherwise counts = (front, back) -- counts[g] for group g
+ * in the first pass (offsets given), counts[s] otherwise -- and, when sizes is
  * given, sizes[g] records the list length for the next walk's layout.  In
  * the second pass (offsets NULL) a group given up on is marked spread
  * (sizes -2); the first pass, when spreadList is given, sends a marked group
@@ -205,13 +206,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     if (s >= slots) return;   /* wave-uniform */
     const long long g = groupList ? groupList[s] : s;
     const int2 rg = slotRange[g];   /* (first body, bodies) */
+    const long long cs = offsets ? g : s;   /* counts by slot id in the first pass */
     if (rg.y == 0) {                /* empty second slot of a whole run */
-        if (gd_lane() == 0) counts[s] = make_int2(0, 0);
+        if (gd_lane() == 0) counts[cs] = make_int2(0, 0);
         return;
     }
     if (spreadList && sizes[g] == GD_SPREAD) {
         if (gd_lane() == 0) {
-            counts[s] = make_int2(-1, -1);
+            counts[cs] = make_int2(-1, -1);
             spreadList[atomicAdd(spreadCount, 1)] = (int)g;
         }
         return;
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
         /* groups deeper than 7/8 of GD_TR_STACK (or overflowing it) */
         if (deepCount && depth > GD_TR_STACK - GD_TR_STACK / 8) atomicAdd(deepCount, 1);
         if (ok) {
-            counts[s] = make_int2(nf, nb);
+            counts[cs] = make_int2(nf, nb);
             if (sizes) sizes[g] = nf + nb;
             if (stats) {
                 const int nvalid = rg.y;
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
                 atomicAdd(&sl[3], (unsigned long long)nb * (unsigned long long)nvalid);
             }
         } else {
-            counts[s] = make_int2(-1, -1);
+            counts[cs] = make_int2(-1, -1);
             if (sizes) sizes[g] = offsets ? -1 : GD_SPREAD;
             const int at = atomicAdd(overflowCount, 1);
             overflowList[at] = (int)g;
@@ -775,6 +777,270 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
 }
 
 /*
+ * Learned corrections of a wavefront's queued cells (first pass, learned
+ * mode): queue[0 .. qn) holds node indices, bit 30 set for the cells of half
+ * B.  Lane l < qn computes cell l's features at its half's centre, the
+ * network runs on the qn cells at once (full 16-row tiles instead of a few
+ * cells per chunk), and every body adds k / r^5 * output of each cell of its
+ * own half (k = m s^3, r from the body) to (ax, ay, az).  map: 64 ints of
+ * LDS.  Returns the number of network evaluations.
+ */
+__device__ __forceinline__ int gd_flush_corr(
+    const float4* __restrict__ wcom, const int2* __restrict__ wlink,
+    const float4* __restrict__ woct, float ext, const gd_bf16x4* __restrict__ W,
+    const float* __restrict__ Bv, float minRatio, const int* queue, int qn, int* map,
+    float gax, float gay, float gaz, float gbx, float gby, float gbz, bool inB, float px,
+    float py, float pz, float& ax, float& ay, float& az) {
+    const unsigned lane = gd_lane();
+    const bool have = (int)lane < qn;
+    const int e = have ? queue[lane] : 0;
+    const int idx = e & 0x3fffffff;
+    const bool half = (e >> 30) & 1;
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f), o0 = c, o1 = c, o2 = c;
+    float size = 1.f;
+    if (have) {
+        c = wcom[idx];
+        o0 = woct[3 * idx];
+        o1 = woct[3 * idx + 1];
+        o2 = woct[3 * idx + 2];
+        size = ldexpf(ext, -(wlink[idx].y & 31));
+    }
+    float f[8], k;
+    gd_cell_features(c, o0, o1, o2, size, half ? gbx : gax, half ? gby : gay,
+                     half ? gbz : gaz, have, minRatio, f, k);
+    if (have) map[lane] = (int)lane;
+    __builtin_amdgcn_wave_barrier();
+    float cx, cy, cz;
+    gd_mlp_cells(W, Bv, qn, map, f, cx, cy, cz);
+    const float kx = k * cx, ky = k * cy, kz = k * cz;
+    const int hb = half ? 1 : 0;
+    for (int kk = 0; kk < qn; ++kk) {
+        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), kk));
+        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), kk));
+        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.z), kk));
+        const float ox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kx), kk));
+        const float oy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ky), kk));
+        const float oz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kz), kk));
+        const int h = __builtin_amdgcn_readlane(hb, kk);
+        const float dx = qx - px, dy = qy - py, dz = qz - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float i1 = __builtin_amdgcn_rsqf(r2);
+        const float i2 = i1 * i1;
+        const float s5 = (inB == (h != 0)) ? i2 * i2 * i1 : 0.f;
+        ax = __builtin_fmaf(s5, ox, ax);
+        ay = __builtin_fmaf(s5, oy, ay);
+        az = __builtin_fmaf(s5, oz, az);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return qn;
+}
+
+/*
+ * First-pass evaluation of one 64-body run by one wavefront: the run's two
+ * group slots (halves A and B; B may be empty) each have their own list.
+ * Lane l is body start + l and belongs to A (l < nA) or B; per chunk step
+ * each lane gathers entry c0 + l of both lists into LDS (sA, sB), and every
+ * lane evaluates the 64 entries of its own half's chunk.  The shorter list
+ * is padded (massless, far away: exactly 0), so the run costs the longer of
+ * the two lists instead of their sum.  cA / cB are the halves' counts, (0, 0)
+ * for a half that is not evaluated here (empty, or left to the later passes).
+ * In learned mode the cells to correct (s / r > minRatio from their half's
+ * centre) are queued (sQueue, 64 ints) and the network runs on full batches
+ * (gd_flush_corr).  sQ holds 4 x 64 float4 (A's and B's prescaled moments).
+ */
+template <int MODE, bool PHI>
+__device__ __forceinline__ void gd_eval_pair(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
+    const int* __restrict__ listA, int capA, int2 cA, const int* __restrict__ listB, int capB,
+    int2 cB, long long start, int nA, int nB, float eps2, const gd_bf16x4* __restrict__ mlpW,
+    const float* __restrict__ mlpB, float minRatio, float4* sA, float4* sB, float4* sQ,
+    int* sMap, int* sQueue, gd_kahan3& tot, float& phs, float& phc,
+    unsigned long long* nnEvals) {
+    const unsigned lane = gd_lane();
+    const bool valid = (int)lane < nA + nB;
+    const bool inB = (int)lane >= nA;
+    const float4 p = pos[start + (valid ? lane : 0)];
+    const float px = p.x, py = p.y, pz = p.z;
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 pad = make_float4(1e18f, 1e18f, 1e18f, 0.f);
+    const float4* myE = inB ? sB : sA;
+
+    /* front lists: monopoles */
+    const int nfA = cA.x, nfB = cB.x;
+    const int nf = nfA > nfB ? nfA : nfB;
+    float4 na = pad, nb = pad;
+    if ((int)lane < nfA) na = wcom[listA[lane]];
+    if ((int)lane < nfB) nb = wcom[listB[lane]];
+    for (int c0 = 0; c0 < nf; c0 += 64) {
+        sA[lane] = na;
+        sB[lane] = nb;
+        __builtin_amdgcn_wave_barrier();
+        na = pad;
+        nb = pad;
+        if (c0 + 64 + (int)lane < nfA) na = wcom[listA[c0 + 64 + lane]];
+        if (c0 + 64 + (int)lane < nfB) nb = wcom[listB[c0 + 64 + lane]];
+        float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+        gd_eval64<PHI>(myE, px, py, pz, eps2, ax, ay, az, ph);
+        gd_kahan_add(tot.x, tot.cx, ax);
+        gd_kahan_add(tot.y, tot.cy, ay);
+        gd_kahan_add(tot.z, tot.cz, az);
+        if (PHI) gd_kahan_add(phs, phc, ph);
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (MODE == 0) return;
+
+    /* back lists: cells */
+    const int nbA = cA.y, nbB = cB.y;
+    const int nbm = nbA > nbB ? nbA : nbB;
+    float gax = 0.f, gay = 0.f, gaz = 0.f, gbx = 0.f, gby = 0.f, gbz = 0.f;
+    if (MODE == 2) {
+        float lx, ly, lz, hx, hy, hz;
+        gd_masked_box(valid && !inB, px, py, pz, lx, ly, lz, hx, hy, hz);
+        if (nA > 0) { gax = 0.5f * (lx + hx); gay = 0.5f * (ly + hy); gaz = 0.5f * (lz + hz); }
+        gd_masked_box(valid && inB, px, py, pz, lx, ly, lz, hx, hy, hz);
+        if (nB > 0) { gbx = 0.5f * (lx + hx); gby = 0.5f * (ly + hy); gbz = 0.5f * (lz + hz); }
+    }
+    const float ext = info->box.extent;
+    float4* sQA0 = sQ;
+    float4* sQA1 = sQ + 64;
+    float4* sQB0 = sQ + 128;
+    float4* sQB1 = sQ + 192;
+    const float4* myQ0 = inB ? sQB0 : sQA0;
+    const float4* myQ1 = inB ? sQB1 : sQA1;
+    int qn = 0;   /* queued cells (learned mode) */
+    float lx = 0.f, ly = 0.f, lz = 0.f;   /* their corrections, since the last Kahan add */
+    for (int c0 = 0; c0 < nbm; c0 += 64) {
+        const int e = c0 + (int)lane;
+        float4 ca = pad, qa0 = zero4, qa1 = zero4, cb = pad, qb0 = zero4, qb1 = zero4;
+        int ia = -1, ib = -1;
+        if (e < nbA) {
+            ia = listA[capA - 1 - e];
+            ca = wcom[ia];
+            qa0 = wquad[2 * ia];
+            qa1 = wquad[2 * ia + 1];
+        }
+        if (e < nbB) {
+            ib = listB[capB - 1 - e];
+            cb = wcom[ib];
+            qb0 = wquad[2 * ib];
+            qb1 = wquad[2 * ib + 1];
+        }
+        sA[lane] = ca;
+        sB[lane] = cb;
+        sQA0[lane] = make_float4(7.5f * qa0.x, 7.5f * qa0.y, 7.5f * qa0.z, 7.5f * qa0.w);
+        sQA1[lane] = make_float4(7.5f * qa1.x, 7.5f * qa1.y, -1.5f * qa1.z, qa1.z);
+        sQB0[lane] = make_float4(7.5f * qb0.x, 7.5f * qb0.y, 7.5f * qb0.z, 7.5f * qb0.w);
+        sQB1[lane] = make_float4(7.5f * qb1.x, 7.5f * qb1.y, -1.5f * qb1.z, qb1.z);
+        __builtin_amdgcn_wave_barrier();
+        float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+        gd_eval64_quad<PHI>(myE, myQ0, myQ1, px, py, pz, eps2, ax, ay, az, ph);
+        if (MODE == 2) {
+            /* queue the cells each half corrects: s / r > minRatio from its centre,
+               with the feature function's own rounding of s / r */
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int id = h ? ib : ia;
+                const float4 cc = h ? cb : ca;
+                bool need = false;
+                if (id >= 0 && cc.w > 0.f) {
+                    const float dx = cc.x - (h ? gbx : gax), dy = cc.y - (h ? gby : gay),
+                                dz = cc.z - (h ? gbz : gaz);
+                    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    const float size = ldexpf(ext, -(wlink[id].y & 31));
+                    need = size * __builtin_amdgcn_rsqf(r2) > minRatio;
+                }
+                const unsigned long long bal = __ballot(need);
+                const int m = __popcll(bal);
+                if (qn + m > 64) {   /* wave-uniform */
+                    *nnEvals += (unsigned long long)gd_flush_corr(
+                        wcom, wlink, woct, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax,
+                        gay, gaz, gbx, gby, gbz, inB, px, py, pz, lx, ly, lz);
+                    qn = 0;
+                }
+                if (need) sQueue[qn + __popcll(bal & gd_lanemask_lt())] = id | (h << 30);
+                qn += m;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        gd_kahan_add(tot.x, tot.cx, ax);
+        gd_kahan_add(tot.y, tot.cy, ay);
+        gd_kahan_add(tot.z, tot.cz, az);
+        if (PHI) gd_kahan_add(phs, phc, ph);
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (MODE == 2) {
+        if (qn > 0)
+            *nnEvals += (unsigned long long)gd_flush_corr(
+                wcom, wlink, woct, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax, gay, gaz,
+                gbx, gby, gbz, inB, px, py, pz, lx, ly, lz);
+        gd_kahan_add(tot.x, tot.cx, lx);
+        gd_kahan_add(tot.y, tot.cy, ly);
+        gd_kahan_add(tot.z, tot.cz, lz);
+    }
+}
+
+/* First-pass evaluation kernel: one wavefront per 64-body run (its two group
+ * slots, gd_eval_pair); EVW wavefronts per block.  A half whose traversal
+ * did not list it (counts -1: second pass or per-body walk) is skipped and
+ * its bodies are written by those passes. */
+template <int MODE, bool PHI, int EVW>
+__global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ slotRange, int runs, const int* __restrict__ lists,
+    const long long* __restrict__ offsets, const int* __restrict__ caps,
+    const int2* __restrict__ counts, float3* __restrict__ acc, float* __restrict__ phi,
+    float eps2, float G, const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB,
+    float minRatio, unsigned long long* __restrict__ stats) {
+    __shared__ float4 sA[EVW][64];
+    __shared__ float4 sB[EVW][64];
+    __shared__ float4 sQ[MODE != 0 ? EVW : 1][MODE != 0 ? 256 : 1];
+    __shared__ int sMap[MODE == 2 ? EVW : 1][64];
+    __shared__ int sQueue[MODE == 2 ? EVW : 1][64];
+    __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
+    __shared__ float sBias[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
+    if (MODE == 2) {
+        /* the network's packed weights and biases, once per block */
+        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64 * EVW) sW[e] = mlpW[e];
+        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64 * EVW) sBias[e] = mlpB[e];
+        __syncthreads();
+        mlpW = sW;
+        mlpB = sBias;
+    }
+    const unsigned lane = gd_lane();
+    const int wave = (int)(threadIdx.x >> 6);
+    const long long r = (long long)gd_xcd_block() * EVW + wave;
+    if (r >= runs) return;   /* wave-uniform */
+    const int2 rgA = slotRange[2 * r], rgB = slotRange[2 * r + 1];
+    int2 cA = counts[2 * r];
+    int2 cB = rgB.y > 0 ? counts[2 * r + 1] : make_int2(-1, -1);
+    const bool actA = cA.x >= 0, actB = cB.x >= 0;
+    if (!actA && !actB) return;
+    if (!actA) cA = make_int2(0, 0);
+    if (!actB) cB = make_int2(0, 0);
+    const int* listA = lists + (actA ? offsets[2 * r] : 0);
+    const int* listB = lists + (actB ? offsets[2 * r + 1] : 0);
+    const int capA = actA ? caps[2 * r] : 0, capB = actB ? caps[2 * r + 1] : 0;
+    gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float phs = 0.f, phc = 0.f;
+    unsigned long long nn = 0;
+    gd_eval_pair<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, listA, capA, cA, listB, capB,
+                            cB, rgA.x, rgA.y, rgB.y, eps2, mlpW, mlpB, minRatio, sA[wave],
+                            sB[wave], sQ[MODE != 0 ? wave : 0], sMap[MODE == 2 ? wave : 0],
+                            sQueue[MODE == 2 ? wave : 0], tot, phs, phc, &nn);
+    if (MODE == 2 && stats && lane == 0 && nn) atomicAdd(&gd_stat_slot(stats)[4], nn);
+    const bool inB = (int)lane >= rgA.y;
+    if ((int)lane < rgA.y + rgB.y && (inB ? actB : actA)) {
+        const long long i = rgA.x + lane;
+        acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
+        if (PHI) phi[i] = -G * phs;
+    }
+}
+
+/*
  * Per-body walk of the groups the traversal could not list (a group whose
  * bodies are spread out -- typically a run of the Morton curve that jumps
  * between distant clusters -- sees a huge list through its bounding box).
@@ -978,7 +1244,7 @@ struct gd_walk_scratch {
     int* overflow1;        /* groups entries */
     int* overflow2;        /* groups entries */
     int* bodyOver;         /* 64 x groups entries: bodies for the deep-stack per-body walk */
-    int2* counts;          /* groups entries */
+    int2* counts;          /* groups + 1 entries (the sentinel slot) */
     int* sizes;            /* groups entries: last list length, -1 unknown */
     long long* offsets;    /* groups entries: first-pass layout */
     int* caps;             /* groups entries */
@@ -1037,7 +1303,7 @@ static void gd_walk_ensure(int groups) {
         GD_HIP(hipMalloc(&g_walk.overflow1, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.overflow2, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.bodyOver, 64 * sizeof(int) * (size_t)groups));
-        GD_HIP(hipMalloc(&g_walk.counts, sizeof(int2) * (size_t)groups));
+        GD_HIP(hipMalloc(&g_walk.counts, sizeof(int2) * ((size_t)groups + 1)));
         GD_HIP(hipMalloc(&g_walk.sizes, sizeof(int) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.offsets, sizeof(long long) * (size_t)groups));
         GD_HIP(hipMalloc(&g_walk.caps, sizeof(int) * (size_t)groups));
@@ -1162,9 +1428,14 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
 #undef GD_PASS1
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
-    gd_launch_eval<MODE, PHI>(groups, b, pos, g_walk.order, g_walk.lists, g_walk.offsets,
-                              g_walk.caps, g_walk.counts, 0, acc, phi, eps2, G, n, W, Bv,
-                              minRatio);
+    {
+        /* one wavefront per run: both halves' lists side by side */
+        constexpr int EVW = MODE == 2 ? 8 : 4;   /* learned: 8 share the staged weights */
+        k_walk_eval_pair<MODE, PHI, EVW><<<gd_div_up(runs, EVW), 64 * EVW>>>(
+            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, runs,
+            g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2, G, W, Bv,
+            minRatio, g_walk.kstats);
+    }
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
     gd_walk_status h;

@@ -116,14 +116,14 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
  * Returns the previous first-pass setting. */
 int gd_walk_set_list_capacity(int entries, int secondPass);
 
-/* Groups of the list walks.  Each run of 64 bodies in Morton order is a
- * group, split in two at its largest key jump when the halves' bounding
- * boxes hold less than `ratio` of the run's box volume (a run that spans
- * two clusters sees fewer cells through two small boxes).  ratio < 0 (the
- * default): theta - 0.2 clamped to [0, 0.3] (no splitting at theta <= 0.2,
- * where the halves' lists share most cells); 0: whole runs; at most 1.  The
- * training samples (gd_corrector_samples) take the groups of their own
- * opening angle.  Returns the previous setting. */
+/* Groups of the list walks.  Each run of 64 bodies in Morton order is split
+ * in two at its largest key jump; each half has its own interaction list
+ * (seen through its own, smaller bounding box) and the two lists are
+ * evaluated side by side by one wavefront.  ratio < 0 (the default): every
+ * run with two distinct keys is split; 0: whole runs; 0 < ratio <= 1: a run
+ * is split only when the halves' boxes hold less than `ratio` of the run's
+ * box volume.  The training samples (gd_corrector_samples) take the same
+ * groups.  Returns the previous setting. */
 float gd_walk_set_group_split(float ratio);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the

@@ -148,6 +148,8 @@ def test_group_slots_partition_and_split_straddling_runs():
     assert slots[7, 1] == 0 and slots[6, 1] == n - 192
     whole, _ = group_slots(keys, pos, 0.0)
     assert (whole[1::2, 1] == 0).all()
-    # the walk's fraction: none at small opening angles, at most 0.3
-    assert split_ratio(0.2) == 0 and split_ratio(0.1) == 0
-    assert abs(split_ratio(0.4) - 0.2) < 1e-6 and split_ratio(0.7) == np.float32(0.3)
+    # the walk's default: every run with distinct keys splits at its largest jump
+    assert split_ratio(0.2) > 1 and split_ratio(0.7) > 1
+    always, _ = group_slots(keys, pos, split_ratio(0.5))
+    assert list(always[0]) != [0, 64] and list(always[2]) == [64, 40]
+    assert (always[1::2, 1] > 0).all()

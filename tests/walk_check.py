@@ -124,17 +124,17 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
 
 
 def split_ratio(theta):
-    """The walk's volume fraction for splitting runs (float32, as the host)."""
-    r = f32(theta) - f32(0.2)
-    return f32(0) if r < 0 else (f32(0.3) if r > f32(0.3) else r)
+    """The walk's default split rule: every run (a fraction above 1 means
+    "always"; the opening angle does not change it)."""
+    return f32(2.0)
 
 
 def group_slots(keys, pos, ratio):
     """The walk's groups: each run of 64 consecutive bodies (sorted order) is
     split in two at its largest key jump (the adjacent pair whose keys differ
-    in the highest bit; the first such pair) when the halves' box volumes add
-    up to less than `ratio` of the run's, restated in float32 with the
-    kernel's operation order.  Returns (slots, slot_of_body): slots is an int array
+    in the highest bit; the first such pair): always when `ratio` > 1, else
+    when the halves' box volumes add up to less than `ratio` of the run's,
+    restated in float32 with the kernel's operation order.  Returns (slots, slot_of_body): slots is an int array
     (2 per run) of (first body, bodies), bodies 0 for an empty slot."""
     keys = np.asarray(keys, np.uint64)
     p = np.asarray(pos[:, :3], f32)
@@ -150,7 +150,9 @@ def group_slots(keys, pos, ratio):
             k = keys[s0:s0 + cnt]
             x = k[:-1] ^ k[1:]
             hb = np.array([int(v).bit_length() for v in x])
-            if hb.max() > 0:
+            if hb.max() > 0 and ratio > 1:
+                j = int(np.argmax(hb)) + 1
+            elif hb.max() > 0:
                 jj = int(np.argmax(hb)) + 1
                 a, b = p[s0:s0 + jj], p[s0 + jj:s0 + cnt]
                 amin, amax, bmin, bmax = a.min(0), a.max(0), b.min(0), b.max(0)
