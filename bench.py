@@ -50,9 +50,9 @@ FP32_PEAK_TFLOPS = 157.3
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
 # (walk, theta, bodies) of the workload they were measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75, 1 << 21): {"bytes": 1056543354,
+    ("learned", 0.75, 1 << 21): {"bytes": 647007095,
                                  "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.7, 1 << 24): {"bytes": 9246210451,
+    ("learned", 0.7, 1 << 24): {"bytes": 6136759387,
                                 "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
 }
 BUDGET = 1e-3
@@ -201,7 +201,7 @@ def eval_roofline(walks, mode, theta, n):
         "bound": "mfma",
         "compute": ("FP32 vector ALU (peak = MI355X FP32; packed FP32 is not used, so 78.6 is "
                     "the practical ceiling)"),
-        "kernel": f"k_walk_eval ({mode} walk, theta={theta})",
+        "kernel": f"k_walk_eval_pair ({mode} walk, theta={theta})",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",

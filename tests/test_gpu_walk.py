@@ -17,7 +17,8 @@ import pytest
 from cudagravitydemo_amd import Simulation, ics, load_library
 from cudagravitydemo_amd import corrector as cr
 from oracle import mlp
-from tests.walk_check import accepted, group_slots, split_ratio, walk_fp64, world_extent
+from tests.walk_check import (accepted, group_slots, potential_fp64, split_ratio, walk_fp64,
+                               world_extent)
 
 pytestmark = pytest.mark.gpu
 
@@ -199,6 +200,42 @@ def test_walk_with_set_group_split(mode, ratio):
         assert prev < 0
         assert st["overflow_groups"] == 0
         _check(sim, mode, lambda i: False, params, ratio=ratio)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("path", ["lists", "per_body"])
+def test_walk_potential_matches_interaction_set_fp64(mode, path):
+    """The walk's potential (gd_walk with phi) against the same interaction
+    sets summed in FP64: list walk (both halves of a run in one wavefront)
+    and per-body walk."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    caps = {"lists": (16384, 0), "per_body": (1, 1)}[path]
+    lib = load_library()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        lib.gd_walk_set_list_capacity(*caps)
+        try:
+            sim.walk(THETA, potential=True)
+        finally:
+            lib.gd_walk_set_list_capacity(0, 0)
+        phi = sim.potentials()
+        ps = sim.positions()
+        slots, slot_of = _slots(sim, mode)
+        total = sim.tree_summary()[0]
+        com, link, quad = sim.walk_arrays(total)
+        lo, hi = sim.bbox()
+        ext = world_extent(lo, hi)
+        worst = 0.0
+        for i in _sample(sim.n):
+            x = ps[i, :3]
+            box = (x, x) if path == "per_body" else _slot_box(ps, slots, slot_of[i])
+            want = -sim.G * potential_fp64(x.astype(np.float64), box, com, link, quad, ext,
+                                            THETA, sim.eps * sim.eps, mode)
+            err = abs(phi[i] - want) / abs(want)
+            worst = max(worst, err)
+            assert err < 1e-5, (i, phi[i], want)
+        print(f"mode {mode} {path}: worst potential error {worst:.2e}", flush=True)
 
 
 def test_walk_counts_match_interaction_sets():

@@ -76,6 +76,30 @@ def quadrupole(x, com, quad, idx, eps2):
     return (cd[:, None] * d - 3.0 * inv5[:, None] * sd).sum(0)
 
 
+def potential_fp64(x, box, com, link, quad, extent, theta, eps2, mode):
+    """Potential (without -G) at point x for the interaction set of `box`:
+    sum m / r over monopoles; quadrupole cells (mode 1) add
+    (3 d.S d - r^2 tr S) / (2 r^5); r softened, as the walk evaluates it."""
+    lo, hi = box
+    leaves, cells = accepted(com, link, extent, lo, hi, theta)
+    x = np.asarray(x, np.float64)
+    mono = leaves if mode else np.concatenate([leaves, cells])
+    d = com[mono, :3].astype(np.float64) - x
+    r2 = (d * d).sum(1) + eps2
+    phi = (com[mono, 3].astype(np.float64) / np.sqrt(r2)).sum()
+    if mode and cells.size:
+        d = com[cells, :3].astype(np.float64) - x
+        m = com[cells, 3].astype(np.float64)
+        q = quad[cells].astype(np.float64)
+        sd = np.stack([q[:, 0] * d[:, 0] + q[:, 3] * d[:, 1] + q[:, 4] * d[:, 2],
+                       q[:, 3] * d[:, 0] + q[:, 1] * d[:, 1] + q[:, 5] * d[:, 2],
+                       q[:, 4] * d[:, 0] + q[:, 5] * d[:, 1] + q[:, 2] * d[:, 2]], 1)
+        qq = (d * sd).sum(1)
+        r2 = (d * d).sum(1) + eps2
+        phi += (m / np.sqrt(r2) + (3 * qq - r2 * q[:, 6]) / (2 * r2 ** 2.5)).sum()
+    return phi
+
+
 def features(x, com, oct_, idx, extent, link):
     """The network's 8 inputs for cells idx seen from point x, and k = m s^3:
     u (3), s/r, (7.5 w - 1.5 T) / (m s^3) (3), u.w / (m s^3), with
