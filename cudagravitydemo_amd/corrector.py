@@ -106,7 +106,7 @@ class SampleSet:
         self.Y = DeviceBuffer(lib, 12 * self.cap)
         self.n = 0
 
-    def add_from(self, sim, targets, theta, min_ratio=0.5):
+    def add_from(self, sim, targets, theta, min_ratio=0.55):
         """Append the (body, cell) samples of one snapshot: the cells whose
         size / distance from the group's centre exceeds min_ratio, the ones
         walk_learned corrects with the same min_ratio; returns (rows added,
@@ -182,7 +182,7 @@ def train(net, samples, epochs=1, batch=8192, lr=2e-3, seed=0, log=None):
     return losses
 
 
-def walk_learned(sim, net, theta, min_ratio=0.5, stats=False):
+def walk_learned(sim, net, theta, min_ratio=0.55, stats=False):
     """The learned walk on sim's current tree (built with gd_set_quadrupole
     level 2).  Cells with size / distance <= min_ratio (from the group's
     centre) keep the analytic quadrupole alone: their residual is small and

@@ -84,7 +84,7 @@ def test_adam_step_matches_closed_form():
 @pytest.mark.parametrize("ratio,min_ratio", [(-1.0, 0.5), (0.0, 0.0)])
 def test_samples_match_direct_sums(ratio, min_ratio):
     """ratio -1: the learned walk's default groups at the sampling angle 0.8
-    (runs split at volume fraction 0.3), cells with s/r > 0.5 only; 0: whole
+    (every run split at its largest key jump), cells with s/r > 0.5 only; 0: whole
     64-body runs, as gd_walk_set_group_split(0) makes the walk keep them, and
     every accepted cell."""
     pos, vel = ics.exponential_disk(8192, seed=13)
