@@ -238,6 +238,45 @@ def test_walk_potential_matches_interaction_set_fp64(mode, path):
         print(f"mode {mode} {path}: worst potential error {worst:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 130, 1000])
+def test_walk_small_and_ragged_sizes(mode, n):
+    """Every body of tiny and ragged systems (a run of one body, runs whose
+    second half is empty or a single body, a ragged last run) against its
+    group's interaction set in FP64."""
+    pos, vel = ics.exponential_disk(max(n, 2), seed=77)
+    pos, vel = pos[:n], vel[:n]
+    params = _net_params()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        _run(sim, mode, 0, params)
+        ps = sim.positions()
+        acc = sim.accelerations()
+        slots, slot_of = _slots(sim, mode)
+        total = sim.tree_summary()[0]
+        com, link, quad = sim.walk_arrays(total)
+        oct_ = sim.third_moments(total) if mode == 2 else None
+        lo, hi = sim.bbox()
+        ext = world_extent(lo, hi)
+        net = (lambda X: mlp.forward(params, X, bf16_ops=True)[0]) if mode == 2 else None
+        for i in range(n):
+            x = ps[i, :3]
+            want, _ = walk_fp64(x.astype(np.float64), _slot_box(ps, slots, slot_of[i]), com, link,
+                                quad, ext, THETA, sim.eps * sim.eps, mode, net, MIN_RATIO,
+                                oct_=oct_)
+            want = sim.G * want
+            if n == 1:
+                assert np.abs(acc[i]).max() == 0.0   # a single body feels nothing
+                continue
+            tol = 1e-5
+            if mode == 2:   # the network's share is held to 1 %, as in _check
+                base, _ = walk_fp64(x.astype(np.float64), _slot_box(ps, slots, slot_of[i]), com,
+                                    link, quad, ext, THETA, sim.eps * sim.eps, 1)
+                tol += 1e-2 * np.linalg.norm(want - sim.G * base) / np.linalg.norm(want)
+            assert np.linalg.norm(acc[i] - want) <= tol * np.linalg.norm(want), (i, acc[i], want)
+
+
 def test_walk_counts_match_interaction_sets():
     """Interaction and cell counts of the list walk equal the CPU sets'."""
     pos, vel = ics.exponential_disk(N, seed=1234)
