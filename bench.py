@@ -50,12 +50,17 @@ FP32_PEAK_TFLOPS = 157.3
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
 # (walk, theta, bodies) of the workload they were measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75, 1 << 21): {"bytes": 647007095,
+    ("learned", 0.75, 1 << 21): {"bytes": 631703972,
                                  "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.7, 1 << 24): {"bytes": 6136759387,
-                                "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
+    ("learned", 0.675, 1 << 24): {"bytes": 6346263135,
+                                  "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
 }
 BUDGET = 1e-3
+# Each walk's opening angle is calibrated against this share of the budget:
+# the learned walk is judged again after its timed steps, on the evolved
+# state, where its error moves by a percent or two, and falls back to the
+# plain walk if it is then over budget.
+CALIBRATION_SHARE = 0.97
 FLOP_MONO = 20
 FLOP_QUAD = 54
 FLOP_CORR = 15
@@ -144,7 +149,8 @@ def median_err(sim, sample, exact):
 
 
 def calibrate(walks, mode, thetas, sample):
-    """Largest theta (thetas descending) meeting the budget on the current state."""
+    """Largest theta (thetas descending) whose median error on the current
+    state is within CALIBRATION_SHARE of the budget."""
     sim = walks.sim
     walks.prepare(mode)
     sim.build_tree_async()
@@ -154,7 +160,7 @@ def calibrate(walks, mode, thetas, sample):
         walks.walk(mode, th)
         med, p99 = median_err(sim, sample, exact)
         tried.append({"theta": th, "median": med, "p99": p99})
-        if med <= BUDGET:
+        if med <= CALIBRATION_SHARE * BUDGET:
             return th, med, p99, tried
     return None, None, None, tried
 
@@ -266,7 +272,7 @@ def main():
     n_samples = samples.n
     samples.close()
 
-    thetas = (1.0, 0.9, 0.85, 0.8, 0.75, 0.7, 0.65, 0.6, 0.5)
+    thetas = (1.0, 0.9, 0.85, 0.8, 0.75, 0.725, 0.7, 0.675, 0.65, 0.625, 0.6, 0.5)
     th_l, med_l, p99_l, tried_l = calibrate(walks, "learned", thetas, sample)
     corrector_on = th_l is not None
     secs_l = timed_steps(walks, "learned", th_l, dt, args.steps, 1) if corrector_on else None
@@ -344,6 +350,7 @@ def main():
             "walk": mode,
             "theta": theta,
             "force_error_budget": BUDGET,
+            "calibration_budget": CALIBRATION_SHARE * BUDGET,
             "judge": "FP64 direct sum, 4096-body seeded subsample",
         },
         "plain": leg(th_p, med_p, p99_p, secs_p, tried_p),

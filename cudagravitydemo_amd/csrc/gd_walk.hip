@@ -172,7 +172,8 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 __device__ __forceinline__ unsigned long long* gd_stat_slot(unsigned long long* stats) {
     return stats + (blockIdx.x & (GD_STAT_SLOTS - 1)) * GD_STAT_STRIDE;
 }
-#define GD_LIST_MIN 16384         /* first-pass room of any group (entries) */
+#define GD_LIST_MIN 4096          /* first-pass room of any group (entries) */
+#define GD_PASS2_FLOOR 16384      /* the second pass never gives up on a shorter list */
 #define GD_SPREAD (-2)            /* sizes[] mark: the group is walked body by body */
 
 /*
@@ -1454,7 +1455,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
         k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
-            cnt, cap2, cap2 / 4 < GD_LIST_MIN ? cap2 / 4 : GD_LIST_MIN, g_walk.slotCap);
+            cnt, cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap);
         GD_LAUNCHED();
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,

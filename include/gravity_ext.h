@@ -108,7 +108,7 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
 /* Interaction-list capacities (entries of 4 bytes).  entries: per 64-body
  * group in the first pass.  0 (the default) packs the first pass's lists
  * back to back, each group getting 1.5 times its previous walk's list length
- * and at least 16384 entries; groups found spread out (walked body by body)
+ * and at least 4096 entries; groups found spread out (walked body by body)
  * skip the list passes, re-checked every 32 walks.  A value > 0 gives every
  * group exactly that many entries and no skipping.  Either call forgets the
  * recorded lengths.  secondPass: per

@@ -348,7 +348,7 @@ def test_second_pass_and_per_body_together(mode):
 
 def test_packed_lists_follow_the_walk():
     """Default layout: lists packed from the previous walk's lengths.  The
-    first walk (no record: 16384 entries each) and the ones after give the
+    first walk (no record: 4096 entries each) and the ones after give the
     same accelerations as a walk with fixed room for every list, and after the
     first walk no compact group leaves the first pass."""
     pos, vel = ics.exponential_disk(N, seed=1234)
