@@ -52,7 +52,7 @@ float gd_walk_split_ratio(float theta);
 
 /* radix sort: one tile = 256 threads x 16 keys, 8-bit digits */
 #define GD_SORT_THREADS 256
-#define GD_SORT_ITEMS 16
+#define GD_SORT_ITEMS 15
 #define GD_SORT_TILE (GD_SORT_THREADS * GD_SORT_ITEMS)
 #define GD_SORT_PASSES 8
 #define GD_SORT_BINS 256

@@ -14,7 +14,7 @@
  *                  digits at once (Morton order makes the high digits long
  *                  runs), first into LDS, then into global counters
  *   k_sort_prefix  exclusive scan of each histogram -> digit base offsets
- *   k_sort_pass x8 one 8-bit digit per pass over 4096-key tiles:
+ *   k_sort_pass x8 one 8-bit digit per pass over 3840-key tiles:
  *                  - tile ids come from an atomic ticket, so a tile only ever
  *                    waits on tiles that started before it;
  *                  - keys are ranked inside the tile with wavefront ballots
