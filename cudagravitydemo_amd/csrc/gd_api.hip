@@ -18,6 +18,7 @@
 
 #include <stdarg.h>
 #include <string.h>
+#include <vector>
 #include <sys/sysinfo.h>
 
 extern "C" {
@@ -368,6 +369,18 @@ void gd_set_quadrupole(treeState* tree, int enable) {
     }
 }
 
+/* The build does not store the (zero) moments of single-body leaves; the
+ * inspection copies report them as zero. */
+static void gd_zero_single_leaves(const gd_builder* b, float4* host, int per, int count) {
+    if (count <= 0 || gd_has_error()) return;
+    std::vector<int2> link((size_t)count);
+    GD_HIP(hipMemcpy(link.data(), b->wlink, sizeof(int2) * (size_t)count, hipMemcpyDeviceToHost));
+    if (gd_has_error()) return;
+    for (int i = 0; i < count; ++i)
+        if (link[i].x < 0 && (link[i].y >> 5) == 1)
+            for (int k = 0; k < per; ++k) host[(size_t)per * i + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 void gd_tree_third_moments(const treeState* tree, void* oct, int count) {
     if (!tree || !tree->builder) {
         GD_FAIL("gd_tree_third_moments: uninitialised tree state");
@@ -383,6 +396,7 @@ void gd_tree_third_moments(const treeState* tree, void* oct, int count) {
         return;
     }
     GD_HIP(hipMemcpy(oct, b->woct, 3 * sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
+    gd_zero_single_leaves(b, static_cast<float4*>(oct), 3, count);
 }
 
 void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* quad, int count) {
@@ -397,8 +411,10 @@ void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* qua
     }
     if (com) GD_HIP(hipMemcpy(com, b->wcom, sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
     if (link) GD_HIP(hipMemcpy(link, b->wlink, sizeof(int2) * (size_t)count, hipMemcpyDeviceToHost));
-    if (quad && b->wquad)
+    if (quad && b->wquad) {
         GD_HIP(hipMemcpy(quad, b->wquad, 2 * sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
+        gd_zero_single_leaves(b, static_cast<float4*>(quad), 2, count);
+    }
 }
 
 void gd_tree_levels(treeState* tree) {

@@ -333,12 +333,23 @@ __device__ __forceinline__ gd_mass_result gd_node_mass(ChildCom childCom,
     return r;
 }
 
+/* A single-body leaf's moments about its centre of mass are zero and are
+ * not stored: no walk reads a leaf's moments (leaves are monopoles), and a
+ * parent that finds such a child (its link: a leaf of count 1) takes them as
+ * zero.  Three quarters of the nodes are such leaves, so this saves most of
+ * the moment traffic of the build (80 B written and read back per leaf). */
+__device__ __forceinline__ bool gd_single_leaf(const int2* __restrict__ wlink, int j) {
+    const int2 lk = wlink[j];
+    return lk.x < 0 && (lk.y >> 5) == 1;
+}
+
 /* Second moments about the centre of mass, S = sum m (x - c)(x - c)^T, as
  * two float4: (Sxx, Syy, Szz, Sxy), (Sxz, Syz, trace, 0).  Leaves sum over
  * their bodies; internal nodes shift each child's moments by the parallel-axis
  * term m_c (c_c - c)(c_c - c)^T. */
 __device__ __forceinline__ void gd_node_moments(const float4* __restrict__ pos,
                                                 const float4* __restrict__ wcom,
+                                                const int2* __restrict__ wlink,
                                                 const float4* __restrict__ wquad, int first,
                                                 int count, const int* child, bool leaf, float4 c,
                                                 float4* q0, float4* q1) {
@@ -359,8 +370,11 @@ __device__ __forceinline__ void gd_node_moments(const float4* __restrict__ pos,
         for (int k = 0; k < 8; ++k) {
             if (child[k] == -1) continue;
             const float4 cc = wcom[child[k]];
-            const float4 a = wquad[2 * child[k]];
-            const float4 b = wquad[2 * child[k] + 1];
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+            if (!gd_single_leaf(wlink, child[k])) {
+                a = wquad[2 * child[k]];
+                b = wquad[2 * child[k] + 1];
+            }
             const float dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
             sxx += __builtin_fmaf(cc.w * dx, dx, a.x);
             syy += __builtin_fmaf(cc.w * dy, dy, a.y);
@@ -380,6 +394,7 @@ __device__ __forceinline__ void gd_node_moments(const float4* __restrict__ pos,
  * O += O_c + (S_c,ij d_k + S_c,ik d_j + S_c,jk d_i) + m_c d_i d_j d_k. */
 __device__ __forceinline__ void gd_node_third(const float4* __restrict__ pos,
                                               const float4* __restrict__ wcom,
+                                              const int2* __restrict__ wlink,
                                               const float4* __restrict__ wquad,
                                               const float4* __restrict__ woct, int first,
                                               int count, const int* child, bool leaf, float4 c,
@@ -408,8 +423,14 @@ __device__ __forceinline__ void gd_node_third(const float4* __restrict__ pos,
             if (child[k] == -1) continue;
             const int j = child[k];
             const float4 cc = wcom[j];
-            const float4 a = wquad[2 * j], b = wquad[2 * j + 1];
-            const float4 p = woct[3 * j], q = woct[3 * j + 1], r = woct[3 * j + 2];
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, p = a, q = a, r = a;
+            if (!gd_single_leaf(wlink, j)) {
+                a = wquad[2 * j];
+                b = wquad[2 * j + 1];
+                p = woct[3 * j];
+                q = woct[3 * j + 1];
+                r = woct[3 * j + 2];
+            }
             const float sxx = a.x, syy = a.y, szz = a.z, sxy = a.w, sxz = b.x, syz = b.y;
             const float dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
             const float mx = cc.w * dx, my = cc.w * dy, mz = cc.w * dz;
@@ -460,15 +481,16 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
         wcom[idx] = m.m;
         wlink[idx] = m.leaf ? make_int2(-(first + 1), (count << 5) | level)
                             : make_int2(m.firstChild, (m.nChild << 5) | level);
-        if (MOM >= 1) {
+        const bool single = m.leaf && count == 1;   /* moments zero, not stored */
+        if (MOM >= 1 && !single) {
             float4 q0, q1;
-            gd_node_moments(pos, wcom, wquad, first, count, child, m.leaf, m.m, &q0, &q1);
+            gd_node_moments(pos, wcom, wlink, wquad, first, count, child, m.leaf, m.m, &q0, &q1);
             wquad[2 * idx] = q0;
             wquad[2 * idx + 1] = q1;
         }
-        if (MOM >= 2) {
+        if (MOM >= 2 && !single) {
             float4 o0, o1, o2;
-            gd_node_third(pos, wcom, wquad, woct, first, count, child, m.leaf, m.m, &o0, &o1,
+            gd_node_third(pos, wcom, wlink, wquad, woct, first, count, child, m.leaf, m.m, &o0, &o1,
                           &o2);
             woct[3 * idx] = o0;
             woct[3 * idx + 1] = o1;
