@@ -237,7 +237,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+              "(no multi-GPU path is built)", file=sys.stderr)
         sys.exit(1)
     from cudagravitydemo_amd import Simulation, ics
     from cudagravitydemo_amd import corrector as cr

@@ -91,7 +91,7 @@ def test_theta_zero_walk():
 
 
 def test_reference_steps_per_step_parity():
-    """100 steps of config 0 (16384 bodies, theta 0.5, eps 0.01, dt 1e-3),
+    """100 steps of the 16384-body exponential disk (theta 0.5, eps 0.01, dt 1e-3),
     each compared with the oracle step run on the GPU's own pre-step state."""
     pos, vel = ics.exponential_disk(16384, seed=1234)
     eps, theta, dt = 0.01, 0.5, 1e-3
@@ -285,7 +285,8 @@ def test_judge_matches_oracle_fp64():
 
 
 def test_production_walk_accuracy_disk16k():
-    """Monopole and quadrupole walks against the FP64 judge on config 0.  The
+    """Monopole and quadrupole walks against the FP64 judge on the 16384-body
+    exponential disk (eps 0.01).  The
     monopole walk's median error falls with theta (about 1e-3 at 0.3); the
     quadrupole walk stays below 1e-3 at theta 0.5."""
     pos, vel = ics.exponential_disk(16384, seed=1234)
