@@ -184,7 +184,8 @@ long long gd_corrector_samples(const treeState* tree, const float4* pos, const i
  * every accepted cell whose size / r (from the group's centre) exceeds
  * minRatio, evaluated once per (64-body group, cell) on the matrix cores and
  * scaled per body by m s^3 / r^5.  Needs a tree built with second and third
- * moments (gd_set_quadrupole(tree, 2)). */
+ * moments (gd_set_quadrupole(tree, 2)).  phi, when given, receives the
+ * quadrupole walk's potential: the network corrects accelerations only. */
 void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc, float* phi,
                      float eps2, float G, float theta, int starCount, const gd_mlp* net,
                      float minRatio, gd_walk_stats* stats);
