@@ -517,27 +517,6 @@ __device__ __forceinline__ void gd_mlp_cells(const gd_bf16x4* __restrict__ W,
     }
 }
 
-/* The network for the lanes with need set: compacts them (map in LDS, 64
- * ints), evaluates, and returns each needing lane its own residual. */
-__device__ __forceinline__ void gd_mlp_lanes(const gd_bf16x4* __restrict__ W,
-                                             const float* __restrict__ Bv, bool need, int* map,
-                                             const float* f, float& ox, float& oy, float& oz) {
-    const unsigned long long bal = __ballot(need);
-    ox = oy = oz = 0.f;
-    if (!bal) return;   /* wave-uniform */
-    const int m = __popcll(bal);
-    const int at = __popcll(bal & gd_lanemask_lt());
-    if (need) map[at] = (int)gd_lane();
-    __builtin_amdgcn_wave_barrier();
-    float cx, cy, cz;
-    gd_mlp_cells(W, Bv, m, map, f, cx, cy, cz);
-    const float rx = __shfl(cx, at), ry = __shfl(cy, at), rz = __shfl(cz, at);
-    if (need) {
-        ox = rx; oy = ry; oz = rz;
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
 /* The network for the lanes with need set, results compacted: returns their
  * number n (wave-uniform); for i < n, corr[i] = (k * (ox, oy, oz), the
  * lane's index as int bits) of the i-th such lane in lane order, with k that
@@ -1047,8 +1026,7 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
  * between distant clusters -- sees a huge list through its bounding box).
  * One wavefront walks one body: 64 nodes are tested per step against the
  * body itself, each accepted node is evaluated at once by the lane that
- * tested it (monopole; in MODE 1/2 quadrupole for cells, and in MODE 2 the
- * network's residual evaluated at the body), per-lane sums are
+ * tested it (monopole; in MODE 1 quadrupole for cells), per-lane sums are
  * Kahan-compensated and combined by a fixed butterfly at the end.  Slot q
  * walks body bodyList[q], or, without bodyList, body q % 64 of group (slot)
  * groupList[q / 64] (when the group has that many bodies); WAVES wavefronts per block, each with an LDS stack of
@@ -1062,28 +1040,17 @@ template <int MODE, bool PHI, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
+    const gd_tree_info* __restrict__ info,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList,
     const int* __restrict__ bodyList, long long count,
     int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
-    int n, const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB,
-    float minRatio, unsigned long long* __restrict__ stats) {
+    int n, unsigned long long* __restrict__ stats) {
+    static_assert(MODE == 0 || MODE == 1, "per-body walk: monopole or quadrupole");
     __shared__ int sStackAll[WAVES][STACK];
-    __shared__ int sMapAll[MODE == 2 ? WAVES : 1][64];
-    __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
-    __shared__ float sB[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
-    if (MODE == 2) {
-        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64 * WAVES) sW[e] = mlpW[e];
-        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64 * WAVES) sB[e] = mlpB[e];
-        __syncthreads();
-        mlpW = sW;
-        mlpB = sB;
-    }
     const unsigned lane = gd_lane();
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
     int* sStack = sStackAll[wave];
-    int* sMap = sMapAll[MODE == 2 ? wave : 0];
     const long long q = (long long)gd_xcd_block() * WAVES + wave;
     if (q >= count) return;   /* wave-uniform */
     long long i;
@@ -1101,7 +1068,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const unsigned long long lt = gd_lanemask_lt();
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
-    unsigned long long nInter = 0, nCell = 0, nOpen = 0, nNet = 0;
+    unsigned long long nInter = 0, nCell = 0, nOpen = 0;
     int top = 1;
     if (lane == 0) sStack[0] = 0;
     while (top > 0) {
@@ -1176,26 +1143,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
                 az = __builtin_fmaf(cd, dz, cs * sdz);
                 ph = __builtin_fmaf(0.5f * inv5, __builtin_fmaf(3.f, q, -r2 * q1.z), mi);
             }
-            if (MODE == 2 && cb) {
-                float4 o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0;
-                if (cell) {
-                    o0 = woct[3 * idx];
-                    o1 = woct[3 * idx + 1];
-                    o2 = woct[3 * idx + 2];
-                }
-                float f[8], k, ox, oy, oz;
-                gd_cell_features(c, o0, o1, o2, size, p.x, p.y, p.z, cell, minRatio, f, k);
-                gd_mlp_lanes(mlpW, mlpB, k > 0.f, sMap, f, ox, oy, oz);
-                nNet += (unsigned long long)__popcll(__ballot(k > 0.f));
-                if (cell) {
-                    const float i1 = __builtin_amdgcn_rsqf(d2);
-                    const float i2 = i1 * i1;
-                    const float s5 = k * (i2 * i2 * i1);
-                    ax = __builtin_fmaf(s5, ox, ax);
-                    ay = __builtin_fmaf(s5, oy, ay);
-                    az = __builtin_fmaf(s5, oz, az);
-                }
-            }
         }
         nInter += (unsigned long long)__popcll(__ballot(accept));
         gd_kahan_add(tot.x, tot.cx, ax);
@@ -1220,7 +1167,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
             atomicAdd(&sl[0], nInter);
             atomicAdd(&sl[1], nOpen);
             atomicAdd(&sl[3], nCell);
-            if (MODE == 2) atomicAdd(&sl[4], nNet);
         }
     }
 }
@@ -1481,17 +1427,16 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
     k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
         <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES>>>(
-            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
-            g_walk.overflow2, nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G,
-            theta, n, W, Bv, minRatio, g_walk.stats);
+            pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, g_walk.overflow2,
+            nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
+            g_walk.stats);
     GD_LAUNCHED();
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
     k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, nullptr,
-        g_walk.bodyOver, h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, W, Bv, minRatio,
-        g_walk.stats);
+        pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
+        h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, g_walk.stats);
     GD_LAUNCHED();
 }
 
