@@ -164,8 +164,10 @@ static void gd_builder_walk_arrays(gd_builder* b, int maxNodes) {
     if (maxNodes <= b->maxNodes) return;
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
+    if (b->meta) GD_HIP(hipFree(b->meta));
     GD_HIP(hipMalloc(&b->wcom, sizeof(float4) * (size_t)maxNodes));
     GD_HIP(hipMalloc(&b->wlink, sizeof(int2) * (size_t)maxNodes));
+    GD_HIP(hipMalloc(&b->meta, sizeof(int4) * (size_t)maxNodes));
     b->maxNodes = maxNodes;
 }
 
@@ -238,6 +240,7 @@ void sumDestroyer(treeBuilder* sum) {
     GD_HIP(hipFree(b->scanTiles));
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
+    if (b->meta) GD_HIP(hipFree(b->meta));
     if (b->wquad) GD_HIP(hipFree(b->wquad));
     if (b->woct) GD_HIP(hipFree(b->woct));
     memset(b, 0, sizeof(*b));

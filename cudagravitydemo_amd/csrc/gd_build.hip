@@ -21,13 +21,15 @@
  *                   node of the level above that contains the body
  *                   (tree.cu:224-231); particleCount = size of the body's
  *                   level-L cell, found by galloping over the sorted keys
- *                   (equal to tree.cu:181-203's sibling / parent-end rule);
- *                   child[] = -1
+ *                   (equal to tree.cu:181-203's sibling / parent-end rule); the
+ *                   first child is the body's node one level down, if it starts
+ *                   one.  Written as a 16-byte meta record per node.
  *   k_build_mass    one launch per level, deepest first (tree.cu:89-179, same
- *                   operation order as the oracle); each node also writes its
- *                   own slot in the parent's child[] (tree.cu:232-233), and its
- *                   compact walk record (centre of mass, first child / body,
- *                   count, level)
+ *                   operation order as the oracle): centre of mass, moments,
+ *                   the compact walk record (centre of mass, first child /
+ *                   body, count, level) and the whole 80-byte node record,
+ *                   child[] included (tree.cu:232-233: a child sits in the slot
+ *                   of its octant)
  * Level tables live in gd_tree_info on the device.
  */
 #include "gd_internal.h"
@@ -160,7 +162,7 @@ __device__ __forceinline__ int gd_cell_size(const uint64_t* __restrict__ keys, i
 
 __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
     const uint64_t* __restrict__ keys, int n, const uint32_t* __restrict__ tileOffsets,
-    const uint32_t* __restrict__ levelTotals, int maxTiles, node* __restrict__ nodes,
+    const uint32_t* __restrict__ levelTotals, int maxTiles, int4* __restrict__ meta,
     int maxNodes, gd_tree_info* info) {
     constexpr int WAVES = GD_BUILD_THREADS / GD_WAVE;
     __shared__ uint64_t sk[GD_BUILD_TILE + 2];
@@ -194,14 +196,9 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
             info->totalNodes = off;
             info->overflow = overflow;
             info->zeroMassNodes = 0;
-            if (!overflow) {
-                /* root, basicGravity.cu:158-169 */
-                int4* r = reinterpret_cast<int4*>(nodes);
-                r[0] = make_int4(0, 0, -1, 0);
-                r[1] = make_int4(n, -1, -1, -1);
-                r[2] = make_int4(-1, -1, -1, -1);
-                r[3] = make_int4(-1, 0, 0, 0);
-            }
+            /* root, basicGravity.cu:158-169: all bodies, no parent; with two
+               or more bodies its first child is the first node of level 1 */
+            if (!overflow) meta[0] = make_int4(0, n, -1, n >= 2 ? 1 : -1);
         }
     }
     gd_load_key_tile(keys, n, tileBase, sk);
@@ -247,24 +244,27 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
         const long long i = tileBase + q;
         const uint64_t key = sk[q + 1];
         int prevRank = 0, prevFlag = 0;
+        bool flag = lo[k] <= 1 && 1 <= hi[k];
+        int rank = (int)sRank[k][wave][1] + __popcll(__ballot(flag) & lt);
         for (int L = 1; L <= waveMaxHi; ++L) {
-            const bool flag = lo[k] <= L && L <= hi[k];
-            const unsigned long long bal = __ballot(flag);
-            const int rank = (int)sRank[k][wave][L] + __popcll(bal & lt);
+            /* the body's node one level down, if any, is this node's first child */
+            const bool flagNext = lo[k] <= L + 1 && L + 1 <= hi[k];
+            int rankNext = 0;
+            if (L + 1 <= waveMaxHi)   /* wave-uniform */
+                rankNext = (int)sRank[k][wave][L + 1] + __popcll(__ballot(flagNext) & lt);
             if (flag) {
                 const int idx = sLevelOff[L] + rank;
                 const int parent = (L == 1) ? 0 : sLevelOff[L - 1] + prevRank + prevFlag - 1;
                 const int shift = 63 - 3 * L;
                 const uint64_t cell = key >> shift;
                 const int count = (L > Bv[k]) ? 1 : gd_cell_size(keys, n, i, cell, shift);
-                int4* w = reinterpret_cast<int4*>(nodes + idx);
-                w[0] = make_int4((int)(uint32_t)cell, (int)(uint32_t)(cell >> 32), parent, (int)i);
-                w[1] = make_int4(count, -1, -1, -1);
-                w[2] = make_int4(-1, -1, -1, -1);
-                w[3] = make_int4(-1, L, 0, 0);
+                meta[idx] = make_int4((int)i, count, parent,
+                                      flagNext ? sLevelOff[L + 1] + rankNext : -1);
             }
             prevRank = rank;
             prevFlag = flag;
+            rank = rankNext;
+            flag = flagNext;
         }
     }
 }
@@ -343,160 +343,303 @@ __device__ __forceinline__ bool gd_single_leaf(const int2* __restrict__ wlink, i
     return lk.x < 0 && (lk.y >> 5) == 1;
 }
 
-/* Second moments about the centre of mass, S = sum m (x - c)(x - c)^T, as
- * two float4: (Sxx, Syy, Szz, Sxy), (Sxz, Syz, trace, 0).  Leaves sum over
- * their bodies; internal nodes shift each child's moments by the parallel-axis
- * term m_c (c_c - c)(c_c - c)^T. */
-__device__ __forceinline__ void gd_node_moments(const float4* __restrict__ pos,
-                                                const float4* __restrict__ wcom,
-                                                const int2* __restrict__ wlink,
-                                                const float4* __restrict__ wquad, int first,
-                                                int count, const int* child, bool leaf, float4 c,
-                                                float4* q0, float4* q1) {
+/* Second moments of a leaf about its centre of mass c, S = sum m (x - c)(x - c)^T,
+ * as two float4: (Sxx, Syy, Szz, Sxy), (Sxz, Syz, trace, 0). */
+__device__ __forceinline__ void gd_leaf_moments(const float4* __restrict__ pos, int first,
+                                                int count, float4 c, float4* q0, float4* q1) {
     float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
-    if (leaf) {
-        for (int b = 0; b < count; ++b) {
-            const float4 p = pos[first + b];
-            const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
-            sxx = __builtin_fmaf(p.w * dx, dx, sxx);
-            syy = __builtin_fmaf(p.w * dy, dy, syy);
-            szz = __builtin_fmaf(p.w * dz, dz, szz);
-            sxy = __builtin_fmaf(p.w * dx, dy, sxy);
-            sxz = __builtin_fmaf(p.w * dx, dz, sxz);
-            syz = __builtin_fmaf(p.w * dy, dz, syz);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (child[k] == -1) continue;
-            const float4 cc = wcom[child[k]];
-            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-            if (!gd_single_leaf(wlink, child[k])) {
-                a = wquad[2 * child[k]];
-                b = wquad[2 * child[k] + 1];
-            }
-            const float dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
-            sxx += __builtin_fmaf(cc.w * dx, dx, a.x);
-            syy += __builtin_fmaf(cc.w * dy, dy, a.y);
-            szz += __builtin_fmaf(cc.w * dz, dz, a.z);
-            sxy += __builtin_fmaf(cc.w * dx, dy, a.w);
-            sxz += __builtin_fmaf(cc.w * dx, dz, b.x);
-            syz += __builtin_fmaf(cc.w * dy, dz, b.y);
-        }
+    for (int b = 0; b < count; ++b) {
+        const float4 p = pos[first + b];
+        const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+        sxx = __builtin_fmaf(p.w * dx, dx, sxx);
+        syy = __builtin_fmaf(p.w * dy, dy, syy);
+        szz = __builtin_fmaf(p.w * dz, dz, szz);
+        sxy = __builtin_fmaf(p.w * dx, dy, sxy);
+        sxz = __builtin_fmaf(p.w * dx, dz, sxz);
+        syz = __builtin_fmaf(p.w * dy, dz, syz);
     }
     *q0 = make_float4(sxx, syy, szz, sxy);
     *q1 = make_float4(sxz, syz, sxx + syy + szz, 0.f);
 }
 
-/* Third moments O_ijk = sum m d_i d_j d_k about the COM c (10 components:
- * xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz).  Leaves sum over their bodies;
- * internal nodes shift each child's moments to c (d = c_child - c):
- * O += O_c + (S_c,ij d_k + S_c,ik d_j + S_c,jk d_i) + m_c d_i d_j d_k. */
-__device__ __forceinline__ void gd_node_third(const float4* __restrict__ pos,
-                                              const float4* __restrict__ wcom,
-                                              const int2* __restrict__ wlink,
-                                              const float4* __restrict__ wquad,
-                                              const float4* __restrict__ woct, int first,
-                                              int count, const int* child, bool leaf, float4 c,
-                                              float4* o0, float4* o1, float4* o2) {
+/* Third moments of a leaf, O_ijk = sum m d_i d_j d_k about c, as three float4
+ * (xxx yyy zzz xxy, xxz xyy yyz xzz, yzz xyz - -). */
+__device__ __forceinline__ void gd_leaf_third(const float4* __restrict__ pos, int first,
+                                              int count, float4 c, float4* o0, float4* o1,
+                                              float4* o2) {
     float xxx = 0.f, yyy = 0.f, zzz = 0.f, xxy = 0.f, xxz = 0.f;
     float xyy = 0.f, yyz = 0.f, xzz = 0.f, yzz = 0.f, xyz = 0.f;
-    if (leaf) {
-        for (int b = 0; b < count; ++b) {
-            const float4 p = pos[first + b];
-            const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
-            const float mx = p.w * dx, my = p.w * dy, mz = p.w * dz;
-            xxx = __builtin_fmaf(mx * dx, dx, xxx);
-            yyy = __builtin_fmaf(my * dy, dy, yyy);
-            zzz = __builtin_fmaf(mz * dz, dz, zzz);
-            xxy = __builtin_fmaf(mx * dx, dy, xxy);
-            xxz = __builtin_fmaf(mx * dx, dz, xxz);
-            xyy = __builtin_fmaf(mx * dy, dy, xyy);
-            yyz = __builtin_fmaf(my * dy, dz, yyz);
-            xzz = __builtin_fmaf(mx * dz, dz, xzz);
-            yzz = __builtin_fmaf(my * dz, dz, yzz);
-            xyz = __builtin_fmaf(mx * dy, dz, xyz);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (child[k] == -1) continue;
-            const int j = child[k];
-            const float4 cc = wcom[j];
-            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, p = a, q = a, r = a;
-            if (!gd_single_leaf(wlink, j)) {
-                a = wquad[2 * j];
-                b = wquad[2 * j + 1];
-                p = woct[3 * j];
-                q = woct[3 * j + 1];
-                r = woct[3 * j + 2];
-            }
-            const float sxx = a.x, syy = a.y, szz = a.z, sxy = a.w, sxz = b.x, syz = b.y;
-            const float dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
-            const float mx = cc.w * dx, my = cc.w * dy, mz = cc.w * dz;
-            xxx += p.x + 3.f * sxx * dx + mx * dx * dx;
-            yyy += p.y + 3.f * syy * dy + my * dy * dy;
-            zzz += p.z + 3.f * szz * dz + mz * dz * dz;
-            xxy += p.w + sxx * dy + 2.f * sxy * dx + mx * dx * dy;
-            xxz += q.x + sxx * dz + 2.f * sxz * dx + mx * dx * dz;
-            xyy += q.y + syy * dx + 2.f * sxy * dy + mx * dy * dy;
-            yyz += q.z + syy * dz + 2.f * syz * dy + my * dy * dz;
-            xzz += q.w + szz * dx + 2.f * sxz * dz + mx * dz * dz;
-            yzz += r.x + szz * dy + 2.f * syz * dz + my * dz * dz;
-            xyz += r.y + sxy * dz + sxz * dy + syz * dx + mx * dy * dz;
-        }
+    for (int b = 0; b < count; ++b) {
+        const float4 p = pos[first + b];
+        const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+        const float mx = p.w * dx, my = p.w * dy, mz = p.w * dz;
+        xxx = __builtin_fmaf(mx * dx, dx, xxx);
+        yyy = __builtin_fmaf(my * dy, dy, yyy);
+        zzz = __builtin_fmaf(mz * dz, dz, zzz);
+        xxy = __builtin_fmaf(mx * dx, dy, xxy);
+        xxz = __builtin_fmaf(mx * dx, dz, xxz);
+        xyy = __builtin_fmaf(mx * dy, dy, xyy);
+        yyz = __builtin_fmaf(my * dy, dz, yyz);
+        xzz = __builtin_fmaf(mx * dz, dz, xzz);
+        yzz = __builtin_fmaf(my * dz, dz, yzz);
+        xyz = __builtin_fmaf(mx * dy, dz, xyz);
     }
     *o0 = make_float4(xxx, yyy, zzz, xxy);
     *o1 = make_float4(xxz, xyy, yyz, xzz);
     *o2 = make_float4(yzz, xyz, 0.f, 0.f);
 }
 
-/* MOM: 0 centre of mass only, 1 also second moments, 2 also third moments */
+/*
+ * An internal node of the engine build: centre of mass in the oracle's
+ * operation order (masses of the children in slot order, then the weighted
+ * positions), then its moments about it.  Every child's records are loaded
+ * without a branch (an empty slot reads the first child again and its
+ * contribution is selected away), so all loads are in flight together
+ * instead of one child's dependent chain at a time.  Second moments shift
+ * each child's by the parallel-axis term m_c d d^T (d = c_child - c); third
+ * moments: O += O_c + (S_c,ij d_k + S_c,ik d_j + S_c,jk d_i) + m_c d_i d_j d_k.
+ * A single-body leaf child has zero moments, which are not stored.
+ */
+template <int MOM>
+__device__ __forceinline__ float4 gd_inner_node(const float4* __restrict__ wcom,
+                                                const int2* __restrict__ wlink,
+                                                const float4* __restrict__ wquad,
+                                                const float4* __restrict__ woct,
+                                                const int* child, int fc, float4* q0,
+                                                float4* q1, float4* o0, float4* o1,
+                                                float4* o2) {
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 cc[8];
+    int jj[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        jj[k] = child[k] != -1 ? child[k] : fc;
+        const float4 v = wcom[jj[k]];
+        cc[k] = child[k] != -1 ? v : z4;
+    }
+    float4 a = z4;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.w += cc[k].w;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        a.x = __builtin_fmaf(cc[k].x, cc[k].w, a.x);
+        a.y = __builtin_fmaf(cc[k].y, cc[k].w, a.y);
+        a.z = __builtin_fmaf(cc[k].z, cc[k].w, a.z);
+    }
+    const float4 c = a.w > 0.f ? make_float4(a.x / a.w, a.y / a.w, a.z / a.w, a.w) : z4;
+    if (MOM == 0) return c;
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    float xxx = 0.f, yyy = 0.f, zzz = 0.f, xxy = 0.f, xxz = 0.f;
+    float xyy = 0.f, yyz = 0.f, xzz = 0.f, yzz = 0.f, xyz = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = jj[k];
+        const bool use = child[k] != -1 && !gd_single_leaf(wlink, j);
+        const float4 la = wquad[2 * j], lb = wquad[2 * j + 1];
+        const float4 qa = use ? la : z4, qb = use ? lb : z4;
+        const float w = cc[k].w;   /* 0 for an empty slot */
+        const float dx = cc[k].x - c.x, dy = cc[k].y - c.y, dz = cc[k].z - c.z;
+        if (MOM >= 2) {
+            const float4 lp = woct[3 * j], lq = woct[3 * j + 1], lr = woct[3 * j + 2];
+            const float4 p = use ? lp : z4, q = use ? lq : z4, r = use ? lr : z4;
+            const float mx = w * dx, my = w * dy, mz = w * dz;
+            xxx += p.x + 3.f * qa.x * dx + mx * dx * dx;
+            yyy += p.y + 3.f * qa.y * dy + my * dy * dy;
+            zzz += p.z + 3.f * qa.z * dz + mz * dz * dz;
+            xxy += p.w + qa.x * dy + 2.f * qa.w * dx + mx * dx * dy;
+            xxz += q.x + qa.x * dz + 2.f * qb.x * dx + mx * dx * dz;
+            xyy += q.y + qa.y * dx + 2.f * qa.w * dy + mx * dy * dy;
+            yyz += q.z + qa.y * dz + 2.f * qb.y * dy + my * dy * dz;
+            xzz += q.w + qa.z * dx + 2.f * qb.x * dz + mx * dz * dz;
+            yzz += r.x + qa.z * dy + 2.f * qb.y * dz + my * dz * dz;
+            xyz += r.y + qa.w * dz + qb.x * dy + qb.y * dx + mx * dy * dz;
+        }
+        sxx += __builtin_fmaf(w * dx, dx, qa.x);
+        syy += __builtin_fmaf(w * dy, dy, qa.y);
+        szz += __builtin_fmaf(w * dz, dz, qa.z);
+        sxy += __builtin_fmaf(w * dx, dy, qa.w);
+        sxz += __builtin_fmaf(w * dx, dz, qb.x);
+        syz += __builtin_fmaf(w * dy, dz, qb.y);
+    }
+    *q0 = make_float4(sxx, syy, szz, sxy);
+    *q1 = make_float4(sxz, syz, sxx + syy + szz, 0.f);
+    if (MOM >= 2) {
+        *o0 = make_float4(xxx, yyy, zzz, xxy);
+        *o1 = make_float4(xxz, xyy, yyz, xzz);
+        *o2 = make_float4(yzz, xyz, 0.f, 0.f);
+    }
+    return c;
+}
+
+/* One finished node record (types.h:52-62), written whole: five 16-byte
+ * stores, so consecutive nodes fill whole cache lines. */
+__device__ __forceinline__ void gd_write_record(node* __restrict__ nodes, int idx, uint64_t path,
+                                                int parent, int first, int count, const int* child,
+                                                int level, float4 com) {
+    int4* r = reinterpret_cast<int4*>(nodes + idx);
+    r[0] = make_int4((int)(uint32_t)path, (int)(uint32_t)(path >> 32), parent, first);
+    r[1] = make_int4(count, child[0], child[1], child[2]);
+    r[2] = make_int4(child[3], child[4], child[5], child[6]);
+    r[3] = make_int4(child[7], level, 0, 0);
+    r[4] = make_int4(__float_as_int(com.x), __float_as_int(com.y), __float_as_int(com.z),
+                     __float_as_int(com.w));
+}
+
+/* One internal node at level `level` (shift = 63 - 3 level): its children
+ * are the consecutive nodes of the next level from its first child on whose
+ * parent it is (at most 8, before childEnd), in octant order; a child's
+ * octant is 3 bits of its first body's key.  Writes the walk arrays and the
+ * whole node record; returns 1 for a massless node. */
+template <int MOM>
+__device__ __forceinline__ int gd_finish_inner(node* __restrict__ nodes,
+                                               const uint64_t* __restrict__ keys,
+                                               const int4* __restrict__ meta, int idx, int level,
+                                               int childEnd, float4* __restrict__ wcom,
+                                               int2* __restrict__ wlink,
+                                               float4* __restrict__ wquad,
+                                               float4* __restrict__ woct) {
+    const int shift = 63 - 3 * level;
+    const int4 mt = meta[idx];
+    const int fc = mt.w;
+    const uint64_t key = keys[mt.x];
+    int kid[8], oct[8];
+    kid[0] = fc;
+    oct[0] = (int)((key >> (shift - 3)) & 7u);
+    bool run = true;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        const int j = fc + k;
+        const int4 mk = j < childEnd ? meta[j] : make_int4(0, 0, -1, -1);
+        run = run && mk.z == idx;
+        kid[k] = run ? j : -1;
+        oct[k] = run ? (int)((keys[mk.x] >> (shift - 3)) & 7u) : 8;
+    }
+    int nc = 0;
+    int child[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        nc += kid[k] >= 0 ? 1 : 0;
+        child[k] = -1;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if (oct[k] == o) child[o] = kid[k];
+    float4 q0, q1, o0, o1, o2;
+    const float4 c =
+        gd_inner_node<MOM>(wcom, wlink, wquad, woct, kid, fc, &q0, &q1, &o0, &o1, &o2);
+    gd_write_record(nodes, idx, key >> shift, mt.z, mt.x, mt.y, child, level, c);
+    wcom[idx] = c;
+    wlink[idx] = make_int2(fc, (nc << 5) | level);
+    if (MOM >= 1) {
+        wquad[2 * idx] = q0;
+        wquad[2 * idx + 1] = q1;
+    }
+    if (MOM >= 2) {
+        woct[3 * idx] = o0;
+        woct[3 * idx + 1] = o1;
+        woct[3 * idx + 2] = o2;
+    }
+    return c.w > 0.f ? 0 : 1;
+}
+
+/*
+ * Engine centre of mass, moments and node records for one level (launched
+ * deepest first).  MOM: 0 centre of mass only, 1 also second moments, 2 also
+ * third moments.  Input: the per-node meta of k_build_nodes (first body,
+ * count, parent, first child or -1).  Output: the walk arrays and the whole
+ * 80-byte node record.  Each wavefront scans 64 nodes at a time: a lane whose
+ * node is a leaf finishes it at once (tree.cu:105-125; three quarters of all
+ * nodes are single-body leaves, which store no moments), the internal nodes
+ * (tree.cu:127-179) are queued in the wavefront's LDS and finished 64 at a
+ * time with all lanes, so the long internal path runs in full wavefronts and
+ * no block-wide barrier holds a wavefront back.
+ */
 template <int MOM>
 __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
                                                     const float4* __restrict__ pos,
+                                                    const uint64_t* __restrict__ keys,
+                                                    const int4* __restrict__ meta,
                                                     gd_tree_info* info, int level,
                                                     float4* __restrict__ wcom,
                                                     int2* __restrict__ wlink,
                                                     float4* __restrict__ wquad,
                                                     float4* __restrict__ woct) {
+    __shared__ int sQueueAll[4][128];
     if (info->overflow) return;
     const int cnt = info->levelCount[level];
     const int off = info->levelOffsets[level];
+    const int childEnd =
+        level < GD_MAX_LEVEL ? info->levelOffsets[level + 1] + info->levelCount[level + 1] : 0;
+    const int shift = 63 - 3 * level;
+    const unsigned lane = gd_lane();
+    const unsigned long long lt = gd_lanemask_lt();
+    const int wave = (int)(threadIdx.x >> 6);
+    int* sQueue = sQueueAll[wave];
+    const int waves = gridDim.x * 4;
+    const int none[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    int queued = 0;   /* wave-uniform */
     int zeros = 0;
-    for (int t = blockIdx.x * 256 + threadIdx.x; t < cnt; t += gridDim.x * 256) {
-        const int idx = off + t;
-        const int4* r = reinterpret_cast<const int4*>(nodes + idx);
-        const int4 w0 = r[0], w1 = r[1], w2 = r[2], w3 = r[3];
-        const int child[8] = {w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, w3.x};
-        const int first = w0.w, count = w1.x;
-        const gd_mass_result m =
-            gd_node_mass([wcom](int j) { return wcom[j]; }, pos, first, count, child);
-        nodes[idx].massData = m.m;
-        zeros += m.zero;
-        if (level > 0) {
-            const int parent = w0.z;
-            nodes[parent].child[(uint32_t)w0.x & 7u] = idx;
+    for (int c0 = (blockIdx.x * 4 + wave) * 64; c0 < cnt; c0 += waves * 64) {
+        const int t = c0 + (int)lane;
+        bool inner = false;
+        if (t < cnt) {
+            const int idx = off + t;
+            const int4 mt = meta[idx];
+            inner = mt.w >= 0;
+            if (!inner) {
+                const int first = mt.x, count = mt.y;
+                const uint64_t key = keys[first];
+                float sx = 0.f, sy = 0.f, sz = 0.f, m = 0.f;
+                for (int b = 0; b < count; ++b) {
+                    const float4 p = pos[first + b];
+                    sx = __builtin_fmaf(p.x, p.w, sx);
+                    sy = __builtin_fmaf(p.y, p.w, sy);
+                    sz = __builtin_fmaf(p.z, p.w, sz);
+                    m += p.w;
+                }
+                float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (m > 0.f) c = make_float4(sx / m, sy / m, sz / m, m);
+                else zeros++;
+                gd_write_record(nodes, idx, key >> shift, mt.z, first, count, none, level, c);
+                wcom[idx] = c;
+                wlink[idx] = make_int2(-(first + 1), (count << 5) | level);
+                if (count > 1) {   /* moments of a single-body leaf are zero, not stored */
+                    if (MOM >= 1) {
+                        float4 q0, q1;
+                        gd_leaf_moments(pos, first, count, c, &q0, &q1);
+                        wquad[2 * idx] = q0;
+                        wquad[2 * idx + 1] = q1;
+                    }
+                    if (MOM >= 2) {
+                        float4 o0, o1, o2;
+                        gd_leaf_third(pos, first, count, c, &o0, &o1, &o2);
+                        woct[3 * idx] = o0;
+                        woct[3 * idx + 1] = o1;
+                        woct[3 * idx + 2] = o2;
+                    }
+                }
+            }
         }
-        wcom[idx] = m.m;
-        wlink[idx] = m.leaf ? make_int2(-(first + 1), (count << 5) | level)
-                            : make_int2(m.firstChild, (m.nChild << 5) | level);
-        const bool single = m.leaf && count == 1;   /* moments zero, not stored */
-        if (MOM >= 1 && !single) {
-            float4 q0, q1;
-            gd_node_moments(pos, wcom, wlink, wquad, first, count, child, m.leaf, m.m, &q0, &q1);
-            wquad[2 * idx] = q0;
-            wquad[2 * idx + 1] = q1;
-        }
-        if (MOM >= 2 && !single) {
-            float4 o0, o1, o2;
-            gd_node_third(pos, wcom, wlink, wquad, woct, first, count, child, m.leaf, m.m, &o0, &o1,
-                          &o2);
-            woct[3 * idx] = o0;
-            woct[3 * idx + 1] = o1;
-            woct[3 * idx + 2] = o2;
+        const unsigned long long bal = __ballot(inner);
+        if (inner) sQueue[queued + __popcll(bal & lt)] = off + t;
+        queued += __popcll(bal);
+        __builtin_amdgcn_wave_barrier();
+        if (queued >= 64) {
+            const int idx = sQueue[lane];
+            const int rest = queued - 64 > (int)lane ? sQueue[64 + lane] : 0;
+            __builtin_amdgcn_wave_barrier();
+            if (queued - 64 > (int)lane) sQueue[lane] = rest;
+            queued -= 64;
+            zeros += gd_finish_inner<MOM>(nodes, keys, meta, idx, level, childEnd, wcom, wlink,
+                                          wquad, woct);
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    if ((int)lane < queued)
+        zeros += gd_finish_inner<MOM>(nodes, keys, meta, sQueue[lane], level, childEnd, wcom,
+                                      wlink, wquad, woct);
     if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
 }
 
@@ -514,7 +657,7 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
                                          b->levelTotals);
     GD_LAUNCHED();
     k_build_nodes<<<tiles, GD_BUILD_THREADS>>>(sortedKeys, n, b->tileOffsets, b->levelTotals,
-                                               b->maxTiles, nodes, maxNodes, b->dinfo);
+                                               b->maxTiles, b->meta, maxNodes, b->dinfo);
     GD_LAUNCHED();
     for (int L = GD_MAX_LEVEL; L >= 0; --L) {
         /* a level holds at most min(8^L, n) nodes */
@@ -524,16 +667,16 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
             bound = p8 < bound ? p8 : bound;
         }
         int blocks = gd_div_up(bound, 256);
-        if (blocks > 4096) blocks = 4096;   /* 16 blocks per CU: every wave slot busy */
+        if (blocks > 4096) blocks = 4096;   /* 16 blocks per CU */
         if (b->quad == 2)
-            k_build_mass<2><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
-                                             b->wquad, b->woct);
+            k_build_mass<2><<<blocks, 256>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
+                                             b->wcom, b->wlink, b->wquad, b->woct);
         else if (b->quad == 1)
-            k_build_mass<1><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
-                                             b->wquad, nullptr);
+            k_build_mass<1><<<blocks, 256>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
+                                             b->wcom, b->wlink, b->wquad, nullptr);
         else
-            k_build_mass<0><<<blocks, 256>>>(nodes, sortedPos, b->dinfo, L, b->wcom, b->wlink,
-                                             nullptr, nullptr);
+            k_build_mass<0><<<blocks, 256>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
+                                             b->wcom, b->wlink, nullptr, nullptr);
         GD_LAUNCHED();
     }
 }

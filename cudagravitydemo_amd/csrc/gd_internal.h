@@ -111,6 +111,9 @@ typedef struct {
     float3* bboxPartMax;
     /* prefixSum scratch */
     uint32_t* scanTiles;             /* tile sums */
+    /* per-node build records, maxNodes entries: first body, body count,
+       parent, first child (-1 for a leaf) */
+    int4* meta;
     /* compact walk arrays, maxNodes entries each */
     float4* wcom;                    /* centre of mass + mass */
     int2* wlink;                     /* .x first child (>= 0) or -(first body + 1) for a leaf,
