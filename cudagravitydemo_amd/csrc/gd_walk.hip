@@ -757,6 +757,100 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
 }
 
 /*
+ * Second-pass evaluation cut into segments.  The few groups that do not fit
+ * the first pass have the longest lists; one wavefront each, they would be
+ * evaluated nearly alone at the end of the walk.  Here slot s's list (of
+ * lists[s * cap ..]) is cut into `segs` parts -- front (monopole) and back
+ * (cell) entries alike -- and wavefront q evaluates part q % segs of slot
+ * q / segs for the group's bodies, writing its per-body sums (x, y, z,
+ * potential) to part[q * 64 + lane]; k_walk_eval_sum adds the parts in order.
+ */
+template <int MODE, bool PHI>
+__global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const float4* __restrict__ woct,
+    const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange,
+    const int* __restrict__ groupList, int slots, int segs,
+    const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
+    float4* __restrict__ part, float eps2,
+    const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio,
+    unsigned long long* __restrict__ stats) {
+    __shared__ float4 sE[GD_EV_WAVES][2 * 64];
+    __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
+    __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
+    __shared__ int sMap[MODE == 2 ? GD_EV_WAVES : 1][64];
+    __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
+    __shared__ float sB[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
+    if (MODE == 2) {
+        for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64 * GD_EV_WAVES)
+            sW[e] = mlpW[e];
+        for (int e = threadIdx.x; e < GD_MLP_BIAS_PAD; e += 64 * GD_EV_WAVES) sB[e] = mlpB[e];
+        __syncthreads();
+        mlpW = sW;
+        mlpB = sB;
+    }
+    const unsigned lane = gd_lane();
+    const int wave = (int)(threadIdx.x >> 6);
+    const long long q = (long long)gd_xcd_block() * GD_EV_WAVES + wave;
+    if (q >= (long long)slots * segs) return;
+    const long long s = q / segs;
+    const int seg = (int)(q - s * segs);
+    const int2 cnt = counts[s];
+    if (cnt.x < 0) return;   /* left to the per-body walk */
+    const int2 rg = slotRange[groupList[s]];
+    if (rg.y == 0) return;
+    const int f0 = (int)((long long)cnt.x * seg / segs);
+    const int f1 = (int)((long long)cnt.x * (seg + 1) / segs);
+    const int b0 = (int)((long long)cnt.y * seg / segs);
+    const int b1 = (int)((long long)cnt.y * (seg + 1) / segs);
+    /* front entries f0.. at list[f0..]; back entry e of the part is the
+       list's back entry b0 + e, at lists[s * cap + cap - 1 - b0 - e] */
+    gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float phs = 0.f, phc = 0.f;
+    unsigned long long nn = 0;
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info,
+                             lists + (size_t)s * cap + f0, cap - f0 - b0,
+                             make_int2(f1 - f0, b1 - b0), rg.x, rg.y, eps2, mlpW, mlpB,
+                             minRatio, sE[wave], sQ0[MODE != 0 ? wave : 0],
+                             sQ1[MODE != 0 ? wave : 0], sMap[MODE == 2 ? wave : 0], tot, phs,
+                             phc, &nn);
+    if (MODE == 2 && stats && lane == 0 && nn) atomicAdd(&gd_stat_slot(stats)[4], nn);
+    part[q * 64 + lane] = make_float4(tot.x, tot.y, tot.z, phs);
+}
+
+/* The segments' sums of each second-pass group, added in segment order with
+ * compensation: one wavefront per slot, lane = body. */
+template <bool PHI>
+__global__ __launch_bounds__(256) void k_walk_eval_sum(const int2* __restrict__ slotRange,
+                                                       const int* __restrict__ groupList,
+                                                       int slots, int segs,
+                                                       const int2* __restrict__ counts,
+                                                       const float4* __restrict__ part,
+                                                       float3* __restrict__ acc,
+                                                       float* __restrict__ phi, float G) {
+    const unsigned lane = gd_lane();
+    const long long s = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= slots) return;
+    if (counts[s].x < 0) return;
+    const int2 rg = slotRange[groupList[s]];
+    if ((int)lane >= rg.y) return;
+    gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float phs = 0.f, phc = 0.f;
+#pragma unroll 8
+    for (int k = 0; k < segs; ++k) {
+        const float4 v = part[((size_t)s * segs + k) * 64 + lane];
+        gd_kahan_add(tot.x, tot.cx, v.x);
+        gd_kahan_add(tot.y, tot.cy, v.y);
+        gd_kahan_add(tot.z, tot.cz, v.z);
+        if (PHI) gd_kahan_add(phs, phc, v.w);
+    }
+    const long long i = rg.x + lane;
+    acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
+    if (PHI) phi[i] = -G * phs;
+}
+
+/*
  * Learned corrections of a wavefront's queued cells (first pass, learned
  * mode): queue[0 .. qn) holds node indices, bit 30 set for the cells of half
  * B.  Lane l < qn computes cell l's features at its half's centre, the
@@ -1211,9 +1305,11 @@ struct gd_walk_scratch {
     long long walks;       /* walks so far (spread marks are cleared every 32) */
     int deepStack;         /* first pass with 2 x GD_TR_STACK stack entries */
     unsigned long long* kstats;   /* the list passes' statistics, or NULL (not asked for) */
+    float4* segPart;       /* GD_SEG_WAVES x 64: per-body sums of second-pass segments */
 };
 
 #define GD_LIST2 262144
+#define GD_SEG_WAVES 2048         /* second pass: groups x segments per launch */
 #define GD_POOL_BYTES (8ll << 30)
 static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
@@ -1235,6 +1331,7 @@ static void gd_walk_ensure(int groups) {
     if (!g_walk.stats) {
         GD_HIP(hipMalloc(&g_walk.stats, GD_STAT_WORDS * sizeof(unsigned long long)));
         GD_HIP(hipMalloc(&g_walk.status, sizeof(gd_walk_status)));
+        GD_HIP(hipMalloc(&g_walk.segPart, (size_t)GD_SEG_WAVES * 64 * sizeof(float4)));
     }
     if (groups > g_walk.groupCap) {
         if (g_walk.overflow1) GD_HIP(hipFree(g_walk.overflow1));
@@ -1408,9 +1505,25 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
             cnt, nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats);
         GD_LAUNCHED();
-        gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool, nullptr,
-                                  nullptr, g_walk.counts2, cap2, acc, phi, eps2, G, n, W, Bv,
-                                  minRatio);
+        /* few groups: each list cut into segments over many wavefronts */
+        const int segs = cnt <= GD_SEG_WAVES / 2 ? (GD_SEG_WAVES / cnt < 64 ? GD_SEG_WAVES / cnt : 64)
+                                                 : 1;
+        if (segs > 1) {
+            k_walk_eval_seg<MODE, PHI>
+                <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
+                    pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
+                    g_walk.overflow1 + b0, cnt, segs, g_walk.pool, g_walk.counts2, cap2,
+                    g_walk.segPart, eps2, W, Bv, minRatio, g_walk.kstats);
+            GD_LAUNCHED();
+            k_walk_eval_sum<PHI><<<gd_div_up(cnt, 4), 256>>>(g_walk.slotRange,
+                                                             g_walk.overflow1 + b0, cnt, segs,
+                                                             g_walk.counts2, g_walk.segPart,
+                                                             acc, phi, G);
+        } else {
+            gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool, nullptr,
+                                      nullptr, g_walk.counts2, cap2, acc, phi, eps2, G, n, W,
+                                      Bv, minRatio);
+        }
         GD_LAUNCHED();
     }
     if (over1 > 0) GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
