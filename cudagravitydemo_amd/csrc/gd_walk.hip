@@ -457,19 +457,50 @@ __device__ __forceinline__ void gd_eval64_quad(const float4* __restrict__ e,
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float inv2 = inv * inv;
-        const float mi = c.w * inv;
-        const float inv5 = inv * inv2 * inv2;
+        const float inv3 = inv * inv2;
+        const float inv5 = inv3 * inv2;
         /* with S' = 7.5 S: sd = S' d, q = d.S' d */
         const float sdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.w, dy, b.x * dz));
         const float sdy = __builtin_fmaf(a.w, dx, __builtin_fmaf(a.y, dy, b.y * dz));
         const float sdz = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, a.z * dz));
         const float q = __builtin_fmaf(dx, sdx, __builtin_fmaf(dy, sdy, dz * sdz));
-        const float cd = __builtin_fmaf(mi, inv2, inv5 * __builtin_fmaf(q, inv2, b.z));
+        const float cd = __builtin_fmaf(c.w, inv3, inv5 * __builtin_fmaf(q, inv2, b.z));
         const float cs = -0.4f * inv5;
         ax = __builtin_fmaf(cd, dx, __builtin_fmaf(cs, sdx, ax));
         ay = __builtin_fmaf(cd, dy, __builtin_fmaf(cs, sdy, ay));
         az = __builtin_fmaf(cd, dz, __builtin_fmaf(cs, sdz, az));
-        if (PHI) ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(0.4f, q, -r2 * b.w), mi);
+        if (PHI) ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(0.4f, q, -r2 * b.w), c.w * inv);
+    }
+}
+
+/* The same for 64 cells stored as one record each, e3[3k .. 3k+2] = (com,
+ * 7.5 S first four, 7.5 Sxz 7.5 Syz -1.5 trS trS): one LDS base address
+ * serves all three reads of an entry. */
+template <bool PHI>
+__device__ __forceinline__ void gd_eval64_quad3(const float4* __restrict__ e3, float px, float py,
+                                                float pz, float eps2, float& ax, float& ay,
+                                                float& az, float& ph) {
+#pragma unroll 2
+    for (int k = 0; k < 64; ++k) {
+        const float4 c = e3[3 * k];
+        const float4 a = e3[3 * k + 1];
+        const float4 b = e3[3 * k + 2];
+        const float dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float inv3 = inv * inv2;
+        const float inv5 = inv3 * inv2;
+        const float sdx = __builtin_fmaf(a.x, dx, __builtin_fmaf(a.w, dy, b.x * dz));
+        const float sdy = __builtin_fmaf(a.w, dx, __builtin_fmaf(a.y, dy, b.y * dz));
+        const float sdz = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, a.z * dz));
+        const float q = __builtin_fmaf(dx, sdx, __builtin_fmaf(dy, sdy, dz * sdz));
+        const float cd = __builtin_fmaf(c.w, inv3, inv5 * __builtin_fmaf(q, inv2, b.z));
+        const float cs = -0.4f * inv5;
+        ax = __builtin_fmaf(cd, dx, __builtin_fmaf(cs, sdx, ax));
+        ay = __builtin_fmaf(cd, dy, __builtin_fmaf(cs, sdy, ay));
+        az = __builtin_fmaf(cd, dz, __builtin_fmaf(cs, sdz, az));
+        if (PHI) ph += __builtin_fmaf(0.5f * inv5, __builtin_fmaf(0.4f, q, -r2 * b.w), c.w * inv);
     }
 }
 
@@ -977,12 +1008,11 @@ __device__ __forceinline__ void gd_eval_pair(
         if (nB > 0) { gbx = 0.5f * (lx + hx); gby = 0.5f * (ly + hy); gbz = 0.5f * (lz + hz); }
     }
     const float ext = info->box.extent;
-    float4* sQA0 = sQ;
-    float4* sQA1 = sQ + 64;
-    float4* sQB0 = sQ + 128;
-    float4* sQB1 = sQ + 192;
-    const float4* myQ0 = inB ? sQB0 : sQA0;
-    const float4* myQ1 = inB ? sQB1 : sQA1;
+    /* cell records: half A's 64 at sQ[0 .. 191], half B's at sQ[192 ..], three
+       float4 each (the monopole buffers sA, sB alias the start of sQ, done) */
+    float4* recA = sQ + 3 * lane;
+    float4* recB = sQ + 192 + 3 * lane;
+    const float4* myRec = sQ + (inB ? 192 : 0);
     int qn = 0;   /* queued cells (learned mode) */
     float lx = 0.f, ly = 0.f, lz = 0.f;   /* their corrections, since the last Kahan add */
     for (int c0 = 0; c0 < nbm; c0 += 64) {
@@ -1001,15 +1031,15 @@ __device__ __forceinline__ void gd_eval_pair(
             qb0 = wquad[2 * ib];
             qb1 = wquad[2 * ib + 1];
         }
-        sA[lane] = ca;
-        sB[lane] = cb;
-        sQA0[lane] = make_float4(7.5f * qa0.x, 7.5f * qa0.y, 7.5f * qa0.z, 7.5f * qa0.w);
-        sQA1[lane] = make_float4(7.5f * qa1.x, 7.5f * qa1.y, -1.5f * qa1.z, qa1.z);
-        sQB0[lane] = make_float4(7.5f * qb0.x, 7.5f * qb0.y, 7.5f * qb0.z, 7.5f * qb0.w);
-        sQB1[lane] = make_float4(7.5f * qb1.x, 7.5f * qb1.y, -1.5f * qb1.z, qb1.z);
+        recA[0] = ca;
+        recA[1] = make_float4(7.5f * qa0.x, 7.5f * qa0.y, 7.5f * qa0.z, 7.5f * qa0.w);
+        recA[2] = make_float4(7.5f * qa1.x, 7.5f * qa1.y, -1.5f * qa1.z, qa1.z);
+        recB[0] = cb;
+        recB[1] = make_float4(7.5f * qb0.x, 7.5f * qb0.y, 7.5f * qb0.z, 7.5f * qb0.w);
+        recB[2] = make_float4(7.5f * qb1.x, 7.5f * qb1.y, -1.5f * qb1.z, qb1.z);
         __builtin_amdgcn_wave_barrier();
         float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
-        gd_eval64_quad<PHI>(myE, myQ0, myQ1, px, py, pz, eps2, ax, ay, az, ph);
+        gd_eval64_quad3<PHI>(myRec, px, py, pz, eps2, ax, ay, az, ph);
         if (MODE == 2) {
             /* queue the cells each half corrects: s / r > minRatio from its centre,
                with the feature function's own rounding of s / r */
@@ -1069,9 +1099,9 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     const int2* __restrict__ counts, float3* __restrict__ acc, float* __restrict__ phi,
     float eps2, float G, const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB,
     float minRatio, unsigned long long* __restrict__ stats) {
-    __shared__ float4 sA[EVW][64];
-    __shared__ float4 sB[EVW][64];
-    __shared__ float4 sQ[MODE != 0 ? EVW : 1][MODE != 0 ? 256 : 1];
+    /* per wavefront: monopole chunks of both halves (2 x 64 float4), then,
+       reusing the same memory, cell records (2 x 64 x 3 float4) */
+    __shared__ float4 sE[EVW][MODE != 0 ? 384 : 128];
     __shared__ int sMap[MODE == 2 ? EVW : 1][64];
     __shared__ int sQueue[MODE == 2 ? EVW : 1][64];
     __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
@@ -1102,8 +1132,8 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     float phs = 0.f, phc = 0.f;
     unsigned long long nn = 0;
     gd_eval_pair<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, listA, capA, cA, listB, capB,
-                            cB, rgA.x, rgA.y, rgB.y, eps2, mlpW, mlpB, minRatio, sA[wave],
-                            sB[wave], sQ[MODE != 0 ? wave : 0], sMap[MODE == 2 ? wave : 0],
+                            cB, rgA.x, rgA.y, rgB.y, eps2, mlpW, mlpB, minRatio, sE[wave],
+                            sE[wave] + 64, sE[wave], sMap[MODE == 2 ? wave : 0],
                             sQueue[MODE == 2 ? wave : 0], tot, phs, phc, &nn);
     if (MODE == 2 && stats && lane == 0 && nn) atomicAdd(&gd_stat_slot(stats)[4], nn);
     const bool inB = (int)lane >= rgA.y;
