@@ -887,15 +887,17 @@ __global__ __launch_bounds__(256) void k_walk_eval_sum(const int2* __restrict__ 
  * B.  Lane l < qn computes cell l's features at its half's centre, the
  * network runs on the qn cells at once (full 16-row tiles instead of a few
  * cells per chunk), and every body adds k / r^5 * output of each cell of its
- * own half (k = m s^3, r from the body) to (ax, ay, az).  map: 64 ints of
- * LDS.  Returns the number of network evaluations.
+ * own half (k = m s^3, r from the body) to (ax, ay, az), reading its half's
+ * records (centre, k * output) from rec, so both halves' lanes work in the
+ * same steps.  map: 64 ints, rec: 256 float4 of LDS.  Returns the number of
+ * network evaluations.
  */
 __device__ __forceinline__ int gd_flush_corr(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
     const float4* __restrict__ woct, float ext, const gd_bf16x4* __restrict__ W,
     const float* __restrict__ Bv, float minRatio, const int* queue, int qn, int* map,
     float gax, float gay, float gaz, float gbx, float gby, float gbz, bool inB, float px,
-    float py, float pz, float& ax, float& ay, float& az) {
+    float py, float pz, float4* rec, float& ax, float& ay, float& az) {
     const unsigned lane = gd_lane();
     const bool have = (int)lane < qn;
     const int e = have ? queue[lane] : 0;
@@ -917,24 +919,40 @@ __device__ __forceinline__ int gd_flush_corr(
     __builtin_amdgcn_wave_barrier();
     float cx, cy, cz;
     gd_mlp_cells(W, Bv, qn, map, f, cx, cy, cz);
-    const float kx = k * cx, ky = k * cy, kz = k * cz;
-    const int hb = half ? 1 : 0;
-    for (int kk = 0; kk < qn; ++kk) {
-        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), kk));
-        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), kk));
-        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.z), kk));
-        const float ox = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kx), kk));
-        const float oy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ky), kk));
-        const float oz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kz), kk));
-        const int h = __builtin_amdgcn_readlane(hb, kk);
-        const float dx = qx - px, dy = qy - py, dz = qz - pz;
+    /* each half's corrected cells as records (centre, k * output) in LDS, the
+       shorter half padded with far-away zero entries; every lane then reads
+       its own half's records, both halves in the same steps */
+    const unsigned long long lt = gd_lanemask_lt();
+    const unsigned long long balA = __ballot(have && !half), balB = __ballot(have && half);
+    const int nA = __popcll(balA), nB = __popcll(balB);
+    const int nmax = nA > nB ? nA : nB;
+    if (have) {
+        const int j = __popcll((half ? balB : balA) & lt);
+        float4* r = rec + (half ? 128 : 0) + 2 * j;
+        r[0] = make_float4(c.x, c.y, c.z, 0.f);
+        r[1] = make_float4(k * cx, k * cy, k * cz, 0.f);
+    }
+    const float4 far = make_float4(1e18f, 1e18f, 1e18f, 0.f), zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((int)lane >= nA && (int)lane < nmax) {
+        rec[2 * lane] = far;
+        rec[2 * lane + 1] = zero;
+    }
+    if ((int)lane >= nB && (int)lane < nmax) {
+        rec[128 + 2 * lane] = far;
+        rec[128 + 2 * lane + 1] = zero;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const float4* my = rec + (inB ? 128 : 0);
+    for (int kk = 0; kk < nmax; ++kk) {
+        const float4 q = my[2 * kk], o = my[2 * kk + 1];
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         const float i1 = __builtin_amdgcn_rsqf(r2);
         const float i2 = i1 * i1;
-        const float s5 = (inB == (h != 0)) ? i2 * i2 * i1 : 0.f;
-        ax = __builtin_fmaf(s5, ox, ax);
-        ay = __builtin_fmaf(s5, oy, ay);
-        az = __builtin_fmaf(s5, oz, az);
+        const float s5 = i2 * i2 * i1;
+        ax = __builtin_fmaf(s5, o.x, ax);
+        ay = __builtin_fmaf(s5, o.y, ay);
+        az = __builtin_fmaf(s5, o.z, az);
     }
     __builtin_amdgcn_wave_barrier();
     return qn;
@@ -951,7 +969,9 @@ __device__ __forceinline__ int gd_flush_corr(
  * for a half that is not evaluated here (empty, or left to the later passes).
  * In learned mode the cells to correct (s / r > minRatio from their half's
  * centre) are queued (sQueue, 64 ints) and the network runs on full batches
- * (gd_flush_corr).  sQ holds 4 x 64 float4 (A's and B's prescaled moments).
+ * (gd_flush_corr).  sQ holds 384 float4: a cell chunk's records, three
+ * float4 per entry and half (sA, sB alias its start for the monopole chunks),
+ * and between chunks the flush's per-half correction records.
  */
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_pair(
@@ -1060,7 +1080,7 @@ __device__ __forceinline__ void gd_eval_pair(
                 if (qn + m > 64) {   /* wave-uniform */
                     *nnEvals += (unsigned long long)gd_flush_corr(
                         wcom, wlink, woct, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax,
-                        gay, gaz, gbx, gby, gbz, inB, px, py, pz, lx, ly, lz);
+                        gay, gaz, gbx, gby, gbz, inB, px, py, pz, sQ, lx, ly, lz);
                     qn = 0;
                 }
                 if (need) sQueue[qn + __popcll(bal & gd_lanemask_lt())] = id | (h << 30);
@@ -1078,7 +1098,7 @@ __device__ __forceinline__ void gd_eval_pair(
         if (qn > 0)
             *nnEvals += (unsigned long long)gd_flush_corr(
                 wcom, wlink, woct, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax, gay, gaz,
-                gbx, gby, gbz, inB, px, py, pz, lx, ly, lz);
+                gbx, gby, gbz, inB, px, py, pz, sQ, lx, ly, lz);
         gd_kahan_add(tot.x, tot.cx, lx);
         gd_kahan_add(tot.y, tot.cy, ly);
         gd_kahan_add(tot.z, tot.cz, lz);
