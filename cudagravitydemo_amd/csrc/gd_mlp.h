@@ -81,7 +81,9 @@ __device__ __forceinline__ gd_f32x4 gd_mfma32(gd_bf16x4 a0, gd_bf16x4 a1, gd_bf1
  * exponential and one reciprocal (both hardware approximations, ~1 ulp),
  * exact limits z (u -> +inf) and 0 (u -> -inf). */
 __device__ __forceinline__ float gd_gelu(float z) {
-    const float u2 = (2.f * GD_GELU_C * 1.4426950408889634f) * __builtin_fmaf(GD_GELU_A * z * z, z, z);
+    /* 2u log2(e) = z (K + K a z^2), K = 2 c log2(e) */
+    constexpr float K = 2.f * GD_GELU_C * 1.4426950408889634f;
+    const float u2 = z * __builtin_fmaf(K * GD_GELU_A, z * z, K);
     const float r = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u2));
     return __builtin_fmaf(-z, r, z);
 }
@@ -95,18 +97,21 @@ __device__ __forceinline__ float gd_gelu_grad(float z) {
 }
 
 /* Inference only (no activations kept): layer 1 on 16x16x16 tiles, the
- * 64-wide layers on 16x16x32 (see gd_mfma32). */
+ * 64-wide layers on 16x16x32 (see gd_mfma32).  The bias enters as the
+ * accumulator's initial value (rows r0..r0+3 of the lane's column). */
 __device__ __forceinline__ gd_f32x4 gd_mlp_infer16(const gd_bf16x4* __restrict__ fwd,
                                                    const float* __restrict__ bias, gd_bf16x4 x) {
     const unsigned l = gd_lane();
     const int r0 = 4 * (int)(l / 16);
-    gd_f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    auto b4 = [&](int at) {
+        const gd_f32x4 v = {bias[at + r0], bias[at + r0 + 1], bias[at + r0 + 2],
+                            bias[at + r0 + 3]};
+        return v;
+    };
     gd_bf16x4 in[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        gd_f32x4 z = gd_mfma(fwd[t * 64 + l], x, zero);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) z[j] += bias[16 * t + r0 + j];
+        const gd_f32x4 z = gd_mfma(fwd[t * 64 + l], x, b4(16 * t));
         in[t] = gd_pack4(gd_gelu(z[0]), gd_gelu(z[1]), gd_gelu(z[2]), gd_gelu(z[3]));
     }
 #pragma unroll
@@ -116,20 +121,16 @@ __device__ __forceinline__ gd_f32x4 gd_mlp_infer16(const gd_bf16x4* __restrict__
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             gd_f32x4 z = gd_mfma32(W[(t * 4 + 0) * 64 + l], W[(t * 4 + 1) * 64 + l], in[0], in[1],
-                                   zero);
+                                   b4(64 * L + 16 * t));
             z = gd_mfma32(W[(t * 4 + 2) * 64 + l], W[(t * 4 + 3) * 64 + l], in[2], in[3], z);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) z[j] += bias[64 * L + 16 * t + r0 + j];
             nxt[t] = gd_pack4(gd_gelu(z[0]), gd_gelu(z[1]), gd_gelu(z[2]), gd_gelu(z[3]));
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) in[t] = nxt[t];
     }
     const gd_bf16x4* W4 = fwd + 36 * 64;
-    gd_f32x4 o = gd_mfma32(W4[0 * 64 + l], W4[1 * 64 + l], in[0], in[1], zero);
+    gd_f32x4 o = gd_mfma32(W4[0 * 64 + l], W4[1 * 64 + l], in[0], in[1], b4(192));
     o = gd_mfma32(W4[2 * 64 + l], W4[3 * 64 + l], in[2], in[3], o);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) o[j] += bias[192 + r0 + j];
     return o;
 }
 
