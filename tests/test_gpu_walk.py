@@ -322,6 +322,28 @@ def test_mixed_lists_and_per_body(mode):
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
+def test_few_long_lists_in_the_second_pass(mode):
+    """First-pass capacity just below the five longest lists: those five go to
+    the second pass, which cuts each list into 64 segments over as many
+    wavefronts and adds the segments' sums in order -- against the
+    interaction sets in FP64."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        sizes = _group_sizes(sim, mode)
+        cap = int(np.sort(sizes)[-6])
+        over, body = _predict(sim, mode, sizes, cap, 262144)
+        assert 1 <= len(over) <= 5 and not body
+        slots, slot_of = _slots(sim, mode)
+        st = _run(sim, mode, cap, params)
+        assert st["overflow_groups"] == len(over) and st["body_groups"] == 0
+        extra = [int(slots[g, 0]) + k for g in over for k in (0, int(slots[g, 1]) - 1)]
+        _check(sim, mode, lambda i: False, params, extra=extra)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_second_pass_and_per_body_together(mode):
     """First-pass capacity at the median list, second-pass capacity at the
     median of the groups that do not fit: those within it go to the second
