@@ -52,7 +52,7 @@ FP32_PEAK_TFLOPS = 157.3
 MEASURED_TRAFFIC = {
     ("learned", 0.75, 1 << 21): {"bytes": 630689294,
                                  "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.675, 1 << 24): {"bytes": 6346263135,
+    ("learned", 0.675, 1 << 24): {"bytes": 6329801814,
                                   "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
     ("learned", 0.7, 1 << 24): {"bytes": 5994216230,
                                 "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
