@@ -23,7 +23,9 @@
  *                   level-L cell, found by galloping over the sorted keys
  *                   (equal to tree.cu:181-203's sibling / parent-end rule); the
  *                   first child is the body's node one level down, if it starts
- *                   one.  Written as a 16-byte meta record per node.
+ *                   one.  Written as a 16-byte meta record per node (first
+ *                   body, count with the node's octant in bits 29-31, parent,
+ *                   first child).
  *   k_build_mass    one launch per level, deepest first (tree.cu:89-179, same
  *                   operation order as the oracle): centre of mass, moments,
  *                   the compact walk record (centre of mass, first child /
@@ -258,7 +260,9 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
                 const int shift = 63 - 3 * L;
                 const uint64_t cell = key >> shift;
                 const int count = (L > Bv[k]) ? 1 : gd_cell_size(keys, n, i, cell, shift);
-                meta[idx] = make_int4((int)i, count, parent,
+                /* count < 2^28 (8 n nodes fit an int): its top bits carry
+                   the node's octant in its parent */
+                meta[idx] = make_int4((int)i, count | ((int)(cell & 7u) << 29), parent,
                                       flagNext ? sLevelOff[L + 1] + rankNext : -1);
             }
             prevRank = rank;
@@ -488,7 +492,8 @@ __device__ __forceinline__ void gd_write_record(node* __restrict__ nodes, int id
 /* One internal node at level `level` (shift = 63 - 3 level): its children
  * are the consecutive nodes of the next level from its first child on whose
  * parent it is (at most 8, before childEnd), in octant order; a child's
- * octant is 3 bits of its first body's key.  Writes the walk arrays and the
+ * octant (3 bits of its first body's key) rides in the top bits of its meta
+ * count, so no child's key is read.  Writes the walk arrays and the
  * whole node record; returns 1 for a massless node. */
 template <int MOM>
 __device__ __forceinline__ int gd_finish_inner(node* __restrict__ nodes,
@@ -512,7 +517,7 @@ __device__ __forceinline__ int gd_finish_inner(node* __restrict__ nodes,
         const int4 mk = j < childEnd ? meta[j] : make_int4(0, 0, -1, -1);
         run = run && mk.z == idx;
         kid[k] = run ? j : -1;
-        oct[k] = run ? (int)((keys[mk.x] >> (shift - 3)) & 7u) : 8;
+        oct[k] = run ? (int)(((unsigned)mk.y >> 29) & 7u) : 8;
     }
     int nc = 0;
     int child[8];
@@ -529,7 +534,7 @@ __device__ __forceinline__ int gd_finish_inner(node* __restrict__ nodes,
     float4 q0, q1, o0, o1, o2;
     const float4 c =
         gd_inner_node<MOM>(wcom, wlink, wquad, woct, kid, fc, &q0, &q1, &o0, &o1, &o2);
-    gd_write_record(nodes, idx, key >> shift, mt.z, mt.x, mt.y, child, level, c);
+    gd_write_record(nodes, idx, key >> shift, mt.z, mt.x, mt.y & 0x1fffffff, child, level, c);
     wcom[idx] = c;
     wlink[idx] = make_int2(fc, (nc << 5) | level);
     if (MOM >= 1) {
@@ -589,7 +594,7 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
             const int4 mt = meta[idx];
             inner = mt.w >= 0;
             if (!inner) {
-                const int first = mt.x, count = mt.y;
+                const int first = mt.x, count = mt.y & 0x1fffffff;
                 const uint64_t key = keys[first];
                 float sx = 0.f, sy = 0.f, sz = 0.f, m = 0.f;
                 for (int b = 0; b < count; ++b) {

@@ -111,8 +111,9 @@ typedef struct {
     float3* bboxPartMax;
     /* prefixSum scratch */
     uint32_t* scanTiles;             /* tile sums */
-    /* per-node build records, maxNodes entries: first body, body count,
-       parent, first child (-1 for a leaf) */
+    /* per-node build records, maxNodes entries: first body, body count
+       (octant in the parent in bits 29-31), parent, first child (-1 for a
+       leaf) */
     int4* meta;
     /* compact walk arrays, maxNodes entries each */
     float4* wcom;                    /* centre of mass + mass */
