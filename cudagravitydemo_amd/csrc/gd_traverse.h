@@ -145,8 +145,8 @@ __device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int
  * later passes walk the group).  A range takes one LDS word where its nodes
  * would take up to eight, which keeps the stack small enough for eight
  * wavefronts per SIMD.  Each step pops up to 64 ranges from the top -- as
- * many as hold at most 128 nodes -- and expands them into pop (128 ints of
- * LDS); lane l then tests nodes l and 64 + l, so that two node loads per
+ * many as hold at most 128 nodes -- and expands them into pop (128 + 8 ints
+ * of LDS); lane l then tests nodes l and 64 + l, so that two node loads per
  * lane are in flight.  Children of the upper nodes go on top: the walk stays
  * depth-first and the stack shallow. */
 template <bool SPLIT>
@@ -175,8 +175,16 @@ __device__ __forceinline__ bool gd_traverse_box(
         const int nr = __popcll(__ballot(take));
         const int nn = __shfl(ex + cnt, nr - 1);
         if (take) {
+            /* all 8 slots of each range, from the last: a range shorter than 8
+               writes past its end into the next ranges' slots (or the 8 ints
+               of slack), which those ranges write again at a lower k, in a
+               later instruction -- no per-lane trip count, no divergence */
             const int f = (int)(rg >> 3);
-            for (int k = 0; k < cnt; ++k) pop[ex + k] = f + k;
+#pragma unroll
+            for (int k = 7; k >= 0; --k) {
+                pop[ex + k] = f + k;
+                __builtin_amdgcn_wave_barrier();
+            }
         }
         __builtin_amdgcn_wave_barrier();
         top -= nr;

@@ -158,7 +158,8 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
 #define GD_TR2_STACK 4096         /* second pass, one wavefront per block (16 KB of LDS) */
-#define GD_POP 128                /* LDS pop buffer per traversing wavefront (nodes) */
+#define GD_POP (128 + 8)          /* LDS pop buffer per traversing wavefront: 128 nodes
+                                     and the 8-int slack of the range expansion */
 #define GD_BODY_STACK1 2048       /* per-body walk, first try (nodes) */
 
 /* Walk statistics: 64 copies of the counters, 128 bytes apart, picked by
