@@ -111,7 +111,7 @@ __device__ __forceinline__ gd_probe gd_test_node(float4 c, int2 lk, int idx, flo
     const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
     r.idx = idx;
     r.leaf = lk.x < 0;
-    r.accept = r.leaf || (size * size < th2 * d2);
+    r.accept = r.leaf | (size * size < th2 * d2);   /* no branch: both are cheap */
     r.open = !r.accept;
     r.nk = r.open ? (lk.y >> 5) : 0;
     r.first = lk.x;
@@ -188,13 +188,14 @@ __device__ __forceinline__ bool gd_traverse_box(
         }
         __builtin_amdgcn_wave_barrier();
         top -= nr;
+        /* lanes without a node read the root's records (no branch) and are
+           masked below */
         const bool hasA = (int)lane < nn, hasB = 64 + (int)lane < nn;
-        const int ia = hasA ? pop[lane] : 0;
-        const int ib = hasB ? pop[64 + lane] : 0;
-        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;
-        int2 la = make_int2(0, 0), lb = la;
-        if (hasA) { ca = wcom[ia]; la = wlink[ia]; }
-        if (hasB) { cb = wcom[ib]; lb = wlink[ib]; }
+        const int pa = pop[lane], pb = pop[64 + lane];
+        const int ia = hasA ? pa : 0;
+        const int ib = hasB ? pb : 0;
+        const float4 ca = wcom[ia], cb = wcom[ib];
+        const int2 la = wlink[ia], lb = wlink[ib];
         gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz);
         gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz);
         if (!hasA) A.accept = A.open = false, A.nk = 0;
