@@ -132,6 +132,23 @@ __device__ __forceinline__ void gd_wave_scan16(int v, unsigned long long lt, int
     }
 }
 
+/* Exclusive prefix and total over the wavefront of v (any int), by the
+ * data-parallel-primitive form of the inclusive scan: shifts by 1, 2, 4, 8
+ * lanes within each 16-lane row, then the last lane of row 0 / 2 added to
+ * rows 1 / 3 and the last lane of row 1 to rows 2 and 3 (lanes a shift
+ * reaches from outside the row, or rows a broadcast does not write, add 0). */
+__device__ __forceinline__ void gd_wave_scan_dpp(int v, int& excl, int& total) {
+    int s = v;
+    s += __builtin_amdgcn_update_dpp(0, s, 0x111, 0xf, 0xf, false);   /* row_shr:1 */
+    s += __builtin_amdgcn_update_dpp(0, s, 0x112, 0xf, 0xf, false);   /* row_shr:2 */
+    s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xf, 0xf, false);   /* row_shr:4 */
+    s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xf, 0xf, false);   /* row_shr:8 */
+    s += __builtin_amdgcn_update_dpp(0, s, 0x142, 0xa, 0xf, false);   /* row_bcast:15 */
+    s += __builtin_amdgcn_update_dpp(0, s, 0x143, 0xc, 0xf, false);   /* row_bcast:31 */
+    excl = s - v;
+    total = __builtin_amdgcn_readlane(s, 63);
+}
+
 /* Traversal by one wavefront for the box [l, h]: writes the accepted nodes to
  * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
  * SPLIT) unless list is NULL (counting only), and returns false as soon as
@@ -170,7 +187,7 @@ __device__ __forceinline__ bool gd_traverse_box(
         const unsigned rg = has ? (unsigned)stack[top - 1 - (int)lane] : 0u;
         const int cnt = has ? (int)(rg & 7u) + 1 : 0;
         int ex, all;
-        gd_wave_scan16(cnt, lt, ex, all);
+        gd_wave_scan_dpp(cnt, ex, all);
         const bool take = has && ex + cnt <= 128;
         const int nr = __popcll(__ballot(take));
         const int nn = __shfl(ex + cnt, nr - 1);
