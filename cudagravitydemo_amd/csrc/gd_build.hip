@@ -254,12 +254,21 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
             int rankNext = 0;
             if (L + 1 <= waveMaxHi)   /* wave-uniform */
                 rankNext = (int)sRank[k][wave][L + 1] + __popcll(__ballot(flagNext) & lt);
+            /* level-L cells end where a body starts a node at a level <= L
+               (its left neighbour differs from it at level L) */
+            const unsigned long long ends = __ballot(lo[k] <= L);
             if (flag) {
                 const int idx = sLevelOff[L] + rank;
                 const int parent = (L == 1) ? 0 : sLevelOff[L - 1] + prevRank + prevFlag - 1;
                 const int shift = 63 - 3 * L;
                 const uint64_t cell = key >> shift;
-                const int count = (L > Bv[k]) ? 1 : gd_cell_size(keys, n, i, cell, shift);
+                /* the cell's size: 1 when the right neighbour is outside it,
+                   from the next cell start among this wavefront's bodies, or
+                   by galloping over the keys when the cell runs past them */
+                const unsigned long long later = ends & ~(lt | (1ull << gd_lane()));
+                const int count = (L > Bv[k]) ? 1
+                                  : later ? (int)__builtin_ctzll(later) - (int)gd_lane()
+                                          : gd_cell_size(keys, n, i, cell, shift);
                 /* count < 2^28 (8 n nodes fit an int): its top bits carry
                    the node's octant in its parent */
                 meta[idx] = make_int4((int)i, count | ((int)(cell & 7u) << 29), parent,
