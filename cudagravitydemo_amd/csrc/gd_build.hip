@@ -563,7 +563,8 @@ __device__ __forceinline__ int gd_finish_inner(node* __restrict__ nodes,
  * deepest first).  MOM: 0 centre of mass only, 1 also second moments, 2 also
  * third moments.  Input: the per-node meta of k_build_nodes (first body,
  * count, parent, first child or -1).  Output: the walk arrays and the whole
- * 80-byte node record.  Each wavefront scans 64 nodes at a time: a lane whose
+ * 80-byte node record (sRec: 64 records of LDS per wavefront, for the leaves'
+ * coalesced stores).  Each wavefront scans 64 nodes at a time: a lane whose
  * node is a leaf finishes it at once (tree.cu:105-125; three quarters of all
  * nodes are single-body leaves, which store no moments), the internal nodes
  * (tree.cu:127-179) are queued in the wavefront's LDS and finished 64 at a
@@ -581,6 +582,7 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
                                                     float4* __restrict__ wquad,
                                                     float4* __restrict__ woct) {
     __shared__ int sQueueAll[4][128];
+    __shared__ int4 sRecAll[4][64 * 5];
     if (info->overflow) return;
     const int cnt = info->levelCount[level];
     const int off = info->levelOffsets[level];
@@ -591,8 +593,8 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
     const unsigned long long lt = gd_lanemask_lt();
     const int wave = (int)(threadIdx.x >> 6);
     int* sQueue = sQueueAll[wave];
+    int4* sRec = sRecAll[wave];
     const int waves = gridDim.x * 4;
-    const int none[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     int queued = 0;   /* wave-uniform */
     int zeros = 0;
     for (int c0 = (blockIdx.x * 4 + wave) * 64; c0 < cnt; c0 += waves * 64) {
@@ -616,7 +618,16 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
                 float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (m > 0.f) c = make_float4(sx / m, sy / m, sz / m, m);
                 else zeros++;
-                gd_write_record(nodes, idx, key >> shift, mt.z, first, count, none, level, c);
+                {   /* the record, staged in LDS (written out below) */
+                    const uint64_t path = key >> shift;
+                    int4* r = sRec + 5 * lane;
+                    r[0] = make_int4((int)(uint32_t)path, (int)(uint32_t)(path >> 32), mt.z, first);
+                    r[1] = make_int4(count, -1, -1, -1);
+                    r[2] = make_int4(-1, -1, -1, -1);
+                    r[3] = make_int4(-1, level, 0, 0);
+                    r[4] = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z),
+                                     __float_as_int(c.w));
+                }
                 wcom[idx] = c;
                 wlink[idx] = make_int2(-(first + 1), (count << 5) | level);
                 if (count > 1) {   /* moments of a single-body leaf are zero, not stored */
@@ -635,6 +646,23 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
                     }
                 }
             }
+        }
+        /* the chunk's leaf records, as 16-byte pieces in address order: lane
+           l of store u writes piece 64 u + l of the chunk's 64 records, so a
+           store covers 1 KB of consecutive records (less the internal nodes'
+           records, written when their node is finished) instead of 64
+           pieces 80 B apart */
+        {
+            const unsigned long long leafMask = __ballot(t < cnt && !inner);
+            __builtin_amdgcn_wave_barrier();
+            int4* base = reinterpret_cast<int4*>(nodes + off + c0);
+#pragma unroll
+            for (int u = 0; u < 5; ++u) {
+                const int e = 64 * u + (int)lane;
+                const int rr = e / 5;
+                if ((leafMask >> rr) & 1ull) base[e] = sRec[e];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
         const unsigned long long bal = __ballot(inner);
         if (inner) sQueue[queued + __popcll(bal & lt)] = off + t;
