@@ -83,7 +83,9 @@ def parse():
     ap.add_argument("--no-reference-walk", action="store_true")
     ap.add_argument("--cpu-group-stride", type=int, default=0,
                     help="walk every k-th 32-body group on the CPU (0: about 8192 groups)")
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--train-targets", type=int, default=8192,
+                    help="target bodies per snapshot for the corrector's samples")
     return ap.parse_args()
 
 
@@ -264,7 +266,7 @@ def main():
     samples = cr.SampleSet(sim.lib, 24_000_000)
     for snap in range(4):
         sim.build_tree_async()
-        samples.add_from(sim, cr.default_targets(n, 4096, seed=snap), 0.9)
+        samples.add_from(sim, cr.default_targets(n, args.train_targets, seed=snap), 0.9)
         for _ in range(5):
             walks.step("quad", 0.6, dt)
     log(f"corrector: {samples.n} samples")
