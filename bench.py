@@ -56,13 +56,16 @@ MEASURED_TRAFFIC = {
                                   "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
     ("learned", 0.7, 1 << 24): {"bytes": 5994216230,
                                 "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
+    ("learned", 0.6875, 1 << 24): {"bytes": 6137657755,
+                                   "profile": "profiles/pmc_walk_eval_learned_theta0.6875_merger16M.md"},
 }
 BUDGET = 1e-3
 # Each walk's opening angle is calibrated against this share of the budget:
 # the learned walk is judged again after its timed steps, on the evolved
-# state, where its error moves by a percent or two, and falls back to the
-# plain walk if it is then over budget.
-CALIBRATION_SHARE = 0.97
+# state, and falls back to the plain walk if it is then over budget.  On the
+# 2^24-body merger the evolved state's error was 0.2-0.7 % below the
+# calibration's in every run seen, so 2 % of the budget is left as margin.
+CALIBRATION_SHARE = 0.98
 FLOP_MONO = 20
 FLOP_QUAD = 54
 FLOP_CORR = 15
@@ -276,7 +279,7 @@ def main():
     n_samples = samples.n
     samples.close()
 
-    thetas = (1.0, 0.9, 0.85, 0.8, 0.75, 0.725, 0.7, 0.675, 0.65, 0.625, 0.6, 0.5)
+    thetas = (1.0, 0.9, 0.85, 0.8, 0.75, 0.725, 0.7, 0.6875, 0.675, 0.65, 0.625, 0.6, 0.5)
     th_l, med_l, p99_l, tried_l = calibrate(walks, "learned", thetas, sample)
     corrector_on = th_l is not None
     secs_l = timed_steps(walks, "learned", th_l, dt, args.steps, 1) if corrector_on else None
