@@ -63,7 +63,7 @@ BUDGET = 1e-3
 # Each walk's opening angle is calibrated against this share of the budget:
 # the learned walk is judged again after its timed steps, on the evolved
 # state, and falls back to the plain walk if it is then over budget.  On the
-# 2^24-body merger the evolved state's error was 0.2-0.7 % below the
+# 2^24-body merger the evolved state's error was 0.2-1.7 % below the
 # calibration's in every run seen, so 2 % of the budget is left as margin.
 CALIBRATION_SHARE = 0.98
 FLOP_MONO = 20
