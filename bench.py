@@ -272,6 +272,7 @@ def main():
         samples.add_from(sim, cr.default_targets(n, args.train_targets, seed=snap), 0.9)
         for _ in range(5):
             walks.step("quad", 0.6, dt)
+    samples.canonical_order()   # the same network from the same samples, run after run
     log(f"corrector: {samples.n} samples")
     losses = cr.train(net, samples, epochs=args.train_epochs)
     train_s = time.perf_counter() - t0

@@ -126,6 +126,25 @@ class SampleSet:
         self.n += int(got)
         return int(got), dropped.value
 
+    def canonical_order(self):
+        """Put the rows in an order that depends only on their values.  The
+        sample kernel claims output rows with an atomic counter, so the order
+        of a snapshot's rows (and with it the minibatches of a seeded
+        shuffle) varies from run to run; sorted by a hash of each row's bytes,
+        the same rows train the same network every time."""
+        if self.n <= 1:
+            return
+        X, Y = self.host()
+        w = np.concatenate([X.view(np.uint32), Y.view(np.uint32)], 1).astype(np.uint64)
+        h = np.zeros(self.n, np.uint64)
+        with np.errstate(over="ignore"):
+            for j in range(w.shape[1]):
+                h = (h ^ w[:, j]) * np.uint64(0x100000001B3)
+                h ^= h >> np.uint64(29)
+        order = np.argsort(h, kind="stable")
+        self.X.upload(np.ascontiguousarray(X[order]))
+        self.Y.upload(np.ascontiguousarray(Y[order]))
+
     def host(self, limit=None):
         n = self.n if limit is None else min(limit, self.n)
         return (from_device(self.lib, self.X.ptr, np.float32, (n, 8)),
