@@ -56,7 +56,7 @@ MEASURED_TRAFFIC = {
                                   "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
     ("learned", 0.7, 1 << 24): {"bytes": 5994216230,
                                 "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
-    ("learned", 0.6875, 1 << 24): {"bytes": 6137657755,
+    ("learned", 0.6875, 1 << 24): {"bytes": 6084437655,
                                    "profile": "profiles/pmc_walk_eval_learned_theta0.6875_merger16M.md"},
 }
 BUDGET = 1e-3
@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--cpu-group-stride", type=int, default=0,
                     help="walk every k-th 32-body group on the CPU (0: about 8192 groups)")
     ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--min-ratio", type=float, default=0.5875,
+                    help="the corrector acts on cells with size / distance above this")
     ap.add_argument("--train-targets", type=int, default=8192,
                     help="target bodies per snapshot for the corrector's samples")
     return ap.parse_args()
@@ -129,9 +131,10 @@ def cpu_baseline(pos, vel, eps, theta, dt, stride):
 class Walks:
     """The three walks on one simulation."""
 
-    def __init__(self, sim, net):
+    def __init__(self, sim, net, min_ratio):
         self.sim = sim
         self.net = net
+        self.min_ratio = min_ratio
 
     def prepare(self, mode):
         # moments are built only for the walks that use them
@@ -140,7 +143,7 @@ class Walks:
     def walk(self, mode, theta, stats=False):
         from cudagravitydemo_amd import corrector as cr
         if mode == "learned":
-            return cr.walk_learned(self.sim, self.net, theta, stats=stats)
+            return cr.walk_learned(self.sim, self.net, theta, self.min_ratio, stats=stats)
         return self.sim.walk(theta, stats=stats)
 
     def step(self, mode, theta, dt):
@@ -252,7 +255,7 @@ def main():
     n = pos.shape[0]
     sim = Simulation(pos, vel, eps=eps, hilo=not args.no_hilo)
     net = cr.Corrector(seed=1)
-    walks = Walks(sim, net)
+    walks = Walks(sim, net, args.min_ratio)
     sample = ics.subsample(n, 4096, seed=99)
     for _ in range(args.warmup):
         walks.step("plain", 0.5, dt)
@@ -269,7 +272,8 @@ def main():
     samples = cr.SampleSet(sim.lib, 24_000_000)
     for snap in range(4):
         sim.build_tree_async()
-        samples.add_from(sim, cr.default_targets(n, args.train_targets, seed=snap), 0.9)
+        samples.add_from(sim, cr.default_targets(n, args.train_targets, seed=snap), 0.9,
+                         args.min_ratio)
         for _ in range(5):
             walks.step("quad", 0.6, dt)
     samples.canonical_order()   # the same network from the same samples, run after run

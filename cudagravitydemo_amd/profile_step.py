@@ -35,6 +35,8 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--walk", choices=["plain", "quad", "learned", "reference"], default="plain")
     ap.add_argument("--params", default=None)
+    ap.add_argument("--min-ratio", type=float, default=0.55,
+                    help="learned walk: the corrector acts on cells with size / distance above this")
     ap.add_argument("--list-cap", type=int, default=0,
                     help="fixed first-pass list capacity per group (0: adaptive)")
     ap.add_argument("--group-split", type=float, default=-1.0,
@@ -65,7 +67,7 @@ def main():
                 sim.step_reference(dt, a.theta)
             elif a.walk == "learned":
                 sim.build_tree_async()
-                cr.walk_learned(sim, net, a.theta)
+                cr.walk_learned(sim, net, a.theta, a.min_ratio)
                 sim.integrate(dt)
             else:
                 sim.step(dt, a.theta)
@@ -78,7 +80,7 @@ def main():
             sim.lib.gd_walk_timing(1)
             sim.build_tree_async()
             if a.walk == "learned":
-                cr.walk_learned(sim, net, a.theta)
+                cr.walk_learned(sim, net, a.theta, a.min_ratio)
             else:
                 sim.walk(a.theta)
             ms = (C.c_float * 3)()
