@@ -81,6 +81,36 @@ def test_adam_step_matches_closed_form():
         net.close()
 
 
+def test_canonical_sample_order_is_reproducible():
+    """The sample kernel claims output rows with an atomic counter; after
+    canonical_order() two fills from the same snapshot hold the same rows in
+    the same order (so the seeded shuffle trains the same network), and the
+    rows are a permutation of what the kernel wrote."""
+    pos, vel = ics.exponential_disk(8192, seed=13)
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.set_quadrupole(2)
+        sim.build_tree_async()
+        targets = ics.subsample(8192, 256, seed=3)
+        rows = []
+        for _ in range(2):
+            s = cr.SampleSet(sim.lib, 400000)
+            try:
+                n, dropped = s.add_from(sim, targets, 0.8, 0.5)
+                assert n > 1000 and dropped == 0
+                X0, Y0 = s.host()
+                s.canonical_order()
+                X, Y = s.host()
+                # the same multiset of rows
+                a = np.concatenate([X0, Y0], 1)
+                b = np.concatenate([X, Y], 1)
+                assert np.array_equal(a[np.lexsort(a.T[::-1])], b[np.lexsort(b.T[::-1])])
+                rows.append((X, Y))
+            finally:
+                s.close()
+        assert np.array_equal(rows[0][0], rows[1][0])
+        assert np.array_equal(rows[0][1], rows[1][1])
+
+
 @pytest.mark.parametrize("ratio,min_ratio", [(-1.0, 0.5), (0.0, 0.0)])
 def test_samples_match_direct_sums(ratio, min_ratio):
     """ratio -1: the learned walk's default groups at the sampling angle 0.8
