@@ -110,7 +110,9 @@ class SampleSet:
         """Append the (body, cell) samples of one snapshot: the cells whose
         size / distance from the group's centre exceeds min_ratio, the ones
         walk_learned corrects with the same min_ratio; returns (rows added,
-        targets dropped)."""
+        targets dropped).  Raises ValueError, adding nothing, when the rows do
+        not fit the set's remaining room: the kernel would keep an arbitrary
+        subset of them."""
         lib = self.lib
         t = to_device(lib, np.ascontiguousarray(targets, np.int32))
         dropped = C.c_int(0)
@@ -123,6 +125,9 @@ class SampleSet:
             check(lib)
         finally:
             t.close()
+        if got > room:
+            raise ValueError(f"sample set full: the snapshot yields {got} rows, "
+                             f"{room} of {self.cap} left")
         self.n += int(got)
         return int(got), dropped.value
 

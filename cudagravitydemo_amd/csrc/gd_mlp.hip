@@ -732,6 +732,14 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         return 0;
     }
     if (nt <= 0 || cap <= 0) return 0;
+    {
+        int overflow = 0;
+        GD_HIP(hipMemcpy(&overflow, &b->dinfo->overflow, sizeof(int), hipMemcpyDeviceToHost));
+        if (overflow) {   /* basicGravity.cu:282 stops on the same condition */
+            GD_FAIL("Node array overflow: the tree does not fit its %d nodes", b->maxNodes);
+            return 0;
+        }
+    }
     int2* pairs = nullptr;
     unsigned long long* count = nullptr;
     int* dropped = nullptr;
@@ -752,13 +760,14 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         int drop = 0;
         GD_HIP(hipMemcpy(&total, count, sizeof(total), hipMemcpyDeviceToHost));
         GD_HIP(hipMemcpy(&drop, dropped, sizeof(int), hipMemcpyDeviceToHost));
-        n = (long long)total < cap ? (long long)total : cap;
+        const long long rows = (long long)total < cap ? (long long)total : cap;
+        n = (long long)total;
         if (droppedTargets) *droppedTargets = drop;
-        if (n > 0) {
-            long long blocks = (n + 3) / 4;
+        if (rows > 0) {
+            long long blocks = (rows + 3) / 4;
             if (blocks > 65536) blocks = 65536;
             k_samples_eval<<<(int)blocks, 256>>>(pos, tree->nodes, b->wcom, b->wquad, b->woct,
-                                                 b->dinfo, targets, centre, pairs, n,
+                                                 b->dinfo, targets, centre, pairs, rows,
                                                  (double)eps2, X, Y);
             GD_LAUNCHED();
         }

@@ -203,6 +203,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     int* __restrict__ spreadList, int* __restrict__ spreadCount, int* __restrict__ deepCount,
     unsigned long long* __restrict__ stats) {
     __shared__ int sStack[WAVES][STACK + GD_POP];
+    if (info->overflow) return;   /* the build ran out of nodes: no tree to walk (the host
+                                     reports it from the walk's status) */
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
     const long long s = (long long)gd_xcd_block() * WAVES + wave;
     if (s >= slots) return;   /* wave-uniform */
@@ -1127,6 +1129,7 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     __shared__ int sQueue[MODE == 2 ? EVW : 1][64];
     __shared__ gd_bf16x4 sW[MODE == 2 ? GD_MLP_FWD_TILES * 64 : 1];
     __shared__ float sBias[MODE == 2 ? GD_MLP_BIAS_PAD : 1];
+    if (info->overflow) return;   /* block-uniform: no tree, nothing listed */
     if (MODE == 2) {
         /* the network's packed weights and biases, once per block */
         for (int e = threadIdx.x; e < GD_MLP_FWD_TILES * 64; e += 64 * EVW) sW[e] = mlpW[e];
@@ -1327,6 +1330,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
  */
 struct gd_walk_status {
     int over1, over2, over3, deep1;
+    int treeOverflow;      /* the build's node-array overflow flag, copied at the walk's start */
+    int pad;
     unsigned long long requested;
 };
 
@@ -1461,10 +1466,13 @@ __global__ __launch_bounds__(256) void k_walk_split(const uint64_t* __restrict__
     }
 }
 
-__global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st) {
+__global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st,
+                             const gd_tree_info* info) {
     for (int i = threadIdx.x; i < GD_STAT_WORDS; i += blockDim.x) stats[i] = 0ull;
     if (threadIdx.x == 0) {
         st->over1 = st->over2 = st->over3 = st->deep1 = 0;
+        st->treeOverflow = info->overflow;
+        st->pad = 0;
         st->requested = 0ull;
     }
 }
@@ -1535,6 +1543,12 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
     gd_walk_status h;
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.treeOverflow) {
+        /* the traversal and evaluation kernels exited at once (basicGravity.cu:282
+           stops the run on the same condition) */
+        GD_FAIL("Node array overflow: the tree does not fit its %d nodes", b->maxNodes);
+        return;
+    }
     const int over1 = h.over1;
     overOut[0] = over1 + h.over2;
     *requested = h.requested;
@@ -1628,7 +1642,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     if (gd_has_error()) return;
     const gd_bf16x4* W = net ? gd_mlp_fwd_tiles(net) : nullptr;
     const float* Bv = net ? gd_mlp_bias(net) : nullptr;
-    k_walk_reset<<<1, 256>>>(g_walk.stats, g_walk.status);
+    k_walk_reset<<<1, 256>>>(g_walk.stats, g_walk.status, b->dinfo);
     /* the list passes count interactions only when asked to (their per-group
        atomics are not free); the per-body walk always reports stack overflows */
     g_walk.kstats = stats ? g_walk.stats : nullptr;

@@ -100,7 +100,10 @@ typedef struct {
  * (gd_set_quadrupole), accepted internal cells are evaluated as quadrupoles.
  * Needs a tree built by buildTree or gd_build_tree_async.  stats may be
  * NULL; when given, counters are accumulated on the device and copied out
- * (synchronises).  phi (may be NULL) receives the softened potential.
+ * (synchronises).  phi (may be NULL) receives the softened potential.  When
+ * the build ran out of nodes (the reference stops there, basicGravity.cu:282)
+ * the walk reads no tree: it fails with "Node array overflow" and leaves acc
+ * unwritten.
  */
 void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, float eps2,
              float G, float theta, int starCount, gd_walk_stats* stats);
@@ -174,9 +177,13 @@ void gd_mlp_adam_state(const gd_mlp* net, float* hostM, float* hostV);
  * centre exceeds minRatio (the cells gd_walk_learned corrects) gives one row
  * of X (the features at the group's centre) and the exact scaled residual in
  * Y (FP64 sum over the cell's bodies minus the softened monopole and
- * quadrupole terms, from the body).  Returns the number of rows written (at
- * most cap); *droppedTargets counts targets whose traversal outgrew its
- * stack. */
+ * quadrupole terms, from the body).  Returns the number of rows the targets
+ * yield; only the first cap are written (rows written = min(return, cap)),
+ * and which rows those are then depends on the order in which the targets'
+ * wavefronts claim them, so a caller that needs every row sizes cap from a
+ * first call.  *droppedTargets counts targets whose traversal outgrew its
+ * stack; the rows they listed before that are kept.  Fails with "Node array
+ * overflow" when the build ran out of nodes. */
 long long gd_corrector_samples(const treeState* tree, const float4* pos, const int* targets,
                                int nt, float theta, float minRatio, float eps2, float* X,
                                float* Y, long long cap, int* droppedTargets);

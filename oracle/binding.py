@@ -49,6 +49,9 @@ def lib():
         L.or_walk.argtypes = [vp, i32, vp, vp, vp, f32, f32, f32, vp, i32, i32, i32,
                               C.POINTER(C.c_longlong)]
         L.or_walk.restype = C.c_longlong
+        L.or_walk_traced.argtypes = [vp, i32, vp, vp, vp, f32, f32, f32, vp,
+                                     C.POINTER(C.c_longlong), i32, vp, vp, i32]
+        L.or_walk_traced.restype = C.c_longlong
         L.or_integrate.argtypes = [vp, vp, vp, i32, f32]
         L.or_direct_sum_fp64.argtypes = [vp, i32, vp, i32, C.c_double, C.c_double, vp]
         _lib = L
@@ -139,6 +142,28 @@ def walk(pos, nodes, bmin, bmax, eps2, G, theta, g0=0, g1=None, gstride=1):
     inter = lib().or_walk(_p(pos), n, _p(nodes), _p(bmin), _p(bmax), eps2, G, theta, _p(acc),
                           g0, g1, gstride, C.byref(opened))
     return acc, int(inter), int(opened.value)
+
+
+def walk_traced(pos, nodes, bmin, bmax, eps2, G, theta, variant=0, per_body=4096):
+    """or_walk over all groups plus each body's accepted node indices.
+    variant > 0 plants one of the oracle's deliberate mistakes (oracle.h).
+    Returns (acc, interactions, opened, lists) with lists[i] an int32 array."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    n = pos.shape[0]
+    acc = np.zeros((n, 3), np.float32)
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    bmax = np.ascontiguousarray(bmax, np.float32)
+    opened = C.c_longlong(0)
+    counts = np.zeros(n, np.int32)
+    lists = np.full((n, per_body), -1, np.int32)
+    inter = lib().or_walk_traced(_p(pos), n, _p(nodes), _p(bmin), _p(bmax), eps2, G, theta,
+                                 _p(acc), C.byref(opened), int(variant), _p(counts), _p(lists),
+                                 per_body)
+    if inter < 0:
+        raise ValueError(f"or_walk_traced failed (variant {variant})")
+    if counts.max(initial=0) > per_body:
+        raise ValueError(f"a body accepted {counts.max()} nodes, more than per_body={per_body}")
+    return acc, int(inter), int(opened.value), [lists[i, :counts[i]] for i in range(n)]
 
 
 def integrate(pos, vel, acc, dt):

@@ -273,56 +273,81 @@ int or_tree_mass(node* nodes, const float4* pos, const int* levelOffsets,
 #define OR_GROUP 32
 #define OR_STACK 168
 
-long long or_walk(const float4* pos, int n, const node* nodes, const float3* bmin,
-                  const float3* bmax, float eps2, float G, float theta, float3* acc,
-                  int g0, int g1, int gstride, long long* opened) {
-    const float ext = bmax->x - bmin->x;          /* tree.cu:506, x extent only */
+/* The walk proper.  variant 0 is the reference's rule; variants 1-5 are
+ * deliberate mistakes that exist only so the pinning tests can show that
+ * their check tells them apart (see or_walk_traced).  When lists != NULL the
+ * accepted nodes of every valid body are recorded (at most perBody each,
+ * counts in bodyCount, which may exceed perBody). */
+static long long walk_core(const float4* pos, int n, const node* nodes, const float3* bmin,
+                           const float3* bmax, float eps2, float G, float theta, float3* acc,
+                           int g0, int g1, int gstride, long long* opened, int variant,
+                           int* bodyCount, int* lists, int perBody) {
+    /* tree.cu:506: the x extent only (variant 2: the widest axis instead) */
+    float ext = bmax->x - bmin->x;
+    if (variant == 2) {
+        const float h = bmax->y - bmin->y, d = bmax->z - bmin->z;
+        ext = fmaxf(ext, fmaxf(h, d));
+    }
     const float th2 = theta * theta;
+    const int gsize = variant == 1 ? 1 : OR_GROUP;   /* variant 1: every lane on its own */
     long long accepted = 0, opens = 0;
     if (gstride < 1) gstride = 1;
+    if (variant == 1) {   /* the same bodies as groups [g0, g1) of 32 */
+        g0 *= OR_GROUP;
+        g1 *= OR_GROUP;
+        gstride *= OR_GROUP;
+    }
     for (int g = g0; g < g1; ++g) {
-        if (g % gstride) continue;
+        if (variant == 1 ? ((g / OR_GROUP) % (gstride / OR_GROUP)) : (g % gstride)) continue;
         float px[OR_GROUP], py[OR_GROUP], pz[OR_GROUP];
         float ax[OR_GROUP], ay[OR_GROUP], az[OR_GROUP];
         int valid[OR_GROUP];
-        for (int l = 0; l < OR_GROUP; ++l) {
-            int i = g * OR_GROUP + l;
+        for (int l = 0; l < gsize; ++l) {
+            int i = g * gsize + l;
             valid[l] = i < n;
             px[l] = valid[l] ? pos[i].x : 0.0f;
             py[l] = valid[l] ? pos[i].y : 0.0f;
             pz[l] = valid[l] ? pos[i].z : 0.0f;
             ax[l] = ay[l] = az[l] = 0.0f;
+            if (valid[l] && bodyCount) bodyCount[i] = 0;
         }
         int stack[OR_STACK], top = -1;
         for (int c = 0; c < 8; ++c)
             if (nodes[0].child[c] != -1) stack[++top] = nodes[0].child[c];
         while (top >= 0) {
-            const node* nd = &nodes[stack[top]];
+            const int ni = stack[top];
+            const node* nd = &nodes[ni];
             const float4 cm = nd->massData;
             float size = (float)((double)ext / pow(2.0, (double)nd->treeLevel));
             int nkids = 0;
             for (int c = 0; c < 8; ++c) nkids += nd->child[c] != -1;
-            int leaf = (nd->particleCount <= 1) || (nkids == 0);
+            /* tree.cu:627 (variant 5: without the no-children clause) */
+            int leaf = (nd->particleCount <= 1) || (variant != 5 && nkids == 0);
             float dx[OR_GROUP], dy[OR_GROUP], dz[OR_GROUP], d2[OR_GROUP];
             int any = 0, wants = 0;
-            for (int l = 0; l < OR_GROUP; ++l) {
+            for (int l = 0; l < gsize; ++l) {
                 dx[l] = px[l] - cm.x;
                 dy[l] = py[l] - cm.y;
                 dz[l] = pz[l] - cm.z;
                 d2[l] = fmaf(dz[l], dz[l], fmaf(dy[l], dy[l], dx[l] * dx[l]));
-                int accept = (size * size) < th2 * d2[l];
-                int want = !(accept || !valid[l] || leaf);
+                int accept = variant == 3 ? (size * size) <= th2 * d2[l]
+                                          : (size * size) < th2 * d2[l];
+                /* tree.cu:629-634: invalid lanes never vote (variant 4: they do) */
+                int want = !(accept || (variant != 4 && !valid[l]) || leaf);
                 any |= want;
-                wants += want;
+                wants += want && valid[l];
             }
             --top;
             if (any) {
                 opens += wants;
                 for (int c = 0; c < 8; ++c)
-                    if (nd->child[c] != -1) stack[++top] = nd->child[c];
+                    if (nd->child[c] != -1) {
+                        if (top + 1 >= OR_STACK) return -1;   /* cannot happen for 21 levels */
+                        stack[++top] = nd->child[c];
+                    }
                 continue;
             }
-            for (int l = 0; l < OR_GROUP; ++l) {
+            for (int l = 0; l < gsize; ++l) {
                 float r2 = d2[l] + eps2;
                 float mag = -(G * cm.w) / r2;
                 float dist = sqrtf(r2);
@@ -330,16 +355,38 @@ long long or_walk(const float4* pos, int n, const node* nodes, const float3* bmi
                 ay[l] = fmaf(mag, dy[l] / dist, ay[l]);
                 az[l] = fmaf(mag, dz[l] / dist, az[l]);
                 accepted += valid[l];
+                if (valid[l] && bodyCount) {
+                    const int i = g * gsize + l;
+                    if (bodyCount[i] < perBody) lists[(size_t)i * perBody + bodyCount[i]] = ni;
+                    bodyCount[i]++;
+                }
             }
         }
-        for (int l = 0; l < OR_GROUP; ++l) {
+        for (int l = 0; l < gsize; ++l) {
             if (!valid[l]) continue;
-            int i = g * OR_GROUP + l;
+            int i = g * gsize + l;
             acc[i].x = ax[l]; acc[i].y = ay[l]; acc[i].z = az[l];
         }
     }
     if (opened) *opened = opens;
     return accepted;
+}
+
+long long or_walk(const float4* pos, int n, const node* nodes, const float3* bmin,
+                  const float3* bmax, float eps2, float G, float theta, float3* acc,
+                  int g0, int g1, int gstride, long long* opened) {
+    return walk_core(pos, n, nodes, bmin, bmax, eps2, G, theta, acc, g0, g1, gstride, opened, 0,
+                     NULL, NULL, 0);
+}
+
+long long or_walk_traced(const float4* pos, int n, const node* nodes, const float3* bmin,
+                         const float3* bmax, float eps2, float G, float theta, float3* acc,
+                         long long* opened, int variant, int* bodyCount, int* lists,
+                         int perBody) {
+    if (variant < 0 || variant > 5) return -1;
+    const int groups = (n + OR_GROUP - 1) / OR_GROUP;
+    return walk_core(pos, n, nodes, bmin, bmax, eps2, G, theta, acc, 0, groups, 1, opened,
+                     variant, bodyCount, lists, perBody);
 }
 
 /* tree.cu:678-703: kick with the new acceleration, then drift with the new velocity. */

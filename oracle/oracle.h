@@ -70,6 +70,19 @@ long long or_walk(const float4* pos, int n, const node* nodes, const float3* bmi
                   const float3* bmax, float eps2, float G, float theta, float3* acc,
                   int g0, int g1, int gstride, long long* opened);
 
+/* or_walk over every group, also recording the nodes each valid body
+ * accepted: bodyCount[i] (may exceed perBody) and lists[i * perBody + k],
+ * in traversal order.  variant 0 is or_walk's rule.  Variants 1-5 are
+ * deliberate mistakes for the pinning tests, which must reject each:
+ *   1 per-lane acceptance (no 32-body vote)   2 node size from the widest axis
+ *   3 '<=' in the opening test                4 invalid lanes vote
+ *   5 leaf rule without its no-children clause
+ * Returns or_walk's count, or -1 for an unknown variant. */
+long long or_walk_traced(const float4* pos, int n, const node* nodes, const float3* bmin,
+                         const float3* bmax, float eps2, float G, float theta, float3* acc,
+                         long long* opened, int variant, int* bodyCount, int* lists,
+                         int perBody);
+
 /* Semi-implicit Euler kick-drift (tree.cu:678-703). */
 void or_integrate(float4* pos, float3* vel, const float3* acc, int n, float dt);
 

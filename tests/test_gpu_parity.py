@@ -90,6 +90,27 @@ def test_theta_zero_walk():
     check_build_and_walk(pos, vel, 0.01, 0.0)
 
 
+@pytest.mark.parametrize("fixture,theta", [("lattice", 0.5), ("lattice", 1.0), ("disk", 0.5)])
+def test_reference_walk_on_vote_fixtures(fixture, theta):
+    """calculateAcceleration on the fixtures that pin the oracle's group-vote
+    rule (exact ties of the opening test, x not the widest axis, invalid
+    lanes in the last group, coincident bodies): bit-exact, and each body's
+    result equals the FP64 monopole sum over its group's enumerated set."""
+    from tests import vote_check as vc
+    from tests.test_oracle import disk_fixture, vote_fixture
+    if fixture == "lattice":
+        pos = vote_fixture()
+        vel = np.zeros((pos.shape[0], 3), np.float32)
+    else:
+        pos, vel = disk_fixture()
+    eps = 0.01
+    _, ref, _, acc, acc_ref = check_build_and_walk(pos, vel, eps, theta)
+    assert np.array_equal(acc, acc_ref)
+    sets, _, _ = vc.group_accepted_sets(ref["pos"], ref["nodes"], ref["bmin"], ref["bmax"], theta)
+    a64, mag = vc.monopole_fp64(ref["pos"], ref["nodes"], sets, float(np.float32(eps * eps)))
+    assert (np.linalg.norm(acc - a64, axis=1) / mag).max() < 2e-6
+
+
 def test_reference_steps_per_step_parity():
     """100 steps of the 16384-body exponential disk (theta 0.5, eps 0.01, dt 1e-3),
     each compared with the oracle step run on the GPU's own pre-step state."""

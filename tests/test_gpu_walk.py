@@ -396,3 +396,46 @@ def test_packed_lists_follow_the_walk():
                 assert np.median(d) < 1e-6
             if k > 0:
                 assert st["overflow_groups"] == spread, (k, st)
+
+
+def test_walks_refuse_an_overflowed_tree():
+    """A tree that needs more than the 8 N nodes the reference allocates
+    (basicGravity.cu:11, 282): the first async build of a fresh state flags
+    it on the device, and the walks and the sample generator fail with the
+    reference's message instead of reading the never-written walk arrays; a
+    good tree in the same process walks normally afterwards."""
+    from cudagravitydemo_amd import GravityError
+    from tests.test_oracle import overflow_fixture
+    pos = overflow_fixture()
+    vel = np.zeros((pos.shape[0], 3), np.float32)
+    print(f"case overflow fixture N={pos.shape[0]}", flush=True)
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(2)
+        sim.build_tree_async()
+        total, overflow, _ = sim.tree_summary()
+        assert overflow == 1 and total > 8 * pos.shape[0]
+        with pytest.raises(GravityError, match="Node array overflow"):
+            sim.walk(THETA)
+        net = cr.Corrector(_net_params())
+        try:
+            with pytest.raises(GravityError, match="Node array overflow"):
+                cr.walk_learned(sim, net, THETA, MIN_RATIO)
+            samples = cr.SampleSet(sim.lib, 1000)
+            try:
+                with pytest.raises(GravityError, match="Node array overflow"):
+                    samples.add_from(sim, np.arange(64, dtype=np.int32), THETA, MIN_RATIO)
+            finally:
+                samples.close()
+        finally:
+            net.close()
+        with pytest.raises(GravityError, match="Node array overflow"):
+            sim.build_tree()
+    good, gvel = ics.exponential_disk(4096, seed=3)
+    with Simulation(good, gvel, eps=EPS) as sim:
+        sim.build_tree_async()
+        sim.walk(THETA)
+        acc = sim.accelerations()
+        sample = ics.subsample(4096, 128, seed=1)
+        exact = sim.direct_sum_fp64(sample)
+        err = np.linalg.norm(acc[sample] - exact, axis=1) / np.linalg.norm(exact, axis=1)
+        assert np.median(err) < 1e-2

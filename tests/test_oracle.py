@@ -9,6 +9,7 @@ import pytest
 
 from cudagravitydemo_amd import ics
 from oracle import binding as ob
+from tests import vote_check as vc
 from tests.octree_check import check_invariants, reference_octree
 
 
@@ -138,6 +139,26 @@ def test_topology_duplicates_and_edge_sizes():
     assert len(nodes) == 22 and nodes[-1]["count"] == 5
 
 
+def overflow_fixture():
+    """512 lattice bodies each with a twin ~2 grid cells away along x: every
+    pair shares its cells down to level 20, so the tree needs 9673 nodes,
+    more than the 8 N = 8192 the reference allocates (basicGravity.cu:11)."""
+    g = np.arange(8, dtype=np.float32)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel(), np.ones(512, np.float32)], 1)
+    twin = p.copy()
+    twin[:, 0] += np.float32(7.0 * 2.0 ** -20)
+    return np.ascontiguousarray(np.concatenate([p, twin]).astype(np.float32))
+
+
+def test_topology_overflow_at_eight_n():
+    pos = overflow_fixture()
+    keys = _sorted_keys(pos)
+    assert ob.build_topology(keys)[0] == -1
+    rc, _, _, _, _, total = ob.build_topology(keys, max_nodes=20 * len(pos))
+    assert rc == 0 and total == 9673 > 8 * len(pos)
+
+
 def test_topology_overflow_reported():
     pos, _ = ics.uniform_cube(256, seed=9)
     rc = ob.build_topology(_sorted_keys(pos), max_nodes=100)[0]
@@ -230,6 +251,146 @@ def test_walk_planted_mistake_is_caught():
                         0.5)
     e = np.linalg.norm(acc[sample] - exact, axis=1) / np.linalg.norm(exact, axis=1)
     assert np.median(e) > 1.0
+
+
+# ---- the reference walk's acceptance rule, pinned independently --------------
+# tests/vote_check.py enumerates each 32-body group's accepted set from the
+# definition (tree.cu:562-676) level by level; the oracle records the nodes
+# each body accepted while replaying the stack.  The fixtures are chosen so
+# that every plausible misreading of the rule changes some body's set.
+
+
+def vote_fixture():
+    """Lattice fixture with exact ties of the opening test, x not the widest
+    axis (x 32, y 45, z 15), 3 valid lanes in the last group and two stacks
+    of coincident bodies; plus a far body at x = 32 that makes the x extent a
+    power of two (node sizes 32 / 2^level)."""
+    g = np.arange(16, dtype=np.float32)
+    x, y, z = np.meshgrid(g, 3 * g, g, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel(), np.ones(4096, np.float32)], 1)
+    rng = np.random.default_rng(7)
+    p = p[np.sort(rng.choice(4096, 4096 - 13, replace=False))]
+    extra = np.concatenate([np.repeat(p[100:101], 10, 0), np.repeat(p[2000:2001], 5, 0),
+                            np.array([[32, 0, 0, 1]], np.float32)])
+    return np.ascontiguousarray(np.concatenate([p, extra]).astype(np.float32))
+
+
+def disk_fixture():
+    """A 3001-body disk stretched 3x along y (x is not the widest axis), 25
+    valid lanes in the last group, 7 coincident bodies."""
+    p, v = ics.exponential_disk(3001, seed=21)
+    p[:, 1] *= 3.0
+    p[500:507] = p[500]
+    return p, v
+
+
+def test_fmaf32_rounds_once():
+    # against exact rational arithmetic, including heavy cancellation
+    from fractions import Fraction
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(4000).astype(np.float32)
+    b = rng.standard_normal(4000).astype(np.float32)
+    c = (-(a.astype(np.float64) * b) * (1 + rng.standard_normal(4000) * 1e-7)).astype(np.float32)
+    c[:2000] = rng.standard_normal(2000).astype(np.float32)
+    got = vc.fmaf32(a, b, c)
+    for i in range(4000):
+        ex = Fraction(float(a[i])) * Fraction(float(b[i])) + Fraction(float(c[i]))
+        f = np.float32(float(ex))
+        cands = [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+        errs = [abs(Fraction(float(x)) - ex) for x in cands]
+        best = [x for x, e in zip(cands, errs) if e == min(errs)]
+        assert got[i] in best, (i, a[i], b[i], c[i])
+    # where rounding the float64 sum to float32 goes wrong: the exact value
+    # 1 + 2^-23 + 2^-24 - 2^-64 lies just below a float32 tie; float64 rounds
+    # it onto the tie and ties-to-even would then round up to 1 + 2^-22
+    a = np.float32(2.0 ** -24 * (1 + 2.0 ** -20))
+    b = np.float32(1 - 2.0 ** -20)
+    c = np.float32(1 + 2.0 ** -23)
+    assert np.float32(a.astype(np.float64) * b + np.float64(c)) == np.float32(1 + 2.0 ** -22)
+    assert vc.fmaf32(a, b, c) == c
+
+
+def _vote_case(pos, vel, theta):
+    t = ob.build_tree(pos, vel)
+    sets, opened, ties = vc.group_accepted_sets(t["pos"], t["nodes"], t["bmin"], t["bmax"], theta)
+    return t, sets, opened, ties
+
+
+@pytest.mark.parametrize("fixture,theta", [("lattice", 0.5), ("lattice", 1.0), ("disk", 0.5),
+                                           ("disk", 0.7)])
+def test_walk_accepted_sets_match_group_vote_definition(fixture, theta):
+    if fixture == "lattice":
+        pos = vote_fixture()
+        vel = np.zeros((pos.shape[0], 3), np.float32)
+    else:
+        pos, vel = disk_fixture()
+    n = pos.shape[0]
+    t, sets, opened, ties = _vote_case(pos, vel, theta)
+    if fixture == "lattice":
+        assert ties > 0            # the strict '<' is exercised
+    eps2 = np.float32(1e-4)
+    acc, inter, op, lists = ob.walk_traced(t["pos"], t["nodes_full"], t["bmin"], t["bmax"], eps2,
+                                           1.0, theta)
+    # the oracle's accepted nodes, body by body, equal the group's set
+    assert vc.compare_with_oracle_lists(sets, lists, n) == []
+    valid = [min(32, n - 32 * g) for g in range(len(sets))]
+    assert inter == sum(len(s) * k for s, k in zip(sets, valid))
+    assert op == opened
+    # or_walk itself takes the same path as the traced replay
+    acc0, inter0, op0 = ob.walk(t["pos"], t["nodes_full"], t["bmin"], t["bmax"], eps2, 1.0, theta)
+    assert np.array_equal(acc0, acc) and (inter0, op0) == (inter, op)
+    # every group's accepted nodes tile the bodies
+    assert vc.check_partition(sets, t["nodes"], n) == []
+    # forces: the oracle's FP32 sums against FP64 sums over the same sets
+    a64, mag = vc.monopole_fp64(t["pos"], t["nodes"], sets, float(eps2))
+    assert (np.linalg.norm(acc - a64, axis=1) / mag).max() < 2e-6
+
+
+def test_walk_accepted_sets_satisfy_the_rule_node_by_node():
+    # the rule restated per node, by parent pointers instead of levels:
+    # an accepted node is a leaf or passes for every valid lane; each of its
+    # proper ancestors below the root is not a leaf and fails for some lane
+    pos, vel = disk_fixture()
+    theta = 0.6
+    t, sets, _, _ = _vote_case(pos, vel, theta)
+    nodes = t["nodes"]
+    n = pos.shape[0]
+    ext = np.float64(t["bmax"][0]) - np.float64(t["bmin"][0])
+    leaf = vc.leaf_flags(nodes)
+    P = t["pos"][:, :3].astype(np.float64)
+    for g in range(0, len(sets), 5):
+        bodies = P[32 * g:min(n, 32 * g + 32)]
+        for v in sets[g]:
+            chain = [int(v)]
+            while nodes["parent"][chain[-1]] > 0:
+                chain.append(int(nodes["parent"][chain[-1]]))
+            for k, u in enumerate(chain):
+                size = ext / 2.0 ** int(nodes["level"][u])
+                d2 = ((bodies - nodes["mass"][u, :3].astype(np.float64)) ** 2).sum(1)
+                margin = size * size - theta * theta * d2     # < 0: the lane passes
+                if k == 0:
+                    assert leaf[u] or (margin < 1e-5 * size * size).all(), (g, u)
+                else:
+                    assert not leaf[u] and (margin > -1e-5 * size * size).any(), (g, u)
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+def test_walk_check_rejects_planted_mistakes(variant):
+    # 1 per-lane acceptance, 2 widest-axis node size, 3 '<=', 4 invalid lanes
+    # vote, 5 leaf rule without its no-children clause: each planted in the
+    # oracle, each must change some body's accepted set on a fixture
+    caught = []
+    for fixture, theta in (("lattice", 0.5), ("lattice", 1.0), ("disk", 0.5)):
+        if fixture == "lattice":
+            pos = vote_fixture()
+            vel = np.zeros((pos.shape[0], 3), np.float32)
+        else:
+            pos, vel = disk_fixture()
+        t, sets, _, _ = _vote_case(pos, vel, theta)
+        _, _, _, lists = ob.walk_traced(t["pos"], t["nodes_full"], t["bmin"], t["bmax"],
+                                        np.float32(1e-4), 1.0, theta, variant=variant)
+        caught.append(len(vc.compare_with_oracle_lists(sets, lists, pos.shape[0])) > 0)
+    assert caught[0] and caught[1], caught   # the lattice catches every variant
 
 
 def test_integrate_kick_then_drift():
