@@ -108,6 +108,8 @@ _PROTOS = {
     "gd_build_tree_async": (None, [C.POINTER(TreeState), vpp, vpp, vpp, vpp]),
     "gd_tree_levels": (None, [C.POINTER(TreeState)]),
     "gd_set_quadrupole": (None, [C.POINTER(TreeState), i32]),
+    "gd_build_timing": (None, [i32]),
+    "gd_build_phase_ms": (i32, [vp]),
     "gd_tree_walk_arrays": (None, [C.POINTER(TreeState), vp, vp, vp, i32]),
     "gd_tree_third_moments": (None, [C.POINTER(TreeState), vp, i32]),
     "gd_walk": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32,
@@ -132,6 +134,7 @@ _PROTOS = {
                                C.POINTER(WalkStats)]),
     "gd_integrate_hilo": (None, [vp, vp, vp, vp, i32, f32]),
     "gd_direct_sum_fp64": (None, [vp, vp, i32, vp, i32, f64, f64, vp]),
+    "gd_force_errors": (None, [vp, vp, i32, vp, vp]),
     "gd_diagnostics": (None, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
     "gd_stream_copy": (None, [vp, vp, C.c_size_t]),
     "gd_event_create": (vp, []),

@@ -3,9 +3,10 @@
 All arithmetic runs in libgravity.so: samples come from gd_corrector_samples
 (exact per-cell sums on the GPU), training from gd_mlp_train_step (BF16 MFMA
 forward/backward + Adam), and the corrected walk from gd_walk_learned.
-The FP64 direct-sum judge decides whether the corrector may be used: when its
-measured median force error exceeds the budget, the caller falls back to the
-plain walk at a smaller opening angle.
+The FP64 direct-sum judge decides whether the corrector may be used:
+JudgedRun checks the walk every k steps and, when the measured median force
+error exceeds the budget, switches the corrector off and continues with the
+walk without it at a smaller opening angle.
 """
 import ctypes as C
 
@@ -160,9 +161,9 @@ class SampleSet:
         self.Y.close()
 
 
-def train(net, samples, epochs=1, batch=8192, lr=2e-3, seed=0, log=None):
-    """Adam on shuffled minibatches of the sample set, cosine-decayed learning
-    rate (lr down to lr / 20).  The targets are trained at unit RMS (divided
+def train(net, samples, epochs=1, batch=8192, lr=2e-3, seed=0, log=None, decay=True):
+    """Adam on shuffled minibatches of the sample set, with the learning
+    rate cosine-decayed from lr to lr / 20 (decay) or constant.  The targets are trained at unit RMS (divided
     by their RMS over the set) and the factor is folded back into the output
     layer afterwards.  Returns per-epoch mean loss (in the unit-RMS scale)."""
     lib = net.lib
@@ -188,7 +189,7 @@ def train(net, samples, epochs=1, batch=8192, lr=2e-3, seed=0, log=None):
             tot, cnt = 0.0, 0
             for b0 in range(0, n - batch + 1 if n >= batch else 1, batch):
                 B = min(batch, n - b0)
-                f = 0.5 * (1.0 + np.cos(np.pi * step / max(total, 1)))
+                f = 0.5 * (1.0 + np.cos(np.pi * step / max(total, 1))) if decay else 1.0
                 rate = lr * (0.05 + 0.95 * f)
                 loss = lib.gd_mlp_train_step(net.ptr, samples.X.ptr, ys.ptr,
                                              order_buf.ptr + 4 * b0, B, rate)
@@ -233,3 +234,140 @@ def median_error(sim, sample):
 
 def default_targets(n, k=8192, seed=11):
     return ics.subsample(n, k, seed=seed)
+
+
+def train_on_snapshots(sim, net, advance, snapshots=4, targets=8192, subsamples=1,
+                       sample_theta=0.9, min_ratio=0.5875, epochs=10, batch=8192, lr=2e-3,
+                       cap=24_000_000, seed=0, log=None, decay=True):
+    """Train `net` on (features -> residual) samples from snapshots of the
+    run in `sim`: at each snapshot the tree is built with second and third
+    moments and `subsamples` seeded sets of `targets` bodies give their
+    samples (gd_corrector_samples at opening angle sample_theta, cells with
+    size / distance above min_ratio); `advance()` moves the run on to the
+    next snapshot.  Returns a dict with the sample count, per-epoch losses and
+    the seconds spent."""
+    import time
+    t0 = time.perf_counter()
+    sim.set_quadrupole(2)
+    samples = SampleSet(sim.lib, cap)
+    try:
+        for snap in range(snapshots):
+            sim.build_tree_async()
+            for k in range(subsamples):
+                got, _ = samples.add_from(
+                    sim, default_targets(sim.n, targets, seed=seed + 1000 * snap + k),
+                    sample_theta, min_ratio)
+                if log:
+                    log(f"snapshot {snap} subsample {k}: {got} samples")
+            if snap + 1 < snapshots:
+                advance()
+        samples.canonical_order()
+        n = samples.n
+        losses = train(net, samples, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log,
+                       decay=decay)
+    finally:
+        samples.close()
+    return {"samples": n, "losses": losses, "seconds": time.perf_counter() - t0}
+
+
+class JudgedRun:
+    """The learned walk with its FP64 judge and automatic fallback.
+
+    Each step is the reference's (basicGravity.cu:774-783): build the tree,
+    walk, kick-drift.  Every `every`-th step, counting from the first, the
+    walk's accelerations on a seeded sample of `sample_size` bodies are
+    compared with the FP64 direct sum (gd_direct_sum_fp64, then
+    gd_force_errors) before the kick-drift.  When the median relative error
+    exceeds `budget` the corrector is switched off for the rest of the run:
+    the step's walk is done again with the fallback walk (by default the
+    quadrupole walk at theta 0.4, the corrector's own method without the
+    network) and every later step uses it.  The fallback's accelerations are
+    judged on the same step as well, so every judged step records the error
+    of the accelerations that were integrated.
+
+    checks: one dict per judged walk (step, walk, theta, median, p99, within).
+    judge_ms: time spent in the judge (HIP events), for rates with and without it.
+    """
+
+    def __init__(self, sim, net, theta, dt, min_ratio=0.5875, budget=1e-3, every=20,
+                 sample_size=4096, sample_seed=4242, fallback_theta=0.4, fallback="quad"):
+        if fallback not in ("quad", "plain"):
+            raise ValueError("fallback is 'quad' or 'plain'")
+        self.sim = sim
+        self.lib = sim.lib
+        self.net = net
+        self.theta = float(theta)
+        self.dt = float(dt)
+        self.min_ratio = float(min_ratio)
+        self.budget = float(budget)
+        self.every = int(every)
+        self.fallback = fallback
+        self.fallback_theta = float(fallback_theta)
+        self.corrector_on = net is not None
+        self.steps = 0
+        self.checks = []
+        self.judge_ms = 0.0
+        self.sample = ics.subsample(sim.n, sample_size, seed=sample_seed)
+        ns = self.sample.shape[0]
+        self._sample = to_device(self.lib, self.sample)
+        self._exact = DeviceBuffer(self.lib, 24 * ns)
+        self._err = DeviceBuffer(self.lib, 8 * ns)
+        self._ev = (self.lib.gd_event_create(), self.lib.gd_event_create())
+        sim.set_quadrupole(2 if self.corrector_on else (1 if fallback == "quad" else 0))
+
+    def _walk(self):
+        if self.corrector_on:
+            walk_learned(self.sim, self.net, self.theta, self.min_ratio)
+        else:
+            self.sim.walk(self.fallback_theta)
+
+    def _judge(self, exact_ready):
+        """Median / p99 error of sim.acc on the sample; computes the direct sum
+        unless this step's is already in place."""
+        lib, ns = self.lib, self.sample.shape[0]
+        lib.gd_event_record(self._ev[0])
+        if not exact_ready:
+            lib.gd_direct_sum_fp64(self.sim.pos, self.sim.lo if self.sim.hilo else None,
+                                   self.sim.n, self._sample.ptr, ns, self.sim.eps ** 2,
+                                   self.sim.G, self._exact.ptr)
+        lib.gd_force_errors(self.sim.acc, self._sample.ptr, ns, self._exact.ptr, self._err.ptr)
+        lib.gd_event_record(self._ev[1])
+        err = from_device(lib, self._err.ptr, np.float64, (ns,))
+        self.judge_ms += lib.gd_event_elapsed_ms(self._ev[0], self._ev[1])
+        check(lib)
+        return float(np.median(err)), float(np.quantile(err, 0.99))
+
+    def step(self):
+        sim = self.sim
+        sim.build_tree_async()
+        self._walk()
+        if self.steps % self.every == 0:
+            med, p99 = self._judge(False)
+            rec = {"step": self.steps, "walk": "learned" if self.corrector_on else self.fallback,
+                   "theta": self.theta if self.corrector_on else self.fallback_theta,
+                   "median": med, "p99": p99, "within": med <= self.budget}
+            self.checks.append(rec)
+            if self.corrector_on and not rec["within"]:
+                # switch the corrector off and redo this step's walk without it
+                # (gd_walk takes quadrupole cells when the moment level is >= 1;
+                # this step's tree holds the second moments either way)
+                self.corrector_on = False
+                sim.set_quadrupole(1 if self.fallback == "quad" else 0)
+                self._walk()
+                med, p99 = self._judge(True)
+                self.checks.append({"step": self.steps, "walk": self.fallback,
+                                    "theta": self.fallback_theta, "median": med, "p99": p99,
+                                    "within": med <= self.budget})
+        sim.integrate(self.dt)
+        self.steps += 1
+
+    def run(self, steps):
+        for _ in range(steps):
+            self.step()
+
+    def close(self):
+        for b in (self._sample, self._exact, self._err):
+            b.close()
+        for e in self._ev:
+            self.lib.gd_event_destroy(e)
+        self._ev = ()

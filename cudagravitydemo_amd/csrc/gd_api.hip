@@ -311,6 +311,28 @@ void createWorldBox(const float4* cudaPositionMassVals, const int starCount,
     outputWorldBox->scale = scale;
 }
 
+/* optional HIP-event timing of the build's stages (null stream) */
+static int g_buildTiming = 0;
+static int g_buildTimed = 0;
+static hipEvent_t g_buildEv[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+static void gd_build_mark(int k) {
+    if (g_buildTiming) GD_HIP(hipEventRecord(g_buildEv[k], nullptr));
+}
+
+void gd_build_timing(int enable) {
+    g_buildTiming = enable ? 1 : 0;
+    if (g_buildTiming && !g_buildEv[0])
+        for (int e = 0; e < 7; ++e) GD_HIP(hipEventCreate(&g_buildEv[e]));
+}
+
+int gd_build_phase_ms(float* ms) {
+    if (!g_buildTimed) return -1;
+    GD_HIP(hipEventSynchronize(g_buildEv[6]));
+    for (int e = 0; e < 6; ++e) GD_HIP(hipEventElapsedTime(&ms[e], g_buildEv[e], g_buildEv[e + 1]));
+    return 0;
+}
+
 void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
                          float3** spareLo) {
     if (!tree || !tree->sorter || !tree->builder || !pos || !vel) {
@@ -321,13 +343,18 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
     if (n <= 0) return;
     gd_builder* b = gd_builder_of(tree->builder);
     gd_builder_walk_arrays(b, tree->maxNodes);
+    if (gd_has_error()) return;
+    gd_build_mark(0);
     /* 1. bbox and quantisation box, on the device */
     gd_launch_bbox(*pos, n, b, tree->boundingData[2], tree->boundingData[3]);
+    gd_build_mark(1);
     /* 2. keys */
     gd_launch_morton_dev(*pos, n, b->dinfo, tree->mortonIn, tree->indexIn);
+    gd_build_mark(2);
     /* 3. sort */
     gd_sort_pairs(gd_sorter_of(tree->sorter), tree->mortonIn, tree->mortonOut, tree->indexIn,
                   tree->indexOut, n);
+    gd_build_mark(3);
     /* 4. reorder into the spare buffers, then swap (basicGravity.cu:217-226) */
     if (posLo && spareLo && *posLo && *spareLo) {
         gd_launch_reorder_lo(*pos, *vel, *posLo, tree->sortedPosMassVals, tree->sortedVelVals,
@@ -345,8 +372,13 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
     float3* tv = *vel;
     *vel = tree->sortedVelVals;
     tree->sortedVelVals = tv;
+    gd_build_mark(4);
     /* 5. topology and centre of mass */
+    gd_build_mid_event = g_buildTiming ? g_buildEv[5] : nullptr;
     gd_build_octree(b, tree->mortonOut, *pos, n, tree->nodes, tree->maxNodes);
+    gd_build_mid_event = nullptr;
+    gd_build_mark(6);
+    if (g_buildTiming) g_buildTimed = 1;
 }
 
 void gd_set_quadrupole(treeState* tree, int enable) {

@@ -685,6 +685,8 @@ __global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
     if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
 }
 
+hipEvent_t gd_build_mid_event = nullptr;
+
 void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* sortedPos, int n,
                      node* nodes, int maxNodes) {
     if (n <= 0) return;
@@ -701,6 +703,7 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
     k_build_nodes<<<tiles, GD_BUILD_THREADS>>>(sortedKeys, n, b->tileOffsets, b->levelTotals,
                                                b->maxTiles, b->meta, maxNodes, b->dinfo);
     GD_LAUNCHED();
+    if (gd_build_mid_event) GD_HIP(hipEventRecord(gd_build_mid_event, nullptr));
     for (int L = GD_MAX_LEVEL; L >= 0; --L) {
         /* a level holds at most min(8^L, n) nodes */
         long long bound = n;

@@ -11,8 +11,7 @@
  * (Dekker 1971 / Knuth TAOCP 4.2.2 two-sum).
  *
  * The judge is an FP64 softened direct sum over all bodies for a sample of
- * bodies: one block per sample body, threads stride over sources, FP64
- * accumulation, wave then block reduction.
+ * bodies (tiles of samples in registers, sources streamed through LDS).
  */
 #include "gd_internal.h"
 
@@ -88,74 +87,188 @@ extern "C" void gd_integrate_hilo(float4* pos, float3* posLo, float3* vel, const
 
 /* ---------------------------------------------------------------------------
  * FP64 direct-sum judge
+ *
+ * k_judge_tile: block (t, c) holds the 256 sample bodies of tile t, one per
+ * thread, in registers (FP64 hi + lo), and streams source chunk c through
+ * LDS, 256 sources at a time, each converted to FP64 once (hi + lo, mass);
+ * every thread sums its sample's softened terms over the chunk in FP64 and
+ * writes the partial sum to part[c][sample].  k_judge_sum adds the chunks'
+ * partials in chunk order, so the result does not depend on scheduling.
+ * The sources are read once per sample tile (ns / 256 times) instead of once
+ * per sample.  A sample's own term is exactly 0 when eps2 > 0 (its offset is
+ * 0); with eps2 == 0 it is skipped explicitly (SKIP), and coincident distinct
+ * bodies then give inf / NaN, as the plain sum would.
  * ------------------------------------------------------------------------- */
+#define GD_JUDGE_TILE 256          /* samples per block, one per thread */
+#define GD_JUDGE_SRC 256           /* sources per LDS stage */
+
+template <bool SKIP>
+__global__ __launch_bounds__(GD_JUDGE_TILE) void k_judge_tile(
+    const float4* __restrict__ pos, const float3* __restrict__ lo, int n,
+    const int* __restrict__ sample, int ns, int chunks, long long chunkLen, double eps2,
+    double3* __restrict__ part) {
+    __shared__ double4 sSrc[GD_JUDGE_SRC];
+    const int tile = blockIdx.x / chunks;
+    const int c = blockIdx.x - tile * chunks;
+    const int k = tile * GD_JUDGE_TILE + threadIdx.x;
+    const int si = k < ns ? sample[k] : -1;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (si >= 0) {
+        const float4 p = pos[si];
+        xi = p.x; yi = p.y; zi = p.z;
+        if (lo) {
+            const float3 l = lo[si];
+            xi += l.x; yi += l.y; zi += l.z;
+        }
+    }
+    const long long j0 = (long long)c * chunkLen;
+    const long long j1 = j0 + chunkLen < n ? j0 + chunkLen : n;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (long long base = j0; base < j1; base += GD_JUDGE_SRC) {
+        const long long j = base + threadIdx.x;
+        double4 q = make_double4(0.0, 0.0, 0.0, 0.0);   /* past the end: massless */
+        if (j < j1) {
+            const float4 p = pos[j];
+            q = make_double4(p.x, p.y, p.z, p.w);
+            if (lo) {
+                const float3 l = lo[j];
+                q.x += l.x; q.y += l.y; q.z += l.z;
+            }
+        }
+        __syncthreads();   /* the previous stage is consumed */
+        sSrc[threadIdx.x] = q;
+        __syncthreads();
+        const int cnt = j1 - base < GD_JUDGE_SRC ? (int)(j1 - base) : GD_JUDGE_SRC;
+#pragma unroll 4
+        for (int e = 0; e < cnt; ++e) {
+            const double4 s = sSrc[e];
+            const double dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
+            const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+            /* m r^-3 from the hardware reciprocal square root y (relative error
+               below 2^-22) corrected to first order: with e = 1 - r2 y^2,
+               r^-3 = y^3 (1 + 3e/2 + O(e^2)) = y^3 (2.5 - 1.5 r2 y^2) + O(e^2),
+               leaving ~1e-13 */
+            const double y = __builtin_amdgcn_rsq(r2);
+            const double y2 = y * y;
+            const double g = __builtin_fma(-1.5 * r2, y2, 2.5);
+            double f = ((s.w * y) * y2) * g;
+            if (SKIP && base + e == (long long)si) f = 0.0;
+            ax = __builtin_fma(f, dx, ax);
+            ay = __builtin_fma(f, dy, ay);
+            az = __builtin_fma(f, dz, az);
+        }
+    }
+    if (k < ns) part[(size_t)c * ns + k] = make_double3(ax, ay, az);
+}
+
+__global__ __launch_bounds__(256) void k_judge_sum(const double3* __restrict__ part, int ns,
+                                                   int chunks, double G,
+                                                   double* __restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= ns) return;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+        const double3 v = part[(size_t)c * ns + k];
+        ax += v.x; ay += v.y; az += v.z;
+    }
+    out[3 * k] = -G * ax;
+    out[3 * k + 1] = -G * ay;
+    out[3 * k + 2] = -G * az;
+}
+
+/* Source chunks per sample tile: enough blocks to fill the chip (about
+ * 2048), each chunk at least 2048 sources, and at most maxParts partial sums
+ * (chunks x ns) in the scratch. */
+static int gd_judge_chunks(int n, int ns, long long maxParts) {
+    const int tiles = gd_div_up(ns, GD_JUDGE_TILE);
+    int chunks = gd_div_up(2048, tiles);
+    const int bySize = gd_div_up(n, 2048);
+    if (chunks > bySize) chunks = bySize;
+    if ((long long)chunks * ns > maxParts) chunks = (int)(maxParts / ns);
+    return chunks < 1 ? 1 : chunks;
+}
+
+static void gd_launch_judge(const float4* pos, const float3* posLo, int n, const int* sample,
+                            int ns, double eps2, double G, double* acc, double3* part,
+                            int chunks) {
+    const long long chunkLen =
+        ((gd_div_up(n, chunks) + GD_JUDGE_SRC - 1) / GD_JUDGE_SRC) * (long long)GD_JUDGE_SRC;
+    const int tiles = gd_div_up(ns, GD_JUDGE_TILE);
+    if (eps2 > 0.0)
+        k_judge_tile<false><<<tiles * chunks, GD_JUDGE_TILE>>>(pos, posLo, n, sample, ns, chunks,
+                                                               chunkLen, eps2, part);
+    else
+        k_judge_tile<true><<<tiles * chunks, GD_JUDGE_TILE>>>(pos, posLo, n, sample, ns, chunks,
+                                                              chunkLen, eps2, part);
+    GD_LAUNCHED();
+    k_judge_sum<<<gd_div_up(ns, 256), 256>>>(part, ns, chunks, G, acc);
+    GD_LAUNCHED();
+}
+
+/* library-held partial sums, grown on demand */
+static double3* g_judgePart = nullptr;
+static long long g_judgeParts = 0;
+#define GD_JUDGE_PARTS (1ll << 22)     /* at most 96 MiB of partials */
+
+extern "C" void gd_direct_sum_fp64(const float4* pos, const float3* posLo, int starCount,
+                                   const int* sample, int ns, double eps2, double G, double* acc) {
+    if (ns <= 0 || starCount <= 0) return;
+    if (eps2 < 0.0) {
+        GD_FAIL("gd_direct_sum_fp64: eps2 = %g is negative", eps2);
+        return;
+    }
+    const int chunks = gd_judge_chunks(starCount, ns, GD_JUDGE_PARTS);
+    const long long need = (long long)chunks * ns;
+    if (need > g_judgeParts) {
+        if (g_judgePart) GD_HIP(hipFree(g_judgePart));
+        g_judgePart = nullptr;
+        g_judgeParts = 0;
+        GD_HIP(hipMalloc(&g_judgePart, (size_t)need * sizeof(double3)));
+        if (gd_has_error()) return;
+        g_judgeParts = need;
+    }
+    gd_launch_judge(pos, posLo, starCount, sample, ns, eps2, G, acc, g_judgePart, chunks);
+}
+
+/* err[k] = |acc[sample[k]] - exact[k]| / |exact[k]| in FP64 (0 when exact is 0). */
+__global__ __launch_bounds__(256) void k_force_errors(const float3* __restrict__ acc,
+                                                      const int* __restrict__ sample, int ns,
+                                                      const double* __restrict__ exact,
+                                                      double* __restrict__ err) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= ns) return;
+    const float3 a = acc[sample[k]];
+    const double ex = exact[3 * k], ey = exact[3 * k + 1], ez = exact[3 * k + 2];
+    const double dx = (double)a.x - ex, dy = (double)a.y - ey, dz = (double)a.z - ez;
+    const double num = sqrt(dx * dx + dy * dy + dz * dz);
+    const double den = sqrt(ex * ex + ey * ey + ez * ez);
+    err[k] = den > 0.0 ? num / den : 0.0;
+}
+
+extern "C" void gd_force_errors(const float3* acc, const int* sample, int ns, const double* exact,
+                                double* err) {
+    if (ns <= 0) return;
+    k_force_errors<<<gd_div_up(ns, 256), 256>>>(acc, sample, ns, exact, err);
+    GD_LAUNCHED();
+}
+
+/* ---------------------------------------------------------------------------
+ * FP64 diagnostics: kinetic energy, momentum, mass, potential energy.
+ * Each block writes its six FP64 sums to part[block]; k_diag_sum adds the
+ * blocks in block order, so the result does not depend on scheduling.
+ * ------------------------------------------------------------------------- */
+#define GD_DIAG_BLOCKS 1024
+
 __device__ __forceinline__ double gd_wave_sum_d(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_direct_fp64(const float4* __restrict__ pos,
-                                                     const float3* __restrict__ lo, int n,
-                                                     const int* __restrict__ sample,
-                                                     double eps2, double G,
-                                                     double* __restrict__ out) {
-    __shared__ double sPart[3][4];
-    const int s = sample[blockIdx.x];
-    const float4 pi = pos[s];
-    double xi = pi.x, yi = pi.y, zi = pi.z;
-    if (lo) {
-        const float3 li = lo[s];
-        xi += li.x; yi += li.y; zi += li.z;
-    }
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) {
-        if (j == s) continue;
-        const float4 pj = pos[j];
-        double xj = pj.x, yj = pj.y, zj = pj.z;
-        if (lo) {
-            const float3 lj = lo[j];
-            xj += lj.x; yj += lj.y; zj += lj.z;
-        }
-        const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
-        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-        const double inv = 1.0 / sqrt(r2);
-        const double f = -G * (double)pj.w * inv * inv * inv;
-        ax += f * dx;
-        ay += f * dy;
-        az += f * dz;
-    }
-    ax = gd_wave_sum_d(ax);
-    ay = gd_wave_sum_d(ay);
-    az = gd_wave_sum_d(az);
-    const int wave = threadIdx.x >> 6;
-    if (gd_lane() == 0) {
-        sPart[0][wave] = ax;
-        sPart[1][wave] = ay;
-        sPart[2][wave] = az;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const double v = sPart[threadIdx.x][0] + sPart[threadIdx.x][1] + sPart[threadIdx.x][2] +
-                         sPart[threadIdx.x][3];
-        out[3 * blockIdx.x + threadIdx.x] = v;
-    }
-}
-
-extern "C" void gd_direct_sum_fp64(const float4* pos, const float3* posLo, int starCount,
-                                   const int* sample, int ns, double eps2, double G, double* acc) {
-    if (ns <= 0 || starCount <= 0) return;
-    k_direct_fp64<<<ns, 256>>>(pos, posLo, starCount, sample, eps2, G, acc);
-    GD_LAUNCHED();
-}
-
-/* ---------------------------------------------------------------------------
- * FP64 diagnostics: kinetic energy, momentum, mass, potential energy
- * ------------------------------------------------------------------------- */
 __global__ __launch_bounds__(256) void k_diag(const float4* __restrict__ pos,
                                               const float3* __restrict__ vel,
                                               const float* __restrict__ phi, int n,
-                                              double* __restrict__ out) {
+                                              double* __restrict__ part) {
     __shared__ double sPart[6][4];
     double ke = 0.0, px = 0.0, py = 0.0, pz = 0.0, m = 0.0, pe = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -178,27 +291,36 @@ __global__ __launch_bounds__(256) void k_diag(const float4* __restrict__ pos,
     }
     __syncthreads();
     if (threadIdx.x < 6) {
-        const double s = sPart[threadIdx.x][0] + sPart[threadIdx.x][1] + sPart[threadIdx.x][2] +
+        const double t = sPart[threadIdx.x][0] + sPart[threadIdx.x][1] + sPart[threadIdx.x][2] +
                          sPart[threadIdx.x][3];
-        atomicAdd(&out[threadIdx.x], s);
+        part[blockIdx.x * 6 + threadIdx.x] = t;
     }
 }
 
+__global__ void k_diag_sum(const double* __restrict__ part, int blocks, double* __restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= 6) return;
+    double t = 0.0;
+    for (int b = 0; b < blocks; ++b) t += part[b * 6 + k];
+    out[k] = t;
+}
+
+static double* g_diagPart = nullptr;   /* GD_DIAG_BLOCKS x 6 partials, then the 6 sums */
+
 extern "C" void gd_diagnostics(const float4* pos, const float3* vel, const float* phi,
                                int starCount, double* out) {
-    double* d = nullptr;
-    GD_HIP(hipMalloc(&d, 6 * sizeof(double)));
+    if (!g_diagPart) GD_HIP(hipMalloc(&g_diagPart, (GD_DIAG_BLOCKS + 1) * 6 * sizeof(double)));
     if (gd_has_error()) return;
-    GD_HIP(hipMemset(d, 0, 6 * sizeof(double)));
+    double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (starCount > 0) {
         int blocks = gd_div_up(starCount, 256);
-        if (blocks > 1024) blocks = 1024;
-        k_diag<<<blocks, 256>>>(pos, vel, phi, starCount, d);
+        if (blocks > GD_DIAG_BLOCKS) blocks = GD_DIAG_BLOCKS;
+        k_diag<<<blocks, 256>>>(pos, vel, phi, starCount, g_diagPart);
         GD_LAUNCHED();
+        k_diag_sum<<<1, 64>>>(g_diagPart, blocks, g_diagPart + GD_DIAG_BLOCKS * 6);
+        GD_LAUNCHED();
+        GD_HIP(hipMemcpy(h, g_diagPart + GD_DIAG_BLOCKS * 6, sizeof(h), hipMemcpyDeviceToHost));
     }
-    double h[6];
-    GD_HIP(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
-    GD_HIP(hipFree(d));
     for (int k = 0; k < 6; ++k) out[k] = h[k];
     out[6] = h[0] + h[5];
 }

@@ -146,6 +146,9 @@ void gd_sort_pairs(gd_sorter* s, const uint64_t* kin, uint64_t* kout, const uint
 void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* sortedPos, int n,
                      node* nodes, int maxNodes);
 void gd_exclusive_scan_u32(gd_builder* b, const uint32_t* in, uint32_t* out, int n);
+/* When not null, gd_build_octree records this event between the topology
+ * kernels and the centre-of-mass levels (build-stage timing). */
+extern hipEvent_t gd_build_mid_event;
 
 /* ---- wave helpers ------------------------------------------------------ */
 #ifdef __HIP_DEVICE_COMPILE__

@@ -53,6 +53,14 @@ void gd_tree_summary(const treeState* tree, int* totalNodes, int* overflow, int*
 void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
                          float3** spareLo);
 
+/* Time every later gd_build_tree_async / buildTree with HIP events on the
+ * null stream (enable = 1) or stop (0).  gd_build_phase_ms fills ms[0..5]
+ * with the last build's bbox, keys, sort, reorder, topology (node counts,
+ * level scan, node records) and centre-of-mass (all levels, with the
+ * moments) times and returns 0, or -1 when nothing was timed (synchronises). */
+void gd_build_timing(int enable);
+int gd_build_phase_ms(float* ms);
+
 /* Makes every later build also compute each node's moments about its centre
  * of mass: enable = 1 the second moments S = sum m (x-c)(x-c)^T (used by the
  * quadrupole walk; 32 B per node of capacity), 2 also the third moments
@@ -211,10 +219,16 @@ void gd_integrate_hilo(float4* pos, float3* posLo, float3* vel, const float3* ac
 void gd_direct_sum_fp64(const float4* pos, const float3* posLo, int starCount,
                         const int* sample, int ns, double eps2, double G, double* acc);
 
+/* Relative force errors against the judge (device arrays):
+ * err[k] = |acc[sample[k]] - exact[3k..3k+2]| / |exact[3k..3k+2]| in FP64. */
+void gd_force_errors(const float3* acc, const int* sample, int ns, const double* exact,
+                     double* err);
+
 /* ---- FP64 diagnostics -----------------------------------------------------
  * out[0..6] (host) = kinetic energy, momentum x/y/z, mass, and the
  * potential energy 0.5 sum m_i phi_i when phi != NULL (else 0), all reduced in
- * FP64 on the device; out[6] = total energy. */
+ * FP64 on the device in a fixed order (the same inputs give the same bits);
+ * out[6] = total energy.  Synchronises. */
 void gd_diagnostics(const float4* pos, const float3* vel, const float* phi, int starCount,
                     double* out);
 

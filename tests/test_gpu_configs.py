@@ -1,0 +1,221 @@
+"""The precision, training and full-size configurations, run as configured.
+
+  precision form   Plummer sphere, 2^22 bodies, hi/lo positions, compensated
+                   FP32 force sums, theta 0.5, eps 0.01, dt 1e-3, 1000 steps:
+                   FP64 energy and momentum, force error vs the FP64 judge.
+  training         snapshots every 50 steps of the 2^21-body merger (eps
+                   0.005, dt 5e-4, 200 steps), ~50M samples from 8192-body
+                   seeded subsamples, Adam lr 1e-3, batch 65536; the corrected
+                   walk at theta 0.8 against the plain walk at theta 0.4 on the
+                   same snapshots.
+  full size        the 2^24-body merger (hi/lo, eps 0.002, dt 2.5e-4), the
+                   learned walk at theta 0.7 for 200 steps, the FP64 judge every
+                   20 steps with automatic fallback; and the fallback forced
+                   by a network whose output is scaled wrong.
+
+The step is the reference's main loop (basicGravity.cu:767-803): build,
+walk, kick-drift.  Each test prints its figures.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from cudagravitydemo_amd import Simulation, ics
+from cudagravitydemo_amd import corrector as cr
+
+pytestmark = pytest.mark.gpu
+
+BUDGET = 1e-3
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _errors(sim, sample, exact):
+    a = sim.accelerations()[sample]
+    e = np.linalg.norm(a - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    return float(np.median(e)), float(np.quantile(e, 0.99))
+
+
+@pytest.mark.timeout(900)
+def test_precision_form_plummer_2e22_1000_steps():
+    n, eps, dt, theta, steps = 1 << 22, 0.01, 1e-3, 0.5, 1000
+    pos, vel = ics.plummer(n, seed=7)
+    _say(f"case precision form: Plummer N={n} theta={theta} eps={eps} dt={dt} steps={steps}")
+    sample = ics.subsample(n, 4096, seed=77)
+    with Simulation(pos, vel, eps=eps, hilo=True) as sim:
+        sim.set_quadrupole(1)
+
+        def measure():
+            """FP64 energy (potential from a theta = 0.25 walk), momentum, and
+            the theta = 0.5 walk's force error on the sample."""
+            sim.build_tree_async()
+            exact = sim.direct_sum_fp64(sample)
+            sim.walk(theta)
+            med, p99 = _errors(sim, sample, exact)
+            sim.walk(0.25, potential=True)
+            d = sim.diagnostics(with_potential=True)
+            v = sim.velocities().astype(np.float64)
+            m = sim.positions()[:, 3].astype(np.float64)
+            scale = float((m * np.linalg.norm(v, axis=1)).sum())
+            return d, med, p99, scale
+
+        d0, med0, p990, pscale = measure()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step(dt, theta)
+        sim.sync()
+        secs = time.perf_counter() - t0
+        d1, med1, p991, _ = measure()
+        dE = (d1["total"] - d0["total"]) / abs(d0["total"])
+        dp = np.linalg.norm([d1[k] - d0[k] for k in ("px", "py", "pz")]) / pscale
+        _say(f"precision form: E0={d0['total']:.9f} E1={d1['total']:.9f} dE/E={dE:.3e} "
+             f"|dp|/sum m|v|={dp:.3e}; force error median/p99 {med0:.2e}/{p990:.2e} -> "
+             f"{med1:.2e}/{p991:.2e}; {1e3 * secs / steps:.2f} ms/step")
+        # Plummer (G = M = a = 1): E = -3 pi / 64
+        assert abs(d0["total"] + 3 * np.pi / 64) < 0.01
+        assert abs(d0["mass"] - 1.0) < 1e-9
+        assert abs(dE) < 1e-4
+        assert dp < 1e-5
+        assert med0 < BUDGET and med1 < BUDGET
+
+
+@pytest.mark.timeout(900)
+def test_training_config_2e21_merger_snapshots():
+    per_disk, eps, dt = 1 << 20, 0.005, 5e-4
+    pos, vel = ics.disk_merger(per_disk, seed=2024)
+    n = pos.shape[0]
+    want, batch, lr = 50_000_000, 65536, 1e-3
+    _say(f"case training: merger N={n}, snapshots every 50 of 200 steps, ~{want} samples, "
+         f"batch {batch}, lr {lr}")
+    with Simulation(pos, vel, eps=eps) as sim:
+        snaps = []
+        for step in range(201):
+            if step % 50 == 0:
+                snaps.append((sim.positions(), sim.velocities()))
+            if step < 200:
+                sim.step(dt, 0.5)   # the plain method at scale: theta 0.5
+        net = cr.Corrector(seed=1)
+        samples = cr.SampleSet(sim.lib, want + 12_000_000)
+        t0 = time.perf_counter()
+        try:
+            sim.set_quadrupole(2)
+            subs = 0
+            for k, (p, v) in enumerate(snaps):
+                sim.set_state(p, v)
+                sim.build_tree_async()
+                goal = want * (k + 1) // len(snaps)
+                while samples.n < goal:
+                    got, dropped = samples.add_from(
+                        sim, cr.default_targets(n, 8192, seed=1000 * k + subs), 0.9, 0.5875)
+                    assert dropped == 0 and got > 0
+                    subs += 1
+            gen_s = time.perf_counter() - t0
+            samples.canonical_order()
+            t1 = time.perf_counter()
+            losses = cr.train(net, samples, epochs=2, batch=batch, lr=lr, decay=False)
+            train_s = time.perf_counter() - t1
+            _say(f"training: {samples.n} samples from {subs} subsamples of 8192 bodies in "
+                 f"{gen_s:.1f} s; 2 epochs in {train_s:.1f} s, loss {losses}")
+            assert samples.n >= want
+        finally:
+            samples.close()
+        assert losses[-1] < 0.7
+
+        sample = ics.subsample(n, 4096, seed=555)   # held out from the training targets
+        e0, e1 = sim.lib.gd_event_create(), sim.lib.gd_event_create()
+
+        def timed(fn):
+            sim.lib.gd_event_record(e0)
+            fn()
+            sim.lib.gd_event_record(e1)
+            return sim.lib.gd_event_elapsed_ms(e0, e1)
+
+        rows = []
+        for k, (p, v) in enumerate(snaps):
+            sim.set_state(p, v)
+            sim.set_quadrupole(2)
+            sim.build_tree_async()
+            exact = sim.direct_sum_fp64(sample)
+            cr.walk_learned(sim, net, 0.8, 0.5875)   # warm the list layout
+            ms_l = timed(lambda: cr.walk_learned(sim, net, 0.8, 0.5875))
+            err_l = _errors(sim, sample, exact)
+            sim.set_quadrupole(1)
+            sim.build_tree_async()
+            sim.walk(0.8)
+            ms_q = timed(lambda: sim.walk(0.8))
+            err_q = _errors(sim, sample, exact)
+            sim.set_quadrupole(0)
+            sim.build_tree_async()
+            sim.walk(0.4)
+            ms_p = timed(lambda: sim.walk(0.4))
+            err_p = _errors(sim, sample, exact)
+            rows.append((err_l[0], err_p[0], err_q[0], ms_l, ms_p, ms_q))
+            _say(f"snapshot {50 * k}: learned theta 0.8 median {err_l[0]:.2e} p99 {err_l[1]:.2e} "
+                 f"{ms_l:.2f} ms | plain theta 0.4 {err_p[0]:.2e}/{err_p[1]:.2e} {ms_p:.2f} ms | "
+                 f"quadrupole theta 0.8 {err_q[0]:.2e}/{err_q[1]:.2e} {ms_q:.2f} ms")
+        net.close()
+        for el, ep, eq, ml, mp, mq in rows:
+            assert el < ep          # more accurate than the plain walk at half the angle
+            assert ml < mp          # and faster
+            assert el < eq          # the network improves on its own method
+
+
+@pytest.mark.timeout(900)
+def test_full_size_learned_run_with_judge_and_fallback():
+    per_disk, eps, dt, theta, steps, every = 1 << 23, 0.002, 2.5e-4, 0.7, 200, 20
+    pos, vel = ics.disk_merger(per_disk, seed=2024)
+    n = pos.shape[0]
+    _say(f"case full size: merger N={n} hi/lo, learned theta {theta}, {steps} steps, "
+         f"judge every {every}")
+    with Simulation(pos, vel, eps=eps, hilo=True) as sim:
+        net = cr.Corrector(seed=1)
+
+        def advance():
+            sim.set_quadrupole(1)
+            for _ in range(5):
+                sim.step(dt, 0.6)
+            sim.set_quadrupole(2)
+
+        info = cr.train_on_snapshots(sim, net, advance, snapshots=4, targets=8192)
+        _say(f"full size: corrector trained on {info['samples']} samples in "
+             f"{info['seconds']:.1f} s, loss {info['losses'][0]:.3f} -> {info['losses'][-1]:.3f}")
+        run = cr.JudgedRun(sim, net, theta, dt, every=every, sample_seed=4242)
+        try:
+            t0 = time.perf_counter()
+            run.run(steps)
+            sim.sync()
+            secs = time.perf_counter() - t0
+            judged = sorted({c["step"] for c in run.checks})
+            for c in run.checks:
+                _say(f"  check step {c['step']:3d} {c['walk']:7s} theta {c['theta']}: "
+                     f"median {c['median']:.3e} p99 {c['p99']:.3e}")
+            nj = len(run.checks)
+            _say(f"full size: {1e3 * secs / steps:.2f} ms/step with the judge, "
+                 f"{1e3 * (secs - run.judge_ms / 1e3) / steps:.2f} without; "
+                 f"{run.judge_ms / nj:.1f} ms per check; corrector on at the end: "
+                 f"{run.corrector_on}")
+            assert judged == list(range(0, steps, every))
+            # at every judged step the integrated accelerations were within budget
+            for s in judged:
+                last = [c for c in run.checks if c["step"] == s][-1]
+                assert last["within"], last
+            assert run.judge_ms / nj < 45.0
+        finally:
+            run.close()
+
+        # a network whose output is scaled wrong: the first check switches it off
+        bad = cr.Corrector(cr.scale_output(net.params(), -25.0))
+        run2 = cr.JudgedRun(sim, bad, theta, dt, every=every, sample_seed=4243)
+        try:
+            run2.run(2)
+            c = run2.checks
+            _say(f"forced fallback: {[(x['walk'], round(x['median'], 6)) for x in c]}")
+            assert len(c) == 2 and c[0]["walk"] == "learned" and not c[0]["within"]
+            assert c[1]["walk"] == "quad" and c[1]["within"] and not run2.corrector_on
+        finally:
+            run2.close()
+            bad.close()
+            net.close()
