@@ -134,6 +134,7 @@ _PROTOS = {
                                C.POINTER(WalkStats)]),
     "gd_integrate_hilo": (None, [vp, vp, vp, vp, i32, f32]),
     "gd_direct_sum_fp64": (None, [vp, vp, i32, vp, i32, f64, f64, vp]),
+    "gd_direct_sum_fp64_noalloc": (None, [vp, vp, i32, vp, i32, f64, f64, vp]),
     "gd_force_errors": (None, [vp, vp, i32, vp, vp]),
     "gd_diagnostics": (None, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
     "gd_stream_copy": (None, [vp, vp, C.c_size_t]),

@@ -230,6 +230,54 @@ extern "C" void gd_direct_sum_fp64(const float4* pos, const float3* posLo, int s
     gd_launch_judge(pos, posLo, starCount, sample, ns, eps2, G, acc, g_judgePart, chunks);
 }
 
+/* ---------------------------------------------------------------------------
+ * The judge without allocation: the same kernels, with the partial sums in a
+ * file-scope device array instead of library-allocated memory.  Each GPU
+ * holds its own copy of the array until the process ends; its address is
+ * looked up once per device (hipGetSymbolAddress) and only that address is
+ * handed to kernels.  Every partial a call reads is written by the same call,
+ * so nothing needs clearing; calls that overlap on one GPU share the array
+ * and race.
+ * ------------------------------------------------------------------------- */
+#define GD_JUDGE_STATIC_PARTS (1 << 19)   /* double3 partials: 12 MiB per GPU */
+#define GD_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[3 * GD_JUDGE_STATIC_PARTS];
+static double3* g_scratchPtr[GD_MAX_DEVICES] = {nullptr};
+
+extern "C" void gd_direct_sum_fp64_noalloc(const float4* pos, const float3* posLo, int starCount,
+                                           const int* sample, int ns, double eps2, double G,
+                                           double* acc) {
+    if (ns <= 0 || starCount <= 0) return;
+    if (ns > GD_JUDGE_STATIC_PARTS) {
+        GD_FAIL("gd_direct_sum_fp64_noalloc: %d samples exceed the static scratch's %d", ns,
+                GD_JUDGE_STATIC_PARTS);
+        return;
+    }
+    if (eps2 < 0.0) {
+        GD_FAIL("gd_direct_sum_fp64_noalloc: eps2 = %g is negative", eps2);
+        return;
+    }
+    int dev = 0;
+    GD_HIP(hipGetDevice(&dev));
+    if (gd_has_error()) return;
+    if (dev < 0 || dev >= GD_MAX_DEVICES) {
+        GD_FAIL("gd_direct_sum_fp64_noalloc: device %d out of range", dev);
+        return;
+    }
+    if (!g_scratchPtr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr) {
+            GD_FAIL("gd_direct_sum_fp64_noalloc: hipGetSymbolAddress failed: %s",
+                    hipGetErrorString(err));
+            return;
+        }
+        g_scratchPtr[dev] = (double3*)ptr;
+    }
+    const int chunks = gd_judge_chunks(starCount, ns, GD_JUDGE_STATIC_PARTS);
+    gd_launch_judge(pos, posLo, starCount, sample, ns, eps2, G, acc, g_scratchPtr[dev], chunks);
+}
+
 /* err[k] = |acc[sample[k]] - exact[k]| / |exact[k]| in FP64 (0 when exact is 0). */
 __global__ __launch_bounds__(256) void k_force_errors(const float3* __restrict__ acc,
                                                       const int* __restrict__ sample, int ns,

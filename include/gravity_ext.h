@@ -219,6 +219,16 @@ void gd_integrate_hilo(float4* pos, float3* posLo, float3* vel, const float3* ac
 void gd_direct_sum_fp64(const float4* pos, const float3* posLo, int starCount,
                         const int* sample, int ns, double eps2, double G, double* acc);
 
+/* gd_direct_sum_fp64 without allocating: no counterpart in the reference;
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops).  The partial sums live in a 12 MiB device array that each
+ * GPU holds until the process ends; ns may not exceed 524288 (an error
+ * otherwise).  Calls that overlap on one GPU share that array and race.
+ * Results agree with gd_direct_sum_fp64's (the same kernels; bit for bit
+ * whenever both split the sources into the same number of chunks). */
+void gd_direct_sum_fp64_noalloc(const float4* pos, const float3* posLo, int starCount,
+                                const int* sample, int ns, double eps2, double G, double* acc);
+
 /* Relative force errors against the judge (device arrays):
  * err[k] = |acc[sample[k]] - exact[3k..3k+2]| / |exact[3k..3k+2]| in FP64. */
 void gd_force_errors(const float3* acc, const int* sample, int ns, const double* exact,
