@@ -248,10 +248,10 @@ def train_on_snapshots(sim, net, advance, snapshots=4, targets=8192, subsamples=
     the seconds spent."""
     import time
     t0 = time.perf_counter()
-    sim.set_quadrupole(2)
     samples = SampleSet(sim.lib, cap)
     try:
         for snap in range(snapshots):
+            sim.set_quadrupole(2)   # advance() may have changed the moment level
             sim.build_tree_async()
             for k in range(subsamples):
                 got, _ = samples.add_from(

@@ -174,14 +174,25 @@ class Simulation:
     def potentials(self):
         return from_device(self.lib, self._phi.ptr, np.float32, (self.n,))
 
-    def set_state(self, pos_mass, vel):
-        """Overwrite the current body arrays (unsorted input is fine)."""
+    def set_state(self, pos_mass, vel, lo=None):
+        """Overwrite the current body arrays (unsorted input is fine); lo, the
+        low part of hi/lo positions in the same order, is zeroed when not given."""
         pos_mass = np.ascontiguousarray(pos_mass, dtype=np.float32)
         vel = np.ascontiguousarray(vel, dtype=np.float32)
         assert pos_mass.shape == (self.n, 4) and vel.shape == (self.n, 3)
         self.lib.gd_memcpy_h2d(self.pos.value, pos_mass.ctypes.data, pos_mass.nbytes)
         self.lib.gd_memcpy_h2d(self.vel.value, vel.ctypes.data, vel.nbytes)
+        if self.hilo:
+            lo = np.zeros((self.n, 3), np.float32) if lo is None else \
+                np.ascontiguousarray(lo, dtype=np.float32)
+            assert lo.shape == (self.n, 3)
+            self.lib.gd_memcpy_h2d(self.lo.value, lo.ctypes.data, lo.nbytes)
         check(self.lib)
+
+    def snapshot(self):
+        """Host copies (pos_mass, vel, lo or None) of the current state, for set_state."""
+        return (self.positions(), self.velocities(),
+                self.positions_lo() if self.hilo else None)
 
     def keys_sorted(self):
         return from_device(self.lib, self.state.mortonOut, np.uint64, (self.n,))
@@ -236,14 +247,16 @@ class Simulation:
         d = self.state.deepestLevel
         return list(self.state.levelOffsets), list(self.state.levelCount), d
 
-    def direct_sum_fp64(self, sample):
-        """FP64 direct-sum accelerations (ns, 3) of the bodies at sorted indices `sample`."""
+    def direct_sum_fp64(self, sample, with_lo=True):
+        """FP64 direct-sum accelerations (ns, 3) of the bodies at sorted indices
+        `sample` (positions hi + lo unless with_lo is False)."""
         sample = np.ascontiguousarray(sample, dtype=np.int32)
         ns = sample.shape[0]
         sbuf = to_device(self.lib, sample)
         obuf = DeviceBuffer(self.lib, 8 * 3 * ns)
         try:
-            self.lib.gd_direct_sum_fp64(self.pos, self.lo if self.hilo else None, self.n,
+            self.lib.gd_direct_sum_fp64(self.pos, self.lo if (self.hilo and with_lo) else None,
+                                        self.n,
                                         sbuf.ptr, ns, self.eps * self.eps, self.G, obuf.ptr)
             check(self.lib)
             return from_device(self.lib, obuf.ptr, np.float64, (ns, 3))
