@@ -5,7 +5,10 @@
  * launcher of the same name and cudaMalloc / cudaMemcpy by gd_malloc /
  * gd_memcpy_*.  No HIP header is included.
  *
- * Usage: gravity_demo [N] [steps] [theta] [eps] [dt] [frames_every] [outfile]
+ * Usage: gravity_demo [N] [steps] [theta] [eps] [dt] [frames_every] [outfile] [icfile]
+ * icfile (optional) holds the initial conditions instead of the built-in disk:
+ * int32 N, float32 (x, y, z, m) x N, float32 (vx, vy, vz) x N; N then comes
+ * from the file.
  * Writes the reference's trajectory format (basicGravity.cu:614-635,793-801):
  * int32 N, int32 frames, RGB[N], then float32[3N] per frame, bodies in
  * original order (the original index of every sorted slot is carried
@@ -56,8 +59,26 @@ static void make_disk(int n, float4* pos, float3* vel) {
     }
 }
 
+/* Reads an initial-conditions file into freshly allocated, zero-padded arrays. */
+static int read_ic_file(const char* name, int* n, float4** pos, float3** vel, int padTo) {
+    FILE* f = fopen(name, "rb");
+    if (!f) return -1;
+    int count = 0;
+    int ok = fread(&count, sizeof(int), 1, f) == 1 && count > 0;
+    if (ok) {
+        const int padded = (count + padTo - 1) / padTo * padTo;
+        *pos = (float4*)calloc((size_t)padded, sizeof(float4));
+        *vel = (float3*)calloc((size_t)padded, sizeof(float3));
+        ok = *pos && *vel && fread(*pos, sizeof(float4), (size_t)count, f) == (size_t)count &&
+             fread(*vel, sizeof(float3), (size_t)count, f) == (size_t)count;
+    }
+    fclose(f);
+    *n = count;
+    return ok ? 0 : -1;
+}
+
 int main(int argc, char** argv) {
-    const int starCount = argc > 1 ? atoi(argv[1]) : 16384;
+    int starCount = argc > 1 ? atoi(argv[1]) : 16384;
     const int tSteps = argc > 2 ? atoi(argv[2]) : 100;
     const float th = argc > 3 ? (float)atof(argv[3]) : theta;
     const float eps = argc > 4 ? (float)atof(argv[4]) : 0.01f;
@@ -65,19 +86,32 @@ int main(int argc, char** argv) {
     const int framesPerWrite = argc > 6 ? atoi(argv[6]) : 5;
     const char* outName = argc > 7 ? argv[7] : "outputDump.bin";
 
+    float4* hostPos = NULL;
+    float3* hostVel = NULL;
+    if (argc > 8 && read_ic_file(argv[8], &starCount, &hostPos, &hostVel, WARPTHREADS) != 0) {
+        printf("Could not read initial conditions from %s\n", argv[8]);
+        return -1;
+    }
+    if (starCount <= 0) {
+        printf("No bodies\n");
+        return -1;
+    }
+
     int accelerationBlocks, treeBuilderBlocks, integrateBlocks, integrationThreadsPerBlock,
         treeBuilderThreadsPerBlock, paddedStarCount;
     blockNumCalculator(&accelerationBlocks, &treeBuilderBlocks, &integrateBlocks,
                        &paddedStarCount, &integrationThreadsPerBlock,
                        &treeBuilderThreadsPerBlock, starCount);
 
-    float4* hostPos = (float4*)calloc((size_t)paddedStarCount, sizeof(float4));
-    float3* hostVel = (float3*)calloc((size_t)paddedStarCount, sizeof(float3));
+    if (!hostPos) {
+        hostPos = (float4*)calloc((size_t)paddedStarCount, sizeof(float4));
+        hostVel = (float3*)calloc((size_t)paddedStarCount, sizeof(float3));
+        make_disk(starCount, hostPos, hostVel);
+    }
     uint32_t* ident = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)starCount);
     uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)starCount);
     uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)starCount);
     float* frame = (float*)malloc(sizeof(float) * 3 * (size_t)starCount);
-    make_disk(starCount, hostPos, hostVel);
     for (int i = 0; i < starCount; ++i) ident[i] = (uint32_t)i;
 
     float4* pos = (float4*)gd_malloc(sizeof(float4) * (size_t)paddedStarCount);

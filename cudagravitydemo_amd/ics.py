@@ -157,3 +157,95 @@ def subsample(n, k=4096, seed=99):
     rng = np.random.Generator(np.random.PCG64(seed))
     k = min(k, n)
     return np.sort(rng.choice(n, size=k, replace=False)).astype(np.int32)
+
+
+def rotation_matrix(yaw, pitch, roll):
+    """The reference's disk rotation (initialConditions.c:199-232).  Its three
+    matrices are written as cglm literals, which cglm stores column by column,
+    so each literal is the transpose of how it reads; the product
+    yaw * pitch * roll (glm_mat3_mul) is then Rz(-yaw) Ry(-pitch) Rx(-roll)
+    with Rz, Ry, Rx the usual right-handed rotations."""
+    cy, sy = np.cos(yaw), np.sin(yaw)
+    cp, sp = np.cos(pitch), np.sin(pitch)
+    cr, sr = np.cos(roll), np.sin(roll)
+    rz = np.array([[cy, sy, 0.0], [-sy, cy, 0.0], [0.0, 0.0, 1.0]])
+    ry = np.array([[cp, 0.0, -sp], [0.0, 1.0, 0.0], [sp, 0.0, cp]])
+    rx = np.array([[1.0, 0.0, 0.0], [0.0, cr, sr], [0.0, -sr, cr]])
+    return rz @ ry @ rx
+
+
+def reference_scenario(n=100000, seed=0, scale_length=4.0, scale_thickness=0.3, total_mass=5.0,
+                       halo_fraction=0.2, angles=(0.0, 0.0, 0.0), centre=(0.0, 0.0, 0.0),
+                       velocity=(0.0, 0.0, 0.0), G=G_UNIT):
+    """The reference's own run (basicGravity.cu:517-545, 694-697): N bodies of
+    equal mass total_mass / N; the first N - int(0.2 N) form the disk of
+    generateLocalCoordsDisk (initialConditions.c:246-302), the rest the
+    'halo' of generateDarkMatterHalo (initialConditions.c:314-382).
+
+    Disk: radius by rejection sampling of R exp(-R/h) under its maximum h/e
+    on [0, 5h] (sampleRadius, :44-69); uniform azimuth; height from the
+    Laplace profile exp(-|z|/z0) by inversion (sampleHeight, :150-162);
+    velocity the circular speed sqrt(G M(<R) / R) of an exponential disk of
+    total_mass, M(<R) = M (1 - (1 + R/h) exp(-R/h)) (:164-176), tangential,
+    no vertical motion; then rotated by rotation_matrix(*angles) and offset by
+    centre / velocity.  Halo: radius uniform in [0, 5h], direction uniform on
+    the sphere (cos theta uniform), at rest relative to `velocity` (:314-361).
+    The reference draws from rand() seeded with the time; this draws from a
+    seeded PCG64 instead."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    halo = int(n * halo_fraction)
+    nd = n - halo
+    h = scale_length
+    R = np.empty(nd)
+    todo = np.arange(nd)
+    ymax = h * np.exp(-1.0)
+    while todo.size:
+        r = 5.0 * h * rng.random(todo.size)
+        y = ymax * rng.random(todo.size)
+        ok = y < r * np.exp(-r / h)
+        R[todo[ok]] = r[ok]
+        todo = todo[~ok]
+    phi = 2.0 * np.pi * rng.random(nd)
+    u = rng.random(nd)
+    u = np.clip(u, 1e-300, 1.0 - 1e-16)
+    z = np.where(u >= 0.5, -np.log(2.0 - 2.0 * u), np.log(2.0 * u)) * scale_thickness
+    menc = total_mass * (1.0 - (1.0 + R / h) * np.exp(-R / h))
+    vc = np.sqrt(G * menc / np.maximum(R, 1e-30))
+    pd = np.stack([R * np.cos(phi), R * np.sin(phi), z], 1)
+    vd = np.stack([-vc * np.sin(phi), vc * np.cos(phi), np.zeros(nd)], 1)
+    rot = rotation_matrix(*angles)
+    pd = pd @ rot.T + np.asarray(centre, float)
+    vd = vd @ rot.T + np.asarray(velocity, float)
+    cth = 2.0 * rng.random(halo) - 1.0
+    ph = 2.0 * np.pi * rng.random(halo)
+    rh = 5.0 * h * rng.random(halo)
+    sth = np.sqrt(1.0 - cth * cth)
+    phalo = np.stack([rh * sth * np.cos(ph), rh * sth * np.sin(ph), rh * cth], 1) + \
+        np.asarray(centre, float)
+    vhalo = np.tile(np.asarray(velocity, float), (halo, 1))
+    pos = np.concatenate([pd, phalo])
+    vel = np.concatenate([vd, vhalo])
+    return _pack(pos, vel, np.full(n, total_mass / n))
+
+
+def write_ic_file(path, pos_mass, vel):
+    """Initial conditions for the C driver: int32 N, float32 (x, y, z, m) x N,
+    float32 (vx, vy, vz) x N, little-endian."""
+    pos_mass = np.ascontiguousarray(pos_mass, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    with open(path, "wb") as f:
+        f.write(np.int32(pos_mass.shape[0]).tobytes())
+        f.write(pos_mass.tobytes())
+        f.write(vel.tobytes())
+
+
+def read_trajectory(path):
+    """The reference's output file (basicGravity.cu:614-635, 793-801): int32 N,
+    int32 frames, RGB[N] (3 bytes each), then float32[3 N] per frame.
+    Returns (N, colours (N, 3) uint8, frames (F, N, 3) float32)."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    n, frames = np.frombuffer(raw[:8].tobytes(), np.int32)
+    colours = raw[8:8 + 3 * n].reshape(n, 3)
+    body = raw[8 + 3 * n:]
+    assert body.size == 12 * n * frames, (body.size, n, frames)
+    return int(n), colours, np.frombuffer(body.tobytes(), np.float32).reshape(frames, n, 3)
