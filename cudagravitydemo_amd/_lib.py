@@ -117,6 +117,8 @@ _PROTOS = {
     "gd_walk_timing": (None, [i32]),
     "gd_walk_set_list_capacity": (i32, [i32, i32]),
     "gd_walk_set_group_split": (f32, [f32]),
+    "gd_walk_set_async": (i32, [i32]),
+    "gd_walk_poll": (i32, []),
     "gd_walk_kernel_ms": (f32, []),
     "gd_walk_phase_ms": (i32, [vp]),
     "gd_mlp_create": (vp, [vp]),

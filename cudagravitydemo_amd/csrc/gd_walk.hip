@@ -173,6 +173,22 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
 __device__ __forceinline__ unsigned long long* gd_stat_slot(unsigned long long* stats) {
     return stats + (blockIdx.x & (GD_STAT_SLOTS - 1)) * GD_STAT_STRIDE;
 }
+/* A launch's slot count: `slots`, or at most *slotsDev when the count is
+ * held on the device (the walk's asynchronous mode sizes grids for a fixed
+ * capacity and reads how many slots are in use there). */
+__device__ __forceinline__ long long gd_slots(int slots, const int* slotsDev) {
+    if (!slotsDev) return slots;
+    const int d = *slotsDev;
+    return d < slots ? d : slots;
+}
+/* Segments per second-pass list when `groups` lists are evaluated at once:
+ * enough wavefronts to fill GD_SEG_WAVES, at most 64 each. */
+__host__ __device__ __forceinline__ int gd_seg_count(int groups) {
+    if (groups <= 0) return 1;
+    if (groups > 2048 / 2) return 1;
+    const int s = 2048 / groups;
+    return s < 64 ? s : 64;
+}
 #define GD_LIST_MIN 4096          /* first-pass room of any group (entries) */
 #define GD_PASS2_FLOOR 16384      /* the second pass never gives up on a shorter list */
 #define GD_SPREAD (-2)            /* sizes[] mark: the group is walked body by body */
@@ -201,13 +217,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     int* __restrict__ lists, int2* __restrict__ counts, int* __restrict__ sizes,
     int* __restrict__ overflowList, int* __restrict__ overflowCount,
     int* __restrict__ spreadList, int* __restrict__ spreadCount, int* __restrict__ deepCount,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, const int* __restrict__ slotsDev = nullptr) {
     __shared__ int sStack[WAVES][STACK + GD_POP];
     if (info->overflow) return;   /* the build ran out of nodes: no tree to walk (the host
                                      reports it from the walk's status) */
     const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
     const long long s = (long long)gd_xcd_block() * WAVES + wave;
-    if (s >= slots) return;   /* wave-uniform */
+    if (s >= gd_slots(slots, slotsDev)) return;   /* wave-uniform */
     const long long g = groupList ? groupList[s] : s;
     const int2 rg = slotRange[g];   /* (first body, bodies) */
     const long long cs = offsets ? g : s;   /* counts by slot id in the first pass */
@@ -399,11 +415,11 @@ __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList, int slots, int cap2,
-    int floor, int* __restrict__ slotCap) {
+    int floor, int* __restrict__ slotCap, const int* __restrict__ slotsDev = nullptr) {
     __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK + GD_POP];
     const int wave = (int)(threadIdx.x >> 6);
     const long long s = (long long)blockIdx.x * GD_TR_WAVES + wave;
-    if (s >= slots) return;   /* wave-uniform */
+    if (s >= gd_slots(slots, slotsDev)) return;   /* wave-uniform */
     const int2 rg = slotRange[groupList[s]];
     const float4 p = pos[rg.x + (rg.y - 1) / 2];
     int nf = 0, nb = 0, depth = 0;
@@ -809,7 +825,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
     const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
     float4* __restrict__ part, float eps2,
     const gd_bf16x4* __restrict__ mlpW, const float* __restrict__ mlpB, float minRatio,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, const int* __restrict__ slotsDev = nullptr) {
     __shared__ float4 sE[GD_EV_WAVES][2 * 64];
     __shared__ float4 sQ0[MODE != 0 ? GD_EV_WAVES : 1][64];
     __shared__ float4 sQ1[MODE != 0 ? GD_EV_WAVES : 1][64];
@@ -827,7 +843,9 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
     const unsigned lane = gd_lane();
     const int wave = (int)(threadIdx.x >> 6);
     const long long q = (long long)gd_xcd_block() * GD_EV_WAVES + wave;
-    if (q >= (long long)slots * segs) return;
+    const long long used = gd_slots(slots, slotsDev);
+    if (segs <= 0) segs = gd_seg_count((int)used);   /* from the device-held count */
+    if (q >= used * segs) return;
     const long long s = q / segs;
     const int seg = (int)(q - s * segs);
     const int2 cnt = counts[s];
@@ -862,10 +880,14 @@ __global__ __launch_bounds__(256) void k_walk_eval_sum(const int2* __restrict__ 
                                                        const int2* __restrict__ counts,
                                                        const float4* __restrict__ part,
                                                        float3* __restrict__ acc,
-                                                       float* __restrict__ phi, float G) {
+                                                       float* __restrict__ phi, float G,
+                                                       const int* __restrict__ slotsDev =
+                                                           nullptr) {
     const unsigned lane = gd_lane();
     const long long s = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (s >= slots) return;
+    const long long used = gd_slots(slots, slotsDev);
+    if (segs <= 0) segs = gd_seg_count((int)used);
+    if (s >= used) return;
     if (counts[s].x < 0) return;
     const int2 rg = slotRange[groupList[s]];
     if ((int)lane >= rg.y) return;
@@ -1183,33 +1205,18 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
  * that cannot happen, since a step pops up to 64 nodes and pushes at most 8
  * children each, and the pushed segments along the stack have strictly
  * increasing minimum level, so the stack never exceeds 22 x 512 entries.
+ * gd_walk_one_body walks body i with one wavefront; k_walk_body hands
+ * slots to wavefronts.
  */
-template <int MODE, bool PHI, int WAVES, int STACK>
-__global__ __launch_bounds__(64 * WAVES) void k_walk_body(
+template <int MODE, bool PHI, int STACK>
+__device__ __forceinline__ void gd_walk_one_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const gd_tree_info* __restrict__ info,
-    const int2* __restrict__ slotRange, const int* __restrict__ groupList,
-    const int* __restrict__ bodyList, long long count,
+    const gd_tree_info* __restrict__ info, long long i, int* sStack,
     int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
-    int n, unsigned long long* __restrict__ stats) {
-    static_assert(MODE == 0 || MODE == 1, "per-body walk: monopole or quadrupole");
-    __shared__ int sStackAll[WAVES][STACK];
+    unsigned long long* __restrict__ stats) {
     const unsigned lane = gd_lane();
-    const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
-    int* sStack = sStackAll[wave];
-    const long long q = (long long)gd_xcd_block() * WAVES + wave;
-    if (q >= count) return;   /* wave-uniform */
-    long long i;
-    if (bodyList) {
-        i = bodyList[q];
-    } else {
-        const int2 rg = slotRange[groupList[q >> 6]];
-        if ((int)(q & 63) >= rg.y) return;   /* wave-uniform */
-        i = rg.x + (q & 63);
-    }
-    if (i >= n) return;   /* wave-uniform */
     const float4 p = pos[i];
     const float ext = info->box.extent;
     const float th2 = theta * theta;
@@ -1319,6 +1326,48 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     }
 }
 
+/* The per-body walk over `count` slots (or countScale * *countDev of them,
+ * grid-stride, when the count is held on the device). */
+template <int MODE, bool PHI, int WAVES, int STACK>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_body(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ slotRange, const int* __restrict__ groupList,
+    const int* __restrict__ bodyList, long long count,
+    int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
+    float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
+    int n, unsigned long long* __restrict__ stats, const int* __restrict__ countDev = nullptr,
+    int countScale = 1) {
+    static_assert(MODE == 0 || MODE == 1, "per-body walk: monopole or quadrupole");
+    __shared__ int sStackAll[WAVES][STACK];
+    const int wave = WAVES > 1 ? (int)(threadIdx.x >> 6) : 0;
+    int* sStack = sStackAll[wave];
+    /* with countDev the count is countScale * *countDev (at most `count`); the
+       grid is then fixed and each wavefront takes every stride-th body */
+    long long todo = count;
+    if (countDev) {
+        const long long d = (long long)countScale * *countDev;
+        todo = d < count ? d : count;
+    }
+    const long long stride = (long long)gridDim.x * WAVES;
+    for (long long q = (long long)gd_xcd_block() * WAVES + wave; q < todo; q += stride) {
+        long long i;
+        if (bodyList) {
+            i = bodyList[q];
+        } else {
+            const int2 rg = slotRange[groupList[q >> 6]];
+            if ((int)(q & 63) >= rg.y) continue;   /* wave-uniform */
+            i = rg.x + (q & 63);
+        }
+        if (i >= n) continue;   /* wave-uniform */
+        gd_walk_one_body<MODE, PHI, STACK>(pos, wcom, wlink, wquad, info, i, sStack,
+                                           overflowBodies, overflowCount, acc, phi, eps2, G,
+                                           theta, stats);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 /*
  * Walk scratch.  counters[0..4]: interactions, opened, stack-overflow flag,
  * cell interactions, network evaluations (64-bit); status: overflow counts
@@ -1362,16 +1411,29 @@ struct gd_walk_scratch {
     int deepStack;         /* first pass with 2 x GD_TR_STACK stack entries */
     unsigned long long* kstats;   /* the list passes' statistics, or NULL (not asked for) */
     float4* segPart;       /* GD_SEG_WAVES x 64: per-body sums of second-pass segments */
+    int async;             /* 1: no host synchronisation inside a walk (gd_walk_set_async) */
+    gd_walk_status* hstatus;   /* pinned copy of the last asynchronous walk's status */
+    hipEvent_t statusEv;   /* recorded after that copy */
+    int statusPending;     /* hstatus not read yet */
 };
 
 #define GD_LIST2 262144
 #define GD_SEG_WAVES 2048         /* second pass: groups x segments per launch */
+static_assert(GD_SEG_WAVES == 2048, "gd_seg_count assumes 2048 segment wavefronts");
 #define GD_POOL_BYTES (8ll << 30)
+#define GD_ASYNC_PASS2 256        /* asynchronous mode: second-pass groups per walk (the rest
+                                     take the per-body walk) */
+#define GD_ASYNC_BODY_BLOCKS 1024 /* asynchronous mode: per-body walk grid (grid-stride) */
+#define GD_ASYNC_DEEP_BLOCKS 256  /* asynchronous mode: deep-stack per-body rerun grid */
+static_assert(GD_ASYNC_PASS2 <= GD_SEG_WAVES / 2, "every asynchronous second-pass list is cut");
 static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                                  nullptr, 0,
                                  nullptr, nullptr, 0,       nullptr, 0,       0,
-                                 GD_LIST2, 0,       0,       nullptr};
+                                 GD_LIST2, 0,       0,       nullptr, nullptr, 0,
+                                 nullptr, nullptr, 0};
+
+static int g_lastRuns = 0;   /* 64-body runs of the last asynchronous walk */
 
 /* optional HIP-event timing of the walk's phases (null stream) */
 static int g_walkTiming = 0;
@@ -1427,6 +1489,8 @@ static void gd_walk_ensure(int groups) {
 static void gd_walk_grow_lists(long long want) {
     if (want <= g_walk.listEntries) return;
     const long long e = want + want / 4;
+    /* an earlier asynchronous walk may still read the pool */
+    if (g_walk.lists) GD_HIP(hipDeviceSynchronize());
     if (g_walk.lists) GD_HIP(hipFree(g_walk.lists));
     GD_HIP(hipMalloc(&g_walk.lists, (size_t)e * sizeof(int)));
     g_walk.listEntries = e;
@@ -1464,6 +1528,15 @@ __global__ __launch_bounds__(256) void k_walk_split(const uint64_t* __restrict__
         slotRange[2 * g] = make_int2(start, j ? j : count);
         slotRange[2 * g + 1] = make_int2(start + (j ? j : count), j ? count - j : 0);
     }
+}
+
+/* Asynchronous mode: first-pass overflow groups beyond the second pass's
+ * capacity `cap` go to the per-body walk's list. */
+__global__ void k_walk_route(const int* __restrict__ over1List, const int* __restrict__ over1,
+                             int cap, int* __restrict__ over2List, int* __restrict__ over2) {
+    const int n1 = *over1;
+    for (int i = cap + blockIdx.x * 256 + threadIdx.x; i < n1; i += gridDim.x * 256)
+        over2List[atomicAdd(over2, 1)] = over1List[i];
 }
 
 __global__ void k_walk_reset(unsigned long long* stats, gd_walk_status* st,
@@ -1541,6 +1614,61 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     }
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
+    constexpr int BMODE0 = MODE == 2 ? 1 : MODE;   /* per-body walk: see pass 3 below */
+    if (g_walk.async) {
+        /* the same passes with every decision on the device: the second pass
+           has fixed grids for GD_ASYNC_PASS2 groups and reads how many it
+           has; groups beyond them, and those it gives up on, take the
+           per-body walk, a grid-stride loop over a device-held count */
+        const int cap2 = g_walk.entries2;
+        const int S2 = GD_ASYNC_PASS2;
+        gd_walk_ensure_pool(S2, cap2);
+        if (gd_has_error()) return;
+        k_walk_route<<<64, 256>>>(g_walk.overflow1, &st->over1, S2, g_walk.overflow2,
+                                  &st->over2);
+        GD_LAUNCHED();
+        k_walk_estimate<SPLIT><<<gd_div_up(S2, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
+            cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap,
+            &st->over1);
+        GD_LAUNCHED();
+        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<S2, 64>>>(
+            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
+            nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
+            g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats, &st->over1);
+        GD_LAUNCHED();
+        k_walk_eval_seg<MODE, PHI><<<GD_SEG_WAVES / GD_EV_WAVES, 64 * GD_EV_WAVES>>>(
+            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
+            g_walk.overflow1, S2, 0, g_walk.pool, g_walk.counts2, cap2, g_walk.segPart, eps2, W,
+            Bv, minRatio, g_walk.kstats, &st->over1);
+        GD_LAUNCHED();
+        k_walk_eval_sum<PHI><<<gd_div_up(S2, 4), 256>>>(g_walk.slotRange, g_walk.overflow1, S2,
+                                                         0, g_walk.counts2, g_walk.segPart, acc,
+                                                         phi, G, &st->over1);
+        GD_LAUNCHED();
+        const long long bodies = 64ll * groups;
+        k_walk_body<BMODE0, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
+            <<<GD_ASYNC_BODY_BLOCKS, 64 * GD_BODY_WAVES>>>(
+                pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, g_walk.overflow2,
+                nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
+                g_walk.stats, &st->over2, 64);
+        GD_LAUNCHED();
+        k_walk_body<BMODE0, PHI, 1, GD_BODY_STACK><<<GD_ASYNC_DEEP_BLOCKS, 64>>>(
+            pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, nullptr,
+            g_walk.bodyOver, bodies, nullptr, nullptr, acc, phi, eps2, G, theta, n,
+            g_walk.stats, &st->over3, 1);
+        GD_LAUNCHED();
+        /* the status goes back without waiting: the next walk (or
+           gd_walk_poll) reads it */
+        GD_HIP(hipMemcpyAsync(g_walk.hstatus, st, sizeof(gd_walk_status),
+                              hipMemcpyDeviceToHost, nullptr));
+        GD_HIP(hipEventRecord(g_walk.statusEv, nullptr));
+        g_walk.statusPending = 1;
+        g_lastRuns = runs;
+        overOut[0] = overOut[2] = overOut[3] = -1;   /* not known on the host */
+        *requested = 0;
+        return;
+    }
     gd_walk_status h;
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
     if (h.treeOverflow) {
@@ -1618,6 +1746,30 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     GD_LAUNCHED();
 }
 
+/* The status of the last asynchronous walk, once its copy has arrived
+ * (wait = 1: wait for it): grows the first-pass pool to what that walk asked
+ * for, picks the traversal stack for the next walk and reports a tree that
+ * did not fit its node array.  Returns -1 after reporting an error. */
+static int gd_walk_take_status(int wait) {
+    if (!g_walk.statusPending) return 0;
+    if (wait) {
+        GD_HIP(hipEventSynchronize(g_walk.statusEv));
+    } else {
+        const hipError_t q = hipEventQuery(g_walk.statusEv);
+        if (q == hipErrorNotReady) return 0;
+        GD_HIP(q);
+    }
+    g_walk.statusPending = 0;
+    const gd_walk_status h = *g_walk.hstatus;
+    if (h.treeOverflow) {
+        GD_FAIL("Node array overflow: the tree of an earlier walk did not fit its nodes");
+        return -1;
+    }
+    g_walk.deepStack = 16ll * h.deep1 > (long long)g_lastRuns;
+    gd_walk_grow_lists((long long)h.requested);
+    return gd_has_error() ? -1 : 0;
+}
+
 static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, float* phi,
                          float eps2, float G, float theta, int starCount, gd_walk_stats* stats,
                          int mode, const gd_mlp* net, float minRatio) {
@@ -1637,6 +1789,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         return;
     }
     const int groups = 2 * gd_div_up(starCount, 64);   /* group slots */
+    if (gd_walk_take_status(0) != 0) return;   /* a finished asynchronous walk's report */
     gd_walk_ensure(groups);
     if (g_walk.fixedCap > 0) gd_walk_grow_lists((long long)groups * g_walk.fixedCap);
     if (gd_has_error()) return;
@@ -1674,7 +1827,13 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     /* room for what this walk asked for, for the next walk (groups that did
        not get theirs went to the second pass) */
     gd_walk_grow_lists((long long)requested);
-    if (over[0] > 0 || stats) {
+    if (g_walk.async && stats) {
+        /* statistics synchronise anyway: take this walk's status now */
+        if (gd_walk_take_status(1) != 0) return;
+        over[0] = g_walk.hstatus->over1 + g_walk.hstatus->over2;
+        over[2] = g_walk.hstatus->over2;
+    }
+    if ((!g_walk.async && over[0] > 0) || stats) {
         unsigned long long raw[GD_STAT_WORDS], h[5] = {0, 0, 0, 0, 0};
         GD_HIP(hipMemcpy(raw, g_walk.stats, sizeof(raw), hipMemcpyDeviceToHost));
         for (int sl = 0; sl < GD_STAT_SLOTS; ++sl)
@@ -1706,6 +1865,21 @@ extern "C" void gd_walk_learned(const treeState* tree, const float4* pos, float3
     }
     gd_walk_impl(tree, pos, acc, phi, eps2, G, theta, starCount, stats, 2, net, minRatio);
 }
+
+extern "C" int gd_walk_set_async(int enable) {
+    const int prev = g_walk.async;
+    if (enable && !g_walk.hstatus) {
+        GD_HIP(hipHostMalloc((void**)&g_walk.hstatus, sizeof(gd_walk_status),
+                             hipHostMallocDefault));
+        GD_HIP(hipEventCreateWithFlags(&g_walk.statusEv, hipEventDisableTiming));
+        if (gd_has_error()) return prev;
+    }
+    if (!enable && gd_walk_take_status(1) != 0) return prev;
+    g_walk.async = enable ? 1 : 0;
+    return prev;
+}
+
+extern "C" int gd_walk_poll(void) { return gd_walk_take_status(1); }
 
 extern "C" int gd_walk_set_list_capacity(int entries, int secondPass) {
     const int prev = g_walk.fixedCap;

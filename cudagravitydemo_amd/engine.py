@@ -152,6 +152,20 @@ class Simulation:
         self.walk(theta)
         self.integrate(dt)
 
+    def set_walk_async(self, enable):
+        """Walks without host synchronisation (gd_walk_set_async); with it on,
+        step() queues its kernels and returns at once.  Returns the previous
+        setting.  The setting belongs to the library, not to this state."""
+        prev = self.lib.gd_walk_set_async(1 if enable else 0)
+        check(self.lib)
+        return bool(prev)
+
+    def poll(self):
+        """Waits for the last asynchronous walk's status; raises GravityError
+        for a deferred error (a tree that did not fit its node array)."""
+        self.lib.gd_walk_poll()
+        check(self.lib)
+
     # ---- inspection -------------------------------------------------------
     def sync(self):
         self.lib.gd_synchronize()

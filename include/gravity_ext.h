@@ -127,6 +127,22 @@ void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi, 
  * Returns the previous first-pass setting. */
 int gd_walk_set_list_capacity(int entries, int secondPass);
 
+/* Asynchronous walks (enable = 1): gd_walk and gd_walk_learned then make
+ * no host synchronisation unless statistics are asked for -- the second
+ * pass runs on fixed grids for up to 256 groups whose count the kernels read
+ * from device memory (groups beyond them take the per-body walk), the
+ * per-body walk loops over a device-held count, and the walk's status comes
+ * back to pinned host memory without waiting.  The next walk (or
+ * gd_walk_poll) reads that status when it has arrived: it sizes the list
+ * pool and reports a tree that did not fit its node array ("Node array
+ * overflow"), one walk late.  Accelerations are those of the synchronous walk
+ * whenever at most 256 groups overflow the first pass.  Returns the previous
+ * setting; switching off waits for the last status. */
+int gd_walk_set_async(int enable);
+/* Waits for the last asynchronous walk's status and applies it; returns 0,
+ * or -1 after reporting an error (a tree that did not fit). */
+int gd_walk_poll(void);
+
 /* Groups of the list walks.  Each run of 64 bodies in Morton order is split
  * in two at its largest key jump; each half has its own interaction list
  * (seen through its own, smaller bounding box) and the two lists are

@@ -439,3 +439,141 @@ def test_walks_refuse_an_overflowed_tree():
         exact = sim.direct_sum_fp64(sample)
         err = np.linalg.norm(acc[sample] - exact, axis=1) / np.linalg.norm(exact, axis=1)
         assert np.median(err) < 1e-2
+
+
+# ---- asynchronous walks: no host synchronisation inside a step -------------
+
+def _async(on):
+    lib = load_library()
+    lib.gd_walk_set_async(1 if on else 0)
+    from cudagravitydemo_amd._lib import check
+    check(lib)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_async_walk_equals_sync_walk(mode):
+    """The five longest lists in the second pass: the asynchronous walk (second
+    pass on fixed grids reading its group count from the device) gives the
+    synchronous walk's accelerations bit for bit, with and without counters."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        sizes = _group_sizes(sim, mode)
+        cap = int(np.sort(sizes)[-6])
+        st = _run(sim, mode, cap, params)
+        want = sim.accelerations()
+        assert 1 <= st["overflow_groups"] <= 5
+        try:
+            _async(True)
+            st2 = _run(sim, mode, cap, params)
+            assert np.array_equal(sim.accelerations(), want)
+            assert st2["overflow_groups"] == st["overflow_groups"]
+            assert st2["interactions"] == st["interactions"]
+            lib = load_library()
+            lib.gd_walk_set_list_capacity(cap, 0)
+            sim.lib.gd_memset(sim.acc, 0xFF, 12 * sim.n)          # NaN everywhere
+            if mode == 2:
+                net = cr.Corrector(params)
+                cr.walk_learned(sim, net, THETA, MIN_RATIO)
+                net.close()
+            else:
+                sim.walk(THETA)
+            sim.poll()
+            assert np.array_equal(sim.accelerations(), want)
+        finally:
+            load_library().gd_walk_set_list_capacity(0, 0)
+            _async(False)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_async_walk_with_more_overflows_than_second_pass_slots(mode):
+    """A first-pass capacity at the 25th percentile of the lists: ~380 of the
+    512 groups leave the first pass, more than the asynchronous second pass
+    holds (256); the rest are routed to the per-body walk on the device.
+    Every body is written and the errors against the FP64 direct sum stay
+    those of the synchronous walk."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    sample = ics.subsample(N, 512, seed=5)
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        exact = sim.direct_sum_fp64(sample)
+        sizes = _group_sizes(sim, mode)
+        cap = int(np.percentile(sizes[sizes > 0], 25))
+        lib = load_library()
+        errs = {}
+        try:
+            for on in (False, True):
+                _async(on)
+                lib.gd_walk_set_list_capacity(cap, 0)
+                sim.lib.gd_memset(sim.acc, 0xFF, 12 * sim.n)
+                st = sim.walk(THETA, stats=True)
+                acc = sim.accelerations()
+                assert np.isfinite(acc).all(), on
+                assert st["overflow_groups"] > 256, st
+                e = np.linalg.norm(acc[sample] - exact, axis=1) / np.linalg.norm(exact, axis=1)
+                errs[on] = (float(np.median(e)), float(e.max()))
+        finally:
+            lib.gd_walk_set_list_capacity(0, 0)
+            _async(False)
+        print(f"mode {mode}: median / max error sync {errs[False]}, async {errs[True]}")
+        assert errs[True][0] <= 1.5 * errs[False][0] + 1e-6
+        assert errs[True][1] <= 2.0 * errs[False][1] + 1e-6
+
+
+def test_async_walk_reports_overflow_at_poll():
+    from cudagravitydemo_amd import GravityError
+    from tests.test_oracle import overflow_fixture
+    pos = overflow_fixture()
+    with Simulation(pos, np.zeros((pos.shape[0], 3), np.float32), eps=EPS) as sim:
+        try:
+            _async(True)
+            sim.build_tree_async()
+            sim.walk(THETA)              # queued; its kernels see the flag and stop
+            with pytest.raises(GravityError, match="Node array overflow"):
+                sim.poll()
+        finally:
+            _async(False)
+
+
+@pytest.mark.parametrize("per_disk", [1 << 13, 1 << 20])
+def test_async_engine_step_does_not_wait(per_disk):
+    """The engine step (build, walk, kick-drift) with asynchronous walks makes
+    no host synchronisation: queueing 10 steps takes a fraction of their
+    device time, where the synchronous walk waits for each step.  Prints the
+    ms per step of both (the 16384-body disk and the 2^21-body merger sizes)."""
+    pos, vel = ics.disk_merger(per_disk, seed=2024)
+    lib = load_library()
+    with Simulation(pos, vel, eps=0.005) as sim:
+        sim.set_quadrupole(1)
+        e0, e1 = lib.gd_event_create(), lib.gd_event_create()
+        res = {}
+        try:
+            for on in (False, True):
+                _async(on)
+                for _ in range(3):
+                    sim.step(5e-4, 0.5)
+                sim.sync()
+                sim.poll()
+                import time
+                lib.gd_event_record(e0)
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    sim.step(5e-4, 0.5)
+                host = time.perf_counter() - t0
+                lib.gd_event_record(e1)
+                dev = lib.gd_event_elapsed_ms(e0, e1) / 1e3
+                sim.poll()
+                res[on] = (host, dev)
+        finally:
+            _async(False)
+            lib.gd_event_destroy(e0)
+            lib.gd_event_destroy(e1)
+        n = 2 * per_disk
+        print(f"N={n}: sync {1e3 * res[False][1] / 10:.3f} ms/step (host {1e3 * res[False][0] / 10:.3f}); "
+              f"async {1e3 * res[True][1] / 10:.3f} ms/step (host queues a step in "
+              f"{1e3 * res[True][0] / 10:.3f} ms)", flush=True)
+        if per_disk >= 1 << 20:
+            assert res[True][0] < 0.5 * res[True][1]
