@@ -140,6 +140,10 @@ _PROTOS = {
     "gd_force_errors": (None, [vp, vp, i32, vp, vp]),
     "gd_diagnostics": (None, [vp, vp, vp, i32, C.POINTER(C.c_double)]),
     "gd_stream_copy": (None, [vp, vp, C.c_size_t]),
+    "gd_graph_begin": (None, []),
+    "gd_graph_end": (vp, []),
+    "gd_graph_launch": (None, [vp]),
+    "gd_graph_destroy": (None, [vp]),
     "gd_event_create": (vp, []),
     "gd_event_record": (None, [vp]),
     "gd_event_elapsed_ms": (f32, [vp, vp]),

@@ -11,7 +11,9 @@
  *   buildTree                                         basicGravity.cu:116-325
  *   roundNumberToMultiple / blockNumCalculator        basicGravity.cu:351-427
  *   G, theta                                          units.c:3-4
- * Every launch goes to the null stream, as in the reference.
+ * Every launch goes to one library stream (gd_stream), which synchronises
+ * with the null stream the reference used, so synchronous copies order after
+ * it as in the reference.
  */
 #include "gd_internal.h"
 #include "gravity_host.h"
@@ -47,6 +49,23 @@ void gd_fail(const char* file, int line, const char* fmt, ...) {
 }
 
 int gd_has_error(void) { return g_lastError[0] != '\0'; }
+
+/* ---------------------------------------------------------------------------
+ * The library's stream: every kernel, asynchronous copy and event of the
+ * library goes to this one stream, created on first use with the default
+ * flags (it synchronises with the null stream, so the synchronous copies of
+ * gd_memcpy_* and hipMemcpy order after it) and the default priority.
+ * ------------------------------------------------------------------------- */
+static hipStream_t g_stream = nullptr;
+static int g_capturing = 0;      /* between gd_graph_begin and gd_graph_end */
+static int g_liveGraphs = 0;     /* instantiated graphs not destroyed yet */
+
+hipStream_t gd_stream(void) {
+    if (!g_stream) GD_HIP(hipStreamCreate(&g_stream));
+    return g_stream;
+}
+int gd_capturing(void) { return g_capturing; }
+int gd_graphs_hold_buffers(void) { return g_capturing || g_liveGraphs > 0; }
 
 extern "C" {
 
@@ -116,7 +135,8 @@ radixSorter* sorterCreate(int maxCount) {
     GD_HIP(hipMalloc(&s->tileCounter, sizeof(uint32_t) * GD_SORT_PASSES));
     GD_HIP(hipMalloc(&s->status, statusBytes));
     GD_HIP(hipMemset(s->status, 0, statusBytes));
-    s->epoch = 1u;
+    GD_HIP(hipMalloc(&s->epoch, sizeof(uint32_t)));
+    GD_HIP(hipMemset(s->epoch, 0, sizeof(uint32_t)));
     s->pub.tempStorage = s->status;
     s->pub.tempStorageBytes = statusBytes + sizeof(uint32_t) * (GD_SORT_PASSES * GD_SORT_BINS +
                                                                 GD_SORT_PASSES) +
@@ -153,6 +173,7 @@ void sorterDestroyer(radixSorter* sorter) {
     GD_HIP(hipFree(s->hist));
     GD_HIP(hipFree(s->tileCounter));
     GD_HIP(hipFree(s->status));
+    GD_HIP(hipFree(s->epoch));
     memset(s, 0, sizeof(*s));
     free(s);
 }
@@ -162,6 +183,11 @@ void sorterDestroyer(radixSorter* sorter) {
  * ------------------------------------------------------------------------- */
 static void gd_builder_walk_arrays(gd_builder* b, int maxNodes) {
     if (maxNodes <= b->maxNodes) return;
+    if (gd_graphs_hold_buffers()) {
+        GD_FAIL("the walk arrays cannot be allocated while a graph holds the library's "
+                "buffers (build once before capturing)");
+        return;
+    }
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
     if (b->meta) GD_HIP(hipFree(b->meta));
@@ -311,13 +337,13 @@ void createWorldBox(const float4* cudaPositionMassVals, const int starCount,
     outputWorldBox->scale = scale;
 }
 
-/* optional HIP-event timing of the build's stages (null stream) */
+/* optional HIP-event timing of the build's stages (library stream) */
 static int g_buildTiming = 0;
 static int g_buildTimed = 0;
 static hipEvent_t g_buildEv[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
 static void gd_build_mark(int k) {
-    if (g_buildTiming) GD_HIP(hipEventRecord(g_buildEv[k], nullptr));
+    if (g_buildTiming) GD_HIP(hipEventRecord(g_buildEv[k], gd_stream()));
 }
 
 void gd_build_timing(int enable) {
@@ -342,6 +368,10 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
     const int n = tree->starCount;
     if (n <= 0) return;
     gd_builder* b = gd_builder_of(tree->builder);
+    if (gd_capturing() && g_buildTiming) {
+        GD_FAIL("buildTree: a captured build cannot be timed (gd_build_timing(0) first)");
+        return;
+    }
     gd_builder_walk_arrays(b, tree->maxNodes);
     if (gd_has_error()) return;
     gd_build_mark(0);
@@ -391,6 +421,12 @@ void gd_set_quadrupole(treeState* tree, int enable) {
         return;
     }
     gd_builder* b = gd_builder_of(tree->builder);
+    if (gd_graphs_hold_buffers() &&
+        ((enable && b->quadNodes < tree->maxNodes) || (enable == 2 && b->octNodes < tree->maxNodes))) {
+        GD_FAIL("gd_set_quadrupole: the moment arrays cannot be allocated while a graph holds "
+                "the library's buffers");
+        return;
+    }
     b->quad = enable;
     if (b->quad && b->quadNodes < tree->maxNodes) {
         if (b->wquad) GD_HIP(hipFree(b->wquad));
@@ -525,6 +561,56 @@ void blockNumCalculator(int* accelerationBlocks, int* treeBuilderBlocks, int* in
 }
 
 /* ---------------------------------------------------------------------------
+ * HIP graphs of the library's work
+ * ------------------------------------------------------------------------- */
+void gd_graph_begin(void) {
+    if (g_capturing) {
+        GD_FAIL("gd_graph_begin: a capture is already open");
+        return;
+    }
+    GD_HIP(hipStreamBeginCapture(gd_stream(), hipStreamCaptureModeRelaxed));
+    if (!gd_has_error()) g_capturing = 1;
+}
+
+void* gd_graph_end(void) {
+    if (!g_capturing) {
+        GD_FAIL("gd_graph_end: no capture is open");
+        return nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t err = hipStreamEndCapture(gd_stream(), &graph);
+    g_capturing = 0;
+    if (err != hipSuccess || !graph) {
+        GD_FAIL("gd_graph_end: capture failed: %s", hipGetErrorString(err));
+        return nullptr;
+    }
+    if (gd_has_error()) {   /* a call inside the capture failed: no graph */
+        GD_HIP(hipGraphDestroy(graph));
+        return nullptr;
+    }
+    hipGraphExec_t exec = nullptr;
+    GD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    GD_HIP(hipGraphDestroy(graph));
+    if (gd_has_error()) return nullptr;
+    ++g_liveGraphs;
+    return (void*)exec;
+}
+
+void gd_graph_launch(void* exec) {
+    if (!exec) {
+        GD_FAIL("gd_graph_launch: no graph");
+        return;
+    }
+    GD_HIP(hipGraphLaunch((hipGraphExec_t)exec, gd_stream()));
+}
+
+void gd_graph_destroy(void* exec) {
+    if (!exec) return;
+    GD_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+    if (g_liveGraphs > 0) --g_liveGraphs;
+}
+
+/* ---------------------------------------------------------------------------
  * Events
  * ------------------------------------------------------------------------- */
 void* gd_event_create(void) {
@@ -532,7 +618,7 @@ void* gd_event_create(void) {
     GD_HIP(hipEventCreate(&e));
     return (void*)e;
 }
-void gd_event_record(void* ev) { GD_HIP(hipEventRecord((hipEvent_t)ev, nullptr)); }
+void gd_event_record(void* ev) { GD_HIP(hipEventRecord((hipEvent_t)ev, gd_stream())); }
 float gd_event_elapsed_ms(void* start, void* stop) {
     float ms = 0.f;
     GD_HIP(hipEventSynchronize((hipEvent_t)stop));

@@ -129,9 +129,9 @@ void gd_launch_bbox(const float4* pos, int n, gd_builder* b, float3* outMin, flo
                                     ? gd_div_up(n, 2 * GD_BBOX_THREADS)
                                     : GD_BBOX_BLOCKS)
                              : 1;
-    k_bbox_partial<<<blocks, GD_BBOX_THREADS>>>(pos, n, b->bboxPartMin, b->bboxPartMax);
+    k_bbox_partial<<<blocks, GD_BBOX_THREADS, 0, gd_stream()>>>(pos, n, b->bboxPartMin, b->bboxPartMax);
     GD_LAUNCHED();
-    k_bbox_final<<<1, 1024>>>(b->bboxPartMin, b->bboxPartMax, blocks, outMin, outMax, b->dinfo);
+    k_bbox_final<<<1, 1024, 0, gd_stream()>>>(b->bboxPartMin, b->bboxPartMax, blocks, outMin, outMax, b->dinfo);
     GD_LAUNCHED();
 }
 
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_morton_val(const float4* __restrict__ p
 void gd_launch_morton_dev(const float4* pos, int n, const gd_tree_info* info, uint64_t* keys,
                           uint32_t* idx) {
     if (n <= 0) return;
-    k_morton_dev<<<gd_div_up(n, 256), 256>>>(pos, n, info, keys, idx);
+    k_morton_dev<<<gd_div_up(n, 256), 256, 0, gd_stream()>>>(pos, n, info, keys, idx);
     GD_LAUNCHED();
 }
 
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_reorder_lo(const float4* __restrict__ p
 void gd_launch_reorder(const float4* posIn, const float3* velIn, float4* posOut, float3* velOut,
                        const uint32_t* idx, int n) {
     if (n <= 0) return;
-    k_reorder<<<gd_div_up(n, 256), 256>>>(posIn, velIn, posOut, velOut, idx, n);
+    k_reorder<<<gd_div_up(n, 256), 256, 0, gd_stream()>>>(posIn, velIn, posOut, velOut, idx, n);
     GD_LAUNCHED();
 }
 
@@ -235,7 +235,7 @@ void gd_launch_reorder_lo(const float4* posIn, const float3* velIn, const float3
                           float4* posOut, float3* velOut, float3* loOut, const uint32_t* idx,
                           int n) {
     if (n <= 0) return;
-    k_reorder_lo<<<gd_div_up(n, 256), 256>>>(posIn, velIn, loIn, posOut, velOut, loOut, idx, n);
+    k_reorder_lo<<<gd_div_up(n, 256), 256, 0, gd_stream()>>>(posIn, velIn, loIn, posOut, velOut, loOut, idx, n);
     GD_LAUNCHED();
 }
 
@@ -247,7 +247,7 @@ extern "C" {
 void mortonEncode(const float4* posMassVals, uint64_t* mortonCodes, uint32_t* originalIndex,
                   int NStars, worldBox rootBox) {
     if (NStars <= 0) return;
-    k_morton_val<<<gd_div_up(NStars, 256), 256>>>(posMassVals, NStars, rootBox, mortonCodes,
+    k_morton_val<<<gd_div_up(NStars, 256), 256, 0, gd_stream()>>>(posMassVals, NStars, rootBox, mortonCodes,
                                                   originalIndex);
     GD_LAUNCHED();
 }
@@ -255,13 +255,13 @@ void mortonEncode(const float4* posMassVals, uint64_t* mortonCodes, uint32_t* or
 void localMinMaxFinder(const float4* posMassVals, unsigned int N, float3* minCorner,
                        float3* maxCorner) {
     /* BOXBLOCKS x BOXTHREADS like tree.h:16-17; each block writes one partial */
-    k_bbox_partial<<<256, GD_BBOX_THREADS>>>(posMassVals, (int)N, minCorner, maxCorner);
+    k_bbox_partial<<<256, GD_BBOX_THREADS, 0, gd_stream()>>>(posMassVals, (int)N, minCorner, maxCorner);
     GD_LAUNCHED();
 }
 
 void globalMinMaxReducer(const float3* minCorner, const float3* maxCorner, int count,
                          float3* outMin, float3* outMax) {
-    k_bbox_final<<<1, 1024>>>(minCorner, maxCorner, count, outMin, outMax, nullptr);
+    k_bbox_final<<<1, 1024, 0, gd_stream()>>>(minCorner, maxCorner, count, outMin, outMax, nullptr);
     GD_LAUNCHED();
 }
 

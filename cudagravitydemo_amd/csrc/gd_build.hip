@@ -695,15 +695,15 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
         GD_FAIL("octree build of %d bodies exceeds the builder's capacity", n);
         return;
     }
-    k_build_count<<<tiles, GD_BUILD_THREADS>>>(sortedKeys, n, b->tileCounts, b->maxTiles);
+    k_build_count<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileCounts, b->maxTiles);
     GD_LAUNCHED();
-    k_build_scan<<<GD_MAX_LEVEL, 1024>>>(b->tileCounts, b->tileOffsets, tiles, b->maxTiles,
+    k_build_scan<<<GD_MAX_LEVEL, 1024, 0, gd_stream()>>>(b->tileCounts, b->tileOffsets, tiles, b->maxTiles,
                                          b->levelTotals);
     GD_LAUNCHED();
-    k_build_nodes<<<tiles, GD_BUILD_THREADS>>>(sortedKeys, n, b->tileOffsets, b->levelTotals,
+    k_build_nodes<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileOffsets, b->levelTotals,
                                                b->maxTiles, b->meta, maxNodes, b->dinfo);
     GD_LAUNCHED();
-    if (gd_build_mid_event) GD_HIP(hipEventRecord(gd_build_mid_event, nullptr));
+    if (gd_build_mid_event) GD_HIP(hipEventRecord(gd_build_mid_event, gd_stream()));
     for (int L = GD_MAX_LEVEL; L >= 0; --L) {
         /* a level holds at most min(8^L, n) nodes */
         long long bound = n;
@@ -714,13 +714,13 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
         int blocks = gd_div_up(bound, 256);
         if (blocks > 4096) blocks = 4096;   /* 16 blocks per CU */
         if (b->quad == 2)
-            k_build_mass<2><<<blocks, 256>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
+            k_build_mass<2><<<blocks, 256, 0, gd_stream()>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
                                              b->wcom, b->wlink, b->wquad, b->woct);
         else if (b->quad == 1)
-            k_build_mass<1><<<blocks, 256>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
+            k_build_mass<1><<<blocks, 256, 0, gd_stream()>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
                                              b->wcom, b->wlink, b->wquad, nullptr);
         else
-            k_build_mass<0><<<blocks, 256>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
+            k_build_mass<0><<<blocks, 256, 0, gd_stream()>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
                                              b->wcom, b->wlink, nullptr, nullptr);
         GD_LAUNCHED();
     }
@@ -796,7 +796,7 @@ extern "C" {
 
 void identifyNodesAtLevel(uint64_t* mortonCodes, int nodeLevel, int starCount, uint32_t* nodeFlags) {
     if (starCount <= 0) return;
-    k_identify<<<gd_div_up(starCount, 256), 256>>>(mortonCodes, nodeLevel, starCount, nodeFlags);
+    k_identify<<<gd_div_up(starCount, 256), 256, 0, gd_stream()>>>(mortonCodes, nodeLevel, starCount, nodeFlags);
     GD_LAUNCHED();
 }
 
@@ -804,7 +804,7 @@ void createNodes(const uint64_t* mortonCodes, const uint32_t* flags, const uint3
                  const uint32_t* offsets, const uint32_t* offsetsPrev, node* nodes, int level,
                  int arrayLevelOffset, int prevArrayLevelOffset, int starCount, int maxNodes) {
     if (starCount <= 0) return;
-    k_create_nodes<<<gd_div_up(starCount, 256), 256>>>(mortonCodes, flags, previousFlags, offsets,
+    k_create_nodes<<<gd_div_up(starCount, 256), 256, 0, gd_stream()>>>(mortonCodes, flags, previousFlags, offsets,
                                                        offsetsPrev, nodes, level, arrayLevelOffset,
                                                        prevArrayLevelOffset, starCount, maxNodes);
     GD_LAUNCHED();
@@ -813,7 +813,7 @@ void createNodes(const uint64_t* mortonCodes, const uint32_t* flags, const uint3
 void updateTreeParticles(node* nodes, int levelStartIndex, int levelNodeCount, int starCount) {
     (void)starCount;
     if (levelNodeCount <= 0) return;
-    k_tree_particles<<<gd_div_up(levelNodeCount, 256), 256>>>(nodes, levelStartIndex,
+    k_tree_particles<<<gd_div_up(levelNodeCount, 256), 256, 0, gd_stream()>>>(nodes, levelStartIndex,
                                                                levelNodeCount);
     GD_LAUNCHED();
 }
@@ -822,7 +822,7 @@ void updateTreeMass(node* nodes, const float4* positionMassVals, int level, int 
                     int levelStartIndex) {
     (void)level;
     if (levelNodes <= 0) return;
-    k_tree_mass_api<<<gd_div_up(levelNodes, 256), 256>>>(nodes, positionMassVals, levelNodes,
+    k_tree_mass_api<<<gd_div_up(levelNodes, 256), 256, 0, gd_stream()>>>(nodes, positionMassVals, levelNodes,
                                                          levelStartIndex);
     GD_LAUNCHED();
 }

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_integrate(float4* __restrict__ pos,
 extern "C" void positionUpdater(float4* positionMassArray, float3* velocityVals,
                                 float3* accelerationVals, int starCount, float timeStep) {
     if (starCount <= 0) return;
-    k_integrate<<<gd_div_up(starCount, 256), 256>>>(positionMassArray, velocityVals,
+    k_integrate<<<gd_div_up(starCount, 256), 256, 0, gd_stream()>>>(positionMassArray, velocityVals,
                                                     accelerationVals, starCount, timeStep);
     GD_LAUNCHED();
 }
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_integrate_hilo(float4* __restrict__ pos
 extern "C" void gd_integrate_hilo(float4* pos, float3* posLo, float3* vel, const float3* acc,
                                   int starCount, float dt) {
     if (starCount <= 0) return;
-    k_integrate_hilo<<<gd_div_up(starCount, 256), 256>>>(pos, posLo, vel, acc, starCount, dt);
+    k_integrate_hilo<<<gd_div_up(starCount, 256), 256, 0, gd_stream()>>>(pos, posLo, vel, acc, starCount, dt);
     GD_LAUNCHED();
 }
 
@@ -195,13 +195,13 @@ static void gd_launch_judge(const float4* pos, const float3* posLo, int n, const
         ((gd_div_up(n, chunks) + GD_JUDGE_SRC - 1) / GD_JUDGE_SRC) * (long long)GD_JUDGE_SRC;
     const int tiles = gd_div_up(ns, GD_JUDGE_TILE);
     if (eps2 > 0.0)
-        k_judge_tile<false><<<tiles * chunks, GD_JUDGE_TILE>>>(pos, posLo, n, sample, ns, chunks,
+        k_judge_tile<false><<<tiles * chunks, GD_JUDGE_TILE, 0, gd_stream()>>>(pos, posLo, n, sample, ns, chunks,
                                                                chunkLen, eps2, part);
     else
-        k_judge_tile<true><<<tiles * chunks, GD_JUDGE_TILE>>>(pos, posLo, n, sample, ns, chunks,
+        k_judge_tile<true><<<tiles * chunks, GD_JUDGE_TILE, 0, gd_stream()>>>(pos, posLo, n, sample, ns, chunks,
                                                               chunkLen, eps2, part);
     GD_LAUNCHED();
-    k_judge_sum<<<gd_div_up(ns, 256), 256>>>(part, ns, chunks, G, acc);
+    k_judge_sum<<<gd_div_up(ns, 256), 256, 0, gd_stream()>>>(part, ns, chunks, G, acc);
     GD_LAUNCHED();
 }
 
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_force_errors(const float3* __restrict__
 extern "C" void gd_force_errors(const float3* acc, const int* sample, int ns, const double* exact,
                                 double* err) {
     if (ns <= 0) return;
-    k_force_errors<<<gd_div_up(ns, 256), 256>>>(acc, sample, ns, exact, err);
+    k_force_errors<<<gd_div_up(ns, 256), 256, 0, gd_stream()>>>(acc, sample, ns, exact, err);
     GD_LAUNCHED();
 }
 
@@ -363,9 +363,9 @@ extern "C" void gd_diagnostics(const float4* pos, const float3* vel, const float
     if (starCount > 0) {
         int blocks = gd_div_up(starCount, 256);
         if (blocks > GD_DIAG_BLOCKS) blocks = GD_DIAG_BLOCKS;
-        k_diag<<<blocks, 256>>>(pos, vel, phi, starCount, g_diagPart);
+        k_diag<<<blocks, 256, 0, gd_stream()>>>(pos, vel, phi, starCount, g_diagPart);
         GD_LAUNCHED();
-        k_diag_sum<<<1, 64>>>(g_diagPart, blocks, g_diagPart + GD_DIAG_BLOCKS * 6);
+        k_diag_sum<<<1, 64, 0, gd_stream()>>>(g_diagPart, blocks, g_diagPart + GD_DIAG_BLOCKS * 6);
         GD_LAUNCHED();
         GD_HIP(hipMemcpy(h, g_diagPart + GD_DIAG_BLOCKS * 6, sizeof(h), hipMemcpyDeviceToHost));
     }
@@ -393,6 +393,6 @@ extern "C" void gd_stream_copy(void* dst, const void* src, size_t bytes) {
     if (n == 0) return;
     long long blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    k_stream_copy<<<(int)blocks, 256>>>((const float4*)src, (float4*)dst, n);
+    k_stream_copy<<<(int)blocks, 256, 0, gd_stream()>>>((const float4*)src, (float4*)dst, n);
     GD_LAUNCHED();
 }

@@ -25,6 +25,14 @@ int gd_has_error(void);
  * opening angle. */
 float gd_walk_split_ratio(float theta);
 
+/* The stream of every launch, asynchronous copy and event of the library. */
+hipStream_t gd_stream(void);
+/* 1 while a gd_graph_begin capture is open. */
+int gd_capturing(void);
+/* 1 while a capture is open or an instantiated graph lives: the buffers a
+ * graph's kernels address must then stay where they are. */
+int gd_graphs_hold_buffers(void);
+
 #define GD_FAIL(...) gd_fail(__FILE__, __LINE__, __VA_ARGS__)
 #define GD_HIP(call)                                                              \
     do {                                                                          \
@@ -91,7 +99,7 @@ typedef struct {
     uint32_t* tileCounter;           /* [8] tile tickets, one per pass */
     unsigned long long* status;      /* [maxTiles][256] look-back status words */
     int maxTiles;
-    uint32_t epoch;                  /* tag of the next pass's status words */
+    uint32_t* epoch;                 /* device: tag of this sort's first pass (passes add 0..7) */
 } gd_sorter;
 
 /*
@@ -132,7 +140,7 @@ typedef struct {
 static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }
 static inline gd_builder* gd_builder_of(treeBuilder* b) { return (gd_builder*)b; }
 
-/* ---- launchers shared between translation units (all on the null stream) */
+/* ---- launchers shared between translation units (all on gd_stream()) */
 void gd_launch_bbox(const float4* pos, int n, gd_builder* b, float3* outMin, float3* outMax);
 void gd_launch_morton_dev(const float4* pos, int n, const gd_tree_info* info, uint64_t* keys,
                           uint32_t* idx);

@@ -423,7 +423,7 @@ struct gd_mlp {
 };
 
 static void gd_mlp_repack(gd_mlp* net) {
-    k_mlp_pack<<<GD_MLP_FWD_TILES + GD_MLP_BWD_TILES, 64>>>(net->master, net->fwd, net->bwd,
+    k_mlp_pack<<<GD_MLP_FWD_TILES + GD_MLP_BWD_TILES, 64, 0, gd_stream()>>>(net->master, net->fwd, net->bwd,
                                                             net->bias);
     GD_LAUNCHED();
 }
@@ -482,7 +482,7 @@ void gd_mlp_forward(const gd_mlp* net, const float* X, float* Y, long long rows)
     if (rows <= 0) return;
     long long blocks = (rows + 15) / 16;
     if (blocks > 16384) blocks = 16384;
-    k_mlp_infer<<<(int)blocks, 64>>>(net->fwd, net->bias, X, Y, rows);
+    k_mlp_infer<<<(int)blocks, 64, 0, gd_stream()>>>(net->fwd, net->bias, X, Y, rows);
     GD_LAUNCHED();
 }
 
@@ -498,7 +498,7 @@ static float gd_mlp_grad_impl(gd_mlp* net, const float* X, const float* Y, const
         net->partialBlocks = blocks;
     }
     if (gd_has_error()) return -1.f;
-    k_mlp_grad<<<blocks, 64>>>(net->fwd, net->bwd, net->bias, X, Y, order, B, perBlock,
+    k_mlp_grad<<<blocks, 64, 0, gd_stream()>>>(net->fwd, net->bwd, net->bias, X, Y, order, B, perBlock,
                                net->partial, net->lossPart);
     GD_LAUNCHED();
     float c1 = 1.f, c2 = 1.f;
@@ -507,7 +507,7 @@ static float gd_mlp_grad_impl(gd_mlp* net, const float* X, const float* Y, const
         c1 = 1.f - powf(0.9f, (float)net->step);
         c2 = 1.f - powf(0.999f, (float)net->step);
     }
-    k_mlp_reduce_adam<<<gd_div_up(GD_MLP_NPARAMS, 256), 256>>>(
+    k_mlp_reduce_adam<<<gd_div_up(GD_MLP_NPARAMS, 256), 256, 0, gd_stream()>>>(
         net->partial, blocks, net->grad, net->master, net->m1, net->m2, doAdam, lr, 0.9f, 0.999f,
         1e-8f, c1, c2);
     GD_LAUNCHED();
@@ -752,7 +752,7 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
     GD_HIP(hipMemset(dropped, 0, sizeof(int)));
     long long n = 0;
     if (!gd_has_error()) {
-        k_samples_list<<<gd_div_up(nt, GD_SL_WAVES), 64 * GD_SL_WAVES>>>(
+        k_samples_list<<<gd_div_up(nt, GD_SL_WAVES), 64 * GD_SL_WAVES, 0, gd_stream()>>>(
             tree->mortonOut, pos, tree->starCount, b->wcom, b->wlink, b->dinfo, targets, nt, theta,
             gd_walk_split_ratio(theta), minRatio, pairs, cap, count, dropped, centre);
         GD_LAUNCHED();
@@ -766,7 +766,7 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         if (rows > 0) {
             long long blocks = (rows + 3) / 4;
             if (blocks > 65536) blocks = 65536;
-            k_samples_eval<<<(int)blocks, 256>>>(pos, tree->nodes, b->wcom, b->wquad, b->woct,
+            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, tree->nodes, b->wcom, b->wquad, b->woct,
                                                  b->dinfo, targets, centre, pairs, rows,
                                                  (double)eps2, X, Y);
             GD_LAUNCHED();

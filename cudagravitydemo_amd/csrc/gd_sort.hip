@@ -78,11 +78,20 @@ __device__ __forceinline__ uint32_t gd_block_exclusive(uint32_t v, uint32_t* tot
 /* ---------------------------------------------------------------------------
  * Radix sort kernels
  * ------------------------------------------------------------------------- */
-__global__ void k_sort_init(uint32_t* hist, uint32_t* tileCounter) {
+/* Clears the digit counts and tile tickets and advances the status-word
+ * epoch by one per pass, on the device (so a captured sort replays with new
+ * tags): passes p = 0..7 tag their words *epoch + p; tags stay in
+ * [1, 2^30 - 2], 0 being the tag of a cleared word. */
+__global__ void k_sort_init(uint32_t* hist, uint32_t* tileCounter, uint32_t* epoch) {
     const int t = threadIdx.x;   /* 256 threads */
 #pragma unroll
     for (int p = 0; p < GD_SORT_PASSES; ++p) hist[p * GD_SORT_BINS + t] = 0u;
     if (t < GD_SORT_PASSES) tileCounter[t] = 0u;
+    if (t == 0) {
+        uint32_t e = *epoch + GD_SORT_PASSES;
+        if (e == GD_SORT_PASSES || e > 0x3fffffffu - GD_SORT_PASSES) e = 1u;
+        *epoch = e;
+    }
 }
 
 #define GD_HIST_ITEMS 8
@@ -155,7 +164,7 @@ __global__ __launch_bounds__(GD_SORT_THREADS) void k_sort_pass(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, int n, int shift,
     const uint32_t* __restrict__ digitBase, uint32_t* tileCounter,
-    unsigned long long* status, uint32_t epoch) {
+    unsigned long long* status, const uint32_t* __restrict__ epochBase, int pass) {
     constexpr int WAVES = GD_SORT_THREADS / GD_WAVE;
     constexpr int WAVE_SPAN = GD_SORT_ITEMS * GD_WAVE;   /* keys per wave */
     __shared__ uint64_t sKeys[GD_SORT_TILE];
@@ -168,6 +177,7 @@ __global__ __launch_bounds__(GD_SORT_THREADS) void k_sort_pass(
     const int t = threadIdx.x;
     const unsigned lane = gd_lane();
     const int wave = t / GD_WAVE;
+    const uint32_t epoch = *epochBase + (uint32_t)pass;
     if (t == 0) sTile = atomicAdd(tileCounter, 1u);
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) sWaveHist[w][t] = 0u;
@@ -294,13 +304,13 @@ void gd_sort_pairs(gd_sorter* s, const uint64_t* kin, uint64_t* kout, const uint
         GD_FAIL("radix sort of %d keys exceeds the sorter's %d tiles", n, s->maxTiles);
         return;
     }
-    k_sort_init<<<1, 256>>>(s->hist, s->tileCounter);
+    k_sort_init<<<1, 256, 0, gd_stream()>>>(s->hist, s->tileCounter, s->epoch);
     GD_LAUNCHED();
     int hb = gd_div_up(n, GD_HIST_CHUNK);
     if (hb > 1024) hb = 1024;
-    k_sort_hist<<<hb, 256>>>(kin, n, s->hist);
+    k_sort_hist<<<hb, 256, 0, gd_stream()>>>(kin, n, s->hist);
     GD_LAUNCHED();
-    k_sort_prefix<<<GD_SORT_PASSES, 256>>>(s->hist);
+    k_sort_prefix<<<GD_SORT_PASSES, 256, 0, gd_stream()>>>(s->hist);
     GD_LAUNCHED();
     const uint64_t* ksrc = kin;
     const uint32_t* vsrc = vin;
@@ -309,12 +319,10 @@ void gd_sort_pairs(gd_sorter* s, const uint64_t* kin, uint64_t* kout, const uint
            the eighth pass lands in keysOut/valsOut */
         uint64_t* kdst = (p & 1) ? kout : s->keysTmp;
         uint32_t* vdst = (p & 1) ? vout : s->valsTmp;
-        k_sort_pass<<<tiles, GD_SORT_THREADS>>>(ksrc, vsrc, kdst, vdst, n, 8 * p,
+        k_sort_pass<<<tiles, GD_SORT_THREADS, 0, gd_stream()>>>(ksrc, vsrc, kdst, vdst, n, 8 * p,
                                                 s->hist + p * GD_SORT_BINS, s->tileCounter + p,
-                                                s->status, s->epoch);
+                                                s->status, s->epoch, p);
         GD_LAUNCHED();
-        s->epoch = (s->epoch + 1u) & 0x3fffffffu;
-        if (s->epoch == 0u) s->epoch = 1u;   /* 0 is the tag of a cleared word */
         ksrc = kdst;
         vsrc = vdst;
     }
@@ -374,10 +382,10 @@ __global__ __launch_bounds__(GD_SCAN_THREADS) void k_scan_down(const uint32_t* _
 void gd_exclusive_scan_u32(gd_builder* b, const uint32_t* in, uint32_t* out, int n) {
     if (n <= 0) return;
     const int tiles = gd_div_up(n, GD_SCAN_TILE);
-    k_scan_reduce<<<tiles, GD_SCAN_THREADS>>>(in, n, b->scanTiles);
+    k_scan_reduce<<<tiles, GD_SCAN_THREADS, 0, gd_stream()>>>(in, n, b->scanTiles);
     GD_LAUNCHED();
-    k_scan_sums<<<1, 1024>>>(b->scanTiles, tiles);
+    k_scan_sums<<<1, 1024, 0, gd_stream()>>>(b->scanTiles, tiles);
     GD_LAUNCHED();
-    k_scan_down<<<tiles, GD_SCAN_THREADS>>>(in, out, n, b->scanTiles);
+    k_scan_down<<<tiles, GD_SCAN_THREADS, 0, gd_stream()>>>(in, out, n, b->scanTiles);
     GD_LAUNCHED();
 }

@@ -122,7 +122,7 @@ extern "C" void calculateAcceleration(const float4* positionMassArray, treeState
                                       float GVal, float theta, int starCount) {
     if (starCount <= 0) return;
     const int groups = gd_div_up(starCount, 32);
-    k_walk_parity<<<gd_div_up(groups, GD_PAR_SLOTS), 256>>>(
+    k_walk_parity<<<gd_div_up(groups, GD_PAR_SLOTS), 256, 0, gd_stream()>>>(
         positionMassArray, tree.nodes, tree.boundingData[2], tree.boundingData[3], accelVals,
         antiSingularitySquared, GVal, theta, starCount);
     GD_LAUNCHED();
@@ -1435,7 +1435,7 @@ static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nu
 
 static int g_lastRuns = 0;   /* 64-body runs of the last asynchronous walk */
 
-/* optional HIP-event timing of the walk's phases (null stream) */
+/* optional HIP-event timing of the walk's phases (library stream) */
 static int g_walkTiming = 0;
 static hipEvent_t g_walkEv[4] = {nullptr, nullptr, nullptr, nullptr};
 static int g_walkTimed = 0;
@@ -1446,6 +1446,12 @@ __global__ void k_walk_fill(int* a, int n, int v) {
 }
 
 static void gd_walk_ensure(int groups) {
+    if (gd_graphs_hold_buffers() &&
+        (!g_walk.stats || groups > g_walk.groupCap || !g_walk.lists)) {
+        GD_FAIL("gd_walk: the walk's buffers cannot be allocated while a graph holds the "
+                "library's buffers (walk once before capturing)");
+        return;
+    }
     if (!g_walk.stats) {
         GD_HIP(hipMalloc(&g_walk.stats, GD_STAT_WORDS * sizeof(unsigned long long)));
         GD_HIP(hipMalloc(&g_walk.status, sizeof(gd_walk_status)));
@@ -1473,7 +1479,7 @@ static void gd_walk_ensure(int groups) {
                          2 * sizeof(long long) * (size_t)gd_div_up(groups, 1024)));
         GD_HIP(hipMalloc(&g_walk.slotRange, sizeof(int2) * ((size_t)groups + 1)));
         GD_HIP(hipMalloc(&g_walk.order, sizeof(int) * (size_t)groups));
-        k_walk_fill<<<gd_div_up(groups, 256), 256>>>(g_walk.sizes, groups, -1);
+        k_walk_fill<<<gd_div_up(groups, 256), 256, 0, gd_stream()>>>(g_walk.sizes, groups, -1);
         GD_LAUNCHED();
         g_walk.groupCap = groups;
     }
@@ -1488,6 +1494,8 @@ static void gd_walk_ensure(int groups) {
 /* Grow the first-pass pool to hold `want` entries with room to spare. */
 static void gd_walk_grow_lists(long long want) {
     if (want <= g_walk.listEntries) return;
+    if (gd_graphs_hold_buffers()) return;   /* a graph addresses the pool: keep it; groups
+                                               without room take the later passes */
     const long long e = want + want / 4;
     /* an earlier asynchronous walk may still read the pool */
     if (g_walk.lists) GD_HIP(hipDeviceSynchronize());
@@ -1498,6 +1506,11 @@ static void gd_walk_grow_lists(long long want) {
 
 static void gd_walk_ensure_pool(int slots, int entries) {
     const long long need = (long long)slots * entries;
+    if (gd_graphs_hold_buffers() && (need > g_walk.poolEntries || slots > g_walk.countSlots)) {
+        GD_FAIL("gd_walk: the second pass's buffers cannot be allocated while a graph holds "
+                "the library's buffers (walk once asynchronously before capturing)");
+        return;
+    }
     if (need > g_walk.poolEntries) {
         if (g_walk.pool) GD_HIP(hipFree(g_walk.pool));
         GD_HIP(hipMalloc(&g_walk.pool, (size_t)need * sizeof(int)));
@@ -1556,7 +1569,7 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            const int* caps, const int2* counts, int cap, float3* acc,
                            float* phi, float eps2, float G, int n, const gd_bf16x4* W,
                            const float* Bv, float minRatio) {
-    k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
+    k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
         pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, groupList, slots,
         lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.kstats);
 }
@@ -1570,17 +1583,17 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     const int runs = gd_div_up(n, 64);
     const int groups = 2 * runs;   /* group slots: two per 64-body run */
     gd_walk_status* st = g_walk.status;
-    k_walk_split<<<gd_div_up(runs, 4), 256>>>(keys, pos, n, runs,
+    k_walk_split<<<gd_div_up(runs, 4), 256, 0, gd_stream()>>>(keys, pos, n, runs,
                                               gd_walk_split_ratio(theta),
                                               g_walk.slotRange);
     GD_LAUNCHED();
     /* pass 1: every group, packed lists */
     const int reset = (g_walk.walks++ % 32) == 0;
     const int lblocks = gd_div_up(groups, 1024);
-    k_walk_layout<<<lblocks, 1024>>>(g_walk.slotRange, g_walk.sizes, groups, g_walk.fixedCap,
+    k_walk_layout<<<lblocks, 1024, 0, gd_stream()>>>(g_walk.slotRange, g_walk.sizes, groups, g_walk.fixedCap,
                                      reset, g_walk.offsets, g_walk.blockSums);
     GD_LAUNCHED();
-    k_walk_place<<<lblocks, 1024>>>(g_walk.slotRange, g_walk.sizes, groups, g_walk.fixedCap,
+    k_walk_place<<<lblocks, 1024, 0, gd_stream()>>>(g_walk.slotRange, g_walk.sizes, groups, g_walk.fixedCap,
                                     g_walk.listEntries,
                                     g_walk.blockSums, g_walk.offsets, g_walk.caps,
                                     g_walk.order, &st->requested);
@@ -1593,7 +1606,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
        worth it for a few groups, not for a large share. */
 #define GD_PASS1(STK)                                                                          \
     k_walk_traverse<SPLIT, GD_TR_WAVES, STK><<<gd_div_up(groups, GD_TR_WAVES),                  \
-                                               64 * GD_TR_WAVES>>>(                             \
+                                               64 * GD_TR_WAVES, 0, gd_stream()>>>(                             \
         pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.order, groups,    \
         g_walk.offsets,                                                                        \
         g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,  \
@@ -1603,17 +1616,17 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     else GD_PASS1(GD_TR_STACK);
 #undef GD_PASS1
     GD_LAUNCHED();
-    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], nullptr));
+    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], gd_stream()));
     {
         /* one wavefront per run: both halves' lists side by side */
         constexpr int EVW = MODE == 2 ? 8 : 4;   /* learned: 8 share the staged weights */
-        k_walk_eval_pair<MODE, PHI, EVW><<<gd_div_up(runs, EVW), 64 * EVW>>>(
+        k_walk_eval_pair<MODE, PHI, EVW><<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, runs,
             g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2, G, W, Bv,
             minRatio, g_walk.kstats);
     }
     GD_LAUNCHED();
-    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], nullptr));
+    if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], gd_stream()));
     constexpr int BMODE0 = MODE == 2 ? 1 : MODE;   /* per-body walk: see pass 3 below */
     if (g_walk.async) {
         /* the same passes with every decision on the device: the second pass
@@ -1624,47 +1637,49 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         const int S2 = GD_ASYNC_PASS2;
         gd_walk_ensure_pool(S2, cap2);
         if (gd_has_error()) return;
-        k_walk_route<<<64, 256>>>(g_walk.overflow1, &st->over1, S2, g_walk.overflow2,
+        k_walk_route<<<64, 256, 0, gd_stream()>>>(g_walk.overflow1, &st->over1, S2, g_walk.overflow2,
                                   &st->over2);
         GD_LAUNCHED();
-        k_walk_estimate<SPLIT><<<gd_div_up(S2, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
+        k_walk_estimate<SPLIT><<<gd_div_up(S2, GD_TR_WAVES), 64 * GD_TR_WAVES, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
             cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap,
             &st->over1);
         GD_LAUNCHED();
-        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<S2, 64>>>(
+        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<S2, 64, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
             nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
-        k_walk_eval_seg<MODE, PHI><<<GD_SEG_WAVES / GD_EV_WAVES, 64 * GD_EV_WAVES>>>(
+        k_walk_eval_seg<MODE, PHI><<<GD_SEG_WAVES / GD_EV_WAVES, 64 * GD_EV_WAVES, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
             g_walk.overflow1, S2, 0, g_walk.pool, g_walk.counts2, cap2, g_walk.segPart, eps2, W,
             Bv, minRatio, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
-        k_walk_eval_sum<PHI><<<gd_div_up(S2, 4), 256>>>(g_walk.slotRange, g_walk.overflow1, S2,
+        k_walk_eval_sum<PHI><<<gd_div_up(S2, 4), 256, 0, gd_stream()>>>(g_walk.slotRange, g_walk.overflow1, S2,
                                                          0, g_walk.counts2, g_walk.segPart, acc,
                                                          phi, G, &st->over1);
         GD_LAUNCHED();
         const long long bodies = 64ll * groups;
         k_walk_body<BMODE0, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
-            <<<GD_ASYNC_BODY_BLOCKS, 64 * GD_BODY_WAVES>>>(
+            <<<GD_ASYNC_BODY_BLOCKS, 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
                 pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, g_walk.overflow2,
                 nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
                 g_walk.stats, &st->over2, 64);
         GD_LAUNCHED();
-        k_walk_body<BMODE0, PHI, 1, GD_BODY_STACK><<<GD_ASYNC_DEEP_BLOCKS, 64>>>(
+        k_walk_body<BMODE0, PHI, 1, GD_BODY_STACK><<<GD_ASYNC_DEEP_BLOCKS, 64, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, nullptr,
             g_walk.bodyOver, bodies, nullptr, nullptr, acc, phi, eps2, G, theta, n,
             g_walk.stats, &st->over3, 1);
         GD_LAUNCHED();
         /* the status goes back without waiting: the next walk (or
-           gd_walk_poll) reads it */
-        GD_HIP(hipMemcpyAsync(g_walk.hstatus, st, sizeof(gd_walk_status),
-                              hipMemcpyDeviceToHost, nullptr));
-        GD_HIP(hipEventRecord(g_walk.statusEv, nullptr));
-        g_walk.statusPending = 1;
-        g_lastRuns = runs;
+           gd_walk_poll) reads it; a captured walk keeps it on the device */
+        if (!gd_capturing()) {
+            GD_HIP(hipMemcpyAsync(g_walk.hstatus, st, sizeof(gd_walk_status),
+                                  hipMemcpyDeviceToHost, gd_stream()));
+            GD_HIP(hipEventRecord(g_walk.statusEv, gd_stream()));
+            g_walk.statusPending = 1;
+            g_lastRuns = runs;
+        }
         overOut[0] = overOut[2] = overOut[3] = -1;   /* not known on the host */
         *requested = 0;
         return;
@@ -1689,11 +1704,11 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     if (gd_has_error()) return;
     for (int b0 = 0; b0 < over1; b0 += slots) {
         const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
-        k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES>>>(
+        k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap);
         GD_LAUNCHED();
-        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64>>>(
+        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats);
@@ -1703,12 +1718,12 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
                                                  : 1;
         if (segs > 1) {
             k_walk_eval_seg<MODE, PHI>
-                <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES>>>(
+                <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
                     pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
                     g_walk.overflow1 + b0, cnt, segs, g_walk.pool, g_walk.counts2, cap2,
                     g_walk.segPart, eps2, W, Bv, minRatio, g_walk.kstats);
             GD_LAUNCHED();
-            k_walk_eval_sum<PHI><<<gd_div_up(cnt, 4), 256>>>(g_walk.slotRange,
+            k_walk_eval_sum<PHI><<<gd_div_up(cnt, 4), 256, 0, gd_stream()>>>(g_walk.slotRange,
                                                              g_walk.overflow1 + b0, cnt, segs,
                                                              g_walk.counts2, g_walk.segPart,
                                                              acc, phi, G);
@@ -1732,7 +1747,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
        ones), which the point-based opening test already serves better. */
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
     k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
-        <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES>>>(
+        <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
             pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, g_walk.overflow2,
             nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
             g_walk.stats);
@@ -1740,7 +1755,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
-    k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64>>>(
+    k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64, 0, gd_stream()>>>(
         pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
         h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, g_walk.stats);
     GD_LAUNCHED();
@@ -1789,13 +1804,19 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         return;
     }
     const int groups = 2 * gd_div_up(starCount, 64);   /* group slots */
-    if (gd_walk_take_status(0) != 0) return;   /* a finished asynchronous walk's report */
+    if (gd_capturing() && (!g_walk.async || stats || g_walkTiming)) {
+        GD_FAIL("gd_walk: a captured walk must be asynchronous, without statistics or timing "
+                "(gd_walk_set_async)");
+        return;
+    }
+    if (!gd_capturing() && gd_walk_take_status(0) != 0) return;   /* a finished
+                                                                     asynchronous walk's report */
     gd_walk_ensure(groups);
     if (g_walk.fixedCap > 0) gd_walk_grow_lists((long long)groups * g_walk.fixedCap);
     if (gd_has_error()) return;
     const gd_bf16x4* W = net ? gd_mlp_fwd_tiles(net) : nullptr;
     const float* Bv = net ? gd_mlp_bias(net) : nullptr;
-    k_walk_reset<<<1, 256>>>(g_walk.stats, g_walk.status, b->dinfo);
+    k_walk_reset<<<1, 256, 0, gd_stream()>>>(g_walk.stats, g_walk.status, b->dinfo);
     /* the list passes count interactions only when asked to (their per-group
        atomics are not free); the per-body walk always reports stack overflows */
     g_walk.kstats = stats ? g_walk.stats : nullptr;
@@ -1803,7 +1824,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     if (g_walkTiming) {
         if (!g_walkEv[0])
             for (int e = 0; e < 4; ++e) GD_HIP(hipEventCreate(&g_walkEv[e]));
-        GD_HIP(hipEventRecord(g_walkEv[0], nullptr));
+        GD_HIP(hipEventRecord(g_walkEv[0], gd_stream()));
     }
     int over[4] = {0, 0, 0, 0};
     unsigned long long requested = 0;
@@ -1821,7 +1842,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
 #undef GD_PASSES
     if (gd_has_error()) return;
     if (g_walkTiming) {
-        GD_HIP(hipEventRecord(g_walkEv[3], nullptr));
+        GD_HIP(hipEventRecord(g_walkEv[3], gd_stream()));
         g_walkTimed = 1;
     }
     /* room for what this walk asked for, for the next walk (groups that did
@@ -1891,7 +1912,7 @@ extern "C" int gd_walk_set_list_capacity(int entries, int secondPass) {
     g_walk.entries2 = secondPass > 0 ? secondPass : GD_LIST2;
     /* forget the list lengths and spread marks recorded so far */
     if (g_walk.sizes) {
-        k_walk_fill<<<gd_div_up(g_walk.groupCap, 256), 256>>>(g_walk.sizes, g_walk.groupCap, -1);
+        k_walk_fill<<<gd_div_up(g_walk.groupCap, 256), 256, 0, gd_stream()>>>(g_walk.sizes, g_walk.groupCap, -1);
         GD_LAUNCHED();
     }
     return prev;

@@ -160,6 +160,33 @@ class Simulation:
         check(self.lib)
         return bool(prev)
 
+    def capture_steps(self, dt, theta, steps=2):
+        """A HIP graph of `steps` engine steps (an even number: the graph must
+        leave the ping-pong buffers where it found them).  Walks must be
+        asynchronous (set_walk_async) and every buffer allocated by eager steps
+        first.  Returns the graph; replay() runs it, close_graph() frees it."""
+        if steps <= 0 or steps % 2:
+            raise ValueError("capture an even number of steps")
+        self.lib.gd_graph_begin()
+        check(self.lib)
+        try:
+            for _ in range(steps):
+                self.build_tree_async()
+                self.walk(theta)
+                self.integrate(dt)
+        finally:
+            graph = self.lib.gd_graph_end()
+        check(self.lib)
+        return graph
+
+    def replay(self, graph):
+        self.lib.gd_graph_launch(graph)
+        check(self.lib)
+
+    def close_graph(self, graph):
+        self.lib.gd_graph_destroy(graph)
+        check(self.lib)
+
     def poll(self):
         """Waits for the last asynchronous walk's status; raises GravityError
         for a deferred error (a tree that did not fit its node array)."""

@@ -44,7 +44,7 @@ void gd_tree_summary(const treeState* tree, int* totalNodes, int* overflow, int*
 
 /*
  * buildTree without any host synchronisation: every kernel of the build is
- * queued on the null stream, the level tables stay on the device and the
+ * queued on the library's stream, the level tables stay on the device and the
  * host fields of *tree (levelOffsets, levelCount, deepestLevel) are NOT
  * updated (gd_tree_levels fetches them).  posLo, when not NULL, is the low
  * part of hi/lo positions (float3 per body) and is permuted with pos/vel
@@ -54,7 +54,7 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
                          float3** spareLo);
 
 /* Time every later gd_build_tree_async / buildTree with HIP events on the
- * null stream (enable = 1) or stop (0).  gd_build_phase_ms fills ms[0..5]
+ * library's stream (enable = 1) or stop (0).  gd_build_phase_ms fills ms[0..5]
  * with the last build's bbox, keys, sort, reorder, topology (node counts,
  * level scan, node records) and centre-of-mass (all levels, with the
  * moments) times and returns 0, or -1 when nothing was timed (synchronises). */
@@ -154,7 +154,7 @@ int gd_walk_poll(void);
 float gd_walk_set_group_split(float ratio);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
- * null stream (enable = 1) or stop (0).  The walk is a traversal kernel that
+ * library's stream (enable = 1) or stop (0).  The walk is a traversal kernel that
  * writes each group's interaction list, an evaluation kernel (the dominant
  * one) and, for groups whose list did not fit, the per-body walk.
  * gd_walk_kernel_ms returns the last evaluation kernel's duration in ms;
@@ -263,8 +263,30 @@ void gd_diagnostics(const float4* pos, const float3* vel, const float* phi, int 
  * calibrating hardware byte counters. */
 void gd_stream_copy(void* dst, const void* src, size_t bytes);
 
+/* ---- HIP graphs ----------------------------------------------------------------
+ * Every launch of the library goes to one stream, created on first use with
+ * the default flags and priority.  gd_graph_begin opens a capture of that
+ * stream (relaxed mode); the library calls made until gd_graph_end become a
+ * graph, instantiated and returned (NULL on failure), and gd_graph_launch
+ * replays it on the same stream.  What can be captured: calls that neither
+ * synchronise nor allocate -- gd_build_tree_async, gd_walk / gd_walk_learned
+ * in the asynchronous mode without statistics or timing, the integrators.
+ * Kernels are captured with the device pointers of the moment: a graph of
+ * an even number of steps returns the ping-pong buffers (buildTree's pointer
+ * swap) to where they were, so it can be replayed step pair after step pair.
+ * While a capture is open or a graph lives, the library does not reallocate
+ * the buffers graphs address (walk lists, moment and walk arrays): a call
+ * that would needs fails instead, and the walk's first pass keeps its list
+ * pool (groups without room take the later passes).  A captured walk leaves
+ * its status on the device (no deferred report): check gd_tree_summary for an
+ * overflowed tree.  Destroy graphs with gd_graph_destroy. */
+void gd_graph_begin(void);
+void* gd_graph_end(void);
+void gd_graph_launch(void* graphExec);
+void gd_graph_destroy(void* graphExec);
+
 /* ---- timing ----------------------------------------------------------------
- * HIP events on the null stream, the stream every entry point launches on. */
+ * HIP events on the library's stream. */
 void* gd_event_create(void);
 void gd_event_record(void* ev);
 float gd_event_elapsed_ms(void* start, void* stop);   /* synchronises on stop */

@@ -23,7 +23,8 @@ radixSorter* sorterCreate(int maxCount);
  * ascending LSD radix sort of numCount (u64 key, u32 value) pairs from
  * keysIn/valsIn (left untouched) into keysOut/valsOut.  All pointers are
  * device pointers; keysOut/valsOut must not alias the inputs.  A NULL sorter
- * or numCount outside [1, capacity] is an error.  Runs on the null stream.
+ * or numCount outside [1, capacity] is an error.  Runs on the library's
+ * stream, which synchronises with the null stream.
  */
 void radixSortPairs(radixSorter* sorter, const uint64_t* keysIn, uint64_t* keysOut,
                     const uint32_t* valsIn, uint32_t* valsOut, int numCount);
@@ -34,7 +35,7 @@ void sorterDestroyer(radixSorter* s);
 /*
  * Launcher for the reference kernel reorderParticles (sorting.cu:99-107,
  * declared at sorting.h:12): posOut[i] = posIn[idx[i]], velOut[i] =
- * velIn[idx[i]] for i < starCount.  Device pointers, null stream.
+ * velIn[idx[i]] for i < starCount.  Device pointers, the library's stream.
  */
 void reorderParticles(const float4* posMassValsIn, const float3* velValsIn,
                       float4* posMassValsOut, float3* velValsOut,
