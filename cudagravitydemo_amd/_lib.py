@@ -107,6 +107,7 @@ _PROTOS = {
                                 C.POINTER(i32)]),
     "gd_build_tree_async": (None, [C.POINTER(TreeState), vpp, vpp, vpp, vpp]),
     "gd_tree_levels": (None, [C.POINTER(TreeState)]),
+    "gd_tree_root_fp64": (None, [C.POINTER(TreeState), C.POINTER(C.c_double)]),
     "gd_set_quadrupole": (None, [C.POINTER(TreeState), i32]),
     "gd_build_timing": (None, [i32]),
     "gd_build_phase_ms": (i32, [vp]),

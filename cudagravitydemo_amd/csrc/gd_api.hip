@@ -223,6 +223,7 @@ treeBuilder* sumCreate(int maxCount) {
     GD_HIP(hipHostMalloc(&b->hinfo, sizeof(gd_tree_info), hipHostMallocDefault));
     GD_HIP(hipMalloc(&b->bboxPartMin, sizeof(float3) * GD_BBOX_BLOCKS));
     GD_HIP(hipMalloc(&b->bboxPartMax, sizeof(float3) * GD_BBOX_BLOCKS));
+    GD_HIP(hipMalloc(&b->bboxPartCom, 4 * sizeof(double) * GD_BBOX_BLOCKS));
     GD_HIP(hipMalloc(&b->scanTiles, sizeof(uint32_t) * (size_t)gd_div_up(maxCount, GD_SCAN_TILE)));
     b->pub.tempStorage = b->scanTiles;
     b->pub.tempStorageBytes = sizeof(uint32_t) * (size_t)gd_div_up(maxCount, GD_SCAN_TILE);
@@ -263,6 +264,7 @@ void sumDestroyer(treeBuilder* sum) {
     GD_HIP(hipHostFree(b->hinfo));
     GD_HIP(hipFree(b->bboxPartMin));
     GD_HIP(hipFree(b->bboxPartMax));
+    GD_HIP(hipFree(b->bboxPartCom));
     GD_HIP(hipFree(b->scanTiles));
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
@@ -376,7 +378,8 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
     if (gd_has_error()) return;
     gd_build_mark(0);
     /* 1. bbox and quantisation box, on the device */
-    gd_launch_bbox(*pos, n, b, tree->boundingData[2], tree->boundingData[3]);
+    gd_launch_bbox(*pos, (posLo && *posLo) ? *posLo : nullptr, n, b, tree->boundingData[2],
+                   tree->boundingData[3]);
     gd_build_mark(1);
     /* 2. keys */
     gd_launch_morton_dev(*pos, n, b->dinfo, tree->mortonIn, tree->indexIn);
@@ -523,6 +526,15 @@ void buildTree(treeState* tree, float4** cudaPosMasVals, float3** cudaVelVals) {
     gd_build_tree_async(tree, cudaPosMasVals, cudaVelVals, nullptr, nullptr);
     if (gd_has_error()) return;
     gd_tree_levels(tree);
+}
+
+void gd_tree_root_fp64(const treeState* tree, double* out) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_tree_root_fp64: uninitialised tree state");
+        return;
+    }
+    gd_builder* b = gd_builder_of(tree->builder);
+    GD_HIP(hipMemcpy(out, b->dinfo->rootCom, 4 * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 void gd_tree_summary(const treeState* tree, int* totalNodes, int* overflow, int* zeroMassNodes) {

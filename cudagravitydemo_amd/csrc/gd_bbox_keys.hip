@@ -61,11 +61,30 @@ __device__ __forceinline__ gd_minmax gd_minmax_empty() {
     return m;
 }
 
-/* Per-block partial min/max over a grid-stride range of bodies. */
+/* FP64 mass and mass-weighted position of one body (hi + lo when lo is given). */
+__device__ __forceinline__ void gd_com_add(double* s, float4 p, const float3* __restrict__ lo,
+                                           int i) {
+    double x = p.x, y = p.y, z = p.z;
+    if (lo) {
+        const float3 l = lo[i];
+        x += l.x; y += l.y; z += l.z;
+    }
+    const double m = p.w;
+    s[0] += m;
+    s[1] = __builtin_fma(m, x, s[1]);
+    s[2] = __builtin_fma(m, y, s[2]);
+    s[3] = __builtin_fma(m, z, s[3]);
+}
+
+/* Per-block partial min/max over a grid-stride range of bodies and, when
+ * pcom is given, the block's FP64 sums m, m x, m y, m z (pcom[4 block ..]). */
 __global__ __launch_bounds__(GD_BBOX_THREADS) void k_bbox_partial(const float4* __restrict__ pos,
+                                                                  const float3* __restrict__ lo,
                                                                   int n, float3* pmin,
-                                                                  float3* pmax) {
+                                                                  float3* pmax, double* pcom) {
+    __shared__ double sCom[GD_BBOX_THREADS / 64][4];
     gd_minmax m = gd_minmax_empty();
+    double cs[4] = {0.0, 0.0, 0.0, 0.0};
     const int stride = gridDim.x * GD_BBOX_THREADS;
     int i = blockIdx.x * GD_BBOX_THREADS + threadIdx.x;
     /* two bodies per iteration keep two 16-byte loads in flight */
@@ -75,14 +94,34 @@ __global__ __launch_bounds__(GD_BBOX_THREADS) void k_bbox_partial(const float4* 
         m.lx = fminf(m.lx, fminf(a.x, b.x)); m.hx = fmaxf(m.hx, fmaxf(a.x, b.x));
         m.ly = fminf(m.ly, fminf(a.y, b.y)); m.hy = fmaxf(m.hy, fmaxf(a.y, b.y));
         m.lz = fminf(m.lz, fminf(a.z, b.z)); m.hz = fmaxf(m.hz, fmaxf(a.z, b.z));
+        if (pcom) {
+            gd_com_add(cs, a, lo, i);
+            gd_com_add(cs, b, lo, i + stride);
+        }
     }
     if (i < n) {
         const float4 a = pos[i];
         m.lx = fminf(m.lx, a.x); m.hx = fmaxf(m.hx, a.x);
         m.ly = fminf(m.ly, a.y); m.hy = fmaxf(m.hy, a.y);
         m.lz = fminf(m.lz, a.z); m.hz = fmaxf(m.hz, a.z);
+        if (pcom) gd_com_add(cs, a, lo, i);
     }
     gd_minmax_block<GD_BBOX_THREADS>(m);
+    if (pcom) {   /* block sum: wavefront butterflies, then the wavefronts in order */
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) cs[k] += __shfl_xor(cs[k], off);
+        }
+        if (gd_lane() == 0)
+            for (int k = 0; k < 4; ++k) sCom[threadIdx.x >> 6][k] = cs[k];
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double t = 0.0;
+            for (int w = 0; w < GD_BBOX_THREADS / 64; ++w) t += sCom[w][threadIdx.x];
+            pcom[4 * blockIdx.x + threadIdx.x] = t;
+        }
+    }
     if (threadIdx.x == 0) {
         pmin[blockIdx.x] = make_float3(m.lx, m.ly, m.lz);
         pmax[blockIdx.x] = make_float3(m.hx, m.hy, m.hz);
@@ -97,7 +136,24 @@ __global__ __launch_bounds__(GD_BBOX_THREADS) void k_bbox_partial(const float4* 
 __global__ __launch_bounds__(1024) void k_bbox_final(const float3* __restrict__ pmin,
                                                      const float3* __restrict__ pmax, int count,
                                                      float3* outMin, float3* outMax,
-                                                     gd_tree_info* info) {
+                                                     gd_tree_info* info,
+                                                     const double* __restrict__ pcom) {
+    if (pcom && info && threadIdx.x < 4) {   /* the blocks' FP64 sums, in block order */
+        double t = 0.0;
+        for (int b = 0; b < count; ++b) t += pcom[4 * b + threadIdx.x];
+        info->rootCom[threadIdx.x] = t;   /* m, m x, m y, m z: divided below */
+    }
+    __syncthreads();
+    if (pcom && info && threadIdx.x == 0) {
+        const double mt = info->rootCom[0];
+        const double cx = mt > 0.0 ? info->rootCom[1] / mt : 0.0;
+        const double cy = mt > 0.0 ? info->rootCom[2] / mt : 0.0;
+        const double cz = mt > 0.0 ? info->rootCom[3] / mt : 0.0;
+        info->rootCom[0] = cx;
+        info->rootCom[1] = cy;
+        info->rootCom[2] = cz;
+        info->rootCom[3] = mt;
+    }
     gd_minmax m = gd_minmax_empty();
     for (int i = threadIdx.x; i < count; i += 1024) {
         const float3 a = pmin[i], b = pmax[i];
@@ -124,14 +180,17 @@ __global__ __launch_bounds__(1024) void k_bbox_final(const float3* __restrict__ 
     }
 }
 
-void gd_launch_bbox(const float4* pos, int n, gd_builder* b, float3* outMin, float3* outMax) {
+void gd_launch_bbox(const float4* pos, const float3* lo, int n, gd_builder* b, float3* outMin,
+                    float3* outMax) {
     const int blocks = n > 0 ? (gd_div_up(n, 2 * GD_BBOX_THREADS) < GD_BBOX_BLOCKS
                                     ? gd_div_up(n, 2 * GD_BBOX_THREADS)
                                     : GD_BBOX_BLOCKS)
                              : 1;
-    k_bbox_partial<<<blocks, GD_BBOX_THREADS, 0, gd_stream()>>>(pos, n, b->bboxPartMin, b->bboxPartMax);
+    k_bbox_partial<<<blocks, GD_BBOX_THREADS, 0, gd_stream()>>>(pos, lo, n, b->bboxPartMin,
+                                                                b->bboxPartMax, b->bboxPartCom);
     GD_LAUNCHED();
-    k_bbox_final<<<1, 1024, 0, gd_stream()>>>(b->bboxPartMin, b->bboxPartMax, blocks, outMin, outMax, b->dinfo);
+    k_bbox_final<<<1, 1024, 0, gd_stream()>>>(b->bboxPartMin, b->bboxPartMax, blocks, outMin,
+                                              outMax, b->dinfo, b->bboxPartCom);
     GD_LAUNCHED();
 }
 
@@ -255,13 +314,15 @@ void mortonEncode(const float4* posMassVals, uint64_t* mortonCodes, uint32_t* or
 void localMinMaxFinder(const float4* posMassVals, unsigned int N, float3* minCorner,
                        float3* maxCorner) {
     /* BOXBLOCKS x BOXTHREADS like tree.h:16-17; each block writes one partial */
-    k_bbox_partial<<<256, GD_BBOX_THREADS, 0, gd_stream()>>>(posMassVals, (int)N, minCorner, maxCorner);
+    k_bbox_partial<<<256, GD_BBOX_THREADS, 0, gd_stream()>>>(posMassVals, nullptr, (int)N, minCorner,
+                                                               maxCorner, nullptr);
     GD_LAUNCHED();
 }
 
 void globalMinMaxReducer(const float3* minCorner, const float3* maxCorner, int count,
                          float3* outMin, float3* outMax) {
-    k_bbox_final<<<1, 1024, 0, gd_stream()>>>(minCorner, maxCorner, count, outMin, outMax, nullptr);
+    k_bbox_final<<<1, 1024, 0, gd_stream()>>>(minCorner, maxCorner, count, outMin, outMax, nullptr,
+                                              nullptr);
     GD_LAUNCHED();
 }
 

@@ -85,6 +85,8 @@ typedef struct {
     int zeroMassNodes;               /* nodes whose mass summed to zero */
     worldBox box;                    /* quantisation box of this step */
     float3 bmin, bmax;               /* global bbox corners */
+    double rootCom[4];               /* FP64 sums over all bodies: centre of mass x, y, z
+                                        (hi + lo positions) and mass */
 } gd_tree_info;
 
 /*
@@ -117,6 +119,7 @@ typedef struct {
     gd_tree_info* hinfo;             /* pinned host mirror */
     float3* bboxPartMin;             /* [GD_BBOX_BLOCKS] */
     float3* bboxPartMax;
+    double* bboxPartCom;             /* [GD_BBOX_BLOCKS][4]: FP64 m, m x, m y, m z */
     /* prefixSum scratch */
     uint32_t* scanTiles;             /* tile sums */
     /* per-node build records, maxNodes entries: first body, body count
@@ -141,7 +144,8 @@ static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }
 static inline gd_builder* gd_builder_of(treeBuilder* b) { return (gd_builder*)b; }
 
 /* ---- launchers shared between translation units (all on gd_stream()) */
-void gd_launch_bbox(const float4* pos, int n, gd_builder* b, float3* outMin, float3* outMax);
+void gd_launch_bbox(const float4* pos, const float3* lo, int n, gd_builder* b, float3* outMin,
+                    float3* outMax);
 void gd_launch_morton_dev(const float4* pos, int n, const gd_tree_info* info, uint64_t* keys,
                           uint32_t* idx);
 void gd_launch_reorder(const float4* posIn, const float3* velIn, float4* posOut, float3* velOut,

@@ -281,6 +281,14 @@ class Simulation:
         check(self.lib)
         return out
 
+    def root_fp64(self):
+        """The last build's root-level centre of mass and mass, summed in FP64
+        over all bodies (hi + lo): array (cx, cy, cz, M)."""
+        out = (C.c_double * 4)()
+        self.lib.gd_tree_root_fp64(C.byref(self.state), out)
+        check(self.lib)
+        return np.array(list(out))
+
     def levels(self):
         """Host copy of levelOffsets / levelCount / deepestLevel."""
         self.lib.gd_tree_levels(C.byref(self.state))

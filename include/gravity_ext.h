@@ -42,6 +42,12 @@ void gd_device_name(char* buf, int bufLen);
  * (synchronises). */
 void gd_tree_summary(const treeState* tree, int* totalNodes, int* overflow, int* zeroMassNodes);
 
+/* The root-level centre of mass in FP64: out[0..2] = sum m x / sum m over
+ * all bodies (hi + lo positions when the build had them), out[3] = sum m,
+ * summed in FP64 in a fixed order by the build's bounding-box pass (the
+ * node array keeps the reference's FP32 centre of mass).  Synchronises. */
+void gd_tree_root_fp64(const treeState* tree, double* out);
+
 /*
  * buildTree without any host synchronisation: every kernel of the build is
  * queued on the library's stream, the level tables stay on the device and the

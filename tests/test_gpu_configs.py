@@ -63,22 +63,27 @@ def test_precision_form_plummer_2e22_1000_steps():
             return d, med, p99, scale
 
         d0, med0, p990, pscale = measure()
+        com0 = sim.root_fp64()
         t0 = time.perf_counter()
         for _ in range(steps):
             sim.step(dt, theta)
         sim.sync()
         secs = time.perf_counter() - t0
         d1, med1, p991, _ = measure()
+        com1 = sim.root_fp64()
+        dcom = float(np.linalg.norm(com1[:3] - com0[:3]))
         dE = (d1["total"] - d0["total"]) / abs(d0["total"])
         dp = np.linalg.norm([d1[k] - d0[k] for k in ("px", "py", "pz")]) / pscale
         _say(f"precision form: E0={d0['total']:.9f} E1={d1['total']:.9f} dE/E={dE:.3e} "
              f"|dp|/sum m|v|={dp:.3e}; force error median/p99 {med0:.2e}/{p990:.2e} -> "
-             f"{med1:.2e}/{p991:.2e}; {1e3 * secs / steps:.2f} ms/step")
+             f"{med1:.2e}/{p991:.2e}; FP64 root centre of mass moved {dcom:.2e}; "
+             f"{1e3 * secs / steps:.2f} ms/step")
         # Plummer (G = M = a = 1): E = -3 pi / 64
         assert abs(d0["total"] + 3 * np.pi / 64) < 0.01
         assert abs(d0["mass"] - 1.0) < 1e-9
         assert abs(dE) < 1e-4
         assert dp < 1e-5
+        assert dcom < 1e-5 and abs(com1[3] - 1.0) < 1e-12
         assert med0 < BUDGET and med1 < BUDGET
 
 

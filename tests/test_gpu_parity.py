@@ -485,3 +485,23 @@ def test_quadrupole_moments_and_walk():
         sim.build_tree_async()
         sim.walk(0.0)
         assert rel_err(sim.accelerations()[sample], exact).max() < 1e-4
+
+
+def test_root_centre_of_mass_fp64():
+    """The build's FP64 root-level sums equal numpy's FP64 centre of mass over
+    hi + lo positions; the node array keeps the reference's FP32 value."""
+    pos, vel = ics.disk_merger(1 << 15, seed=3)
+    with Simulation(pos, vel, eps=0.005, hilo=True) as sim:
+        for _ in range(5):
+            sim.step(1e-3, 0.5)                 # the low parts become non-zero
+        sim.build_tree_async()
+        got = sim.root_fp64()
+        p = sim.positions().astype(np.float64)
+        lo = sim.positions_lo().astype(np.float64)
+        assert np.abs(lo).max() > 0
+        x, m = p[:, :3] + lo, p[:, 3]
+        want = np.r_[(m[:, None] * x).sum(0) / m.sum(), m.sum()]
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-13), (got, want)
+        nodes = sim.nodes_raw(1).view(ob.NODE_DTYPE).reshape(-1)
+        assert nodes[0]["mass"][3] == np.float32(nodes[0]["mass"][3])
+        assert abs(float(nodes[0]["mass"][3]) - want[3]) < 1e-6 * want[3]
