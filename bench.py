@@ -215,7 +215,7 @@ def stage_profile(walks, mode, theta, dt, n, hilo, reps=5):
     moments = mode != "plain"
     lo = 24 if hilo else 0
     byts = {
-        "bbox": 16 * n,
+        "bbox": (16 + (12 if hilo else 0)) * n,   # + the low parts for the FP64 root sums
         "keys": 28 * n,
         "sort": 200 * n,
         "reorder": (60 + lo) * n,

@@ -61,14 +61,9 @@ __device__ __forceinline__ gd_minmax gd_minmax_empty() {
     return m;
 }
 
-/* FP64 mass and mass-weighted position of one body (hi + lo when lo is given). */
-__device__ __forceinline__ void gd_com_add(double* s, float4 p, const float3* __restrict__ lo,
-                                           int i) {
-    double x = p.x, y = p.y, z = p.z;
-    if (lo) {
-        const float3 l = lo[i];
-        x += l.x; y += l.y; z += l.z;
-    }
+/* FP64 mass and mass-weighted position of one body, hi + lo parts. */
+__device__ __forceinline__ void gd_com_add(double* s, float4 p, float3 l) {
+    const double x = (double)p.x + l.x, y = (double)p.y + l.y, z = (double)p.z + l.z;
     const double m = p.w;
     s[0] += m;
     s[1] = __builtin_fma(m, x, s[1]);
@@ -76,8 +71,11 @@ __device__ __forceinline__ void gd_com_add(double* s, float4 p, const float3* __
     s[3] = __builtin_fma(m, z, s[3]);
 }
 
-/* Per-block partial min/max over a grid-stride range of bodies and, when
- * pcom is given, the block's FP64 sums m, m x, m y, m z (pcom[4 block ..]). */
+/* Per-block partial min/max over a grid-stride range of bodies and, with
+ * COM, the block's FP64 sums m, m x, m y, m z (pcom[4 block ..]) over hi + lo
+ * positions (LO: lo is read, else the low parts are zero).  Every load of an
+ * iteration is issued before any is used. */
+template <bool COM, bool LO>
 __global__ __launch_bounds__(GD_BBOX_THREADS) void k_bbox_partial(const float4* __restrict__ pos,
                                                                   const float3* __restrict__ lo,
                                                                   int n, float3* pmin,
@@ -85,29 +83,33 @@ __global__ __launch_bounds__(GD_BBOX_THREADS) void k_bbox_partial(const float4* 
     __shared__ double sCom[GD_BBOX_THREADS / 64][4];
     gd_minmax m = gd_minmax_empty();
     double cs[4] = {0.0, 0.0, 0.0, 0.0};
+    const float3 z3 = make_float3(0.f, 0.f, 0.f);
     const int stride = gridDim.x * GD_BBOX_THREADS;
     int i = blockIdx.x * GD_BBOX_THREADS + threadIdx.x;
     /* two bodies per iteration keep two 16-byte loads in flight */
     for (; i + stride < n; i += 2 * stride) {
         const float4 a = pos[i];
         const float4 b = pos[i + stride];
+        const float3 la = LO ? lo[i] : z3;
+        const float3 lb = LO ? lo[i + stride] : z3;
         m.lx = fminf(m.lx, fminf(a.x, b.x)); m.hx = fmaxf(m.hx, fmaxf(a.x, b.x));
         m.ly = fminf(m.ly, fminf(a.y, b.y)); m.hy = fmaxf(m.hy, fmaxf(a.y, b.y));
         m.lz = fminf(m.lz, fminf(a.z, b.z)); m.hz = fmaxf(m.hz, fmaxf(a.z, b.z));
-        if (pcom) {
-            gd_com_add(cs, a, lo, i);
-            gd_com_add(cs, b, lo, i + stride);
+        if (COM) {
+            gd_com_add(cs, a, la);
+            gd_com_add(cs, b, lb);
         }
     }
     if (i < n) {
         const float4 a = pos[i];
+        const float3 la = LO ? lo[i] : z3;
         m.lx = fminf(m.lx, a.x); m.hx = fmaxf(m.hx, a.x);
         m.ly = fminf(m.ly, a.y); m.hy = fmaxf(m.hy, a.y);
         m.lz = fminf(m.lz, a.z); m.hz = fmaxf(m.hz, a.z);
-        if (pcom) gd_com_add(cs, a, lo, i);
+        if (COM) gd_com_add(cs, a, la);
     }
     gd_minmax_block<GD_BBOX_THREADS>(m);
-    if (pcom) {   /* block sum: wavefront butterflies, then the wavefronts in order */
+    if (COM) {   /* block sum: wavefront butterflies, then the wavefronts in order */
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
 #pragma unroll
@@ -138,21 +140,25 @@ __global__ __launch_bounds__(1024) void k_bbox_final(const float3* __restrict__ 
                                                      float3* outMin, float3* outMax,
                                                      gd_tree_info* info,
                                                      const double* __restrict__ pcom) {
-    if (pcom && info && threadIdx.x < 4) {   /* the blocks' FP64 sums, in block order */
-        double t = 0.0;
-        for (int b = 0; b < count; ++b) t += pcom[4 * b + threadIdx.x];
-        info->rootCom[threadIdx.x] = t;   /* m, m x, m y, m z: divided below */
-    }
-    __syncthreads();
-    if (pcom && info && threadIdx.x == 0) {
-        const double mt = info->rootCom[0];
-        const double cx = mt > 0.0 ? info->rootCom[1] / mt : 0.0;
-        const double cy = mt > 0.0 ? info->rootCom[2] / mt : 0.0;
-        const double cz = mt > 0.0 ? info->rootCom[3] / mt : 0.0;
-        info->rootCom[0] = cx;
-        info->rootCom[1] = cy;
-        info->rootCom[2] = cz;
-        info->rootCom[3] = mt;
+    __shared__ double sSum[1024][4];
+    if (pcom && info) {   /* uniform: the blocks' FP64 sums by a fixed pairwise tree */
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sSum[t][k] = t < count ? pcom[4 * t + k] : 0.0;
+        __syncthreads();
+        for (int h = 512; h > 0; h >>= 1) {
+            if (t < h)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sSum[t][k] += sSum[t + h][k];
+            __syncthreads();
+        }
+        if (t == 0) {
+            const double mt = sSum[0][0];
+            info->rootCom[0] = mt > 0.0 ? sSum[0][1] / mt : 0.0;
+            info->rootCom[1] = mt > 0.0 ? sSum[0][2] / mt : 0.0;
+            info->rootCom[2] = mt > 0.0 ? sSum[0][3] / mt : 0.0;
+            info->rootCom[3] = mt;
+        }
     }
     gd_minmax m = gd_minmax_empty();
     for (int i = threadIdx.x; i < count; i += 1024) {
@@ -186,8 +192,12 @@ void gd_launch_bbox(const float4* pos, const float3* lo, int n, gd_builder* b, f
                                     ? gd_div_up(n, 2 * GD_BBOX_THREADS)
                                     : GD_BBOX_BLOCKS)
                              : 1;
-    k_bbox_partial<<<blocks, GD_BBOX_THREADS, 0, gd_stream()>>>(pos, lo, n, b->bboxPartMin,
-                                                                b->bboxPartMax, b->bboxPartCom);
+    if (lo)
+        k_bbox_partial<true, true><<<blocks, GD_BBOX_THREADS, 0, gd_stream()>>>(
+            pos, lo, n, b->bboxPartMin, b->bboxPartMax, b->bboxPartCom);
+    else
+        k_bbox_partial<true, false><<<blocks, GD_BBOX_THREADS, 0, gd_stream()>>>(
+            pos, nullptr, n, b->bboxPartMin, b->bboxPartMax, b->bboxPartCom);
     GD_LAUNCHED();
     k_bbox_final<<<1, 1024, 0, gd_stream()>>>(b->bboxPartMin, b->bboxPartMax, blocks, outMin,
                                               outMax, b->dinfo, b->bboxPartCom);
@@ -314,7 +324,7 @@ void mortonEncode(const float4* posMassVals, uint64_t* mortonCodes, uint32_t* or
 void localMinMaxFinder(const float4* posMassVals, unsigned int N, float3* minCorner,
                        float3* maxCorner) {
     /* BOXBLOCKS x BOXTHREADS like tree.h:16-17; each block writes one partial */
-    k_bbox_partial<<<256, GD_BBOX_THREADS, 0, gd_stream()>>>(posMassVals, nullptr, (int)N, minCorner,
+    k_bbox_partial<false, false><<<256, GD_BBOX_THREADS, 0, gd_stream()>>>(posMassVals, nullptr, (int)N, minCorner,
                                                                maxCorner, nullptr);
     GD_LAUNCHED();
 }

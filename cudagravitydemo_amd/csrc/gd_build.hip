@@ -405,10 +405,14 @@ __device__ __forceinline__ void gd_leaf_third(const float4* __restrict__ pos, in
 /*
  * An internal node of the engine build: centre of mass in the oracle's
  * operation order (masses of the children in slot order, then the weighted
- * positions), then its moments about it.  Every child's records are loaded
- * without a branch (an empty slot reads the first child again and its
- * contribution is selected away), so all loads are in flight together
- * instead of one child's dependent chain at a time.  Second moments shift
+ * positions), then its moments about it.  Every child's centre of mass and
+ * link are loaded without a branch (an empty slot reads the first child
+ * again and its contribution is selected away), so all loads are in flight
+ * together instead of one child's dependent chain at a time; the moment
+ * records are then loaded only for children that store them (the lanes'
+ * other loads are masked off, so they cost no memory traffic: single-body
+ * leaves, three quarters of the nodes, would otherwise add 80 B of reads
+ * each).  Second moments shift
  * each child's by the parallel-axis term m_c d d^T (d = c_child - c); third
  * moments: O += O_c + (S_c,ij d_k + S_c,ik d_j + S_c,jk d_i) + m_c d_i d_j d_k.
  * A single-body leaf child has zero moments, which are not stored.
@@ -424,11 +428,13 @@ __device__ __forceinline__ float4 gd_inner_node(const float4* __restrict__ wcom,
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 cc[8];
     int jj[8];
+    bool mom[8];   /* the child stores moments: not an empty slot, not a single-body leaf */
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         jj[k] = child[k] != -1 ? child[k] : fc;
         const float4 v = wcom[jj[k]];
         cc[k] = child[k] != -1 ? v : z4;
+        if (MOM != 0) mom[k] = child[k] != -1 && !gd_single_leaf(wlink, jj[k]);
     }
     float4 a = z4;
 #pragma unroll
@@ -447,14 +453,23 @@ __device__ __forceinline__ float4 gd_inner_node(const float4* __restrict__ wcom,
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int j = jj[k];
-        const bool use = child[k] != -1 && !gd_single_leaf(wlink, j);
-        const float4 la = wquad[2 * j], lb = wquad[2 * j + 1];
-        const float4 qa = use ? la : z4, qb = use ? lb : z4;
+        const bool use = mom[k];
+        /* only children that store moments are read (a single-body leaf's
+           moments are zero and were never written) */
+        float4 qa = z4, qb = z4;
+        if (use) {
+            qa = wquad[2 * j];
+            qb = wquad[2 * j + 1];
+        }
         const float w = cc[k].w;   /* 0 for an empty slot */
         const float dx = cc[k].x - c.x, dy = cc[k].y - c.y, dz = cc[k].z - c.z;
         if (MOM >= 2) {
-            const float4 lp = woct[3 * j], lq = woct[3 * j + 1], lr = woct[3 * j + 2];
-            const float4 p = use ? lp : z4, q = use ? lq : z4, r = use ? lr : z4;
+            float4 p = z4, q = z4, r = z4;
+            if (use) {
+                p = woct[3 * j];
+                q = woct[3 * j + 1];
+                r = woct[3 * j + 2];
+            }
             const float mx = w * dx, my = w * dy, mz = w * dz;
             xxx += p.x + 3.f * qa.x * dx + mx * dx * dx;
             yyy += p.y + 3.f * qa.y * dy + my * dy * dy;

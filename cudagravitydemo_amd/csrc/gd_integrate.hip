@@ -303,7 +303,7 @@ extern "C" void gd_force_errors(const float3* acc, const int* sample, int ns, co
 /* ---------------------------------------------------------------------------
  * FP64 diagnostics: kinetic energy, momentum, mass, potential energy.
  * Each block writes its six FP64 sums to part[block]; k_diag_sum adds the
- * blocks in block order, so the result does not depend on scheduling.
+ * blocks by a fixed pairwise tree, so the result does not depend on scheduling.
  * ------------------------------------------------------------------------- */
 #define GD_DIAG_BLOCKS 1024
 
@@ -345,12 +345,21 @@ __global__ __launch_bounds__(256) void k_diag(const float4* __restrict__ pos,
     }
 }
 
-__global__ void k_diag_sum(const double* __restrict__ part, int blocks, double* __restrict__ out) {
-    const int k = threadIdx.x;
-    if (k >= 6) return;
-    double t = 0.0;
-    for (int b = 0; b < blocks; ++b) t += part[b * 6 + k];
-    out[k] = t;
+/* The blocks' partials by a fixed pairwise tree (1024 threads, one block each). */
+__global__ __launch_bounds__(1024) void k_diag_sum(const double* __restrict__ part, int blocks,
+                                                   double* __restrict__ out) {
+    __shared__ double sSum[GD_DIAG_BLOCKS][6];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sSum[t][k] = t < blocks ? part[t * 6 + k] : 0.0;
+    __syncthreads();
+    for (int h = GD_DIAG_BLOCKS / 2; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sSum[t][k] += sSum[t + h][k];
+        __syncthreads();
+    }
+    if (t < 6) out[t] = sSum[0][t];
 }
 
 static double* g_diagPart = nullptr;   /* GD_DIAG_BLOCKS x 6 partials, then the 6 sums */
@@ -365,7 +374,8 @@ extern "C" void gd_diagnostics(const float4* pos, const float3* vel, const float
         if (blocks > GD_DIAG_BLOCKS) blocks = GD_DIAG_BLOCKS;
         k_diag<<<blocks, 256, 0, gd_stream()>>>(pos, vel, phi, starCount, g_diagPart);
         GD_LAUNCHED();
-        k_diag_sum<<<1, 64, 0, gd_stream()>>>(g_diagPart, blocks, g_diagPart + GD_DIAG_BLOCKS * 6);
+        k_diag_sum<<<1, GD_DIAG_BLOCKS, 0, gd_stream()>>>(g_diagPart, blocks,
+                                                          g_diagPart + GD_DIAG_BLOCKS * 6);
         GD_LAUNCHED();
         GD_HIP(hipMemcpy(h, g_diagPart + GD_DIAG_BLOCKS * 6, sizeof(h), hipMemcpyDeviceToHost));
     }
