@@ -109,6 +109,8 @@ _PROTOS = {
     "gd_tree_levels": (None, [C.POINTER(TreeState)]),
     "gd_tree_root_fp64": (None, [C.POINTER(TreeState), C.POINTER(C.c_double)]),
     "gd_set_quadrupole": (None, [C.POINTER(TreeState), i32]),
+    "gd_set_node_records": (None, [C.POINTER(TreeState), i32]),
+    "gd_write_node_records": (None, [C.POINTER(TreeState)]),
     "gd_build_timing": (None, [i32]),
     "gd_build_phase_ms": (i32, [vp]),
     "gd_tree_walk_arrays": (None, [C.POINTER(TreeState), vp, vp, vp, i32]),

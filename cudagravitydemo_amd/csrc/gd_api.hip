@@ -361,8 +361,8 @@ int gd_build_phase_ms(float* ms) {
     return 0;
 }
 
-void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
-                         float3** spareLo) {
+static void gd_build_tree(treeState* tree, float4** pos, float3** vel, float3** posLo,
+                          float3** spareLo, int records) {
     if (!tree || !tree->sorter || !tree->builder || !pos || !vel) {
         GD_FAIL("buildTree: uninitialised tree state");
         return;
@@ -408,10 +408,37 @@ void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** p
     gd_build_mark(4);
     /* 5. topology and centre of mass */
     gd_build_mid_event = g_buildTiming ? g_buildEv[5] : nullptr;
-    gd_build_octree(b, tree->mortonOut, *pos, n, tree->nodes, tree->maxNodes);
+    gd_build_octree(b, tree->mortonOut, *pos, n, tree->nodes, tree->maxNodes, records);
     gd_build_mid_event = nullptr;
     gd_build_mark(6);
     if (g_buildTiming) g_buildTimed = 1;
+}
+
+void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
+                         float3** spareLo) {
+    const int records = tree && tree->builder ? !gd_builder_of(tree->builder)->noRecords : 1;
+    gd_build_tree(tree, pos, vel, posLo, spareLo, records);
+}
+
+void gd_set_node_records(treeState* tree, int enable) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_set_node_records: uninitialised tree state");
+        return;
+    }
+    gd_builder_of(tree->builder)->noRecords = enable ? 0 : 1;
+}
+
+void gd_write_node_records(treeState* tree) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_write_node_records: uninitialised tree state");
+        return;
+    }
+    gd_builder* b = gd_builder_of(tree->builder);
+    if (b->lastBodies <= 0) {
+        GD_FAIL("gd_write_node_records: no tree has been built");
+        return;
+    }
+    gd_build_node_records(b, tree->mortonOut, tree->nodes, tree->maxNodes);
 }
 
 void gd_set_quadrupole(treeState* tree, int enable) {
@@ -523,7 +550,7 @@ void gd_tree_levels(treeState* tree) {
 }
 
 void buildTree(treeState* tree, float4** cudaPosMasVals, float3** cudaVelVals) {
-    gd_build_tree_async(tree, cudaPosMasVals, cudaVelVals, nullptr, nullptr);
+    gd_build_tree(tree, cudaPosMasVals, cudaVelVals, nullptr, nullptr, 1);
     if (gd_has_error()) return;
     gd_tree_levels(tree);
 }

@@ -27,11 +27,15 @@
  *                   body, count with the node's octant in bits 29-31, parent,
  *                   first child).
  *   k_build_mass    one launch per level, deepest first (tree.cu:89-179, same
- *                   operation order as the oracle): centre of mass, moments,
- *                   the compact walk record (centre of mass, first child /
- *                   body, count, level) and the whole 80-byte node record,
- *                   child[] included (tree.cu:232-233: a child sits in the slot
- *                   of its octant)
+ *                   operation order as the oracle): one thread per node of the
+ *                   level below, leaves finished from their bodies, each
+ *                   internal node's centre of mass and moments summed from its
+ *                   children's in LDS
+ *   k_node_records  the whole 80-byte node record, child[] included
+ *                   (tree.cu:232-233: a child sits in the slot of its octant),
+ *                   from the meta and the walk arrays; a build runs it unless
+ *                   the caller asked for the compact walk arrays only (the
+ *                   engine's walk reads nothing else)
  * Level tables live in gd_tree_info on the device.
  */
 #include "gd_internal.h"
@@ -346,16 +350,6 @@ __device__ __forceinline__ gd_mass_result gd_node_mass(ChildCom childCom,
     return r;
 }
 
-/* A single-body leaf's moments about its centre of mass are zero and are
- * not stored: no walk reads a leaf's moments (leaves are monopoles), and a
- * parent that finds such a child (its link: a leaf of count 1) takes them as
- * zero.  Three quarters of the nodes are such leaves, so this saves most of
- * the moment traffic of the build (80 B written and read back per leaf). */
-__device__ __forceinline__ bool gd_single_leaf(const int2* __restrict__ wlink, int j) {
-    const int2 lk = wlink[j];
-    return lk.x < 0 && (lk.y >> 5) == 1;
-}
-
 /* Second moments of a leaf about its centre of mass c, S = sum m (x - c)(x - c)^T,
  * as two float4: (Sxx, Syy, Szz, Sxy), (Sxz, Syz, trace, 0). */
 __device__ __forceinline__ void gd_leaf_moments(const float4* __restrict__ pos, int first,
@@ -403,313 +397,319 @@ __device__ __forceinline__ void gd_leaf_third(const float4* __restrict__ pos, in
 }
 
 /*
- * An internal node of the engine build: centre of mass in the oracle's
- * operation order (masses of the children in slot order, then the weighted
- * positions), then its moments about it.  Every child's centre of mass and
- * link are loaded without a branch (an empty slot reads the first child
- * again and its contribution is selected away), so all loads are in flight
- * together instead of one child's dependent chain at a time; the moment
- * records are then loaded only for children that store them (the lanes'
- * other loads are masked off, so they cost no memory traffic: single-body
- * leaves, three quarters of the nodes, would otherwise add 80 B of reads
- * each).  Second moments shift
- * each child's by the parallel-axis term m_c d d^T (d = c_child - c); third
- * moments: O += O_c + (S_c,ij d_k + S_c,ik d_j + S_c,jk d_i) + m_c d_i d_j d_k.
- * A single-body leaf child has zero moments, which are not stored.
+ * Centre of mass and moments: one launch per level L, deepest first, over
+ * the nodes of level L + 1 (the children of level L's internal nodes).
+ *
+ * A block takes a tile of 256 consecutive children, one per thread, and
+ * finishes the parents whose first child lies in the tile's first 248 (a
+ * parent has at most 8 children, so all of theirs are in the tile; the next
+ * tile starts 248 further on).  Children of one parent are consecutive and
+ * in octant order, so a parent is a run of equal parent indices.
+ *   1. each thread loads its child: a leaf's centre of mass and moments are
+ *      computed from its bodies (tree.cu:105-125) and written out, an
+ *      internal child's are loaded (its level's launch wrote them); every
+ *      load of the tile is independent and coalesced.
+ *   2. the first thread of each run sums its children's masses, then the
+ *      weighted positions, in octant order (tree.cu:127-179) -> the
+ *      parent's centre of mass c, handed to each child of the run.
+ *   3. each child thread forms its terms of the parent's moments about c
+ *      (parallel-axis shifts: S += S_c + m d d^T, O += O_c + (S_c,ij d_k +
+ *      S_c,ik d_j + S_c,jk d_i) + m d_i d_j d_k, d = c_child - c), and the
+ *      run's first thread adds them in octant order.
+ * Every sum is formed term by term in the oracle's order; an octant without
+ * a child adds exactly +0 in the oracle and no partial sum is ever -0 (each
+ * starts at +0 and adds products or fmaf results whose addend is +0 or a sum
+ * of the same kind), so leaving the empty octants out changes no bit.
+ * A single-body leaf has zero moments, which are not stored.
+ *
+ * MOM: 0 centre of mass only, 1 also second moments, 2 also third moments.
+ * Input: the per-node meta of k_build_nodes (first body, count, parent,
+ * first child or -1).  Output: the walk arrays of levels L + 1 (leaves) and
+ * L (internal nodes).  L = -1 finishes a root that is a leaf (one body).
  */
+#define GD_MASS_TILE 256
+#define GD_MASS_STEP (GD_MASS_TILE - 8)    /* first children per tile */
+#define GD_MASS_TERMS (MOM == 2 ? 16 : (MOM == 1 ? 6 : 1))
+
 template <int MOM>
-__device__ __forceinline__ float4 gd_inner_node(const float4* __restrict__ wcom,
-                                                const int2* __restrict__ wlink,
-                                                const float4* __restrict__ wquad,
-                                                const float4* __restrict__ woct,
-                                                const int* child, int fc, float4* q0,
-                                                float4* q1, float4* o0, float4* o1,
-                                                float4* o2) {
+__global__ __launch_bounds__(GD_MASS_TILE) void k_build_mass(const float4* __restrict__ pos,
+                                                             const int4* __restrict__ meta,
+                                                             gd_tree_info* info, int level,
+                                                             float4* __restrict__ wcom,
+                                                             int2* __restrict__ wlink,
+                                                             float4* __restrict__ wquad,
+                                                             float4* __restrict__ woct) {
+    __shared__ int sPar[GD_MASS_TILE + 1];          /* parent of tile child t - 1 */
+    __shared__ float4 sCom[GD_MASS_TILE];           /* children's centres of mass */
+    __shared__ float4 sC[GD_MASS_TILE];             /* their parent's */
+    __shared__ float sTerm[GD_MASS_TERMS][GD_MASS_TILE];
+    if (info->overflow) return;
+    const int cl = level + 1;
+    const int cnt = info->levelCount[cl];
+    const int coff = info->levelOffsets[cl];
+    const int tid = (int)threadIdx.x;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 cc[8];
-    int jj[8];
-    bool mom[8];   /* the child stores moments: not an empty slot, not a single-body leaf */
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        jj[k] = child[k] != -1 ? child[k] : fc;
-        const float4 v = wcom[jj[k]];
-        cc[k] = child[k] != -1 ? v : z4;
-        if (MOM != 0) mom[k] = child[k] != -1 && !gd_single_leaf(wlink, jj[k]);
+    int zeros = 0;
+    const long long stride = (long long)gridDim.x * GD_MASS_STEP;
+    /* the tile's meta is loaded one tile ahead, so that its latency overlaps
+       the previous tile's loads and LDS phases */
+    long long t0 = (long long)blockIdx.x * GD_MASS_STEP;
+    int4 mtNext = make_int4(0, 0, -2, -1);
+    int prevNext = -3;
+    if (t0 + tid < cnt) {
+        mtNext = meta[coff + (int)(t0 + tid)];
+        if (tid == 0 && t0 > 0) prevNext = meta[coff + (int)t0 - 1].z;
     }
-    float4 a = z4;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) a.w += cc[k].w;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        a.x = __builtin_fmaf(cc[k].x, cc[k].w, a.x);
-        a.y = __builtin_fmaf(cc[k].y, cc[k].w, a.y);
-        a.z = __builtin_fmaf(cc[k].z, cc[k].w, a.z);
-    }
-    const float4 c = a.w > 0.f ? make_float4(a.x / a.w, a.y / a.w, a.z / a.w, a.w) : z4;
-    if (MOM == 0) return c;
-    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
-    float xxx = 0.f, yyy = 0.f, zzz = 0.f, xxy = 0.f, xxz = 0.f;
-    float xyy = 0.f, yyz = 0.f, xzz = 0.f, yzz = 0.f, xyz = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int j = jj[k];
-        const bool use = mom[k];
-        /* only children that store moments are read (a single-body leaf's
-           moments are zero and were never written) */
-        float4 qa = z4, qb = z4;
-        if (use) {
-            qa = wquad[2 * j];
-            qb = wquad[2 * j + 1];
-        }
-        const float w = cc[k].w;   /* 0 for an empty slot */
-        const float dx = cc[k].x - c.x, dy = cc[k].y - c.y, dz = cc[k].z - c.z;
-        if (MOM >= 2) {
-            float4 p = z4, q = z4, r = z4;
-            if (use) {
-                p = woct[3 * j];
-                q = woct[3 * j + 1];
-                r = woct[3 * j + 2];
+    for (; t0 < cnt; t0 += stride) {
+        const long long i = t0 + tid;
+        const bool valid = i < cnt;
+        const int j = coff + (int)i;
+        const int4 mt = mtNext;
+        const int prev = prevNext;
+        {
+            const long long in = i + stride;
+            mtNext = make_int4(0, 0, -2, -1);
+            prevNext = -3;
+            if (in < cnt) {
+                mtNext = meta[coff + (int)in];
+                if (tid == 0) prevNext = meta[coff + (int)in - 1].z;
             }
-            const float mx = w * dx, my = w * dy, mz = w * dz;
-            xxx += p.x + 3.f * qa.x * dx + mx * dx * dx;
-            yyy += p.y + 3.f * qa.y * dy + my * dy * dy;
-            zzz += p.z + 3.f * qa.z * dz + mz * dz * dz;
-            xxy += p.w + qa.x * dy + 2.f * qa.w * dx + mx * dx * dy;
-            xxz += q.x + qa.x * dz + 2.f * qb.x * dx + mx * dx * dz;
-            xyy += q.y + qa.y * dx + 2.f * qa.w * dy + mx * dy * dy;
-            yyz += q.z + qa.y * dz + 2.f * qb.y * dy + my * dy * dz;
-            xzz += q.w + qa.z * dx + 2.f * qb.x * dz + mx * dz * dz;
-            yzz += r.x + qa.z * dy + 2.f * qb.y * dz + my * dz * dz;
-            xyz += r.y + qa.w * dz + qb.x * dy + qb.y * dx + mx * dy * dz;
         }
-        sxx += __builtin_fmaf(w * dx, dx, qa.x);
-        syy += __builtin_fmaf(w * dy, dy, qa.y);
-        szz += __builtin_fmaf(w * dz, dz, qa.z);
-        sxy += __builtin_fmaf(w * dx, dy, qa.w);
-        sxz += __builtin_fmaf(w * dx, dz, qb.x);
-        syz += __builtin_fmaf(w * dy, dz, qb.y);
-    }
-    *q0 = make_float4(sxx, syy, szz, sxy);
-    *q1 = make_float4(sxz, syz, sxx + syy + szz, 0.f);
-    if (MOM >= 2) {
-        *o0 = make_float4(xxx, yyy, zzz, xxy);
-        *o1 = make_float4(xxz, xyy, yyz, xzz);
-        *o2 = make_float4(yzz, xyz, 0.f, 0.f);
-    }
-    return c;
-}
-
-/* One finished node record (types.h:52-62), written whole: five 16-byte
- * stores, so consecutive nodes fill whole cache lines. */
-__device__ __forceinline__ void gd_write_record(node* __restrict__ nodes, int idx, uint64_t path,
-                                                int parent, int first, int count, const int* child,
-                                                int level, float4 com) {
-    int4* r = reinterpret_cast<int4*>(nodes + idx);
-    r[0] = make_int4((int)(uint32_t)path, (int)(uint32_t)(path >> 32), parent, first);
-    r[1] = make_int4(count, child[0], child[1], child[2]);
-    r[2] = make_int4(child[3], child[4], child[5], child[6]);
-    r[3] = make_int4(child[7], level, 0, 0);
-    r[4] = make_int4(__float_as_int(com.x), __float_as_int(com.y), __float_as_int(com.z),
-                     __float_as_int(com.w));
-}
-
-/* One internal node at level `level` (shift = 63 - 3 level): its children
- * are the consecutive nodes of the next level from its first child on whose
- * parent it is (at most 8, before childEnd), in octant order; a child's
- * octant (3 bits of its first body's key) rides in the top bits of its meta
- * count, so no child's key is read.  Writes the walk arrays and the
- * whole node record; returns 1 for a massless node. */
-template <int MOM>
-__device__ __forceinline__ int gd_finish_inner(node* __restrict__ nodes,
-                                               const uint64_t* __restrict__ keys,
-                                               const int4* __restrict__ meta, int idx, int level,
-                                               int childEnd, float4* __restrict__ wcom,
-                                               int2* __restrict__ wlink,
-                                               float4* __restrict__ wquad,
-                                               float4* __restrict__ woct) {
-    const int shift = 63 - 3 * level;
-    const int4 mt = meta[idx];
-    const int fc = mt.w;
-    const uint64_t key = keys[mt.x];
-    int kid[8], oct[8];
-    kid[0] = fc;
-    oct[0] = (int)((key >> (shift - 3)) & 7u);
-    bool run = true;
+        int par = -2;
+        float4 ck = z4, qa = z4, qb = z4, p = z4, q = z4, r = z4;
+        if (valid) {
+            par = mt.z;
+            if (mt.w < 0) {   /* a leaf: from its bodies */
+                const int first = mt.x, count = mt.y & 0x1fffffff;
+                float sx = 0.f, sy = 0.f, sz = 0.f, m = 0.f;
+                for (int b = 0; b < count; ++b) {
+                    const float4 x = pos[first + b];
+                    sx = __builtin_fmaf(x.x, x.w, sx);
+                    sy = __builtin_fmaf(x.y, x.w, sy);
+                    sz = __builtin_fmaf(x.z, x.w, sz);
+                    m += x.w;
+                }
+                if (m > 0.f) ck = make_float4(sx / m, sy / m, sz / m, m);
+                /* the tile's last 8 children are the next tile's first 8:
+                   only one of the two writes them */
+                const bool own = tid < GD_MASS_STEP;
+                if (own) {
+                    zeros += m > 0.f ? 0 : 1;
+                    wcom[j] = ck;
+                    wlink[j] = make_int2(-(first + 1), (count << 5) | cl);
+                }
+                if (count > 1) {   /* moments of a single-body leaf are zero, not stored */
+                    if (MOM >= 1) {
+                        gd_leaf_moments(pos, first, count, ck, &qa, &qb);
+                        if (own) {
+                            wquad[2 * j] = qa;
+                            wquad[2 * j + 1] = qb;
+                        }
+                    }
+                    if (MOM >= 2) {
+                        gd_leaf_third(pos, first, count, ck, &p, &q, &r);
+                        if (own) {
+                            woct[3 * j] = p;
+                            woct[3 * j + 1] = q;
+                            woct[3 * j + 2] = r;
+                        }
+                    }
+                }
+            } else {          /* internal: written by the previous launch */
+                ck = wcom[j];
+                if (MOM >= 1) {
+                    qa = wquad[2 * j];
+                    qb = wquad[2 * j + 1];
+                }
+                if (MOM >= 2) {
+                    p = woct[3 * j];
+                    q = woct[3 * j + 1];
+                    r = woct[3 * j + 2];
+                }
+            }
+        }
+        sPar[tid + 1] = par;
+        if (tid == 0) sPar[0] = prev;
+        sCom[tid] = ck;
+        __syncthreads();
+        /* 2. the first child of each run (a parent; none for the root) */
+        const bool head = valid && par >= 0 && tid < GD_MASS_STEP && sPar[tid] != par;
+        int nc = 0;
+        float4 c = z4;
+        if (head) {
+            nc = 1;
+            while (nc < 8 && sPar[tid + nc + 1] == par) ++nc;
+            float4 a = z4;
+            for (int k = 0; k < nc; ++k) a.w += sCom[tid + k].w;
+            for (int k = 0; k < nc; ++k) {
+                const float4 v = sCom[tid + k];
+                a.x = __builtin_fmaf(v.x, v.w, a.x);
+                a.y = __builtin_fmaf(v.y, v.w, a.y);
+                a.z = __builtin_fmaf(v.z, v.w, a.z);
+            }
+            c = a.w > 0.f ? make_float4(a.x / a.w, a.y / a.w, a.z / a.w, a.w) : z4;
+            for (int k = 0; k < nc; ++k) sC[tid + k] = c;
+        }
+        if (MOM >= 1) {
+            __syncthreads();
+            /* 3. each child's terms about its parent's centre of mass (a
+               child whose run is finished by another tile writes unused ones) */
+            {
+                const float4 cp = sC[tid];
+                const float w = ck.w;
+                const float dx = ck.x - cp.x, dy = ck.y - cp.y, dz = ck.z - cp.z;
+                sTerm[0][tid] = __builtin_fmaf(w * dx, dx, qa.x);
+                sTerm[1][tid] = __builtin_fmaf(w * dy, dy, qa.y);
+                sTerm[2][tid] = __builtin_fmaf(w * dz, dz, qa.z);
+                sTerm[3][tid] = __builtin_fmaf(w * dx, dy, qa.w);
+                sTerm[4][tid] = __builtin_fmaf(w * dx, dz, qb.x);
+                sTerm[5][tid] = __builtin_fmaf(w * dy, dz, qb.y);
+                if (MOM >= 2) {
+                    const float mx = w * dx, my = w * dy, mz = w * dz;
+                    sTerm[6 % GD_MASS_TERMS][tid] = p.x + 3.f * qa.x * dx + mx * dx * dx;
+                    sTerm[7 % GD_MASS_TERMS][tid] = p.y + 3.f * qa.y * dy + my * dy * dy;
+                    sTerm[8 % GD_MASS_TERMS][tid] = p.z + 3.f * qa.z * dz + mz * dz * dz;
+                    sTerm[9 % GD_MASS_TERMS][tid] = p.w + qa.x * dy + 2.f * qa.w * dx + mx * dx * dy;
+                    sTerm[10 % GD_MASS_TERMS][tid] = q.x + qa.x * dz + 2.f * qb.x * dx + mx * dx * dz;
+                    sTerm[11 % GD_MASS_TERMS][tid] = q.y + qa.y * dx + 2.f * qa.w * dy + mx * dy * dy;
+                    sTerm[12 % GD_MASS_TERMS][tid] = q.z + qa.y * dz + 2.f * qb.y * dy + my * dy * dz;
+                    sTerm[13 % GD_MASS_TERMS][tid] = q.w + qa.z * dx + 2.f * qb.x * dz + mx * dz * dz;
+                    sTerm[14 % GD_MASS_TERMS][tid] = r.x + qa.z * dy + 2.f * qb.y * dz + my * dz * dz;
+                    sTerm[15 % GD_MASS_TERMS][tid] =
+                        r.y + qa.w * dz + qb.x * dy + qb.y * dx + mx * dy * dz;
+                }
+            }
+            __syncthreads();
+        }
+        if (head) {
+            const int P = par;
+            wcom[P] = c;
+            wlink[P] = make_int2(j, (nc << 5) | level);
+            zeros += c.w > 0.f ? 0 : 1;
+            if (MOM >= 1) {
+                float sum[GD_MASS_TERMS];
 #pragma unroll
-    for (int k = 1; k < 8; ++k) {
-        const int j = fc + k;
-        const int4 mk = j < childEnd ? meta[j] : make_int4(0, 0, -1, -1);
-        run = run && mk.z == idx;
-        kid[k] = run ? j : -1;
-        oct[k] = run ? (int)(((unsigned)mk.y >> 29) & 7u) : 8;
-    }
-    int nc = 0;
-    int child[8];
+                for (int e = 0; e < GD_MASS_TERMS; ++e) sum[e] = 0.f;
+                for (int k = 0; k < nc; ++k)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        nc += kid[k] >= 0 ? 1 : 0;
-        child[k] = -1;
+                    for (int e = 0; e < GD_MASS_TERMS; ++e) sum[e] += sTerm[e][tid + k];
+                wquad[2 * P] = make_float4(sum[0], sum[1], sum[2], sum[3]);
+                wquad[2 * P + 1] = make_float4(sum[4], sum[5], sum[0] + sum[1] + sum[2], 0.f);
+                if (MOM >= 2) {
+                    woct[3 * P] = make_float4(sum[6 % GD_MASS_TERMS], sum[7 % GD_MASS_TERMS],
+                                              sum[8 % GD_MASS_TERMS], sum[9 % GD_MASS_TERMS]);
+                    woct[3 * P + 1] = make_float4(sum[10 % GD_MASS_TERMS], sum[11 % GD_MASS_TERMS],
+                                                  sum[12 % GD_MASS_TERMS], sum[13 % GD_MASS_TERMS]);
+                    woct[3 * P + 2] =
+                        make_float4(sum[14 % GD_MASS_TERMS], sum[15 % GD_MASS_TERMS], 0.f, 0.f);
+                }
+            }
+        }
+        __syncthreads();   /* the next tile reuses the LDS */
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int o = 0; o < 8; ++o)
-            if (oct[k] == o) child[o] = kid[k];
-    float4 q0, q1, o0, o1, o2;
-    const float4 c =
-        gd_inner_node<MOM>(wcom, wlink, wquad, woct, kid, fc, &q0, &q1, &o0, &o1, &o2);
-    gd_write_record(nodes, idx, key >> shift, mt.z, mt.x, mt.y & 0x1fffffff, child, level, c);
-    wcom[idx] = c;
-    wlink[idx] = make_int2(fc, (nc << 5) | level);
-    if (MOM >= 1) {
-        wquad[2 * idx] = q0;
-        wquad[2 * idx + 1] = q1;
-    }
-    if (MOM >= 2) {
-        woct[3 * idx] = o0;
-        woct[3 * idx + 1] = o1;
-        woct[3 * idx + 2] = o2;
-    }
-    return c.w > 0.f ? 0 : 1;
+    if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
 }
 
 /*
- * Engine centre of mass, moments and node records for one level (launched
- * deepest first).  MOM: 0 centre of mass only, 1 also second moments, 2 also
- * third moments.  Input: the per-node meta of k_build_nodes (first body,
- * count, parent, first child or -1).  Output: the walk arrays and the whole
- * 80-byte node record (sRec: 64 records of LDS per wavefront, for the leaves'
- * coalesced stores).  Each wavefront scans 64 nodes at a time: a lane whose
- * node is a leaf finishes it at once (tree.cu:105-125; three quarters of all
- * nodes are single-body leaves, which store no moments), the internal nodes
- * (tree.cu:127-179) are queued in the wavefront's LDS and finished 64 at a
- * time with all lanes, so the long internal path runs in full wavefronts and
- * no block-wide barrier holds a wavefront back.
+ * The 80-byte node records (types.h:52-62) of every node, from the meta and
+ * the walk arrays: path = first body's key >> (63 - 3 level), parent, first
+ * body, count, child[] by octant (a child's octant is 3 bits of its first
+ * body's key; the first child's comes from the node's own key, the others'
+ * ride in the top bits of their meta count), level, centre of mass.  Each
+ * wavefront stages 64 consecutive records in LDS and stores them as 16-byte
+ * pieces in address order (a store covers 1 KB of consecutive records).
  */
-template <int MOM>
-__global__ __launch_bounds__(256) void k_build_mass(node* __restrict__ nodes,
-                                                    const float4* __restrict__ pos,
-                                                    const uint64_t* __restrict__ keys,
-                                                    const int4* __restrict__ meta,
-                                                    gd_tree_info* info, int level,
-                                                    float4* __restrict__ wcom,
-                                                    int2* __restrict__ wlink,
-                                                    float4* __restrict__ wquad,
-                                                    float4* __restrict__ woct) {
-    __shared__ int sQueueAll[4][128];
+__global__ __launch_bounds__(256) void k_node_records(node* __restrict__ nodes,
+                                                      const uint64_t* __restrict__ keys,
+                                                      const int4* __restrict__ meta,
+                                                      const gd_tree_info* __restrict__ info,
+                                                      const float4* __restrict__ wcom,
+                                                      const int2* __restrict__ wlink) {
     __shared__ int4 sRecAll[4][64 * 5];
+    __shared__ int sOff[GD_LEVELS + 1];
     if (info->overflow) return;
-    const int cnt = info->levelCount[level];
-    const int off = info->levelOffsets[level];
-    const int childEnd =
-        level < GD_MAX_LEVEL ? info->levelOffsets[level + 1] + info->levelCount[level + 1] : 0;
-    const int shift = 63 - 3 * level;
+    if (threadIdx.x < GD_LEVELS) sOff[threadIdx.x] = info->levelOffsets[threadIdx.x];
+    if (threadIdx.x == 0) sOff[GD_LEVELS] = info->totalNodes;
+    __syncthreads();
+    const int total = sOff[GD_LEVELS];
     const unsigned lane = gd_lane();
-    const unsigned long long lt = gd_lanemask_lt();
     const int wave = (int)(threadIdx.x >> 6);
-    int* sQueue = sQueueAll[wave];
     int4* sRec = sRecAll[wave];
     const int waves = gridDim.x * 4;
-    int queued = 0;   /* wave-uniform */
-    int zeros = 0;
-    for (int c0 = (blockIdx.x * 4 + wave) * 64; c0 < cnt; c0 += waves * 64) {
-        const int t = c0 + (int)lane;
-        bool inner = false;
-        if (t < cnt) {
-            const int idx = off + t;
+    for (int c0 = (blockIdx.x * 4 + wave) * 64; c0 < total; c0 += waves * 64) {
+        const int idx = c0 + (int)lane;
+        if (idx < total) {
             const int4 mt = meta[idx];
-            inner = mt.w >= 0;
-            if (!inner) {
-                const int first = mt.x, count = mt.y & 0x1fffffff;
-                const uint64_t key = keys[first];
-                float sx = 0.f, sy = 0.f, sz = 0.f, m = 0.f;
-                for (int b = 0; b < count; ++b) {
-                    const float4 p = pos[first + b];
-                    sx = __builtin_fmaf(p.x, p.w, sx);
-                    sy = __builtin_fmaf(p.y, p.w, sy);
-                    sz = __builtin_fmaf(p.z, p.w, sz);
-                    m += p.w;
-                }
-                float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (m > 0.f) c = make_float4(sx / m, sy / m, sz / m, m);
-                else zeros++;
-                {   /* the record, staged in LDS (written out below) */
-                    const uint64_t path = key >> shift;
-                    int4* r = sRec + 5 * lane;
-                    r[0] = make_int4((int)(uint32_t)path, (int)(uint32_t)(path >> 32), mt.z, first);
-                    r[1] = make_int4(count, -1, -1, -1);
-                    r[2] = make_int4(-1, -1, -1, -1);
-                    r[3] = make_int4(-1, level, 0, 0);
-                    r[4] = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z),
-                                     __float_as_int(c.w));
-                }
-                wcom[idx] = c;
-                wlink[idx] = make_int2(-(first + 1), (count << 5) | level);
-                if (count > 1) {   /* moments of a single-body leaf are zero, not stored */
-                    if (MOM >= 1) {
-                        float4 q0, q1;
-                        gd_leaf_moments(pos, first, count, c, &q0, &q1);
-                        wquad[2 * idx] = q0;
-                        wquad[2 * idx + 1] = q1;
-                    }
-                    if (MOM >= 2) {
-                        float4 o0, o1, o2;
-                        gd_leaf_third(pos, first, count, c, &o0, &o1, &o2);
-                        woct[3 * idx] = o0;
-                        woct[3 * idx + 1] = o1;
-                        woct[3 * idx + 2] = o2;
-                    }
-                }
-            }
-        }
-        /* the chunk's leaf records, as 16-byte pieces in address order: lane
-           l of store u writes piece 64 u + l of the chunk's 64 records, so a
-           store covers 1 KB of consecutive records (less the internal nodes'
-           records, written when their node is finished) instead of 64
-           pieces 80 B apart */
-        {
-            const unsigned long long leafMask = __ballot(t < cnt && !inner);
-            __builtin_amdgcn_wave_barrier();
-            int4* base = reinterpret_cast<int4*>(nodes + off + c0);
+            int level = 0;
 #pragma unroll
-            for (int u = 0; u < 5; ++u) {
-                const int e = 64 * u + (int)lane;
-                const int rr = e / 5;
-                if ((leafMask >> rr) & 1ull) base[e] = sRec[e];
+            for (int L = 1; L <= GD_MAX_LEVEL; ++L) level += idx >= sOff[L] ? 1 : 0;
+            const int shift = 63 - 3 * level;
+            const uint64_t key = keys[mt.x];
+            const float4 c = wcom[idx];
+            int child[8];
+#pragma unroll
+            for (int o = 0; o < 8; ++o) child[o] = -1;
+            if (mt.w >= 0) {
+                const int fc = mt.w;
+                const int nc = wlink[idx].y >> 5;
+                int oct[8];
+                oct[0] = (int)((key >> (shift - 3)) & 7u);
+#pragma unroll
+                for (int k = 1; k < 8; ++k)
+                    oct[k] = k < nc ? (int)(((unsigned)meta[fc + k].y >> 29) & 7u) : 8;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+#pragma unroll
+                    for (int o = 0; o < 8; ++o)
+                        if (oct[k] == o) child[o] = fc + k;
             }
-            __builtin_amdgcn_wave_barrier();
+            const uint64_t path = key >> shift;
+            int4* r = sRec + 5 * lane;
+            r[0] = make_int4((int)(uint32_t)path, (int)(uint32_t)(path >> 32), mt.z, mt.x);
+            r[1] = make_int4(mt.y & 0x1fffffff, child[0], child[1], child[2]);
+            r[2] = make_int4(child[3], child[4], child[5], child[6]);
+            r[3] = make_int4(child[7], level, 0, 0);
+            r[4] = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z),
+                             __float_as_int(c.w));
         }
-        const unsigned long long bal = __ballot(inner);
-        if (inner) sQueue[queued + __popcll(bal & lt)] = off + t;
-        queued += __popcll(bal);
         __builtin_amdgcn_wave_barrier();
-        if (queued >= 64) {
-            const int idx = sQueue[lane];
-            const int rest = queued - 64 > (int)lane ? sQueue[64 + lane] : 0;
-            __builtin_amdgcn_wave_barrier();
-            if (queued - 64 > (int)lane) sQueue[lane] = rest;
-            queued -= 64;
-            zeros += gd_finish_inner<MOM>(nodes, keys, meta, idx, level, childEnd, wcom, wlink,
-                                          wquad, woct);
-            __builtin_amdgcn_wave_barrier();
+        int4* base = reinterpret_cast<int4*>(nodes + c0);
+        const int pieces = 5 * (total - c0 < 64 ? total - c0 : 64);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const int e = 64 * u + (int)lane;
+            if (e < pieces) base[e] = sRec[e];
         }
+        __builtin_amdgcn_wave_barrier();
     }
-    if ((int)lane < queued)
-        zeros += gd_finish_inner<MOM>(nodes, keys, meta, sQueue[lane], level, childEnd, wcom,
-                                      wlink, wquad, woct);
-    if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
 }
 
 hipEvent_t gd_build_mid_event = nullptr;
 
+static int gd_node_blocks(int maxNodes, int n, int perBlock, int maxBlocks) {
+    /* every node at once: at most maxNodes (and 8 n) of them */
+    int blocks = gd_div_up(maxNodes < 8ll * n ? maxNodes : 8ll * n, perBlock);
+    if (blocks > maxBlocks) blocks = maxBlocks;   /* the kernels stride */
+    return blocks < 1 ? 1 : blocks;
+}
+
+void gd_build_node_records(gd_builder* b, const uint64_t* sortedKeys, node* nodes, int maxNodes) {
+    const int n = b->lastBodies;
+    if (n <= 0) return;
+    k_node_records<<<gd_node_blocks(maxNodes, n, 256, 8192), 256, 0, gd_stream()>>>(
+        nodes, sortedKeys, b->meta, b->dinfo, b->wcom, b->wlink);
+    GD_LAUNCHED();
+}
+
 void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* sortedPos, int n,
-                     node* nodes, int maxNodes) {
+                     node* nodes, int maxNodes, int records) {
     if (n <= 0) return;
     const int tiles = gd_div_up(n, GD_BUILD_TILE);
     if (tiles > b->maxTiles) {
         GD_FAIL("octree build of %d bodies exceeds the builder's capacity", n);
         return;
     }
+    b->lastBodies = n;
     k_build_count<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileCounts, b->maxTiles);
     GD_LAUNCHED();
     k_build_scan<<<GD_MAX_LEVEL, 1024, 0, gd_stream()>>>(b->tileCounts, b->tileOffsets, tiles, b->maxTiles,
@@ -719,26 +719,32 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
                                                b->maxTiles, b->meta, maxNodes, b->dinfo);
     GD_LAUNCHED();
     if (gd_build_mid_event) GD_HIP(hipEventRecord(gd_build_mid_event, gd_stream()));
-    for (int L = GD_MAX_LEVEL; L >= 0; --L) {
-        /* a level holds at most min(8^L, n) nodes */
+    /* level L's launch covers the children at L + 1 (level 21 holds leaves
+       only); L = -1 only when the root is a leaf */
+    for (int L = GD_MAX_LEVEL - 1; L >= (n == 1 ? -1 : 0); --L) {
+        /* level L + 1 holds at most min(8^(L+1), n) nodes */
         long long bound = n;
-        if (L < 8) {
-            const long long p8 = 1ll << (3 * L);
+        if (L + 1 < 8) {
+            const long long p8 = 1ll << (3 * (L + 1));
             bound = p8 < bound ? p8 : bound;
         }
-        int blocks = gd_div_up(bound, 256);
-        if (blocks > 4096) blocks = 4096;   /* 16 blocks per CU */
+        int lb = gd_div_up(bound, GD_MASS_STEP);
+        if (lb > 2048) lb = 2048;   /* 8 blocks per CU; the kernel strides */
         if (b->quad == 2)
-            k_build_mass<2><<<blocks, 256, 0, gd_stream()>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
-                                             b->wcom, b->wlink, b->wquad, b->woct);
+            k_build_mass<2><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
+                                                                  b->wcom, b->wlink, b->wquad,
+                                                                  b->woct);
         else if (b->quad == 1)
-            k_build_mass<1><<<blocks, 256, 0, gd_stream()>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
-                                             b->wcom, b->wlink, b->wquad, nullptr);
+            k_build_mass<1><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
+                                                                  b->wcom, b->wlink, b->wquad,
+                                                                  nullptr);
         else
-            k_build_mass<0><<<blocks, 256, 0, gd_stream()>>>(nodes, sortedPos, sortedKeys, b->meta, b->dinfo, L,
-                                             b->wcom, b->wlink, nullptr, nullptr);
+            k_build_mass<0><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
+                                                                  b->wcom, b->wlink, nullptr,
+                                                                  nullptr);
         GD_LAUNCHED();
     }
+    if (records) gd_build_node_records(b, sortedKeys, nodes, maxNodes);
 }
 
 /* ---------------------------------------------------------------------------

@@ -131,6 +131,8 @@ typedef struct {
     int2* wlink;                     /* .x first child (>= 0) or -(first body + 1) for a leaf,
                                         .y = (children or bodies) << 5 | level */
     int maxNodes;
+    int noRecords;                   /* 1: builds leave the 80-byte node array alone */
+    int lastBodies;                  /* bodies of the last octree build */
     /* second moments about the COM, 2 float4 per node, when quad != 0 */
     float4* wquad;
     int quad;
@@ -156,7 +158,10 @@ void gd_launch_reorder_lo(const float4* posIn, const float3* velIn, const float3
 void gd_sort_pairs(gd_sorter* s, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
                    uint32_t* vout, int n);
 void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* sortedPos, int n,
-                     node* nodes, int maxNodes);
+                     node* nodes, int maxNodes, int records);
+/* The 80-byte node records of the last gd_build_octree, from its per-node
+ * meta and walk arrays (bit-identical to a build that writes them). */
+void gd_build_node_records(gd_builder* b, const uint64_t* sortedKeys, node* nodes, int maxNodes);
 void gd_exclusive_scan_u32(gd_builder* b, const uint32_t* in, uint32_t* out, int n);
 /* When not null, gd_build_octree records this event between the topology
  * kernels and the centre-of-mass levels (build-stage timing). */

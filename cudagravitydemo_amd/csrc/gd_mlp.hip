@@ -632,7 +632,8 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
  * cell's bodies - softened monopole - analytic quadrupole) / (m s^3 / r^5),
  * in FP64: what the quadrupole cell misses, which the network learns. */
 __global__ __launch_bounds__(256) void k_samples_eval(
-    const float4* __restrict__ pos, const node* __restrict__ nodes,
+    const float4* __restrict__ pos, const int4* __restrict__ meta,
+    const int2* __restrict__ wlink,
     const float4* __restrict__ wcom, const float4* __restrict__ wquad,
     const float4* __restrict__ woct,
     const gd_tree_info* __restrict__ info, const int* __restrict__ targets,
@@ -644,8 +645,9 @@ __global__ __launch_bounds__(256) void k_samples_eval(
         const int2 pr = pairs[p];
         const float4 x = pos[targets[pr.x]];
         const int idx = pr.y;
-        const int first = nodes[idx].firstParticleIndex;
-        const int count = nodes[idx].particleCount;
+        const int4 mt = meta[idx];   /* first body, body count (octant in bits 29-31) */
+        const int first = mt.x;
+        const int count = mt.y & 0x1fffffff;
         double ax = 0.0, ay = 0.0, az = 0.0;
         for (int j = lane; j < count; j += 64) {
             const float4 b = pos[first + j];
@@ -665,7 +667,7 @@ __global__ __launch_bounds__(256) void k_samples_eval(
         if (lane == 0) {
             const float4 c = wcom[idx];
             const float4 q0 = wquad[2 * idx], q1 = wquad[2 * idx + 1];
-            const double size = ldexp((double)info->box.extent, -nodes[idx].treeLevel);
+            const double size = ldexp((double)info->box.extent, -(wlink[idx].y & 31));
             const double dx = (double)c.x - x.x, dy = (double)c.y - x.y, dz = (double)c.z - x.z;
             const double rc2 = dx * dx + dy * dy + dz * dz;
             const double rs2 = rc2 + eps2;
@@ -766,7 +768,7 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         if (rows > 0) {
             long long blocks = (rows + 3) / 4;
             if (blocks > 65536) blocks = 65536;
-            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, tree->nodes, b->wcom, b->wquad, b->woct,
+            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, b->meta, b->wlink, b->wcom, b->wquad, b->woct,
                                                  b->dinfo, targets, centre, pairs, rows,
                                                  (double)eps2, X, Y);
             GD_LAUNCHED();

@@ -91,6 +91,8 @@ class Simulation:
         else:
             self.lo = C.c_void_p(None)
             self.lo_spare = C.c_void_p(None)
+        self._records = True          # engine builds write the node array too
+        self._records_stale = False   # the node array lags the last build
         self.closed = False
 
     # ---- the reference's step -------------------------------------------
@@ -98,9 +100,11 @@ class Simulation:
         """buildTree (basicGravity.cu:116-325): sorts the bodies, swaps pointers."""
         self.lib.buildTree(C.byref(self.state), C.byref(self.pos), C.byref(self.vel))
         check(self.lib)
+        self._records_stale = False
 
     def walk_reference(self, theta):
         """calculateAcceleration (tree.cu:562-676) into self.acc."""
+        self._records_now()
         self.lib.calculateAcceleration(self.pos, self.state, self.acc, self.eps * self.eps,
                                        self.G, theta, self.n)
         check(self.lib)
@@ -126,6 +130,22 @@ class Simulation:
         self.lib.gd_build_tree_async(C.byref(self.state), C.byref(self.pos), C.byref(self.vel),
                                      lo, spare)
         check(self.lib)
+        self._records_stale = not self._records
+
+    def set_node_records(self, enable):
+        """Whether engine builds also write the reference's 80-byte node array
+        (gd_set_node_records; on by default).  Off, the engine step writes
+        only the walk arrays, and the node array is written from them when
+        something reads it (nodes_raw, walk_reference)."""
+        self._records = bool(enable)
+        self.lib.gd_set_node_records(C.byref(self.state), 1 if enable else 0)
+        check(self.lib)
+
+    def _records_now(self):
+        if self._records_stale:
+            self.lib.gd_write_node_records(C.byref(self.state))
+            check(self.lib)
+            self._records_stale = False
 
     def set_quadrupole(self, enable):
         """Compute second moments in later builds (the walk then uses them);
@@ -258,6 +278,7 @@ class Simulation:
 
     def nodes_raw(self, count=None):
         """The node array as raw bytes, shape (count, 80)."""
+        self._records_now()
         if count is None:
             count = self.tree_summary()[0]
         return from_device(self.lib, self.state.nodes, np.uint8, (count, 80))

@@ -59,6 +59,16 @@ void gd_tree_root_fp64(const treeState* tree, double* out);
 void gd_build_tree_async(treeState* tree, float4** pos, float3** vel, float3** posLo,
                          float3** spareLo);
 
+/* Whether later gd_build_tree_async builds also write the reference's 80-byte
+ * node array tree->nodes (enable = 1, the default) or only the compact walk
+ * arrays the engine's walks read (0; tree->nodes then keeps what it held).
+ * buildTree always writes the node array.  gd_write_node_records writes it
+ * for the last build, on the library's stream, byte for byte as a build
+ * with records on would have (it reads the build's keys, tree->mortonOut,
+ * so call it before anything else writes them). */
+void gd_set_node_records(treeState* tree, int enable);
+void gd_write_node_records(treeState* tree);
+
 /* Time every later gd_build_tree_async / buildTree with HIP events on the
  * library's stream (enable = 1) or stop (0).  gd_build_phase_ms fills ms[0..5]
  * with the last build's bbox, keys, sort, reorder, topology (node counts,
