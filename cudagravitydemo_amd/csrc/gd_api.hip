@@ -269,8 +269,7 @@ void sumDestroyer(treeBuilder* sum) {
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
     if (b->meta) GD_HIP(hipFree(b->meta));
-    if (b->wquad) GD_HIP(hipFree(b->wquad));
-    if (b->woct) GD_HIP(hipFree(b->woct));
+    if (b->wmom) GD_HIP(hipFree(b->wmom));
     memset(b, 0, sizeof(*b));
     free(b);
 }
@@ -451,35 +450,33 @@ void gd_set_quadrupole(treeState* tree, int enable) {
         return;
     }
     gd_builder* b = gd_builder_of(tree->builder);
-    if (gd_graphs_hold_buffers() &&
-        ((enable && b->quadNodes < tree->maxNodes) || (enable == 2 && b->octNodes < tree->maxNodes))) {
+    if (gd_graphs_hold_buffers() && enable && b->momNodes < tree->maxNodes) {
         GD_FAIL("gd_set_quadrupole: the moment arrays cannot be allocated while a graph holds "
                 "the library's buffers");
         return;
     }
     b->quad = enable;
-    if (b->quad && b->quadNodes < tree->maxNodes) {
-        if (b->wquad) GD_HIP(hipFree(b->wquad));
-        GD_HIP(hipMalloc(&b->wquad, 2 * sizeof(float4) * (size_t)tree->maxNodes));
-        b->quadNodes = tree->maxNodes;
-    }
-    if (b->quad == 2 && b->octNodes < tree->maxNodes) {
-        if (b->woct) GD_HIP(hipFree(b->woct));
-        GD_HIP(hipMalloc(&b->woct, 3 * sizeof(float4) * (size_t)tree->maxNodes));
-        b->octNodes = tree->maxNodes;
+    if (b->quad && b->momNodes < tree->maxNodes) {
+        if (b->wmom) GD_HIP(hipFree(b->wmom));
+        GD_HIP(hipMalloc(&b->wmom, 4 * sizeof(float4) * (size_t)tree->maxNodes));
+        b->momNodes = tree->maxNodes;
     }
 }
 
-/* The build does not store the (zero) moments of single-body leaves; the
- * inspection copies report them as zero. */
-static void gd_zero_single_leaves(const gd_builder* b, float4* host, int per, int count) {
-    if (count <= 0 || gd_has_error()) return;
+/* The moment records of nodes [0, count) in host memory; the build does not
+ * store the (zero) moments of single-body leaves, which read as zero here. */
+static std::vector<float4> gd_moment_records(const gd_builder* b, int count) {
+    std::vector<float4> rec((size_t)4 * (count > 0 ? count : 0));
+    if (count <= 0 || gd_has_error()) return rec;
+    GD_HIP(hipMemcpy(rec.data(), b->wmom, 4 * sizeof(float4) * (size_t)count,
+                     hipMemcpyDeviceToHost));
     std::vector<int2> link((size_t)count);
     GD_HIP(hipMemcpy(link.data(), b->wlink, sizeof(int2) * (size_t)count, hipMemcpyDeviceToHost));
-    if (gd_has_error()) return;
+    if (gd_has_error()) return rec;
     for (int i = 0; i < count; ++i)
         if (link[i].x < 0 && (link[i].y >> 5) == 1)
-            for (int k = 0; k < per; ++k) host[(size_t)per * i + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = 0; k < 4; ++k) rec[(size_t)4 * i + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return rec;
 }
 
 void gd_tree_third_moments(const treeState* tree, void* oct, int count) {
@@ -492,12 +489,20 @@ void gd_tree_third_moments(const treeState* tree, void* oct, int count) {
         GD_FAIL("gd_tree_third_moments: count %d outside [0, %d]", count, b->maxNodes);
         return;
     }
-    if (b->quad != 2 || !b->woct) {
+    if (b->quad != 2 || !b->wmom) {
         GD_FAIL("gd_tree_third_moments: third moments are off (gd_set_quadrupole(tree, 2))");
         return;
     }
-    GD_HIP(hipMemcpy(oct, b->woct, 3 * sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
-    gd_zero_single_leaves(b, static_cast<float4*>(oct), 3, count);
+    const std::vector<float4> rec = gd_moment_records(b, count);
+    if (gd_has_error()) return;
+    float4* out = static_cast<float4*>(oct);
+    for (int i = 0; i < count; ++i) {
+        const float4 m1 = rec[4 * (size_t)i + 1], m2 = rec[4 * (size_t)i + 2],
+                     m3 = rec[4 * (size_t)i + 3];
+        out[3 * (size_t)i] = make_float4(m1.z, m1.w, m2.x, m2.y);
+        out[3 * (size_t)i + 1] = make_float4(m2.z, m2.w, m3.x, m3.y);
+        out[3 * (size_t)i + 2] = make_float4(m3.z, m3.w, 0.f, 0.f);
+    }
 }
 
 void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* quad, int count) {
@@ -512,9 +517,16 @@ void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* qua
     }
     if (com) GD_HIP(hipMemcpy(com, b->wcom, sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
     if (link) GD_HIP(hipMemcpy(link, b->wlink, sizeof(int2) * (size_t)count, hipMemcpyDeviceToHost));
-    if (quad && b->wquad) {
-        GD_HIP(hipMemcpy(quad, b->wquad, 2 * sizeof(float4) * (size_t)count, hipMemcpyDeviceToHost));
-        gd_zero_single_leaves(b, static_cast<float4*>(quad), 2, count);
+    if (quad && b->quad && b->wmom) {
+        const std::vector<float4> rec = gd_moment_records(b, count);
+        if (gd_has_error()) return;
+        float4* out = static_cast<float4*>(quad);
+        for (int i = 0; i < count; ++i) {
+            const float4 m0 = rec[4 * (size_t)i], m1 = rec[4 * (size_t)i + 1];
+            const float tr = m0.x + m0.y + m0.z;   /* as the walks form it */
+            out[2 * (size_t)i] = m0;
+            out[2 * (size_t)i + 1] = make_float4(m1.x, m1.y, tr, 0.f);
+        }
     }
 }
 

@@ -20,8 +20,9 @@
  *                   (tile offset + wave offsets + lane rank); the parent is the
  *                   node of the level above that contains the body
  *                   (tree.cu:224-231); particleCount = size of the body's
- *                   level-L cell, found by galloping over the sorted keys
- *                   (equal to tree.cu:181-203's sibling / parent-end rule); the
+ *                   level-L cell, found among the wavefront's bodies or by a
+ *                   64-way search of the sorted keys (equal to
+ *                   tree.cu:181-203's sibling / parent-end rule); the
  *                   first child is the body's node one level down, if it starts
  *                   one.  Written as a 16-byte meta record per node (first
  *                   body, count with the node's octant in bits 29-31, parent,
@@ -147,23 +148,41 @@ __global__ __launch_bounds__(1024) void k_build_scan(const uint32_t* __restrict_
     if (threadIdx.x == 0) levelTotals[L] = sCarry;
 }
 
-/* Cell size: first j > i whose level-L cell differs, minus i (galloping). */
-__device__ __forceinline__ int gd_cell_size(const uint64_t* __restrict__ keys, int n, long long i,
-                                            uint64_t cell, int shift) {
-    long long same = i + 1;     /* keys[i+1] is known to share the cell */
-    long long step = 2;
-    long long probe = i + step;
-    while (probe < n && (keys[probe] >> shift) == cell) {
-        same = probe;
-        step <<= 1;
-        probe = i + step;
+/* End of a cell that runs past a wavefront's bodies: the first j in
+ * [lo, hi) whose key is outside `cell` (keys >> shift), or hi, where
+ * keys[lo - 1] is known to be inside and the end to be at most hi.  Called by the whole wavefront with uniform
+ * arguments.  Each round the 64 lanes probe 64 keys spaced `step` apart from
+ * lo: while none is outside, lo moves past them and the step grows 64-fold
+ * (1, 64, 4096, ...: the first round reads the next 64 keys at once); once
+ * one is, the boundary lies between two probes and the next rounds divide
+ * that gap 64 ways.  A cell of S bodies takes about 2 log64 S rounds instead
+ * of one lane's 2 log2 S dependent loads.  Membership is monotone (sorted
+ * keys: inside, then outside). */
+__device__ __forceinline__ long long gd_wave_cell_end(const uint64_t* __restrict__ keys,
+                                                      long long lo, long long hi, uint64_t cell,
+                                                      int shift) {
+    /* the end lies in [lo, hi] */
+    long long step = 1;
+    bool bounded = false;      /* hi is a probe known to be outside */
+    const long long lane = (long long)gd_lane();
+    while (lo < hi) {
+        if (bounded) step = (hi - lo + 63) / 64;
+        const long long p = lo + lane * step;
+        const bool out = p >= hi || (keys[p] >> shift) != cell;
+        const unsigned long long b = __ballot(out);
+        if (b == 0ull) {       /* all 64 probes inside */
+            lo += 63 * step + 1;
+            if (!bounded) step *= 64;
+            continue;
+        }
+        const int f = __builtin_ctzll(b);
+        if (f == 0) return lo;
+        const long long h = lo + (long long)f * step;
+        hi = h < hi ? h : hi;
+        lo += (long long)(f - 1) * step + 1;
+        bounded = true;
     }
-    long long diff = probe < n ? probe : n;
-    while (diff - same > 1) {
-        const long long mid = same + ((diff - same) >> 1);
-        if ((keys[mid] >> shift) == cell) same = mid; else diff = mid;
-    }
-    return (int)(diff - i);
+    return lo;
 }
 
 __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
@@ -250,6 +269,9 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
         const long long i = tileBase + q;
         const uint64_t key = sk[q + 1];
         int prevRank = 0, prevFlag = 0;
+        /* the cells that run past the wavefront's last body are nested (each
+           holds that body), so each one's end bounds the next, deeper one's */
+        long long waveHi = n;
         bool flag = lo[k] <= 1 && 1 <= hi[k];
         int rank = (int)sRank[k][wave][1] + __popcll(__ballot(flag) & lt);
         for (int L = 1; L <= waveMaxHi; ++L) {
@@ -261,6 +283,23 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
             /* level-L cells end where a body starts a node at a level <= L
                (its left neighbour differs from it at level L) */
             const unsigned long long ends = __ballot(lo[k] <= L);
+            const unsigned long long later = ends & ~(lt | (1ull << gd_lane()));
+            /* at most one lane per level starts a cell that runs past the
+               wavefront's last body (every later body is inside it): the
+               wavefront finds its end together */
+            long long far = 0;
+            {
+                const unsigned long long g = __ballot(flag && L <= Bv[k] && later == 0ull);
+                if (g) {
+                    const int src = __builtin_ctzll(g);
+                    const uint64_t c = __shfl(key, src) >> (63 - 3 * L);
+                    const long long waveEnd = tileBase + k * GD_BUILD_THREADS +
+                                              (threadIdx.x & ~(GD_WAVE - 1)) + GD_WAVE;
+                    far = gd_wave_cell_end(keys, waveEnd < n ? waveEnd : n, waveHi, c,
+                                           63 - 3 * L);
+                    waveHi = far;
+                }
+            }
             if (flag) {
                 const int idx = sLevelOff[L] + rank;
                 const int parent = (L == 1) ? 0 : sLevelOff[L - 1] + prevRank + prevFlag - 1;
@@ -268,11 +307,10 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
                 const uint64_t cell = key >> shift;
                 /* the cell's size: 1 when the right neighbour is outside it,
                    from the next cell start among this wavefront's bodies, or
-                   by galloping over the keys when the cell runs past them */
-                const unsigned long long later = ends & ~(lt | (1ull << gd_lane()));
+                   from the wavefront's search when the cell runs past them */
                 const int count = (L > Bv[k]) ? 1
                                   : later ? (int)__builtin_ctzll(later) - (int)gd_lane()
-                                          : gd_cell_size(keys, n, i, cell, shift);
+                                          : (int)(far - i);
                 /* count < 2^28 (8 n nodes fit an int): its top bits carry
                    the node's octant in its parent */
                 meta[idx] = make_int4((int)i, count | ((int)(cell & 7u) << 29), parent,
@@ -397,143 +435,168 @@ __device__ __forceinline__ void gd_leaf_third(const float4* __restrict__ pos, in
 }
 
 /*
- * Centre of mass and moments: one launch per level L, deepest first, over
- * the nodes of level L + 1 (the children of level L's internal nodes).
+ * Centre of mass and moments: one launch per level L, deepest first.  Launch
+ * L finishes every node of level L and writes each one's walk arrays once,
+ * in index order, so consecutive threads fill whole 64-byte memory sectors
+ * (a sector written in parts by different launches costs a read-modify-write
+ * in memory, about five times the time of a whole one).
  *
- * A block takes a tile of 256 consecutive children, one per thread, and
- * finishes the parents whose first child lies in the tile's first 248 (a
- * parent has at most 8 children, so all of theirs are in the tile; the next
- * tile starts 248 further on).  Children of one parent are consecutive and
- * in octant order, so a parent is a run of equal parent indices.
- *   1. each thread loads its child: a leaf's centre of mass and moments are
- *      computed from its bodies (tree.cu:105-125) and written out, an
- *      internal child's are loaded (its level's launch wrote them); every
- *      load of the tile is independent and coalesced.
- *   2. the first thread of each run sums its children's masses, then the
- *      weighted positions, in octant order (tree.cu:127-179) -> the
- *      parent's centre of mass c, handed to each child of the run.
+ * A block takes a tile of 256 consecutive nodes of level L + 1 (the children
+ * of level L's internal nodes), one per thread, and finishes the parents
+ * whose first child lies in the tile's first 248 (a parent has at most 8
+ * children, so all of theirs are in the tile; the next tile starts 248
+ * further on).  Children of one parent are consecutive and in octant order.
+ *   1. each thread loads its child's centre of mass and moments (the
+ *      previous launch wrote them); the loads of a tile are coalesced.
+ *   2. the first thread of each run of equal parents sums its children's
+ *      masses, then the weighted positions, in octant order (tree.cu:127-179)
+ *      -> the parent's centre of mass c, handed to each child of the run.
  *   3. each child thread forms its terms of the parent's moments about c
  *      (parallel-axis shifts: S += S_c + m d d^T, O += O_c + (S_c,ij d_k +
  *      S_c,ik d_j + S_c,jk d_i) + m d_i d_j d_k, d = c_child - c), and the
  *      run's first thread adds them in octant order.
+ *   4. the tile owns the level-L nodes from its first parent up to the next
+ *      tile's first parent (the first tile from the level's start, the last
+ *      to its end): their internal nodes take the results of step 3 (found
+ *      through their first child), their leaves are computed from their
+ *      bodies (tree.cu:105-125); all are written in index order.
+ * A level without children (the deepest, or the root of one body) is all
+ * leaves, spread over the blocks in chunks of 256.
  * Every sum is formed term by term in the oracle's order; an octant without
  * a child adds exactly +0 in the oracle and no partial sum is ever -0 (each
  * starts at +0 and adds products or fmaf results whose addend is +0 or a sum
  * of the same kind), so leaving the empty octants out changes no bit.
- * A single-body leaf has zero moments, which are not stored.
+ * A single-body leaf has zero moments; its moment record is not written.
  *
  * MOM: 0 centre of mass only, 1 also second moments, 2 also third moments.
  * Input: the per-node meta of k_build_nodes (first body, count, parent,
- * first child or -1).  Output: the walk arrays of levels L + 1 (leaves) and
- * L (internal nodes).  L = -1 finishes a root that is a leaf (one body).
+ * first child or -1).  Output: the walk arrays of level L.
  */
 #define GD_MASS_TILE 256
 #define GD_MASS_STEP (GD_MASS_TILE - 8)    /* first children per tile */
-#define GD_MASS_TERMS (MOM == 2 ? 16 : (MOM == 1 ? 6 : 1))
+
+/* Finishes the level-L nodes [x0, x1) (step 4): leaves from their bodies,
+ * internal nodes from the tile's results at the tile slot h of their first
+ * child (tileBase = that level's index of slot 0): centre of mass sC[h],
+ * children sNc[h], moment sums sTerm[e * 256 + h]. */
+template <int MOM>
+__device__ __forceinline__ int gd_finish_range(int x0, int x1, int level,
+                                               const float4* __restrict__ pos,
+                                               const int4* __restrict__ meta, long long tileBase,
+                                               const float4* sC, const float* sTerm,
+                                               const int* sNc, float4* __restrict__ wcom,
+                                               int2* __restrict__ wlink,
+                                               float4* __restrict__ wmom) {
+    int zeros = 0;
+    for (int xb = x0; xb < x1; xb += GD_MASS_TILE) {
+        const int X = xb + (int)threadIdx.x;
+        if (X >= x1) continue;
+        const int4 mt = meta[X];
+        if (mt.w >= 0) {   /* internal: its first child's tile slot */
+            const int h = (int)(mt.w - tileBase);
+            const float4 c = sC[h];
+            wcom[X] = c;
+            wlink[X] = make_int2(mt.w, (sNc[h] << 5) | level);
+            if (MOM >= 1) {
+                float v[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    v[e] = (MOM >= 2 || e < 6) ? sTerm[e * GD_MASS_TILE + h] : 0.f;
+                wmom[4 * X] = make_float4(v[0], v[1], v[2], v[3]);
+                wmom[4 * X + 1] = make_float4(v[4], v[5], v[6], v[7]);
+                wmom[4 * X + 2] = make_float4(v[8], v[9], v[10], v[11]);
+                wmom[4 * X + 3] = make_float4(v[12], v[13], v[14], v[15]);
+            }
+            zeros += c.w > 0.f ? 0 : 1;
+        } else {           /* leaf */
+            const int first = mt.x, count = mt.y & 0x1fffffff;
+            float sx = 0.f, sy = 0.f, sz = 0.f, m = 0.f;
+            for (int b = 0; b < count; ++b) {
+                const float4 x = pos[first + b];
+                sx = __builtin_fmaf(x.x, x.w, sx);
+                sy = __builtin_fmaf(x.y, x.w, sy);
+                sz = __builtin_fmaf(x.z, x.w, sz);
+                m += x.w;
+            }
+            float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (m > 0.f) c = make_float4(sx / m, sy / m, sz / m, m);
+            else zeros++;
+            wcom[X] = c;
+            wlink[X] = make_int2(-(first + 1), (count << 5) | level);
+            if (MOM >= 1 && count > 1) {   /* a single-body leaf's moments are zero */
+                float4 q0, q1, o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0;
+                gd_leaf_moments(pos, first, count, c, &q0, &q1);
+                if (MOM >= 2) gd_leaf_third(pos, first, count, c, &o0, &o1, &o2);
+                gd_mom_store(wmom, X, q0, q1.x, q1.y, o0, o1, o2);
+            }
+        }
+    }
+    return zeros;
+}
 
 template <int MOM>
-__global__ __launch_bounds__(GD_MASS_TILE) void k_build_mass(const float4* __restrict__ pos,
+__global__ __launch_bounds__(GD_MASS_TILE) __attribute__((amdgpu_waves_per_eu(6)))
+void k_build_mass(const float4* __restrict__ pos,
                                                              const int4* __restrict__ meta,
                                                              gd_tree_info* info, int level,
                                                              float4* __restrict__ wcom,
                                                              int2* __restrict__ wlink,
-                                                             float4* __restrict__ wquad,
-                                                             float4* __restrict__ woct) {
-    __shared__ int sPar[GD_MASS_TILE + 1];          /* parent of tile child t - 1 */
-    __shared__ float4 sCom[GD_MASS_TILE];           /* children's centres of mass */
-    __shared__ float4 sC[GD_MASS_TILE];             /* their parent's */
-    __shared__ float sTerm[GD_MASS_TERMS][GD_MASS_TILE];
+                                                             float4* __restrict__ wmom) {
+    constexpr int TERMS = MOM == 2 ? 16 : (MOM == 1 ? 6 : 1);
+    __shared__ float4 sCom[GD_MASS_TILE];     /* children's centres of mass */
+    __shared__ float4 sC[GD_MASS_TILE];       /* their parent's */
+    __shared__ float sTerm[TERMS * GD_MASS_TILE];   /* [term][child] */
+    __shared__ int sPar[GD_MASS_TILE + 1];    /* parent of tile child t - 1 */
+    __shared__ int sNc[GD_MASS_TILE];         /* a run's children, at its first child */
+    __shared__ int sRange[2];
     if (info->overflow) return;
     const int cl = level + 1;
-    const int cnt = info->levelCount[cl];
-    const int coff = info->levelOffsets[cl];
+    const int cnt = cl <= GD_MAX_LEVEL ? info->levelCount[cl] : 0;
+    const int coff = cl <= GD_MAX_LEVEL ? info->levelOffsets[cl] : 0;
+    const int lvStart = info->levelOffsets[level];
+    const int lvEnd = lvStart + info->levelCount[level];
     const int tid = (int)threadIdx.x;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     int zeros = 0;
-    const long long stride = (long long)gridDim.x * GD_MASS_STEP;
-    /* the tile's meta is loaded one tile ahead, so that its latency overlaps
-       the previous tile's loads and LDS phases */
-    long long t0 = (long long)blockIdx.x * GD_MASS_STEP;
-    int4 mtNext = make_int4(0, 0, -2, -1);
-    int prevNext = -3;
-    if (t0 + tid < cnt) {
-        mtNext = meta[coff + (int)(t0 + tid)];
-        if (tid == 0 && t0 > 0) prevNext = meta[coff + (int)t0 - 1].z;
+    if (cnt == 0) {   /* no children: every node of the level is a leaf */
+        for (long long x0 = lvStart + (long long)blockIdx.x * GD_MASS_TILE; x0 < lvEnd;
+             x0 += (long long)gridDim.x * GD_MASS_TILE) {
+            const int x1 = x0 + GD_MASS_TILE < lvEnd ? (int)x0 + GD_MASS_TILE : lvEnd;
+            zeros += gd_finish_range<MOM>((int)x0, x1, level, pos, meta, 0, sC, sTerm, sNc,
+                                          wcom, wlink, wmom);
+        }
+        if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
+        return;
     }
-    for (; t0 < cnt; t0 += stride) {
+    const long long stride = (long long)gridDim.x * GD_MASS_STEP;
+    for (long long t0 = (long long)blockIdx.x * GD_MASS_STEP; t0 < cnt; t0 += stride) {
         const long long i = t0 + tid;
         const bool valid = i < cnt;
         const int j = coff + (int)i;
-        const int4 mt = mtNext;
-        const int prev = prevNext;
-        {
-            const long long in = i + stride;
-            mtNext = make_int4(0, 0, -2, -1);
-            prevNext = -3;
-            if (in < cnt) {
-                mtNext = meta[coff + (int)in];
-                if (tid == 0) prevNext = meta[coff + (int)in - 1].z;
-            }
-        }
+        /* 1. the child */
         int par = -2;
-        float4 ck = z4, qa = z4, qb = z4, p = z4, q = z4, r = z4;
+        bool hasMom = false;
+        float4 ck = z4, m0 = z4, m1 = z4, m2 = z4, m3 = z4;
         if (valid) {
+            const int4 mt = meta[j];
+            ck = wcom[j];
             par = mt.z;
-            if (mt.w < 0) {   /* a leaf: from its bodies */
-                const int first = mt.x, count = mt.y & 0x1fffffff;
-                float sx = 0.f, sy = 0.f, sz = 0.f, m = 0.f;
-                for (int b = 0; b < count; ++b) {
-                    const float4 x = pos[first + b];
-                    sx = __builtin_fmaf(x.x, x.w, sx);
-                    sy = __builtin_fmaf(x.y, x.w, sy);
-                    sz = __builtin_fmaf(x.z, x.w, sz);
-                    m += x.w;
-                }
-                if (m > 0.f) ck = make_float4(sx / m, sy / m, sz / m, m);
-                /* the tile's last 8 children are the next tile's first 8:
-                   only one of the two writes them */
-                const bool own = tid < GD_MASS_STEP;
-                if (own) {
-                    zeros += m > 0.f ? 0 : 1;
-                    wcom[j] = ck;
-                    wlink[j] = make_int2(-(first + 1), (count << 5) | cl);
-                }
-                if (count > 1) {   /* moments of a single-body leaf are zero, not stored */
-                    if (MOM >= 1) {
-                        gd_leaf_moments(pos, first, count, ck, &qa, &qb);
-                        if (own) {
-                            wquad[2 * j] = qa;
-                            wquad[2 * j + 1] = qb;
-                        }
-                    }
-                    if (MOM >= 2) {
-                        gd_leaf_third(pos, first, count, ck, &p, &q, &r);
-                        if (own) {
-                            woct[3 * j] = p;
-                            woct[3 * j + 1] = q;
-                            woct[3 * j + 2] = r;
-                        }
-                    }
-                }
-            } else {          /* internal: written by the previous launch */
-                ck = wcom[j];
-                if (MOM >= 1) {
-                    qa = wquad[2 * j];
-                    qb = wquad[2 * j + 1];
-                }
+            hasMom = mt.w >= 0 || (mt.y & 0x1fffffff) > 1;
+            if (MOM >= 1 && hasMom) {
+                m0 = wmom[4 * j];
+                m1 = wmom[4 * j + 1];
                 if (MOM >= 2) {
-                    p = woct[3 * j];
-                    q = woct[3 * j + 1];
-                    r = woct[3 * j + 2];
+                    m2 = wmom[4 * j + 2];
+                    m3 = wmom[4 * j + 3];
                 }
             }
         }
         sPar[tid + 1] = par;
-        if (tid == 0) sPar[0] = prev;
+        if (tid == 0) sPar[0] = (valid && i > 0) ? meta[j - 1].z : -3;
         sCom[tid] = ck;
         __syncthreads();
-        /* 2. the first child of each run (a parent; none for the root) */
-        const bool head = valid && par >= 0 && tid < GD_MASS_STEP && sPar[tid] != par;
+        /* 2. the first child of each run: the parent's centre of mass */
+        const bool head = valid && tid < GD_MASS_STEP && sPar[tid] != par;
         int nc = 0;
         float4 c = z4;
         if (head) {
@@ -550,61 +613,74 @@ __global__ __launch_bounds__(GD_MASS_TILE) void k_build_mass(const float4* __res
             c = a.w > 0.f ? make_float4(a.x / a.w, a.y / a.w, a.z / a.w, a.w) : z4;
             for (int k = 0; k < nc; ++k) sC[tid + k] = c;
         }
+        if (tid == 0) {
+            /* the owned range of level L: from the parent of the tile's first
+               run (any 8 consecutive children hold a first child) to that of
+               the first run at slot 248 on */
+            int a = lvEnd, b = lvEnd;
+            if (t0 == 0) a = lvStart;
+            else
+                for (int s = 0; s < 8 && t0 + s < cnt; ++s)
+                    if (sPar[s] != sPar[s + 1]) { a = sPar[s + 1]; break; }
+            for (int s = GD_MASS_STEP; s < GD_MASS_TILE && t0 + s < cnt; ++s)
+                if (sPar[s] != sPar[s + 1]) { b = sPar[s + 1]; break; }
+            sRange[0] = a;
+            sRange[1] = b;
+        }
         if (MOM >= 1) {
             __syncthreads();
             /* 3. each child's terms about its parent's centre of mass (a
-               child whose run is finished by another tile writes unused ones) */
+               child whose run another tile finishes writes unused ones) */
             {
                 const float4 cp = sC[tid];
+                const float4 qa = m0;
+                const float4 qb = make_float4(m1.x, m1.y, 0.f, 0.f);
                 const float w = ck.w;
                 const float dx = ck.x - cp.x, dy = ck.y - cp.y, dz = ck.z - cp.z;
-                sTerm[0][tid] = __builtin_fmaf(w * dx, dx, qa.x);
-                sTerm[1][tid] = __builtin_fmaf(w * dy, dy, qa.y);
-                sTerm[2][tid] = __builtin_fmaf(w * dz, dz, qa.z);
-                sTerm[3][tid] = __builtin_fmaf(w * dx, dy, qa.w);
-                sTerm[4][tid] = __builtin_fmaf(w * dx, dz, qb.x);
-                sTerm[5][tid] = __builtin_fmaf(w * dy, dz, qb.y);
+                float t[TERMS];
+                t[0] = __builtin_fmaf(w * dx, dx, qa.x);
+                t[1 % TERMS] = __builtin_fmaf(w * dy, dy, qa.y);
+                t[2 % TERMS] = __builtin_fmaf(w * dz, dz, qa.z);
+                t[3 % TERMS] = __builtin_fmaf(w * dx, dy, qa.w);
+                t[4 % TERMS] = __builtin_fmaf(w * dx, dz, qb.x);
+                t[5 % TERMS] = __builtin_fmaf(w * dy, dz, qb.y);
                 if (MOM >= 2) {
+                    /* the child's third moments: (xxx yyy) in m1.zw, (zzz xxy xxz xyy)
+                       in m2, (yyz xzz yzz xyz) in m3 */
                     const float mx = w * dx, my = w * dy, mz = w * dz;
-                    sTerm[6 % GD_MASS_TERMS][tid] = p.x + 3.f * qa.x * dx + mx * dx * dx;
-                    sTerm[7 % GD_MASS_TERMS][tid] = p.y + 3.f * qa.y * dy + my * dy * dy;
-                    sTerm[8 % GD_MASS_TERMS][tid] = p.z + 3.f * qa.z * dz + mz * dz * dz;
-                    sTerm[9 % GD_MASS_TERMS][tid] = p.w + qa.x * dy + 2.f * qa.w * dx + mx * dx * dy;
-                    sTerm[10 % GD_MASS_TERMS][tid] = q.x + qa.x * dz + 2.f * qb.x * dx + mx * dx * dz;
-                    sTerm[11 % GD_MASS_TERMS][tid] = q.y + qa.y * dx + 2.f * qa.w * dy + mx * dy * dy;
-                    sTerm[12 % GD_MASS_TERMS][tid] = q.z + qa.y * dz + 2.f * qb.y * dy + my * dy * dz;
-                    sTerm[13 % GD_MASS_TERMS][tid] = q.w + qa.z * dx + 2.f * qb.x * dz + mx * dz * dz;
-                    sTerm[14 % GD_MASS_TERMS][tid] = r.x + qa.z * dy + 2.f * qb.y * dz + my * dz * dz;
-                    sTerm[15 % GD_MASS_TERMS][tid] =
-                        r.y + qa.w * dz + qb.x * dy + qb.y * dx + mx * dy * dz;
+                    t[6 % TERMS] = m1.z + 3.f * qa.x * dx + mx * dx * dx;
+                    t[7 % TERMS] = m1.w + 3.f * qa.y * dy + my * dy * dy;
+                    t[8 % TERMS] = m2.x + 3.f * qa.z * dz + mz * dz * dz;
+                    t[9 % TERMS] = m2.y + qa.x * dy + 2.f * qa.w * dx + mx * dx * dy;
+                    t[10 % TERMS] = m2.z + qa.x * dz + 2.f * qb.x * dx + mx * dx * dz;
+                    t[11 % TERMS] = m2.w + qa.y * dx + 2.f * qa.w * dy + mx * dy * dy;
+                    t[12 % TERMS] = m3.x + qa.y * dz + 2.f * qb.y * dy + my * dy * dz;
+                    t[13 % TERMS] = m3.y + qa.z * dx + 2.f * qb.x * dz + mx * dz * dz;
+                    t[14 % TERMS] = m3.z + qa.z * dy + 2.f * qb.y * dz + my * dz * dz;
+                    t[15 % TERMS] = m3.w + qa.w * dz + qb.x * dy + qb.y * dx + mx * dy * dz;
                 }
+#pragma unroll
+                for (int e = 0; e < TERMS; ++e) sTerm[e * GD_MASS_TILE + tid] = t[e];
             }
             __syncthreads();
-        }
-        if (head) {
-            const int P = par;
-            wcom[P] = c;
-            wlink[P] = make_int2(j, (nc << 5) | level);
-            zeros += c.w > 0.f ? 0 : 1;
-            if (MOM >= 1) {
-                float sum[GD_MASS_TERMS];
+            /* the run's sums, in octant order, into its first child's slot
+               (only this run's first thread reads its children's slots) */
+            if (head) {
+                float sum[TERMS];
 #pragma unroll
-                for (int e = 0; e < GD_MASS_TERMS; ++e) sum[e] = 0.f;
+                for (int e = 0; e < TERMS; ++e) sum[e] = 0.f;
                 for (int k = 0; k < nc; ++k)
 #pragma unroll
-                    for (int e = 0; e < GD_MASS_TERMS; ++e) sum[e] += sTerm[e][tid + k];
-                wquad[2 * P] = make_float4(sum[0], sum[1], sum[2], sum[3]);
-                wquad[2 * P + 1] = make_float4(sum[4], sum[5], sum[0] + sum[1] + sum[2], 0.f);
-                if (MOM >= 2) {
-                    woct[3 * P] = make_float4(sum[6 % GD_MASS_TERMS], sum[7 % GD_MASS_TERMS],
-                                              sum[8 % GD_MASS_TERMS], sum[9 % GD_MASS_TERMS]);
-                    woct[3 * P + 1] = make_float4(sum[10 % GD_MASS_TERMS], sum[11 % GD_MASS_TERMS],
-                                                  sum[12 % GD_MASS_TERMS], sum[13 % GD_MASS_TERMS]);
-                    woct[3 * P + 2] =
-                        make_float4(sum[14 % GD_MASS_TERMS], sum[15 % GD_MASS_TERMS], 0.f, 0.f);
-                }
+                    for (int e = 0; e < TERMS; ++e) sum[e] += sTerm[e * GD_MASS_TILE + tid + k];
+#pragma unroll
+                for (int e = 0; e < TERMS; ++e) sTerm[e * GD_MASS_TILE + tid] = sum[e];
             }
         }
+        if (head) sNc[tid] = nc;
+        __syncthreads();
+        /* 4. the tile's level-L nodes */
+        zeros += gd_finish_range<MOM>(sRange[0], sRange[1], level, pos, meta, coff + t0, sC,
+                                      sTerm, sNc, wcom, wlink, wmom);
         __syncthreads();   /* the next tile reuses the LDS */
     }
     if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
@@ -719,9 +795,9 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
                                                b->maxTiles, b->meta, maxNodes, b->dinfo);
     GD_LAUNCHED();
     if (gd_build_mid_event) GD_HIP(hipEventRecord(gd_build_mid_event, gd_stream()));
-    /* level L's launch covers the children at L + 1 (level 21 holds leaves
-       only); L = -1 only when the root is a leaf */
-    for (int L = GD_MAX_LEVEL - 1; L >= (n == 1 ? -1 : 0); --L) {
+    /* launch L finishes level L from its children at L + 1 (level 21 holds
+       leaves only; an empty level returns at once) */
+    for (int L = GD_MAX_LEVEL; L >= 0; --L) {
         /* level L + 1 holds at most min(8^(L+1), n) nodes */
         long long bound = n;
         if (L + 1 < 8) {
@@ -729,19 +805,19 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
             bound = p8 < bound ? p8 : bound;
         }
         int lb = gd_div_up(bound, GD_MASS_STEP);
-        if (lb > 2048) lb = 2048;   /* 8 blocks per CU; the kernel strides */
+        /* at most the blocks that fit at once (6, 7, 8 per CU for third,
+           second, no moments); the kernel strides */
+        const int fit = b->quad == 2 ? 1536 : (b->quad == 1 ? 1792 : 2048);
+        if (lb > fit) lb = fit;
         if (b->quad == 2)
             k_build_mass<2><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
-                                                                  b->wcom, b->wlink, b->wquad,
-                                                                  b->woct);
+                                                                  b->wcom, b->wlink, b->wmom);
         else if (b->quad == 1)
             k_build_mass<1><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
-                                                                  b->wcom, b->wlink, b->wquad,
-                                                                  nullptr);
+                                                                  b->wcom, b->wlink, b->wmom);
         else
             k_build_mass<0><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
-                                                                  b->wcom, b->wlink, nullptr,
-                                                                  nullptr);
+                                                                  b->wcom, b->wlink, nullptr);
         GD_LAUNCHED();
     }
     if (records) gd_build_node_records(b, sortedKeys, nodes, maxNodes);

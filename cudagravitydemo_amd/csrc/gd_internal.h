@@ -133,13 +133,11 @@ typedef struct {
     int maxNodes;
     int noRecords;                   /* 1: builds leave the 80-byte node array alone */
     int lastBodies;                  /* bodies of the last octree build */
-    /* second moments about the COM, 2 float4 per node, when quad != 0 */
-    float4* wquad;
+    /* moments about the COM, one 64-byte record per node (gd_mom_* below):
+       second moments when quad != 0, third moments too when quad == 2 */
+    float4* wmom;
     int quad;
-    int quadNodes;
-    /* third moments about the COM, 3 float4 per node, when quad == 2 */
-    float4* woct;
-    int octNodes;
+    int momNodes;
 } gd_builder;
 
 static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }
@@ -176,6 +174,35 @@ __device__ __forceinline__ unsigned gd_lane() { return __lane_id(); }
 __device__ __forceinline__ unsigned long long gd_lanemask_lt() {
     unsigned l = __lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+/* Moment record of a node: 64 bytes, one aligned 64-byte memory sector
+ * (so a build writes whole sectors), 4 float4:
+ *   (Sxx Syy Szz Sxy) (Sxz Syz Oxxx Oyyy) (Ozzz Oxxy Oxxz Oxyy) (Oyyz Oxzz Oyzz Oxyz)
+ * S = sum m d d^T and O = sum m d d d about the centre of mass.  Readers get
+ * the second moments as (Sxx Syy Szz Sxy), (Sxz Syz trace 0), the trace
+ * summed as Sxx + Syy + Szz, and the third moments as (xxx yyy zzz xxy),
+ * (xxz xyy yyz xzz), (yzz xyz 0 0). */
+__device__ __forceinline__ void gd_mom_quad(const float4* __restrict__ wmom, int i, float4& q0,
+                                            float4& q1) {
+    const float4 a = wmom[4 * i], b = wmom[4 * i + 1];
+    q0 = a;
+    q1 = make_float4(b.x, b.y, a.x + a.y + a.z, 0.f);
+}
+__device__ __forceinline__ void gd_mom_oct(const float4* __restrict__ wmom, int i, float4& o0,
+                                           float4& o1, float4& o2) {
+    const float4 b = wmom[4 * i + 1], c = wmom[4 * i + 2], d = wmom[4 * i + 3];
+    o0 = make_float4(b.z, b.w, c.x, c.y);
+    o1 = make_float4(c.z, c.w, d.x, d.y);
+    o2 = make_float4(d.z, d.w, 0.f, 0.f);
+}
+__device__ __forceinline__ void gd_mom_store(float4* __restrict__ wmom, int i, float4 q0,
+                                             float sxz, float syz, float4 o0, float4 o1,
+                                             float4 o2) {
+    wmom[4 * i] = q0;
+    wmom[4 * i + 1] = make_float4(sxz, syz, o0.x, o0.y);
+    wmom[4 * i + 2] = make_float4(o0.z, o0.w, o1.x, o1.y);
+    wmom[4 * i + 3] = make_float4(o1.z, o1.w, o2.x, o2.y);
 }
 
 /* The dispatcher hands workgroups to the 8 XCDs round-robin (block b runs on

@@ -634,8 +634,7 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
 __global__ __launch_bounds__(256) void k_samples_eval(
     const float4* __restrict__ pos, const int4* __restrict__ meta,
     const int2* __restrict__ wlink,
-    const float4* __restrict__ wcom, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct,
+    const float4* __restrict__ wcom, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int* __restrict__ targets,
     const float4* __restrict__ centre, const int2* __restrict__ pairs, long long npairs,
     double eps2, float* __restrict__ X, float* __restrict__ Y) {
@@ -666,7 +665,8 @@ __global__ __launch_bounds__(256) void k_samples_eval(
         }
         if (lane == 0) {
             const float4 c = wcom[idx];
-            const float4 q0 = wquad[2 * idx], q1 = wquad[2 * idx + 1];
+            float4 q0, q1;
+            gd_mom_quad(wmom, idx, q0, q1);
             const double size = ldexp((double)info->box.extent, -(wlink[idx].y & 31));
             const double dx = (double)c.x - x.x, dy = (double)c.y - x.y, dz = (double)c.z - x.z;
             const double rc2 = dx * dx + dy * dy + dz * dz;
@@ -694,7 +694,8 @@ __global__ __launch_bounds__(256) void k_samples_eval(
             const double r = sqrt(gx * gx + gy * gy + gz * gz);
             const double ux = gx / r, uy = gy / r, uz = gz / r;
             const double k3 = (double)c.w * size * size * size;
-            const float4 o0 = woct[3 * idx], o1 = woct[3 * idx + 1], o2 = woct[3 * idx + 2];
+            float4 o0, o1, o2;
+            gd_mom_oct(wmom, idx, o0, o1, o2);
             const double xx = ux * ux, yy = uy * uy, zz = uz * uz;
             const double xy2 = 2.0 * ux * uy, xz2 = 2.0 * ux * uz, yz2 = 2.0 * uy * uz;
             /* O_ijk u_j u_k and the trace vector T_i = O_ijj */
@@ -728,7 +729,7 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         return 0;
     }
     gd_builder* b = gd_builder_of(tree->builder);
-    if (!(b->quad == 2 && b->wquad && b->woct)) {
+    if (!(b->quad == 2 && b->wmom)) {
         GD_FAIL("gd_corrector_samples: build the tree with second and third moments first "
                 "(gd_set_quadrupole(tree, 2))");
         return 0;
@@ -768,7 +769,7 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         if (rows > 0) {
             long long blocks = (rows + 3) / 4;
             if (blocks > 65536) blocks = 65536;
-            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, b->meta, b->wlink, b->wcom, b->wquad, b->woct,
+            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, b->meta, b->wlink, b->wcom, b->wmom,
                                                  b->dinfo, targets, centre, pairs, rows,
                                                  (double)eps2, X, Y);
             GD_LAUNCHED();

@@ -646,8 +646,7 @@ __device__ __forceinline__ void gd_cell_features(float4 c, float4 o0, float4 o1,
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_group(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int* __restrict__ list, int cap, int2 cnt,
     long long start, int count, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sE, float4* sQ0, float4* sQ1,
@@ -701,8 +700,7 @@ __device__ __forceinline__ void gd_eval_group(
         if (e < nb) {
             idx = list[cap - 1 - e];
             c = wcom[idx];
-            q0 = wquad[2 * idx];
-            q1 = wquad[2 * idx + 1];
+            gd_mom_quad(wmom, idx, q0, q1);
         }
         float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
         sE[lane] = c;
@@ -714,9 +712,7 @@ __device__ __forceinline__ void gd_eval_group(
             float4 o0 = zero4, o1 = zero4, o2 = zero4;
             float size = 1.f;
             if (idx >= 0) {
-                o0 = woct[3 * idx];
-                o1 = woct[3 * idx + 1];
-                o2 = woct[3 * idx + 2];
+                gd_mom_oct(wmom, idx, o0, o1, o2);
                 size = ldexpf(ext, -(wlink[idx].y & 31));
             }
             float f[8], k;
@@ -754,8 +750,7 @@ __device__ __forceinline__ void gd_eval_group(
 template <int MODE, bool PHI>
 __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange,
     const int* __restrict__ groupList, int slots,
     const int* __restrict__ lists, const long long* __restrict__ offsets,
@@ -792,7 +787,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     unsigned long long nn = 0;
     const int* list = offsets ? lists + offsets[g] : lists + (size_t)s * cap;
     const int size = offsets ? caps[g] : cap;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, list, size, cnt, rg.x, rg.y,
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wmom, info, list, size, cnt, rg.x, rg.y,
                              eps2,
                              mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
@@ -818,8 +813,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
 template <int MODE, bool PHI>
 __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange,
     const int* __restrict__ groupList, int slots, int segs,
     const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
@@ -861,7 +855,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
     unsigned long long nn = 0;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wquad, woct, info,
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wmom, info,
                              lists + (size_t)s * cap + f0, cap - f0 - b0,
                              make_int2(f1 - f0, b1 - b0), rg.x, rg.y, eps2, mlpW, mlpB,
                              minRatio, sE[wave], sQ0[MODE != 0 ? wave : 0],
@@ -919,7 +913,7 @@ __global__ __launch_bounds__(256) void k_walk_eval_sum(const int2* __restrict__ 
  */
 __device__ __forceinline__ int gd_flush_corr(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
-    const float4* __restrict__ woct, float ext, const gd_bf16x4* __restrict__ W,
+    const float4* __restrict__ wmom, float ext, const gd_bf16x4* __restrict__ W,
     const float* __restrict__ Bv, float minRatio, const int* queue, int qn, int* map,
     float gax, float gay, float gaz, float gbx, float gby, float gbz, bool inB, float px,
     float py, float pz, float4* rec, float& ax, float& ay, float& az) {
@@ -932,9 +926,7 @@ __device__ __forceinline__ int gd_flush_corr(
     float size = 1.f;
     if (have) {
         c = wcom[idx];
-        o0 = woct[3 * idx];
-        o1 = woct[3 * idx + 1];
-        o2 = woct[3 * idx + 2];
+        gd_mom_oct(wmom, idx, o0, o1, o2);
         size = ldexpf(ext, -(wlink[idx].y & 31));
     }
     float f[8], k;
@@ -1001,8 +993,7 @@ __device__ __forceinline__ int gd_flush_corr(
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_pair(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
     const int* __restrict__ listA, int capA, int2 cA, const int* __restrict__ listB, int capB,
     int2 cB, long long start, int nA, int nB, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sA, float4* sB, float4* sQ,
@@ -1067,14 +1058,12 @@ __device__ __forceinline__ void gd_eval_pair(
         if (e < nbA) {
             ia = listA[capA - 1 - e];
             ca = wcom[ia];
-            qa0 = wquad[2 * ia];
-            qa1 = wquad[2 * ia + 1];
+            gd_mom_quad(wmom, ia, qa0, qa1);
         }
         if (e < nbB) {
             ib = listB[capB - 1 - e];
             cb = wcom[ib];
-            qb0 = wquad[2 * ib];
-            qb1 = wquad[2 * ib + 1];
+            gd_mom_quad(wmom, ib, qb0, qb1);
         }
         recA[0] = ca;
         recA[1] = make_float4(7.5f * qa0.x, 7.5f * qa0.y, 7.5f * qa0.z, 7.5f * qa0.w);
@@ -1104,7 +1093,7 @@ __device__ __forceinline__ void gd_eval_pair(
                 const int m = __popcll(bal);
                 if (qn + m > 64) {   /* wave-uniform */
                     *nnEvals += (unsigned long long)gd_flush_corr(
-                        wcom, wlink, woct, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax,
+                        wcom, wlink, wmom, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax,
                         gay, gaz, gbx, gby, gbz, inB, px, py, pz, sQ, lx, ly, lz);
                     qn = 0;
                 }
@@ -1122,7 +1111,7 @@ __device__ __forceinline__ void gd_eval_pair(
     if (MODE == 2) {
         if (qn > 0)
             *nnEvals += (unsigned long long)gd_flush_corr(
-                wcom, wlink, woct, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax, gay, gaz,
+                wcom, wlink, wmom, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax, gay, gaz,
                 gbx, gby, gbz, inB, px, py, pz, sQ, lx, ly, lz);
         gd_kahan_add(tot.x, tot.cx, lx);
         gd_kahan_add(tot.y, tot.cy, ly);
@@ -1137,8 +1126,7 @@ __device__ __forceinline__ void gd_eval_pair(
 template <int MODE, bool PHI, int EVW>
 __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
-    const float4* __restrict__ woct, const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
     const int2* __restrict__ slotRange, int runs, const int* __restrict__ lists,
     const long long* __restrict__ offsets, const int* __restrict__ caps,
     const int2* __restrict__ counts, float3* __restrict__ acc, float* __restrict__ phi,
@@ -1177,7 +1165,7 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
     unsigned long long nn = 0;
-    gd_eval_pair<MODE, PHI>(pos, wcom, wlink, wquad, woct, info, listA, capA, cA, listB, capB,
+    gd_eval_pair<MODE, PHI>(pos, wcom, wlink, wmom, info, listA, capA, cA, listB, capB,
                             cB, rgA.x, rgA.y, rgB.y, eps2, mlpW, mlpB, minRatio, sE[wave],
                             sE[wave] + 64, sE[wave], sMap[MODE == 2 ? wave : 0],
                             sQueue[MODE == 2 ? wave : 0], tot, phs, phc, &nn);
@@ -1211,7 +1199,7 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
 template <int MODE, bool PHI, int STACK>
 __device__ __forceinline__ void gd_walk_one_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, long long i, int* sStack,
     int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
@@ -1283,8 +1271,7 @@ __device__ __forceinline__ void gd_walk_one_body(
             nCell += (unsigned long long)__popcll(cb);
             float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
             if (cell) {
-                q0 = wquad[2 * idx];
-                q1 = wquad[2 * idx + 1];
+                gd_mom_quad(wmom, idx, q0, q1);
                 const float inv2 = inv * inv;
                 const float inv5 = inv * inv2 * inv2;
                 const float sdx = __builtin_fmaf(q0.x, dx, __builtin_fmaf(q0.w, dy, q1.x * dz));
@@ -1331,7 +1318,7 @@ __device__ __forceinline__ void gd_walk_one_body(
 template <int MODE, bool PHI, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wquad,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList,
     const int* __restrict__ bodyList, long long count,
@@ -1361,7 +1348,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
             i = rg.x + (q & 63);
         }
         if (i >= n) continue;   /* wave-uniform */
-        gd_walk_one_body<MODE, PHI, STACK>(pos, wcom, wlink, wquad, info, i, sStack,
+        gd_walk_one_body<MODE, PHI, STACK>(pos, wcom, wlink, wmom, info, i, sStack,
                                            overflowBodies, overflowCount, acc, phi, eps2, G,
                                            theta, stats);
         __builtin_amdgcn_wave_barrier();
@@ -1570,7 +1557,7 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            float* phi, float eps2, float G, int n, const gd_bf16x4* W,
                            const float* Bv, float minRatio) {
     k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, groupList, slots,
+        pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, groupList, slots,
         lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.kstats);
 }
 
@@ -1621,7 +1608,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         /* one wavefront per run: both halves' lists side by side */
         constexpr int EVW = MODE == 2 ? 8 : 4;   /* learned: 8 share the staged weights */
         k_walk_eval_pair<MODE, PHI, EVW><<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange, runs,
+            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
             g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2, G, W, Bv,
             minRatio, g_walk.kstats);
     }
@@ -1651,7 +1638,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
         k_walk_eval_seg<MODE, PHI><<<GD_SEG_WAVES / GD_EV_WAVES, 64 * GD_EV_WAVES, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
+            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange,
             g_walk.overflow1, S2, 0, g_walk.pool, g_walk.counts2, cap2, g_walk.segPart, eps2, W,
             Bv, minRatio, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
@@ -1662,12 +1649,12 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         const long long bodies = 64ll * groups;
         k_walk_body<BMODE0, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
             <<<GD_ASYNC_BODY_BLOCKS, 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
-                pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, g_walk.overflow2,
+                pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
                 nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
                 g_walk.stats, &st->over2, 64);
         GD_LAUNCHED();
         k_walk_body<BMODE0, PHI, 1, GD_BODY_STACK><<<GD_ASYNC_DEEP_BLOCKS, 64, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, nullptr,
+            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, nullptr,
             g_walk.bodyOver, bodies, nullptr, nullptr, acc, phi, eps2, G, theta, n,
             g_walk.stats, &st->over3, 1);
         GD_LAUNCHED();
@@ -1719,7 +1706,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         if (segs > 1) {
             k_walk_eval_seg<MODE, PHI>
                 <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
-                    pos, b->wcom, b->wlink, b->wquad, b->woct, b->dinfo, g_walk.slotRange,
+                    pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange,
                     g_walk.overflow1 + b0, cnt, segs, g_walk.pool, g_walk.counts2, cap2,
                     g_walk.segPart, eps2, W, Bv, minRatio, g_walk.kstats);
             GD_LAUNCHED();
@@ -1748,7 +1735,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
     k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
         <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, g_walk.overflow2,
+            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
             nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
             g_walk.stats);
     GD_LAUNCHED();
@@ -1756,7 +1743,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
     k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64, 0, gd_stream()>>>(
-        pos, b->wcom, b->wlink, b->wquad, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
+        pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
         h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, g_walk.stats);
     GD_LAUNCHED();
 }
@@ -1794,11 +1781,11 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
     }
     if (starCount <= 0) return;
     gd_builder* b = gd_builder_of(tree->builder);
-    if (mode != 0 && !(b->quad && b->wquad)) {
+    if (mode != 0 && !(b->quad && b->wmom)) {
         GD_FAIL("gd_walk: this walk needs the second moments (gd_set_quadrupole before the build)");
         return;
     }
-    if (mode == 2 && !(b->quad == 2 && b->woct)) {
+    if (mode == 2 && !(b->quad == 2 && b->wmom)) {
         GD_FAIL("gd_walk_learned: the network needs the third moments "
                 "(gd_set_quadrupole(tree, 2) before the build)");
         return;
@@ -1873,7 +1860,7 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
 extern "C" void gd_walk(const treeState* tree, const float4* pos, float3* acc, float* phi,
                         float eps2, float G, float theta, int starCount, gd_walk_stats* stats) {
     const gd_builder* b = tree && tree->builder ? gd_builder_of(tree->builder) : nullptr;
-    const int mode = (b && b->quad && b->wquad) ? 1 : 0;
+    const int mode = (b && b->quad && b->wmom) ? 1 : 0;
     gd_walk_impl(tree, pos, acc, phi, eps2, G, theta, starCount, stats, mode, nullptr, 0.f);
 }
 
