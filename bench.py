@@ -212,6 +212,9 @@ def stage_profile(walks, mode, theta, dt, n, hilo, reps=5):
     wm = np.array(wms[1:]).mean(0)
     im = float(np.mean(ims[1:]))
     nodes = sim.tree_summary()[0]
+    link = np.empty((nodes, 2), np.int32)
+    lib.gd_tree_walk_arrays(C.byref(sim.state), None, link.ctypes.data, None, nodes)
+    inner = int((link[:, 0] >= 0).sum())
     moments = mode != "plain"
     lo = 24 if hilo else 0
     byts = {
@@ -219,7 +222,12 @@ def stage_profile(walks, mode, theta, dt, n, hilo, reps=5):
         "keys": 28 * n,
         "sort": 200 * n,
         "reorder": (60 + lo) * n,
-        "build_com": 24 * n + (80 + (24 if moments else 0)) * nodes,
+        # topology: keys read twice, 16 B meta per node written; centre of
+        # mass: meta read twice (as a child, as a node), 16 B centre of mass
+        # + 8 B link written and the centre of mass read back as a child,
+        # every body's position once; with moments the 64-byte record of
+        # each internal node written and read back as a child
+        "build_com": 32 * n + 88 * nodes + (128 * inner if moments else 0),
         "integrate": (64 + lo) * n,
     }
     ms = {"bbox": bm[0], "keys": bm[1], "sort": bm[2], "reorder": bm[3],
@@ -230,8 +238,10 @@ def stage_profile(walks, mode, theta, dt, n, hilo, reps=5):
         out[k] = {"ms": float(v), "bytes": int(byts[k]), "GB_per_s": gbs,
                   "frac_of_8TBps": gbs / HBM_PEAK_GBS}
     out["build_com"].update({"topology_ms": float(bm[4]), "centre_of_mass_ms": float(bm[5]),
-                             "nodes": int(nodes),
-                             "bytes_formula": "24 N + (80 + 24 with moments) M, M nodes"})
+                             "nodes": int(nodes), "internal_nodes": inner,
+                             "bytes_formula": "32 N + 88 M + 128 I with moments "
+                                              "(M nodes, I internal)",
+                             "node_records": "on demand (not written by the step)"})
     out["walk"] = {"traversal_ms": float(wm[0]), "evaluation_ms": float(wm[1]),
                    "second_pass_and_per_body_ms": float(wm[2])}
     out["step_ms_sum"] = float(bm.sum() + wm.sum() + im)
@@ -392,6 +402,9 @@ def main():
     n = pos.shape[0]
     log(f"{n} bodies")
     sim = Simulation(pos, vel, eps=eps, hilo=hilo)
+    # the engine's walks read only the walk arrays; the reference's 80-byte
+    # node array is written on demand (the reference-walk leg asks for it)
+    sim.set_node_records(False)
     net = cr.Corrector(seed=1)
     walks = Walks(sim, net, args.min_ratio)
     cal = ics.subsample(n, 4096, seed=CAL_SEED)

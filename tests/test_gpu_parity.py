@@ -505,3 +505,35 @@ def test_root_centre_of_mass_fp64():
         nodes = sim.nodes_raw(1).view(ob.NODE_DTYPE).reshape(-1)
         assert nodes[0]["mass"][3] == np.float32(nodes[0]["mass"][3])
         assert abs(float(nodes[0]["mass"][3]) - want[3]) < 1e-6 * want[3]
+
+
+@pytest.mark.parametrize("n,mom", [(1, 2), (7, 0), (5000, 1), (1 << 16, 2)])
+def test_node_records_on_demand(n, mom):
+    """An engine build with the node array off writes the same walk arrays,
+    and gd_write_node_records then the same 80-byte node array, byte for
+    byte, as a build with it on (same bodies, same moments)."""
+    if n >= 1000:
+        pos, vel = ics.disk_merger(n // 2, seed=12)
+    else:
+        rng = np.random.default_rng(12)
+        pos = rng.random((n, 4)).astype(np.float32)
+        pos[:, 3] += 0.5
+        vel = np.zeros((n, 3), np.float32)
+    assert pos.shape[0] == n
+    print(f"case records on demand N={n} moments={mom}", flush=True)
+    got = {}
+    for rec in (True, False):
+        with Simulation(pos, vel, eps=0.01) as sim:
+            sim.set_quadrupole(mom)
+            sim.set_node_records(rec)
+            sim.build_tree_async()
+            total = sim.tree_summary()[0]
+            com, link, quad = sim.walk_arrays(total)
+            quad = quad if mom else None
+            third = sim.third_moments(total) if mom == 2 else None
+            got[rec] = (sim.nodes_raw(total).copy(), com, link, quad, third)
+    for a, b in zip(got[True], got[False]):
+        if a is None:
+            assert b is None
+        else:
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
