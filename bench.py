@@ -220,7 +220,7 @@ def stage_profile(walks, mode, theta, dt, n, hilo, reps=5):
     byts = {
         "bbox": (16 + (12 if hilo else 0)) * n,   # + the low parts for the FP64 root sums
         "keys": 28 * n,
-        "sort": 200 * n,
+        "sort": 256 * n,   # 8 passes x (8 B counted + 12 B read + 12 B written)
         "reorder": (60 + lo) * n,
         # topology: keys read twice, 16 B meta per node written; centre of
         # mass: meta read twice (as a child, as a node), 16 B centre of mass

@@ -131,15 +131,16 @@ radixSorter* sorterCreate(int maxCount) {
     const size_t statusBytes = sizeof(unsigned long long) * (size_t)s->maxTiles * GD_SORT_BINS;
     GD_HIP(hipMalloc(&s->keysTmp, sizeof(uint64_t) * (size_t)maxCount));
     GD_HIP(hipMalloc(&s->valsTmp, sizeof(uint32_t) * (size_t)maxCount));
-    GD_HIP(hipMalloc(&s->hist, sizeof(uint32_t) * GD_SORT_PASSES * GD_SORT_BINS));
-    GD_HIP(hipMalloc(&s->tileCounter, sizeof(uint32_t) * GD_SORT_PASSES));
+    GD_HIP(hipMalloc(&s->hist, sizeof(uint32_t) * GD_SORT_PASSES * GD_SORT_GROUPS * GD_SORT_BINS));
+    GD_HIP(hipMalloc(&s->tileCounter, sizeof(uint32_t) * GD_SORT_PASSES * GD_SORT_GROUPS));
     GD_HIP(hipMalloc(&s->status, statusBytes));
     GD_HIP(hipMemset(s->status, 0, statusBytes));
     GD_HIP(hipMalloc(&s->epoch, sizeof(uint32_t)));
     GD_HIP(hipMemset(s->epoch, 0, sizeof(uint32_t)));
     s->pub.tempStorage = s->status;
-    s->pub.tempStorageBytes = statusBytes + sizeof(uint32_t) * (GD_SORT_PASSES * GD_SORT_BINS +
-                                                                GD_SORT_PASSES) +
+    s->pub.tempStorageBytes = statusBytes +
+                              sizeof(uint32_t) * GD_SORT_PASSES * GD_SORT_GROUPS *
+                                  (GD_SORT_BINS + 1) +
                               (sizeof(uint64_t) + sizeof(uint32_t)) * (size_t)maxCount;
     s->pub.capacity = maxCount;
     if (gd_has_error()) {

@@ -64,6 +64,7 @@ int gd_graphs_hold_buffers(void);
 #define GD_SORT_TILE (GD_SORT_THREADS * GD_SORT_ITEMS)
 #define GD_SORT_PASSES 8
 #define GD_SORT_BINS 256
+#define GD_SORT_GROUPS 8            /* tile ranges, one per XCD */
 
 /* exclusive scan (prefixSum): one tile = 256 threads x 8 items */
 #define GD_SCAN_THREADS 256
@@ -97,8 +98,8 @@ typedef struct {
     radixSorter pub;
     uint64_t* keysTmp;               /* ping-pong key buffer, capacity */
     uint32_t* valsTmp;               /* ping-pong value buffer, capacity */
-    uint32_t* hist;                  /* [8][256] digit counts, then exclusive offsets */
-    uint32_t* tileCounter;           /* [8] tile tickets, one per pass */
+    uint32_t* hist;                  /* [8 passes][8 ranges][256] digit counts, then offsets */
+    uint32_t* tileCounter;           /* [8 passes][8 ranges] tile tickets */
     unsigned long long* status;      /* [maxTiles][256] look-back status words */
     int maxTiles;
     uint32_t* epoch;                 /* device: tag of this sort's first pass (passes add 0..7) */

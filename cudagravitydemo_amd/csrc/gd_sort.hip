@@ -9,14 +9,25 @@
  *
  * Sort design (gfx950, 64-wide wavefronts):
  *   k_sort_init    clears the 8x256 digit histograms and 8 tile tickets
- *   k_sort_hist    one read of the keys builds all 8 digit histograms; each
- *                  thread walks 8 consecutive keys and adds runs of equal
- *                  digits at once (Morton order makes the high digits long
- *                  runs), first into LDS, then into global counters
- *   k_sort_prefix  exclusive scan of each histogram -> digit base offsets
+ *   k_sort_hist    before each pass, that pass's digit counts per tile range
+ *                  (below); each thread walks 8 consecutive keys and adds runs
+ *                  of equal digits at once, first into LDS, then into global
+ *                  counters
+ *   k_sort_prefix  that pass's counts -> each range's digit offsets
  *   k_sort_pass x8 one 8-bit digit per pass over 3840-key tiles:
- *                  - tile ids come from an atomic ticket, so a tile only ever
- *                    waits on tiles that started before it;
+ *                  - the tiles are split into 8 contiguous ranges, one per
+ *                    XCD (block b runs on XCD b % 8): the blocks of XCD x
+ *                    take the tiles of range x in order from an atomic ticket
+ *                    of their own, so a tile only ever waits on tiles that
+ *                    started before it, and consecutive tiles -- which write
+ *                    the two halves of the 64-byte sectors where their runs
+ *                    of a digit meet -- run in one L2, where those halves
+ *                    merge instead of each costing the memory a
+ *                    read-modify-write (a digit's run in a tile is ~15 keys,
+ *                    so most sectors are such boundaries);
+ *                  - the histogram pass counts each range's digits apart, so
+ *                    each range starts from its own offsets and the
+ *                    look-back never leaves its range;
  *                  - keys are ranked inside the tile with wavefront ballots
  *                    (8 ballots give the lanes sharing a digit) and per-wave
  *                    digit counters in LDS, which keeps the order stable;
@@ -28,7 +39,7 @@
  *                    cleared between passes;
  *                  - keys are staged in LDS in digit order and written out
  *                    as runs of consecutive addresses.
- * Traffic per key: 8 B histogram read + 8 x (12 B read + 12 B write).
+ * Traffic per key: 8 x (8 B histogram read + 12 B read + 12 B write).
  */
 #include "gd_internal.h"
 
@@ -84,9 +95,8 @@ __device__ __forceinline__ uint32_t gd_block_exclusive(uint32_t v, uint32_t* tot
  * [1, 2^30 - 2], 0 being the tag of a cleared word. */
 __global__ void k_sort_init(uint32_t* hist, uint32_t* tileCounter, uint32_t* epoch) {
     const int t = threadIdx.x;   /* 256 threads */
-#pragma unroll
-    for (int p = 0; p < GD_SORT_PASSES; ++p) hist[p * GD_SORT_BINS + t] = 0u;
-    if (t < GD_SORT_PASSES) tileCounter[t] = 0u;
+    for (int p = 0; p < GD_SORT_PASSES * GD_SORT_GROUPS; ++p) hist[p * GD_SORT_BINS + t] = 0u;
+    if (t < GD_SORT_PASSES * GD_SORT_GROUPS) tileCounter[t] = 0u;
     if (t == 0) {
         uint32_t e = *epoch + GD_SORT_PASSES;
         if (e == GD_SORT_PASSES || e > 0x3fffffffu - GD_SORT_PASSES) e = 1u;
@@ -97,18 +107,56 @@ __global__ void k_sort_init(uint32_t* hist, uint32_t* tileCounter, uint32_t* epo
 #define GD_HIST_ITEMS 8
 #define GD_HIST_CHUNK (256 * GD_HIST_ITEMS)
 
-__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, int n,
-                                                   uint32_t* hist) {
-    __shared__ uint32_t sh[GD_SORT_PASSES][GD_SORT_BINS];
-    const int t = threadIdx.x;
+/* Range g of the tiles: the blocks b = g, g + 8, ... of a grid of one block
+ * per tile, i.e. (tiles - g + 7) / 8 tiles, the first tiles % 8 ranges one
+ * more than the others, in order. */
+__device__ __forceinline__ void gd_sort_range(int tiles, int g, int* start, int* count) {
+    const int q = tiles / GD_SORT_GROUPS, r = tiles % GD_SORT_GROUPS;
+    *start = g * q + (g < r ? g : r);
+    *count = q + (g < r ? 1 : 0);
+}
+
+/* hist[g][d] (range g's count of digit d) -> the global position where
+ * range g's keys of digit d start: all keys of smaller digits, then those
+ * of digit d in earlier ranges.  One block of 256 threads, thread = digit. */
+__device__ __forceinline__ void gd_sort_offsets(uint32_t* h) {
+    const int d = threadIdx.x;
+    uint32_t c[GD_SORT_GROUPS], tot = 0;
 #pragma unroll
-    for (int p = 0; p < GD_SORT_PASSES; ++p) sh[p][t] = 0u;
+    for (int g = 0; g < GD_SORT_GROUPS; ++g) {
+        c[g] = h[g * GD_SORT_BINS + d];
+        tot += c[g];
+    }
+    uint32_t run = gd_block_exclusive<256>(tot, nullptr);
+#pragma unroll
+    for (int g = 0; g < GD_SORT_GROUPS; ++g) {
+        h[g * GD_SORT_BINS + d] = run;
+        run += c[g];
+    }
+}
+
+/* One pass's digit counts per tile range, over that pass's input: blocks
+ * g, g + 8, ... count range g's keys into hist[g][digit] (the ranges'
+ * counts change from pass to pass, as the keys move between ranges). */
+__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, int n,
+                                                   int shift, uint32_t* hist) {
+    __shared__ uint32_t sh[GD_SORT_BINS];
+    const int t = threadIdx.x;
+    const int g = blockIdx.x % GD_SORT_GROUPS;
+    const int nb = (gridDim.x - g + GD_SORT_GROUPS - 1) / GD_SORT_GROUPS;   /* blocks of range g */
+    const int bi = blockIdx.x / GD_SORT_GROUPS;
+    int rs, rc;
+    gd_sort_range((int)(((long long)n + GD_SORT_TILE - 1) / GD_SORT_TILE), g, &rs, &rc);
+    const long long k0 = (long long)rs * GD_SORT_TILE;
+    const long long kEnd = (long long)(rs + rc) * GD_SORT_TILE;
+    const long long k1 = kEnd < n ? kEnd : n;
+    sh[t] = 0u;
     __syncthreads();
-    for (long long base = (long long)blockIdx.x * GD_HIST_CHUNK; base < n;
-         base += (long long)gridDim.x * GD_HIST_CHUNK) {
+    for (long long base = k0 + (long long)bi * GD_HIST_CHUNK; base < k1;
+         base += (long long)nb * GD_HIST_CHUNK) {
         const long long i0 = base + (long long)t * GD_HIST_ITEMS;
-        if (i0 >= n) continue;
-        const int cnt = (n - i0) < GD_HIST_ITEMS ? (int)(n - i0) : GD_HIST_ITEMS;
+        if (i0 >= k1) continue;
+        const int cnt = (k1 - i0) < GD_HIST_ITEMS ? (int)(k1 - i0) : GD_HIST_ITEMS;
         uint64_t k[GD_HIST_ITEMS];
         if (cnt == GD_HIST_ITEMS) {
             const ulonglong2* src = reinterpret_cast<const ulonglong2*>(keys + i0);
@@ -122,38 +170,27 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
 #pragma unroll
             for (int j = 0; j < GD_HIST_ITEMS; ++j) k[j] = j < cnt ? keys[i0 + j] : 0ull;
         }
-#pragma unroll
-        for (int p = 0; p < GD_SORT_PASSES; ++p) {
-            uint32_t prev = (uint32_t)(k[0] >> (8 * p)) & 0xffu;
-            uint32_t run = 1;
-            for (int j = 1; j < cnt; ++j) {
-                const uint32_t d = (uint32_t)(k[j] >> (8 * p)) & 0xffu;
-                if (d == prev) {
-                    ++run;
-                } else {
-                    atomicAdd(&sh[p][prev], run);
-                    prev = d;
-                    run = 1;
-                }
+        uint32_t prev = (uint32_t)(k[0] >> shift) & 0xffu;
+        uint32_t run = 1;
+        for (int j = 1; j < cnt; ++j) {
+            const uint32_t d = (uint32_t)(k[j] >> shift) & 0xffu;
+            if (d == prev) {
+                ++run;
+            } else {
+                atomicAdd(&sh[prev], run);
+                prev = d;
+                run = 1;
             }
-            atomicAdd(&sh[p][prev], run);
         }
+        atomicAdd(&sh[prev], run);
     }
     __syncthreads();
-#pragma unroll
-    for (int p = 0; p < GD_SORT_PASSES; ++p) {
-        const uint32_t c = sh[p][t];
-        if (c) atomicAdd(&hist[p * GD_SORT_BINS + t], c);
-    }
+    const uint32_t c = sh[t];
+    if (c) atomicAdd(&hist[g * GD_SORT_BINS + t], c);
 }
 
-/* hist[p][*] -> exclusive offsets, one block (256 threads) per pass. */
-__global__ __launch_bounds__(256) void k_sort_prefix(uint32_t* hist) {
-    uint32_t* h = hist + blockIdx.x * GD_SORT_BINS;
-    const uint32_t v = h[threadIdx.x];
-    const uint32_t ex = gd_block_exclusive<256>(v, nullptr);
-    h[threadIdx.x] = ex;
-}
+/* One pass's counts -> offsets (one block of 256 threads). */
+__global__ __launch_bounds__(256) void k_sort_prefix(uint32_t* hist) { gd_sort_offsets(hist); }
 
 __device__ __forceinline__ unsigned long long gd_pack_status(uint32_t epoch, uint32_t flag,
                                                              uint32_t count) {
@@ -178,12 +215,17 @@ __global__ __launch_bounds__(GD_SORT_THREADS) void k_sort_pass(
     const unsigned lane = gd_lane();
     const int wave = t / GD_WAVE;
     const uint32_t epoch = *epochBase + (uint32_t)pass;
-    if (t == 0) sTile = atomicAdd(tileCounter, 1u);
+    const int g = blockIdx.x % GD_SORT_GROUPS;   /* the XCD's tile range */
+    if (t == 0) sTile = atomicAdd(&tileCounter[g], 1u);
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) sWaveHist[w][t] = 0u;
     __syncthreads();
-    const int tile = (int)sTile;
+    int rs, rc;
+    gd_sort_range((int)gridDim.x, g, &rs, &rc);
+    const int inRange = (int)sTile;               /* < rc: range g has rc blocks */
+    const int tile = rs + inRange;
     const long long tileBase = (long long)tile * GD_SORT_TILE;
+    digitBase += g * GD_SORT_BINS;
 
     /* load: item j of this lane sits at tileBase + wave*WAVE_SPAN + j*64 + lane */
     uint64_t key[GD_SORT_ITEMS];
@@ -238,7 +280,7 @@ __global__ __launch_bounds__(GD_SORT_THREADS) void k_sort_pass(
         }
     }
     unsigned long long* myStatus = status + (size_t)tile * GD_SORT_BINS + t;
-    if (tile == 0) {
+    if (inRange == 0) {
         __hip_atomic_store(myStatus, gd_pack_status(epoch, GD_FLAG_PREFIX, tot),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -248,9 +290,10 @@ __global__ __launch_bounds__(GD_SORT_THREADS) void k_sort_pass(
     const uint32_t tileStart = gd_block_exclusive<GD_SORT_THREADS>(tot, nullptr);
     sTileStart[t] = tileStart;
 
-    /* decoupled look-back for digit t */
+    /* decoupled look-back for digit t (the range's first tile publishes an
+       inclusive prefix, so the walk never leaves the range) */
     uint32_t excl = 0;
-    if (tile > 0) {
+    if (inRange > 0) {
         const uint32_t want = (epoch & 0x3fffffffu);
         int pred = tile - 1;
         while (true) {
@@ -308,19 +351,22 @@ void gd_sort_pairs(gd_sorter* s, const uint64_t* kin, uint64_t* kout, const uint
     GD_LAUNCHED();
     int hb = gd_div_up(n, GD_HIST_CHUNK);
     if (hb > 1024) hb = 1024;
-    k_sort_hist<<<hb, 256, 0, gd_stream()>>>(kin, n, s->hist);
-    GD_LAUNCHED();
-    k_sort_prefix<<<GD_SORT_PASSES, 256, 0, gd_stream()>>>(s->hist);
-    GD_LAUNCHED();
+    hb = gd_div_up(hb, GD_SORT_GROUPS) * GD_SORT_GROUPS;   /* whole rounds of the 8 ranges */
     const uint64_t* ksrc = kin;
     const uint32_t* vsrc = vin;
     for (int p = 0; p < GD_SORT_PASSES; ++p) {
+        uint32_t* h = s->hist + p * GD_SORT_GROUPS * GD_SORT_BINS;
+        k_sort_hist<<<hb, 256, 0, gd_stream()>>>(ksrc, n, 8 * p, h);
+        GD_LAUNCHED();
+        k_sort_prefix<<<1, 256, 0, gd_stream()>>>(h);
+        GD_LAUNCHED();
         /* even passes write the scratch pair, odd passes the output pair, so
            the eighth pass lands in keysOut/valsOut */
         uint64_t* kdst = (p & 1) ? kout : s->keysTmp;
         uint32_t* vdst = (p & 1) ? vout : s->valsTmp;
         k_sort_pass<<<tiles, GD_SORT_THREADS, 0, gd_stream()>>>(ksrc, vsrc, kdst, vdst, n, 8 * p,
-                                                s->hist + p * GD_SORT_BINS, s->tileCounter + p,
+                                                s->hist + p * GD_SORT_GROUPS * GD_SORT_BINS,
+                                                s->tileCounter + p * GD_SORT_GROUPS,
                                                 s->status, s->epoch, p);
         GD_LAUNCHED();
         ksrc = kdst;
