@@ -237,6 +237,10 @@ def stage_profile(walks, mode, theta, dt, n, hilo, reps=5):
         gbs = byts[k] / (v * 1e-3) / 1e9
         out[k] = {"ms": float(v), "bytes": int(byts[k]), "GB_per_s": gbs,
                   "frac_of_8TBps": gbs / HBM_PEAK_GBS}
+    # the sort counts each pass's digits per tile range over that pass's
+    # input (8 B per key per pass) on top of an 8-pass LSD sort's minimum
+    out["sort"].update({"bytes_minimum_8_pass_lsd": 200 * n,
+                        "frac_of_8TBps_on_minimum": 200 * n / (bm[2] * 1e-3) / 1e9 / HBM_PEAK_GBS})
     out["build_com"].update({"topology_ms": float(bm[4]), "centre_of_mass_ms": float(bm[5]),
                              "nodes": int(nodes), "internal_nodes": inner,
                              "bytes_formula": "32 N + 88 M + 128 I with moments "
@@ -277,22 +281,28 @@ def eval_roofline(walks, mode, theta, n):
     else:
         executed = FLOP_INTERACTION * mono + (FLOP_QUAD + FLOP_CORR) * cells
     mfma = FLOP_MLP * st.get("network_evals", 0) if mode == "learned" else 0
-    algo = FLOP_INTERACTION * inter
+    # 20 FLOP per accepted body-node interaction + 9 per opening test that
+    # opened a node (the tests run in the traversal kernel: ~0.3 % here)
+    algo = FLOP_INTERACTION * inter + FLOP_OPEN * st["opened"]
     achieved = algo / (ev_ms * 1e-3) / 1e12
     tr = MEASURED_TRAFFIC.get((mode, theta, n)) or {}
     return {
         "bound": "mfma",
-        "compute": ("FP32 (vector ALU; the walk has no matrix-core work except the learned "
-                    "walk's network, reported apart); peak = MI355X FP32 157.3 TFLOP/s"),
+        "compute": ("FP32 vector ALU (the walk has no matrix-core work except the learned "
+                    "walk's network, reported apart); peak = MI355X FP32 157.3 TFLOP/s, which "
+                    "counts packed v_pk_fma_f32; this build runs with packed FP32 off, so "
+                    "78.6 TFLOP/s is the reachable ceiling (frac_of_unpacked_fp32)"),
         "kernel": f"k_walk_eval_pair ({mode} walk, theta={theta})",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
+        "frac_of_unpacked_fp32": achieved / (FP32_PEAK_TFLOPS / 2),
         "traffic": tr.get("bytes"),
         "traffic_unit": "bytes per launch (HBM read + write)",
         "traffic_source": tr.get("profile"),
-        "numerator": "20 FLOP x accepted body-node interactions per launch",
+        "numerator": ("20 FLOP x accepted body-node interactions + 9 FLOP x opening tests that "
+                      "opened a node, per walk"),
         "flop_per_launch": int(algo),
         "kernel_ms": ev_ms,
         "executed_flop_per_launch": int(executed),

@@ -52,6 +52,8 @@ def main():
     if a.walk == "learned":
         net = cr.Corrector(np.load(a.params) if a.params else None, seed=1)
     with Simulation(pos, vel, eps=eps, hilo=a.hilo) as sim:
+        if a.walk != "reference":
+            sim.set_node_records(False)   # as bench.py's engine step
         if a.list_cap:
             sim.lib.gd_walk_set_list_capacity(a.list_cap, 0)
         sim.lib.gd_walk_set_group_split(a.group_split)
