@@ -58,13 +58,11 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
 # (walk, theta, bodies) of the workload they were measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75, 1 << 21): {"bytes": 630689294,
+    ("learned", 0.75, 1 << 21): {"bytes": 695638019,
                                  "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.675, 1 << 24): {"bytes": 6329801814,
+    ("learned", 0.675, 1 << 24): {"bytes": 7069520245,
                                   "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
-    ("learned", 0.7, 1 << 24): {"bytes": 5994216230,
-                                "profile": "profiles/pmc_walk_eval_learned_theta0.7_merger16M.md"},
-    ("learned", 0.6875, 1 << 24): {"bytes": 6084437655,
+    ("learned", 0.6875, 1 << 24): {"bytes": 6812063984,
                                    "profile": "profiles/pmc_walk_eval_learned_theta0.6875_merger16M.md"},
 }
 BUDGET = 1e-3
