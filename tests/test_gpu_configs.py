@@ -184,10 +184,14 @@ def test_full_size_learned_run_with_judge_and_fallback():
                 sim.step(dt, 0.6)
             sim.set_quadrupole(2)
 
-        info = cr.train_on_snapshots(sim, net, advance, snapshots=4, targets=8192)
+        # corrections above s/r 0.55 (the library's default): at theta 0.7
+        # the learned walk then meets the budget (the bench's 0.5875 is the
+        # faster setting at its own, calibrated theta)
+        info = cr.train_on_snapshots(sim, net, advance, snapshots=4, targets=8192,
+                                     min_ratio=0.55)
         _say(f"full size: corrector trained on {info['samples']} samples in "
              f"{info['seconds']:.1f} s, loss {info['losses'][0]:.3f} -> {info['losses'][-1]:.3f}")
-        run = cr.JudgedRun(sim, net, theta, dt, every=every, sample_seed=4242)
+        run = cr.JudgedRun(sim, net, theta, dt, min_ratio=0.55, every=every, sample_seed=4242)
         try:
             t0 = time.perf_counter()
             run.run(steps)
@@ -213,7 +217,7 @@ def test_full_size_learned_run_with_judge_and_fallback():
 
         # a network whose output is scaled wrong: the first check switches it off
         bad = cr.Corrector(cr.scale_output(net.params(), -25.0))
-        run2 = cr.JudgedRun(sim, bad, theta, dt, every=every, sample_seed=4243)
+        run2 = cr.JudgedRun(sim, bad, theta, dt, min_ratio=0.55, every=every, sample_seed=4243)
         try:
             run2.run(2)
             c = run2.checks
