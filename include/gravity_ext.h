@@ -79,9 +79,9 @@ int gd_build_phase_ms(float* ms);
 
 /* Makes every later build also compute each node's moments about its centre
  * of mass: enable = 1 the second moments S = sum m (x-c)(x-c)^T (used by the
- * quadrupole walk; 32 B per node of capacity), 2 also the third moments
- * O = sum m (x-c)(x-c)(x-c) (the learned walk's inputs; 48 B more per
- * node); 0 stops both. */
+ * quadrupole walk), 2 also the third moments O = sum m (x-c)(x-c)(x-c) (the
+ * learned walk's inputs); 0 stops both.  Both live in one 64-byte record per
+ * node of capacity (allocated on the first call that enables them). */
 void gd_set_quadrupole(treeState* tree, int enable);
 
 /* Copies the third moments of nodes [0, count) to host memory, 3 float4 per
