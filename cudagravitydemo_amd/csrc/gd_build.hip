@@ -488,10 +488,15 @@ __device__ __forceinline__ int gd_finish_range(int x0, int x1, int level,
                                                int2* __restrict__ wlink,
                                                float4* __restrict__ wmom) {
     int zeros = 0;
+    /* a long range (leaf-heavy levels) is walked with the next chunk's meta
+       in flight while this chunk's leaves load their bodies */
+    int4 mtNext = make_int4(0, 0, 0, -1);
+    if (x0 + (int)threadIdx.x < x1) mtNext = meta[x0 + (int)threadIdx.x];
     for (int xb = x0; xb < x1; xb += GD_MASS_TILE) {
         const int X = xb + (int)threadIdx.x;
+        const int4 mt = mtNext;
+        if (X + GD_MASS_TILE < x1) mtNext = meta[X + GD_MASS_TILE];
         if (X >= x1) continue;
-        const int4 mt = meta[X];
         if (mt.w >= 0) {   /* internal: its first child's tile slot */
             const int h = (int)(mt.w - tileBase);
             const float4 c = sC[h];
