@@ -217,6 +217,8 @@ treeBuilder* sumCreate(int maxCount) {
     const size_t tiles = (size_t)GD_LEVELS * b->maxTiles;
     GD_HIP(hipMalloc(&b->tileCounts, sizeof(uint32_t) * tiles));
     GD_HIP(hipMalloc(&b->tileOffsets, sizeof(uint32_t) * tiles));
+    GD_HIP(hipMalloc(&b->tileFirst, sizeof(int) * tiles));
+    GD_HIP(hipMalloc(&b->tileNext, sizeof(int) * tiles));
     GD_HIP(hipMalloc(&b->levelTotals, sizeof(uint32_t) * GD_LEVELS));
     GD_HIP(hipMemset(b->levelTotals, 0, sizeof(uint32_t) * GD_LEVELS));
     GD_HIP(hipMalloc(&b->dinfo, sizeof(gd_tree_info)));
@@ -260,6 +262,8 @@ void sumDestroyer(treeBuilder* sum) {
     GD_HIP(hipFree(b->pub.offsetsPong));
     GD_HIP(hipFree(b->tileCounts));
     GD_HIP(hipFree(b->tileOffsets));
+    GD_HIP(hipFree(b->tileFirst));
+    GD_HIP(hipFree(b->tileNext));
     GD_HIP(hipFree(b->levelTotals));
     GD_HIP(hipFree(b->dinfo));
     GD_HIP(hipHostFree(b->hinfo));

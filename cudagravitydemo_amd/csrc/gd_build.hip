@@ -14,15 +14,19 @@
  * (-1 where there is no neighbour), that is  max(A+1, 1) <= L <= min(max(A,B)+1, 21):
  * every body starts a contiguous range of levels, usually one or two.
  *
- *   k_build_count   per 1024-body tile, node starts per level (wave ballots)
- *   k_build_scan    per level, exclusive scan of the tile counts
+ *   k_build_count   per 1024-body tile and level, the node starts and the
+ *                   first body that begins a new cell (wave ballots)
+ *   k_build_scan    per level, exclusive scan of the tile counts, and for each
+ *                   tile the first cell start after it (suffix minimum)
  *   k_build_nodes   node index = level base + rank among that level's starts
  *                   (tile offset + wave offsets + lane rank); the parent is the
  *                   node of the level above that contains the body
  *                   (tree.cu:224-231); particleCount = size of the body's
- *                   level-L cell, found among the wavefront's bodies or by a
- *                   64-way search of the sorted keys (equal to
- *                   tree.cu:181-203's sibling / parent-end rule); the
+ *                   level-L cell: up to the next body that begins a new
+ *                   level-L cell, among the wavefront's bodies or, for a cell
+ *                   that runs past them, from the first starts of the later
+ *                   wavefronts and tiles (equal to tree.cu:181-203's sibling /
+ *                   parent-end rule); the
  *                   first child is the body's node one level down, if it starts
  *                   one.  Written as a 16-byte meta record per node (first
  *                   body, count with the node's octant in bits 29-31, parent,
@@ -76,122 +80,191 @@ __device__ __forceinline__ void gd_body_levels(const uint64_t* sk, int q, long l
     *B = b;
 }
 
+/* Wave-uniform bounds of the levels at which the wavefront's bodies start
+ * nodes: the smallest L in [1, 21] with some lo <= L (22 if none) and the
+ * largest L in [1, 21] with some hi >= L (0 if none), by binary search over
+ * ballots. */
+__device__ __forceinline__ int gd_wave_min_lo(int lo) {
+    int a = 1, b = GD_MAX_LEVEL + 1;
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (__ballot(lo <= mid)) b = mid;
+        else a = mid + 1;
+    }
+    return a;
+}
+__device__ __forceinline__ int gd_wave_max_hi(int hi) {
+    int a = 0, b = GD_MAX_LEVEL;
+    while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (__ballot(hi >= mid)) a = mid;
+        else b = mid - 1;
+    }
+    return a;
+}
+
+/* One wavefront's node starts per level and the first of its bodies that
+ * begins a new level-L cell (n if none), lane L holding level L's values.
+ * Above hi's maximum no body starts a node, and a body that begins a new
+ * cell there also begins one at that maximum (lo <= hi for every body that
+ * starts nodes; one that duplicates its left neighbour's key has lo = 22),
+ * so the first start there is the one at the maximum. */
+__device__ __forceinline__ void gd_wave_level_starts(int lo, int hi, int minLo, int maxHi,
+                                                     int waveBase, int n, int* cntV,
+                                                     int* firstV) {
+    const int lane = (int)gd_lane();
+    int c = 0, f = n;
+    for (int L = minLo; L <= maxHi; ++L) {
+        const unsigned long long b = __ballot(lo <= L);
+        const unsigned long long s = __ballot(L <= hi) & b;
+        const int cL = (int)__popcll(s);
+        const int fL = b ? waveBase + (int)__builtin_ctzll(b) : n;
+        if (lane == L) {
+            c = cL;
+            f = fL;
+        }
+    }
+    if (minLo <= maxHi) {
+        const int fm = __builtin_amdgcn_readlane(f, maxHi);
+        if (lane > maxHi) f = fm;
+    }
+    *cntV = c;
+    *firstV = f;
+}
+
 /* ---------------------------------------------------------------------------
  * Topology
  * ------------------------------------------------------------------------- */
 __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_count(const uint64_t* __restrict__ keys,
                                                                   int n, uint32_t* tileCounts,
-                                                                  int maxTiles) {
+                                                                  int* tileFirst, int maxTiles) {
     constexpr int WAVES = GD_BUILD_THREADS / GD_WAVE;
     __shared__ uint64_t sk[GD_BUILD_TILE + 2];
     __shared__ uint32_t sCount[WAVES][GD_LEVELS];
+    __shared__ int sFirst[WAVES][GD_LEVELS];
     const long long tileBase = (long long)blockIdx.x * GD_BUILD_TILE;
     const int wave = threadIdx.x / GD_WAVE;
     gd_load_key_tile(keys, n, tileBase, sk);
-    if (threadIdx.x < WAVES * GD_LEVELS) (&sCount[0][0])[threadIdx.x] = 0u;
     __syncthreads();
+    const int lane = (int)gd_lane();
+    int cnt = 0, first = n;
     for (int k = 0; k < GD_BUILD_ITEMS; ++k) {
         const int q = k * GD_BUILD_THREADS + threadIdx.x;
         const long long i = tileBase + q;
         int lo = 99, hi = -1, B;
         if (i < n) gd_body_levels(sk, q, i, n, &lo, &hi, &B);
-        for (int L = 1; L <= GD_MAX_LEVEL; ++L) {
-            const unsigned long long bal = __ballot(lo <= L && L <= hi);
-            if (gd_lane() == 0) sCount[wave][L] += (uint32_t)__popcll(bal);
-        }
+        const int minLo = gd_wave_min_lo(lo), maxHi = gd_wave_max_hi(hi);
+        int c, f;
+        gd_wave_level_starts(lo, hi, minLo, maxHi,
+                             (int)tileBase + k * GD_BUILD_THREADS + wave * GD_WAVE, n, &c, &f);
+        cnt += c;
+        first = f < first ? f : first;
+    }
+    if (lane >= 1 && lane <= GD_MAX_LEVEL) {
+        sCount[wave][lane] = (uint32_t)cnt;
+        sFirst[wave][lane] = first;
     }
     __syncthreads();
     if (threadIdx.x >= 1 && threadIdx.x <= GD_MAX_LEVEL) {
         uint32_t s = 0;
-        for (int w = 0; w < WAVES; ++w) s += sCount[w][threadIdx.x];
+        int f = n;
+        for (int w = 0; w < WAVES; ++w) {
+            s += sCount[w][threadIdx.x];
+            f = sFirst[w][threadIdx.x] < f ? sFirst[w][threadIdx.x] : f;
+        }
         tileCounts[(size_t)threadIdx.x * maxTiles + blockIdx.x] = s;
+        tileFirst[(size_t)threadIdx.x * maxTiles + blockIdx.x] = f;
     }
 }
 
-/* Block L-1 scans level L's tile counts; levelTotals[L] gets the sum. */
+/* Block L-1 scans level L's tile counts; levelTotals[L] gets the sum.  It also
+ * gives each tile the first body after it that begins a new level-L cell
+ * (tileNext, n if none): the end of a level-L cell that runs past the tile. */
 __global__ __launch_bounds__(1024) void k_build_scan(const uint32_t* __restrict__ tileCounts,
-                                                     uint32_t* tileOffsets, int tiles,
+                                                     uint32_t* tileOffsets,
+                                                     const int* __restrict__ tileFirst,
+                                                     int* tileNext, int tiles, int n,
                                                      int maxTiles, uint32_t* levelTotals) {
     const int L = blockIdx.x + 1;
     const uint32_t* in = tileCounts + (size_t)L * maxTiles;
     uint32_t* out = tileOffsets + (size_t)L * maxTiles;
     __shared__ uint32_t sWave[16];
+    __shared__ int sWaveMin[16];
     __shared__ uint32_t sCarry;
-    if (threadIdx.x == 0) sCarry = 0;
+    __shared__ int sMinCarry;
+    const unsigned lane = gd_lane();
+    const int w = threadIdx.x / GD_WAVE;
+    if (threadIdx.x == 0) {
+        sCarry = 0;
+        sMinCarry = n;
+    }
     __syncthreads();
     for (int base = 0; base < tiles; base += 1024) {
         const int i = base + threadIdx.x;
         const uint32_t v = i < tiles ? in[i] : 0u;
         /* wave inclusive scan */
         uint32_t inc = v;
-        const unsigned lane = gd_lane();
 #pragma unroll
         for (int off = 1; off < GD_WAVE; off <<= 1) {
             const uint32_t o = __shfl_up(inc, off);
             if (lane >= (unsigned)off) inc += o;
         }
-        if (lane == GD_WAVE - 1) sWave[threadIdx.x / GD_WAVE] = inc;
+        if (lane == GD_WAVE - 1) sWave[w] = inc;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t run = sCarry;
-            for (int w = 0; w < 16; ++w) {
-                const uint32_t c = sWave[w];
-                sWave[w] = run;
+            for (int x = 0; x < 16; ++x) {
+                const uint32_t c = sWave[x];
+                sWave[x] = run;
                 run += c;
             }
             sCarry = run;
         }
         __syncthreads();
-        if (i < tiles) out[i] = sWave[threadIdx.x / GD_WAVE] + inc - v;
+        if (i < tiles) out[i] = sWave[w] + inc - v;
         __syncthreads();
     }
     if (threadIdx.x == 0) levelTotals[L] = sCarry;
-}
 
-/* End of a cell that runs past a wavefront's bodies: the first j in
- * [lo, hi) whose key is outside `cell` (keys >> shift), or hi, where
- * keys[lo - 1] is known to be inside and the end to be at most hi.  Called by the whole wavefront with uniform
- * arguments.  Each round the 64 lanes probe 64 keys spaced `step` apart from
- * lo: while none is outside, lo moves past them and the step grows 64-fold
- * (1, 64, 4096, ...: the first round reads the next 64 keys at once); once
- * one is, the boundary lies between two probes and the next rounds divide
- * that gap 64 ways.  A cell of S bodies takes about 2 log64 S rounds instead
- * of one lane's 2 log2 S dependent loads.  Membership is monotone (sorted
- * keys: inside, then outside). */
-__device__ __forceinline__ long long gd_wave_cell_end(const uint64_t* __restrict__ keys,
-                                                      long long lo, long long hi, uint64_t cell,
-                                                      int shift) {
-    /* the end lies in [lo, hi] */
-    long long step = 1;
-    bool bounded = false;      /* hi is a probe known to be outside */
-    const long long lane = (long long)gd_lane();
-    while (lo < hi) {
-        if (bounded) step = (hi - lo + 63) / 64;
-        const long long p = lo + lane * step;
-        const bool out = p >= hi || (keys[p] >> shift) != cell;
-        const unsigned long long b = __ballot(out);
-        if (b == 0ull) {       /* all 64 probes inside */
-            lo += 63 * step + 1;
-            if (!bounded) step *= 64;
-            continue;
+    /* exclusive suffix minimum of tileFirst, last chunk first */
+    const int* fin = tileFirst + (size_t)L * maxTiles;
+    int* fout = tileNext + (size_t)L * maxTiles;
+    for (int base = ((tiles - 1) / 1024) * 1024; base >= 0; base -= 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < tiles ? fin[i] : n;
+        int inc = v;   /* minimum over this lane and the wave's later lanes */
+#pragma unroll
+        for (int off = 1; off < GD_WAVE; off <<= 1) {
+            const int o = __shfl_down(inc, off);
+            if (lane + off < (unsigned)GD_WAVE) inc = o < inc ? o : inc;
         }
-        const int f = __builtin_ctzll(b);
-        if (f == 0) return lo;
-        const long long h = lo + (long long)f * step;
-        hi = h < hi ? h : hi;
-        lo += (long long)(f - 1) * step + 1;
-        bounded = true;
+        if (lane == 0) sWaveMin[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = sMinCarry;
+            for (int x = 15; x >= 0; --x) {
+                const int c = sWaveMin[x];
+                sWaveMin[x] = run;
+                run = c < run ? c : run;
+            }
+            sMinCarry = run;
+        }
+        __syncthreads();
+        int after = __shfl_down(inc, 1);
+        if (lane == GD_WAVE - 1) after = n;
+        if (i < tiles) fout[i] = after < sWaveMin[w] ? after : sWaveMin[w];
+        __syncthreads();
     }
-    return lo;
 }
 
 __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
     const uint64_t* __restrict__ keys, int n, const uint32_t* __restrict__ tileOffsets,
-    const uint32_t* __restrict__ levelTotals, int maxTiles, int4* __restrict__ meta,
-    int maxNodes, gd_tree_info* info) {
+    const int* __restrict__ tileNext, const uint32_t* __restrict__ levelTotals, int maxTiles,
+    int4* __restrict__ meta, int maxNodes, gd_tree_info* info) {
     constexpr int WAVES = GD_BUILD_THREADS / GD_WAVE;
     __shared__ uint64_t sk[GD_BUILD_TILE + 2];
     __shared__ uint32_t sRank[GD_BUILD_ITEMS][WAVES][GD_LEVELS];
+    __shared__ int sEnd[GD_BUILD_ITEMS][WAVES][GD_LEVELS];
     __shared__ int sLevelOff[GD_LEVELS];
     __shared__ int sOverflow;
     const long long tileBase = (long long)blockIdx.x * GD_BUILD_TILE;
@@ -230,23 +303,29 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
     __syncthreads();
     if (sOverflow) return;
 
-    /* pass 1: per (item, wave, level) counts of node starts */
+    /* pass 1: per (item, wave, level) the node starts and the first body that
+       begins a new level-L cell (n if none) */
     int lo[GD_BUILD_ITEMS], hi[GD_BUILD_ITEMS], Bv[GD_BUILD_ITEMS];
-    int waveMaxHi = 0;
+    int minLo[GD_BUILD_ITEMS], maxHi[GD_BUILD_ITEMS];
+    const int lane = (int)gd_lane();
     for (int k = 0; k < GD_BUILD_ITEMS; ++k) {
         const int q = k * GD_BUILD_THREADS + threadIdx.x;
         const long long i = tileBase + q;
         lo[k] = 99; hi[k] = -1; Bv[k] = -1;
         if (i < n) gd_body_levels(sk, q, i, n, &lo[k], &hi[k], &Bv[k]);
-        for (int L = 1; L <= GD_MAX_LEVEL; ++L) {
-            const unsigned long long bal = __ballot(lo[k] <= L && L <= hi[k]);
-            if (gd_lane() == 0) sRank[k][wave][L] = (uint32_t)__popcll(bal);
+        minLo[k] = gd_wave_min_lo(lo[k]);
+        maxHi[k] = gd_wave_max_hi(hi[k]);
+        int c, f;
+        gd_wave_level_starts(lo[k], hi[k], minLo[k], maxHi[k],
+                             (int)tileBase + k * GD_BUILD_THREADS + wave * GD_WAVE, n, &c, &f);
+        if (lane >= 1 && lane <= GD_MAX_LEVEL) {
+            sRank[k][wave][lane] = (uint32_t)c;
+            sEnd[k][wave][lane] = f;
         }
-        waveMaxHi = hi[k] > waveMaxHi ? hi[k] : waveMaxHi;
     }
     __syncthreads();
-    /* exclusive prefix over (item, wave) in body order, from the tile's offset */
     if (threadIdx.x >= 1 && threadIdx.x <= GD_MAX_LEVEL) {
+        /* exclusive prefix over (item, wave) in body order, from the tile's offset */
         const int L = threadIdx.x;
         uint32_t run = tileOffsets[(size_t)L * maxTiles + blockIdx.x];
         for (int k = 0; k < GD_BUILD_ITEMS; ++k)
@@ -255,71 +334,69 @@ __global__ __launch_bounds__(GD_BUILD_THREADS) void k_build_nodes(
                 sRank[k][w][L] = run;
                 run += c;
             }
+    } else if (threadIdx.x >= GD_WAVE + 1 && threadIdx.x <= GD_WAVE + GD_MAX_LEVEL) {
+        /* exclusive suffix minimum in body order, from the first cell start
+           after the tile: sEnd becomes the end of a level-L cell that runs
+           past the (item, wave)'s last body */
+        const int L = threadIdx.x - GD_WAVE;
+        int e = tileNext[(size_t)L * maxTiles + blockIdx.x];
+        for (int k = GD_BUILD_ITEMS - 1; k >= 0; --k)
+            for (int w = WAVES - 1; w >= 0; --w) {
+                const int f = sEnd[k][w][L];
+                sEnd[k][w][L] = e;
+                e = f < e ? f : e;
+            }
     }
     __syncthreads();
 
-    /* pass 2: write the nodes */
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const int o = __shfl_xor(waveMaxHi, off);
-        waveMaxHi = o > waveMaxHi ? o : waveMaxHi;
-    }
+    /* pass 2: write the nodes, level by level over the item's wave-uniform
+       range of levels with node starts */
     for (int k = 0; k < GD_BUILD_ITEMS; ++k) {
+        const int L0 = minLo[k], L1 = maxHi[k];
+        if (L0 > L1) continue;
         const int q = k * GD_BUILD_THREADS + threadIdx.x;
         const long long i = tileBase + q;
         const uint64_t key = sk[q + 1];
-        int prevRank = 0, prevFlag = 0;
-        /* the cells that run past the wavefront's last body are nested (each
-           holds that body), so each one's end bounds the next, deeper one's */
-        long long waveHi = n;
-        bool flag = lo[k] <= 1 && 1 <= hi[k];
-        int rank = (int)sRank[k][wave][1] + __popcll(__ballot(flag) & lt);
-        for (int L = 1; L <= waveMaxHi; ++L) {
+        /* no body of the wavefront starts a node at L0 - 1: each one's
+           level-(L0-1) node is the last one started before the wavefront */
+        int prevRank = L0 >= 2 ? (int)sRank[k][wave][L0 - 1] : 0, prevFlag = 0;
+        bool flag = lo[k] <= L0 && L0 <= hi[k];
+        unsigned long long fb = __ballot(flag);
+        int rank = (int)sRank[k][wave][L0] + __popcll(fb & lt);
+        for (int L = L0; L <= L1; ++L) {
             /* the body's node one level down, if any, is this node's first child */
             const bool flagNext = lo[k] <= L + 1 && L + 1 <= hi[k];
+            unsigned long long fbNext = 0ull;
             int rankNext = 0;
-            if (L + 1 <= waveMaxHi)   /* wave-uniform */
-                rankNext = (int)sRank[k][wave][L + 1] + __popcll(__ballot(flagNext) & lt);
-            /* level-L cells end where a body starts a node at a level <= L
-               (its left neighbour differs from it at level L) */
-            const unsigned long long ends = __ballot(lo[k] <= L);
-            const unsigned long long later = ends & ~(lt | (1ull << gd_lane()));
-            /* at most one lane per level starts a cell that runs past the
-               wavefront's last body (every later body is inside it): the
-               wavefront finds its end together */
-            long long far = 0;
-            {
-                const unsigned long long g = __ballot(flag && L <= Bv[k] && later == 0ull);
-                if (g) {
-                    const int src = __builtin_ctzll(g);
-                    const uint64_t c = __shfl(key, src) >> (63 - 3 * L);
-                    const long long waveEnd = tileBase + k * GD_BUILD_THREADS +
-                                              (threadIdx.x & ~(GD_WAVE - 1)) + GD_WAVE;
-                    far = gd_wave_cell_end(keys, waveEnd < n ? waveEnd : n, waveHi, c,
-                                           63 - 3 * L);
-                    waveHi = far;
-                }
+            if (L + 1 <= L1) {   /* wave-uniform */
+                fbNext = __ballot(flagNext);
+                rankNext = (int)sRank[k][wave][L + 1] + __popcll(fbNext & lt);
             }
-            if (flag) {
-                const int idx = sLevelOff[L] + rank;
-                const int parent = (L == 1) ? 0 : sLevelOff[L - 1] + prevRank + prevFlag - 1;
-                const int shift = 63 - 3 * L;
-                const uint64_t cell = key >> shift;
-                /* the cell's size: 1 when the right neighbour is outside it,
-                   from the next cell start among this wavefront's bodies, or
-                   from the wavefront's search when the cell runs past them */
-                const int count = (L > Bv[k]) ? 1
-                                  : later ? (int)__builtin_ctzll(later) - (int)gd_lane()
-                                          : (int)(far - i);
-                /* count < 2^28 (8 n nodes fit an int): its top bits carry
-                   the node's octant in its parent */
-                meta[idx] = make_int4((int)i, count | ((int)(cell & 7u) << 29), parent,
-                                      flagNext ? sLevelOff[L + 1] + rankNext : -1);
+            if (fb) {   /* wave-uniform: some body of the wavefront starts a level-L node */
+                /* level-L cells end where a body begins a new level-L cell */
+                const unsigned long long ends = __ballot(lo[k] <= L);
+                const unsigned long long later = ends & ~(lt | (1ull << lane));
+                if (flag) {
+                    const int idx = sLevelOff[L] + rank;
+                    const int parent = (L == 1) ? 0 : sLevelOff[L - 1] + prevRank + prevFlag - 1;
+                    const uint64_t cell = key >> (63 - 3 * L);
+                    /* the cell's size: 1 when the right neighbour is outside it,
+                       from the next cell start among this wavefront's bodies, or
+                       from the first one after them */
+                    const int count = (L > Bv[k]) ? 1
+                                      : later ? (int)__builtin_ctzll(later) - lane
+                                              : sEnd[k][wave][L] - (int)i;
+                    /* count < 2^28 (8 n nodes fit an int): its top bits carry
+                       the node's octant in its parent */
+                    meta[idx] = make_int4((int)i, count | ((int)(cell & 7u) << 29), parent,
+                                          flagNext ? sLevelOff[L + 1] + rankNext : -1);
+                }
             }
             prevRank = rank;
             prevFlag = flag;
             rank = rankNext;
             flag = flagNext;
+            fb = fbNext;
         }
     }
 }
@@ -791,13 +868,17 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
         return;
     }
     b->lastBodies = n;
-    k_build_count<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileCounts, b->maxTiles);
+    k_build_count<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileCounts,
+                                                               b->tileFirst, b->maxTiles);
     GD_LAUNCHED();
-    k_build_scan<<<GD_MAX_LEVEL, 1024, 0, gd_stream()>>>(b->tileCounts, b->tileOffsets, tiles, b->maxTiles,
-                                         b->levelTotals);
+    k_build_scan<<<GD_MAX_LEVEL, 1024, 0, gd_stream()>>>(b->tileCounts, b->tileOffsets, b->tileFirst,
+                                                         b->tileNext, tiles, n, b->maxTiles,
+                                                         b->levelTotals);
     GD_LAUNCHED();
-    k_build_nodes<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileOffsets, b->levelTotals,
-                                               b->maxTiles, b->meta, maxNodes, b->dinfo);
+    k_build_nodes<<<tiles, GD_BUILD_THREADS, 0, gd_stream()>>>(sortedKeys, n, b->tileOffsets,
+                                                               b->tileNext, b->levelTotals,
+                                                               b->maxTiles, b->meta, maxNodes,
+                                                               b->dinfo);
     GD_LAUNCHED();
     if (gd_build_mid_event) GD_HIP(hipEventRecord(gd_build_mid_event, gd_stream()));
     /* launch L finishes level L from its children at L + 1 (level 21 holds

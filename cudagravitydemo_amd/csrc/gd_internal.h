@@ -115,6 +115,9 @@ typedef struct {
     int maxTiles;                    /* build tiles at capacity */
     uint32_t* tileCounts;            /* [22][maxTiles] node starts per level and tile */
     uint32_t* tileOffsets;           /* [22][maxTiles] exclusive scan of tileCounts */
+    int* tileFirst;                  /* [22][maxTiles] first body of the tile that begins a
+                                        new cell of the level (n if none) */
+    int* tileNext;                   /* [22][maxTiles] the same for the bodies after the tile */
     uint32_t* levelTotals;           /* [22] */
     gd_tree_info* dinfo;             /* device */
     gd_tree_info* hinfo;             /* pinned host mirror */
