@@ -19,6 +19,10 @@ sample is within CALIBRATION_SHARE of the 1e-3 budget:
              different (held-out) 4096-body sample every --judge-every steps
              and, over budget, switches the corrector off and continues with
              the quadrupole walk at theta 0.4.
+The quadrupole and learned walks sum the cells far from each group (h / R
+below --far, h the group box's half diagonal, R the distance to the cell)
+through one local expansion per group instead of body by body
+(gd_walk_set_far_field); the plain walk evaluates every node body by body.
 `value` is the learned run's body-steps/s with its judge steps included;
 `ms_per_step_without_judge` leaves them out.  Each leg is judged again on the
 held-out sample after its timed steps.
@@ -76,6 +80,7 @@ FLOP_OPEN = 9              # per opening test that opened the node
 # the evaluation work actually executed per interaction, by kind
 FLOP_QUAD = 54
 FLOP_CORR = 15
+FLOP_FAR = 316             # per far (group, cell) pair: its local-expansion terms
 FLOP_MLP = 17792
 THETAS = (1.0, 0.9, 0.85, 0.8, 0.75, 0.725, 0.7, 0.6875, 0.675, 0.65, 0.625, 0.6, 0.55, 0.5)
 PLAIN_THETAS = (0.3, 0.25, 0.2, 0.175, 0.15, 0.1)
@@ -101,6 +106,9 @@ def parse():
                     help="the corrector acts on cells with size / distance above this")
     ap.add_argument("--train-targets", type=int, default=8192,
                     help="target bodies per snapshot for the corrector's samples")
+    ap.add_argument("--far", type=float, default=0.5,
+                    help="quadrupole and learned walks: cells with h / R below this go "
+                         "through their group's local expansion (0: off)")
     return ap.parse_args()
 
 
@@ -272,12 +280,15 @@ def eval_roofline(walks, mode, theta, n):
     inter = st["interactions"]
     cells = st["cell_interactions"]
     mono = inter - cells
+    far_pairs = st.get("far_cells", 0)
+    near_cells = cells - st.get("far_interactions", 0)
     if mode == "plain":
         executed = FLOP_INTERACTION * inter
     elif mode == "quad":
-        executed = FLOP_INTERACTION * mono + FLOP_QUAD * cells
+        executed = FLOP_INTERACTION * mono + FLOP_QUAD * near_cells + FLOP_FAR * far_pairs
     else:
-        executed = FLOP_INTERACTION * mono + (FLOP_QUAD + FLOP_CORR) * cells
+        executed = (FLOP_INTERACTION * mono + FLOP_QUAD * near_cells + FLOP_FAR * far_pairs
+                    + FLOP_CORR * cells)
     mfma = FLOP_MLP * st.get("network_evals", 0) if mode == "learned" else 0
     # 20 FLOP per accepted body-node interaction + 9 per opening test that
     # opened a node (the tests run in the traversal kernel: ~0.3 % here)
@@ -290,7 +301,9 @@ def eval_roofline(walks, mode, theta, n):
                     "walk's network, reported apart); peak = MI355X FP32 157.3 TFLOP/s, which "
                     "counts packed v_pk_fma_f32; this build runs with packed FP32 off, so "
                     "78.6 TFLOP/s is the reachable ceiling (frac_of_unpacked_fp32)"),
-        "kernel": f"k_walk_eval_pair ({mode} walk, theta={theta})",
+        "kernel": (f"k_walk_eval_pair ({mode} walk, theta={theta})" if mode == "plain" else
+                   f"k_walk_far + k_walk_eval_pair ({mode} walk, theta={theta}, far field "
+                   f"{walks.sim.far})"),
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -305,7 +318,11 @@ def eval_roofline(walks, mode, theta, n):
         "kernel_ms": ev_ms,
         "executed_flop_per_launch": int(executed),
         "executed_tflops": executed / (ev_ms * 1e-3) / 1e12,
-        "executed_model": "20 per monopole, 54 per quadrupole cell, +15 per corrected cell",
+        "executed_model": ("20 per monopole, 54 per quadrupole cell evaluated body by body, 316 "
+                           "per far (group, cell) pair through the group's local expansion, "
+                           "+15 per corrected cell"),
+        "far_cells_per_group": far_pairs / max(1, 2 * ((n + 63) // 64)),
+        "far_share_of_cell_interactions": (st.get("far_interactions", 0) / cells) if cells else 0.0,
         "mfma_flop_per_launch": int(mfma),
         "mfma_tflops_bf16": mfma / (ev_ms * 1e-3) / 1e12,
         "traversal_ms": float(phases[:, 0].mean()),
@@ -413,6 +430,8 @@ def main():
     # the engine's walks read only the walk arrays; the reference's 80-byte
     # node array is written on demand (the reference-walk leg asks for it)
     sim.set_node_records(False)
+    sim.far = args.far
+    sim.set_far_field(args.far)
     net = cr.Corrector(seed=1)
     walks = Walks(sim, net, args.min_ratio)
     cal = ics.subsample(n, 4096, seed=CAL_SEED)
@@ -496,6 +515,7 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (host only)")
         cpu = cpu_baseline(sim, final, eps, ref["theta"] if ref else 0.5, dt, args.cpu_groups)
+    sim.set_far_field(0.0)
     net.close()
     sim.close()
 
@@ -525,6 +545,7 @@ def main():
             "walk": "learned" if corrector_on else "learned, fell back to quadrupole theta=0.4",
             "theta": th_l,
             "force_error_budget": BUDGET,
+            "far_field": args.far,
             "calibration_budget": CALIBRATION_SHARE * BUDGET,
             "judge": (f"FP64 direct sum; calibration sample seed {CAL_SEED}, judge sample "
                       f"seed {JUDGE_SEED} (4096 bodies each)"),

@@ -53,7 +53,8 @@ class TreeState(C.Structure):
 class WalkStats(C.Structure):
     _fields_ = [("interactions", C.c_ulonglong), ("opened", C.c_ulonglong),
                 ("overflowGroups", C.c_ulonglong), ("cellInteractions", C.c_ulonglong),
-                ("bodyGroups", C.c_ulonglong), ("networkEvals", C.c_ulonglong)]
+                ("bodyGroups", C.c_ulonglong), ("networkEvals", C.c_ulonglong),
+                ("farCells", C.c_ulonglong), ("farInteractions", C.c_ulonglong)]
 
 
 assert C.sizeof(Node) == 80 and C.sizeof(TreeState) == 304 and C.sizeof(WorldBox) == 20
@@ -120,6 +121,7 @@ _PROTOS = {
     "gd_walk_timing": (None, [i32]),
     "gd_walk_set_list_capacity": (i32, [i32, i32]),
     "gd_walk_set_group_split": (f32, [f32]),
+    "gd_walk_set_far_field": (f32, [f32]),
     "gd_walk_set_async": (i32, [i32]),
     "gd_walk_poll": (i32, []),
     "gd_walk_kernel_ms": (f32, []),

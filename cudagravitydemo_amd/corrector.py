@@ -220,7 +220,9 @@ def walk_learned(sim, net, theta, min_ratio=0.55, stats=False):
     if st is not None:
         return {"interactions": st.interactions, "opened": st.opened,
                 "overflow_groups": st.overflowGroups, "cell_interactions": st.cellInteractions,
-                "body_groups": st.bodyGroups, "network_evals": st.networkEvals}
+                "body_groups": st.bodyGroups, "network_evals": st.networkEvals,
+                "far_cells": st.farCells,
+                "far_interactions": st.farInteractions}
     return None
 
 

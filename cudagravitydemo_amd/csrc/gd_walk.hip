@@ -25,6 +25,7 @@
  *    stack does not fit are walked body by body (k_walk_body).
  */
 #include "gd_internal.h"
+#include "gd_far.h"
 #include "gd_mlp.h"
 #include "gd_traverse.h"
 
@@ -976,6 +977,210 @@ __device__ __forceinline__ int gd_flush_corr(
 }
 
 /*
+ * The cells of a half's list in the far split (entries 0..n-1 at
+ * list[cap - 1 - e]).  The 64 lanes take 64 entries at a time (indices two
+ * chunks ahead, records one chunk ahead, so the gathers overlap the work); a
+ * cell whose centre of mass is far from the half's box (h2 < far2 R^2, R
+ * from the box centre g) goes with its record to the LDS queue sRec (fq
+ * entries of 3 float4, < 64 between calls), and each time 64 are queued every
+ * lane adds one of them to its local-expansion coefficients L (gd_far.h);
+ * the near cells are written back, in order, to the front of the region, for
+ * the evaluation's per-body loop.  KEEP (learned walk): a cell the network
+ * corrects (s / r > minRatio from g, the evaluation's own test) stays near,
+ * so the evaluation finds it.  Returns the near count.
+ */
+template <bool KEEP>
+__device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
+                                             const int2* __restrict__ wlink,
+                                             const float4* __restrict__ wmom, float ext,
+                                             int* __restrict__ list, int cap, int n, float gx,
+                                             float gy, float gz, float h2, float far2, float eps2,
+                                             float minRatio, float4* sRec, gd_local& L, int& fq,
+                                             int& nFar) {
+    const unsigned lane = gd_lane();
+    const unsigned long long lt = gd_lanemask_lt();
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    /* entry e of the region */
+    auto at = [&](int e) -> int& { return list[cap - 1 - e]; };
+    int nNear = 0;
+    /* chunk 0's records and chunk 1's indices in flight */
+    int idxN = (int)lane < n ? at((int)lane) : -1;
+    float4 cN = zero, q0N = zero, q1N = zero;
+    if (idxN >= 0) {
+        cN = wcom[idxN];
+        gd_mom_quad(wmom, idxN, q0N, q1N);
+    }
+    int idxNN = 64 + (int)lane < n ? at(64 + (int)lane) : -1;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const int idx = idxN;
+        const float4 c = cN, q0 = q0N, q1 = q1N;
+        /* next chunk's records, the chunk after's indices (entries >= c0 + 64:
+           the write-backs below go to entries < c0 + 64) */
+        idxN = idxNN;
+        cN = zero; q0N = zero; q1N = zero;
+        if (idxN >= 0) {
+            cN = wcom[idxN];
+            gd_mom_quad(wmom, idxN, q0N, q1N);
+        }
+        idxNN = c0 + 128 + (int)lane < n ? at(c0 + 128 + (int)lane) : -1;
+        const float dx = c.x - gx, dy = c.y - gy, dz = c.z - gz;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        bool isFar = idx >= 0 && h2 < far2 * r2;
+        if (KEEP && isFar && c.w > 0.f) {
+            const float rq = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            const float size = ldexpf(ext, -(wlink[idx].y & 31));
+            if (size * __builtin_amdgcn_rsqf(rq) > minRatio) isFar = false;
+        }
+        const bool isNear = idx >= 0 && !isFar;
+        const unsigned long long nbal = __ballot(isNear);
+        /* in place: entry e moves to nNear + rank <= e, already read by this wave */
+        if (isNear) at(nNear + __popcll(nbal & lt)) = idx;
+        nNear += __popcll(nbal);
+        const unsigned long long fbal = __ballot(isFar);
+        if (isFar) {
+            float4* q = sRec + 3 * (fq + __popcll(fbal & lt));
+            q[0] = c;
+            q[1] = q0;
+            q[2] = q1;
+        }
+        fq += __popcll(fbal);
+        __builtin_amdgcn_wave_barrier();
+        if (fq >= 64) {   /* wave-uniform: a full wave of far cells */
+            const float4 cj = sRec[3 * lane], a = sRec[3 * lane + 1], b = sRec[3 * lane + 2];
+            gd_local_add(L, gx - cj.x, gy - cj.y, gz - cj.z, cj.w, a, b, eps2);
+            nFar += 64;
+            const int rest = fq - 64;
+            float4 v0 = zero, v1 = zero, v2 = zero;
+            if ((int)lane < rest) {
+                v0 = sRec[3 * (64 + lane)];
+                v1 = sRec[3 * (64 + lane) + 1];
+                v2 = sRec[3 * (64 + lane) + 2];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if ((int)lane < rest) {
+                sRec[3 * lane] = v0;
+                sRec[3 * lane + 1] = v1;
+                sRec[3 * lane + 2] = v2;
+            }
+            fq = rest;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    return nNear;
+}
+
+/*
+ * Far split of one half's list by the run's wavefront: its cells (nb, at the
+ * back) through gd_far_region; then the queue's last entries, the sum of the
+ * lanes' coefficients, and the expansion at each body's position, added to
+ * (fx, fy, fz) by the lanes with `mine` (the half's bodies).  sRec: the queue
+ * (128 x 3 float4), then the reduction (64 x GD_LOCAL_STRIDE floats).  Writes
+ * the near counts to nearF (the leaves, nf: all near) and nearB.
+ */
+template <bool KEEP>
+__device__ __forceinline__ void gd_far_half(const float4* __restrict__ wcom,
+                                            const int2* __restrict__ wlink,
+                                            const float4* __restrict__ wmom, float ext,
+                                            int* __restrict__ list, int cap, int nf, int nb,
+                                            float gx, float gy, float gz, float h2, float far2,
+                                            float eps2, float minRatio, bool mine, float px,
+                                            float py, float pz, float4* sRec, int& nearF,
+                                            int& nearB, float& fx, float& fy, float& fz) {
+    const unsigned lane = gd_lane();
+    gd_local L;
+    gd_local_zero(L);
+    int fq = 0, nFar = 0;
+    /* leaves stay in the per-body loop: they are mostly near, and streaming
+       them through the split cost more than the far ones saved */
+    nearF = nf;
+    nearB = gd_far_region<KEEP>(wcom, wlink, wmom, ext, list, cap, nb, gx, gy, gz, h2, far2,
+                                eps2, minRatio, sRec, L, fq, nFar);
+    if (fq > 0) {
+        if ((int)lane < fq) {
+            const float4 cj = sRec[3 * lane], a = sRec[3 * lane + 1], b = sRec[3 * lane + 2];
+            gd_local_add(L, gx - cj.x, gy - cj.y, gz - cj.z, cj.w, a, b, eps2);
+        }
+        nFar += fq;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (nFar > 0) {   /* wave-uniform */
+        /* the queue's records are consumed: its LDS holds the reduction */
+        gd_local_allreduce(L, reinterpret_cast<float*>(sRec));
+        gd_local_finish(L);
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        gd_local_eval(L, px - gx, py - gy, pz - gz, ax, ay, az);
+        if (mine) {
+            fx += ax;
+            fy += ay;
+            fz += az;
+        }
+    }
+}
+
+/*
+ * Far-field kernel of the first pass (runs before k_walk_eval_pair, with the
+ * same run-to-wavefront mapping): for each half of run r with a list, the far
+ * cells are summed through the half's local expansion (gd_far_half), the
+ * half's cell count becomes its near count (the near cells compacted in
+ * place), and each body's far field (without G) is written to acc, which the
+ * evaluation then adds to.  KEEP: learned walk.
+ */
+template <int EVW, bool KEEP>
+__global__ __launch_bounds__(64 * EVW) void k_walk_far(
+    const float4* __restrict__ pos, const float4* __restrict__ wcom,
+    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange, int runs,
+    int* __restrict__ lists, const long long* __restrict__ offsets,
+    const int* __restrict__ caps, int2* __restrict__ counts, float3* __restrict__ acc,
+    float eps2, float far2, float minRatio, unsigned long long* __restrict__ stats) {
+    /* per wavefront: the far queue (128 x 3 float4), later the reduction */
+    constexpr int REC = (3 * 128 > 16 * GD_LOCAL_STRIDE) ? 3 * 128 : 16 * GD_LOCAL_STRIDE;
+    __shared__ float4 sRec[EVW][REC];
+    if (info->overflow) return;   /* block-uniform: no tree, nothing listed */
+    const unsigned lane = gd_lane();
+    const int wave = (int)(threadIdx.x >> 6);
+    const long long r = (long long)gd_xcd_block() * EVW + wave;
+    if (r >= runs) return;   /* wave-uniform */
+    const int2 rgA = slotRange[2 * r], rgB = slotRange[2 * r + 1];
+    const int2 cA = counts[2 * r];
+    const int2 cB = rgB.y > 0 ? counts[2 * r + 1] : make_int2(-1, -1);
+    const bool actA = cA.x >= 0, actB = cB.x >= 0;
+    if (!actA && !actB) return;
+    const int nA = rgA.y, nB = rgB.y;
+    const bool valid = (int)lane < nA + nB;
+    const bool inB = (int)lane >= nA;
+    const float4 p = pos[rgA.x + (valid ? lane : 0)];
+    const float ext = info->box.extent;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const bool act = h ? actB : actA;
+        const int2 cn = h ? cB : cA;
+        if (!act || cn.y <= 0) continue;   /* wave-uniform */
+        float lx, ly, lz, hx, hy, hz;
+        gd_masked_box(valid && (h ? inB : !inB), p.x, p.y, p.z, lx, ly, lz, hx, hy, hz);
+        const float gx = 0.5f * (lx + hx), gy = 0.5f * (ly + hy), gz = 0.5f * (lz + hz);
+        const float ex = hx - lx, ey = hy - ly, ez = hz - lz;
+        const float h2 = 0.25f * ((ex * ex + ey * ey) + ez * ez);
+        const long long s = 2 * r + h;
+        int nearF, nearB;
+        gd_far_half<KEEP>(wcom, wlink, wmom, ext, lists + offsets[s], caps[s], cn.x, cn.y, gx, gy,
+                          gz, h2, far2, eps2, minRatio, valid && (h ? inB : !inB), p.x, p.y, p.z,
+                          sRec[wave], nearF, nearB, fx, fy, fz);
+        if (lane == 0) {
+            counts[s] = make_int2(nearF, nearB);
+            const int moved = cn.x + cn.y - nearF - nearB;
+            if (stats && moved > 0) {
+                unsigned long long* slot = gd_stat_slot(stats);
+                atomicAdd(&slot[5], (unsigned long long)moved);
+                atomicAdd(&slot[6], (unsigned long long)moved * (unsigned long long)(h ? nB : nA));
+            }
+        }
+    }
+    if (valid && (inB ? actB : actA)) acc[rgA.x + lane] = make_float3(fx, fy, fz);
+}
+
+/*
  * First-pass evaluation of one 64-body run by one wavefront: the run's two
  * group slots (halves A and B; B may be empty) each have their own list.
  * Lane l is body start + l and belongs to A (l < nA) or B; per chunk step
@@ -1123,7 +1328,7 @@ __device__ __forceinline__ void gd_eval_pair(
  * slots, gd_eval_pair); EVW wavefronts per block.  A half whose traversal
  * did not list it (counts -1: second pass or per-body walk) is skipped and
  * its bodies are written by those passes. */
-template <int MODE, bool PHI, int EVW>
+template <int MODE, bool PHI, int EVW, bool FAR>
 __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
     const int2* __restrict__ wlink, const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
@@ -1173,6 +1378,13 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     const bool inB = (int)lane >= rgA.y;
     if ((int)lane < rgA.y + rgB.y && (inB ? actB : actA)) {
         const long long i = rgA.x + lane;
+        if (FAR) {
+            /* the far cells' field, from k_walk_far */
+            const float3 f = acc[i];
+            gd_kahan_add(tot.x, tot.cx, f.x);
+            gd_kahan_add(tot.y, tot.cy, f.y);
+            gd_kahan_add(tot.z, tot.cz, f.z);
+        }
         acc[i] = make_float3(G * tot.x, G * tot.y, G * tot.z);
         if (PHI) phi[i] = -G * phs;
     }
@@ -1421,6 +1633,10 @@ static gd_walk_scratch g_walk = {nullptr, nullptr, nullptr, nullptr, nullptr, nu
                                  nullptr, nullptr, 0};
 
 static int g_lastRuns = 0;   /* 64-body runs of the last asynchronous walk */
+/* Far-field ratio of the first-pass evaluation (0: off): in the quadrupole
+ * and learned walks without potential, a cell with h / R below it is summed
+ * through its group's local expansion (gd_far_half). */
+static float g_farRatio = 0.f;
 
 /* optional HIP-event timing of the walk's phases (library stream) */
 static int g_walkTiming = 0;
@@ -1607,10 +1823,30 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     {
         /* one wavefront per run: both halves' lists side by side */
         constexpr int EVW = MODE == 2 ? 8 : 4;   /* learned: 8 share the staged weights */
-        k_walk_eval_pair<MODE, PHI, EVW><<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
-            g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2, G, W, Bv,
-            minRatio, g_walk.kstats);
+        const float far = g_farRatio;
+        bool farDone = false;
+        if constexpr (MODE != 0 && !PHI) {
+            if (far > 0.f) {
+                k_walk_far<EVW, MODE == 2><<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
+                    pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
+                    g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, eps2,
+                    far * far, minRatio, g_walk.kstats);
+                GD_LAUNCHED();
+                k_walk_eval_pair<MODE, PHI, EVW, true>
+                    <<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
+                        pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
+                        g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2,
+                        G, W, Bv, minRatio, g_walk.kstats);
+                farDone = true;
+            }
+        }
+        if (!farDone) {
+            k_walk_eval_pair<MODE, PHI, EVW, false>
+                <<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
+                    pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
+                    g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2, G,
+                    W, Bv, minRatio, g_walk.kstats);
+        }
     }
     GD_LAUNCHED();
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[2], gd_stream()));
@@ -1842,10 +2078,10 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         over[2] = g_walk.hstatus->over2;
     }
     if ((!g_walk.async && over[0] > 0) || stats) {
-        unsigned long long raw[GD_STAT_WORDS], h[5] = {0, 0, 0, 0, 0};
+        unsigned long long raw[GD_STAT_WORDS], h[7] = {0, 0, 0, 0, 0, 0, 0};
         GD_HIP(hipMemcpy(raw, g_walk.stats, sizeof(raw), hipMemcpyDeviceToHost));
         for (int sl = 0; sl < GD_STAT_SLOTS; ++sl)
-            for (int k = 0; k < 5; ++k) h[k] += raw[sl * GD_STAT_STRIDE + k];
+            for (int k = 0; k < 7; ++k) h[k] += raw[sl * GD_STAT_STRIDE + k];
         if (h[2]) GD_FAIL("gd_walk: a body's traversal stack exceeded %d entries", GD_BODY_STACK);
         if (!stats) return;
         stats->interactions = h[0];
@@ -1854,6 +2090,8 @@ static void gd_walk_impl(const treeState* tree, const float4* pos, float3* acc, 
         stats->cellInteractions = h[3];
         stats->bodyGroups = (unsigned long long)over[2];
         stats->networkEvals = h[4];
+        stats->farCells = h[5];
+        stats->farInteractions = h[6];
     }
 }
 
@@ -1902,6 +2140,16 @@ extern "C" int gd_walk_set_list_capacity(int entries, int secondPass) {
         k_walk_fill<<<gd_div_up(g_walk.groupCap, 256), 256, 0, gd_stream()>>>(g_walk.sizes, g_walk.groupCap, -1);
         GD_LAUNCHED();
     }
+    return prev;
+}
+
+extern "C" float gd_walk_set_far_field(float ratio) {
+    const float prev = g_farRatio;
+    if (!(ratio >= 0.f && ratio < 1.f)) {   /* NaN included */
+        GD_FAIL("gd_walk_set_far_field: ratio %g outside [0, 1)", (double)ratio);
+        return prev;
+    }
+    g_farRatio = ratio;
     return prev;
 }
 

@@ -164,8 +164,20 @@ class Simulation:
             return {"interactions": st.interactions, "opened": st.opened,
                     "overflow_groups": st.overflowGroups,
                     "cell_interactions": st.cellInteractions,
-                    "body_groups": st.bodyGroups, "network_evals": st.networkEvals}
+                    "body_groups": st.bodyGroups, "network_evals": st.networkEvals,
+                "far_cells": st.farCells,
+                "far_interactions": st.farInteractions}
         return None
+
+    def set_far_field(self, ratio):
+        """Far cells of the quadrupole and learned list walks through one local
+        expansion per group (gd_walk_set_far_field): cells with h / R below
+        `ratio` (h: the group box's half diagonal, R: distance from its centre);
+        0 evaluates every cell body by body.  Returns the previous setting.  The
+        setting belongs to the library, not to this state."""
+        prev = self.lib.gd_walk_set_far_field(float(ratio))
+        check(self.lib)
+        return float(prev)
 
     def step(self, dt, theta):
         self.build_tree_async()

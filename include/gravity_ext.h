@@ -109,6 +109,10 @@ typedef struct {
     unsigned long long bodyGroups;       /* of those, groups walked body by body */
     unsigned long long networkEvals;     /* learned walk: (group or body, cell) pairs
                                             evaluated by the network */
+    unsigned long long farCells;         /* (group, cell) pairs summed through the
+                                            group's local expansion (far field) */
+    unsigned long long farInteractions;  /* of cellInteractions, the body-cell pairs
+                                            those expansions stand for */
 } gd_walk_stats;
 
 /*
@@ -168,6 +172,20 @@ int gd_walk_poll(void);
  * box volume.  The training samples (gd_corrector_samples) take the same
  * groups.  Returns the previous setting. */
 float gd_walk_set_group_split(float ratio);
+
+/* Far field of the list walks (quadrupole and learned walks without
+ * potential).  With 0 < ratio < 1, a cell of a group's list whose centre of
+ * mass is at distance R from the centre of the group's bounding box, with
+ * h / R < ratio (h the box's half diagonal), is not evaluated body by body:
+ * its monopole and quadrupole field is summed, once per group, into a local
+ * (Taylor) expansion about the box centre, third order for the monopoles and
+ * second order for the second moments, which each body then evaluates at
+ * its own position (truncation O((h/R)^4) and O((h/R)^3) of those terms).
+ * Leaves, and in the learned walk the cells the network corrects, stay body
+ * by body.  The interaction list is unchanged.  0 (the default) evaluates
+ * every cell body by body.  Second-pass and per-body walks always do.
+ * Returns the previous setting. */
+float gd_walk_set_far_field(float ratio);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
  * library's stream (enable = 1) or stop (0).  The walk is a traversal kernel that

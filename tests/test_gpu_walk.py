@@ -74,7 +74,7 @@ def _sample(n, extra=()):
     return np.unique(s)
 
 
-def _check(sim, mode, per_body_of, params, extra=(), ratio=None):
+def _check(sim, mode, per_body_of, params, extra=(), ratio=None, far_of=None):
     ps = sim.positions()
     acc = sim.accelerations()
     slots, slot_of = _slots(sim, mode, ratio=ratio)
@@ -91,13 +91,14 @@ def _check(sim, mode, per_body_of, params, extra=(), ratio=None):
         box = (x, x) if pb else _slot_box(ps, slots, slot_of[i])
         # the per-body walk of the learned mode evaluates the quadrupole alone
         m = 1 if (mode == 2 and pb) else mode
+        far = 0.0 if (pb or far_of is None) else far_of(i)
         want, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
-                            sim.eps * sim.eps, m, net, MIN_RATIO, oct_=oct_)
+                            sim.eps * sim.eps, m, net, MIN_RATIO, oct_=oct_, far=far)
         want = sim.G * want
         tol = 1e-5
         if m == 2:
             base, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
-                                sim.eps * sim.eps, 1)
+                                sim.eps * sim.eps, 1, far=far)
             tol += 1e-2 * np.linalg.norm(want - sim.G * base) / np.linalg.norm(want)
         err = np.linalg.norm(acc[i] - want) / np.linalg.norm(want)
         worst = max(worst, err / tol)
@@ -173,6 +174,58 @@ def test_walk_matches_interaction_set_fp64(mode, path):
             a1 = sim.accelerations()
             d = np.linalg.norm(a2 - a1, axis=1) / np.linalg.norm(a1, axis=1)
             assert np.median(d) > 1e-4
+
+
+@pytest.mark.parametrize("mode,path,far", [(1, "lists", 0.5), (2, "lists", 0.5),
+                                            (1, "mixed", 0.3)])
+def test_far_field_walk_matches_its_expansion_fp64(mode, path, far):
+    """gd_walk_set_far_field: each group's far cells (h / R < far) through the
+    group's local expansion, the rest body by body -- against walk_fp64 with
+    the same split and the expansion summed in FP64 (tests/walk_check.py,
+    pinned by its truncation orders), 1e-5.  Groups that leave the first pass
+    (path "mixed": lists longer than the median) evaluate every cell.  The
+    expansion's truncation is above the bar, so the check has teeth."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    lib = load_library()
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        sizes = _group_sizes(sim, mode)
+        cap = 16384 if path == "lists" else int(np.median(sizes[sizes > 0]))
+        over, body = _predict(sim, mode, sizes, cap, 262144)
+        slots, slot_of = _slots(sim, mode)
+        prev = sim.set_far_field(far)
+        try:
+            st = _run(sim, mode, cap, params)
+        finally:
+            sim.set_far_field(prev)
+        assert prev == 0.0
+        assert st["overflow_groups"] == len(over) and st["body_groups"] == len(body)
+        if path == "mixed":
+            assert 0 < len(over) < (slots[:, 1] > 0).sum()
+        _check(sim, mode, lambda i: slot_of[i] in body, params,
+               far_of=lambda i: 0.0 if slot_of[i] in over else far)
+        # teeth: the exact per-body sum of the same sets is off by more than the bar
+        ps = sim.positions()
+        acc = sim.accelerations()
+        total = sim.tree_summary()[0]
+        com, link, quad = sim.walk_arrays(total)
+        lo, hi = sim.bbox()
+        ext = world_extent(lo, hi)
+        dev = []
+        for i in _sample(sim.n):
+            if slot_of[i] in over:
+                continue
+            exact, _ = walk_fp64(ps[i, :3].astype(np.float64), _slot_box(ps, slots, slot_of[i]),
+                                 com, link, quad, ext, THETA, sim.eps * sim.eps, 1)
+            a1 = acc[i] if mode == 1 else None
+            if a1 is not None:
+                dev.append(np.linalg.norm(a1 - sim.G * exact) / np.linalg.norm(sim.G * exact))
+        if dev:
+            print(f"far {far}: median deviation from the exact set sum {np.median(dev):.2e}, "
+                  f"max {np.max(dev):.2e}", flush=True)
+            assert np.max(dev) > 2e-5 and np.median(dev) < 1e-3
 
 
 @pytest.mark.parametrize("mode,ratio", [(0, 0.0), (1, 1.0), (2, 0.0)])

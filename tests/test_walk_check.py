@@ -14,8 +14,9 @@ import pytest
 from cudagravitydemo_amd import ics
 from oracle import binding as ob
 from oracle import mlp
-from tests.walk_check import (accepted, features, group_slots, monopole, quadrupole,
-                               split_ratio, world_extent)
+from tests.walk_check import (accepted, far_split, features, group_slots, kernel_derivative,
+                               local_coefficients, local_eval, monopole, quadrupole,
+                               split_ratio, walk_fp64, world_extent)
 
 
 @pytest.fixture(scope="module")
@@ -121,6 +122,101 @@ def test_quadrupole_term_against_brute_force():
     X, k = features(x, com, oct_, np.array([0]), 4.0, link)
     want = mlp.features(x, c, m, 1.0, O)
     assert np.allclose(X[0], want, rtol=1e-12, atol=1e-15) and abs(k[0] - m) < 1e-12
+
+
+def test_kernel_derivatives_are_the_gradients_of_each_other():
+    """kernel_derivative(n + 1) is the gradient of kernel_derivative(n)
+    (central differences), and n = 1 is -y / rho^3."""
+    rng = np.random.default_rng(9)
+    y = rng.normal(size=(4, 3))
+    eps2 = 0.02
+    r2 = (y * y).sum(1) + eps2
+    assert np.allclose(kernel_derivative(y, eps2, 1), -y / r2[:, None] ** 1.5, rtol=1e-13)
+    h = 1e-5
+    for n in range(1, 5):
+        up = kernel_derivative(y, eps2, n + 1)
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = h
+            fd = (kernel_derivative(y + e, eps2, n) - kernel_derivative(y - e, eps2, n)) / (2 * h)
+            assert np.allclose(fd, up[..., j], rtol=1e-6, atol=1e-8), (n, j)
+
+
+def _one_cell():
+    rng = np.random.default_rng(12)
+    b = rng.normal(size=(60, 3)) * [0.1, 0.06, 0.03]
+    m = rng.uniform(0.5, 1.5, 60)
+    c = (m[:, None] * b).sum(0) / m.sum()
+    y = b - c
+    S = np.einsum("k,ki,kj->ij", m, y, y)
+    com = np.array([[c[0], c[1], c[2], m.sum()]])
+    quad = np.array([[S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2], np.trace(S), 0]])
+    return com, quad
+
+
+@pytest.mark.parametrize("part,order", [("monopole", 4), ("second moments", 3)])
+def test_local_expansion_truncation_order(part, order):
+    """The local expansion about g of a cell's field: its error against the
+    field evaluated at g + t u falls as t^4 for the monopole (kept to third
+    order) and t^3 for the second-moment term (kept to second order), so
+    every kept term is right; the errors shrink 2^4 and 2^3 fold per halving."""
+    com, quad = _one_cell()
+    g = com[0, :3] + np.array([1.1, -0.6, 0.8])
+    u = np.array([0.3, -0.5, 0.8])
+    eps2 = 1e-4
+    idx = np.array([0])
+    zero = 0.0 * quad
+    Cm = local_coefficients(g, com, zero, idx, eps2)
+    Cq = local_coefficients(g, com, quad, idx, eps2)
+    errs = []
+    for t in (0.08, 0.04, 0.02):
+        x = g + t * u
+        if part == "monopole":
+            got = local_eval(Cm, (x - g)[None])[0]
+            want = quadrupole(x, com, zero, idx, eps2)
+        else:
+            got = local_eval(Cq, (x - g)[None])[0] - local_eval(Cm, (x - g)[None])[0]
+            want = quadrupole(x, com, quad, idx, eps2) - quadrupole(x, com, zero, idx, eps2)
+        errs.append(np.linalg.norm(got - want))
+    for a, b in zip(errs, errs[1:]):
+        assert 0.85 * 2 ** order < a / b < 1.15 * 2 ** order, errs
+    # at delta = 0 the expansion is the field itself
+    assert np.allclose(local_eval(Cq, np.zeros((1, 3)))[0],
+                       quadrupole(g, com, quad, idx, eps2), rtol=1e-12)
+
+
+def test_far_field_walk_stays_near_the_exact_set_sum(tree):
+    """walk_fp64 with far cells through the expansion differs from the exact
+    sum of the same interaction set by the truncation only: below 1e-3 of the
+    force at h / R < 0.3, and the far split follows its definition."""
+    t, com, link, ext = tree["t"], tree["com"], tree["link"], tree["ext"]
+    ps = t["pos"]
+    n = ps.shape[0]
+    nodes = tree["nodes"]
+    # second moments about each node's centre of mass, from its bodies
+    quad = np.zeros((com.shape[0], 8))
+    for j in range(com.shape[0]):
+        a, k = nodes["first"][j], nodes["count"][j]
+        b = ps[a:a + k].astype(np.float64)
+        d = b[:, :3] - com[j, :3].astype(np.float64)
+        S = np.einsum("k,ki,kj->ij", b[:, 3], d, d)
+        quad[j] = [S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2], np.trace(S), 0]
+    worst, fars = 0.0, 0
+    for g0 in range(0, n - 32, 640):
+        lo, hi = ps[g0:g0 + 32, :3].min(0), ps[g0:g0 + 32, :3].max(0)
+        _, cells = accepted(com, link, ext, lo, hi, 0.5)
+        fm, gc = far_split(cells, com, lo, hi, 0.3)
+        h = 0.5 * np.linalg.norm(hi.astype(np.float64) - lo)
+        R = np.linalg.norm(com[cells, :3].astype(np.float64) - gc, axis=1)
+        clear = np.abs(h / R - 0.3) > 1e-5
+        assert np.array_equal(fm[clear], (h / R < 0.3)[clear])
+        fars += int(fm.sum())
+        for i in (g0, g0 + 31):
+            x = ps[i, :3].astype(np.float64)
+            a0, _ = walk_fp64(x, (lo, hi), com, link, quad, ext, 0.5, 1e-4, 1)
+            a1, _ = walk_fp64(x, (lo, hi), com, link, quad, ext, 0.5, 1e-4, 1, far=0.3)
+            worst = max(worst, np.linalg.norm(a1 - a0) / np.linalg.norm(a0))
+    assert fars > 0 and worst < 1e-3, (fars, worst)
 
 
 def test_group_slots_partition_and_split_straddling_runs():

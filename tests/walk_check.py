@@ -76,6 +76,100 @@ def quadrupole(x, com, quad, idx, eps2):
     return (cd[:, None] * d - 3.0 * inv5[:, None] * sd).sum(0)
 
 
+def _pairings(n, k):
+    """All ways to take k disjoint unordered pairs from positions 0..n-1:
+    a list of (pairs, rest)."""
+    import itertools
+    out = []
+    for pairs in itertools.combinations(itertools.combinations(range(n), 2), k):
+        used = [p for pr in pairs for p in pr]
+        if len(set(used)) == 2 * k:
+            out.append((pairs, tuple(c for c in range(n) if c not in used)))
+    return out
+
+
+def _dfact(n):
+    return 1 if n <= 0 else n * _dfact(n - 2)
+
+
+def kernel_derivative(y, eps2, n):
+    """n-th derivative tensor of K(y) = (|y|^2 + eps2)^(-1/2) for points y
+    (k, 3): shape (k, 3, ..., 3).  Closed form: sum over j of
+    (-1)^(n-j) (2n-2j-1)!! rho^-(2n-2j+1) times the sum, over the ways of
+    pairing 2j of the n indices, of the Kronecker deltas of the pairs times
+    the y components of the other indices."""
+    y = np.asarray(y, np.float64)
+    r2 = (y * y).sum(1) + eps2
+    T = np.zeros((y.shape[0],) + (3,) * n)
+    import itertools
+    for j in range(n // 2 + 1):
+        coef = (-1) ** (n - j) * _dfact(2 * n - 2 * j - 1)
+        power = r2 ** (-(2 * n - 2 * j + 1) / 2.0)
+        pats = _pairings(n, j)
+        for ind in itertools.product(range(3), repeat=n):
+            val = np.zeros(y.shape[0])
+            for pairs, rest in pats:
+                if all(ind[a] == ind[b] for a, b in pairs):
+                    v = np.ones(y.shape[0])
+                    for r in rest:
+                        v = v * y[:, ind[r]]
+                    val += v
+            T[(slice(None),) + ind] += coef * power * val
+    return T
+
+
+def local_coefficients(g, com, quad, idx, eps2):
+    """Local expansion about the point g of the field of cells idx (monopole
+    and second moments S about their centres of mass): the symmetric tensors
+    C1 (3), C2 (3x3), C3 (3x3x3), C4 (3x3x3x3) with
+    a(g + delta) = C1 + C2 delta + C3 delta delta / 2 + C4 delta^3 / 6, where
+    C_n = sum over cells of m D_n(y) + S_ab D_(n+2)(y)_(...ab) / 2 for n <= 3
+    and C4 = sum of m D_4(y): y = g - centre of mass, D_n the n-th derivative
+    tensor of the softened kernel (the field of a cell at x is
+    m grad K(x - c) + S_ab grad d_a d_b K(x - c) / 2).  The truncation error
+    is O(delta^4) for the monopoles and O(delta^3) for the second moments."""
+    y = np.asarray(g, np.float64)[None, :] - com[idx, :3].astype(np.float64)
+    m = com[idx, 3].astype(np.float64)
+    q = quad[idx].astype(np.float64)
+    S = np.empty((idx.size, 3, 3))
+    S[:, 0, 0], S[:, 1, 1], S[:, 2, 2] = q[:, 0], q[:, 1], q[:, 2]
+    S[:, 0, 1] = S[:, 1, 0] = q[:, 3]
+    S[:, 0, 2] = S[:, 2, 0] = q[:, 4]
+    S[:, 1, 2] = S[:, 2, 1] = q[:, 5]
+    D = {n: kernel_derivative(y, eps2, n) for n in range(1, 6)}
+    C1 = np.einsum("k,ki->i", m, D[1]) + 0.5 * np.einsum("kab,kiab->i", S, D[3])
+    C2 = np.einsum("k,kij->ij", m, D[2]) + 0.5 * np.einsum("kab,kijab->ij", S, D[4])
+    C3 = np.einsum("k,kijl->ijl", m, D[3]) + 0.5 * np.einsum("kab,kijlab->ijl", S, D[5])
+    C4 = np.einsum("k,kijlm->ijlm", m, D[4])
+    return C1, C2, C3, C4
+
+
+def local_eval(C, delta):
+    """The expansion C of local_coefficients at offsets delta (b, 3)."""
+    C1, C2, C3, C4 = C
+    d = np.asarray(delta, np.float64)
+    return (C1[None, :] + np.einsum("ij,bj->bi", C2, d)
+            + 0.5 * np.einsum("ijl,bj,bl->bi", C3, d, d)
+            + np.einsum("ijlm,bj,bl,bm->bi", C4, d, d, d) / 6.0)
+
+
+def far_split(cells, com, lo, hi, far):
+    """The walk's far cells of a group box [lo, hi]: h^2 < far^2 R^2 with h
+    the box's half diagonal and R the distance from the box centre to the
+    cell's centre of mass, in float32 with the kernel's operation order.
+    Returns (far mask over the cells, box centre g)."""
+    lo = np.asarray(lo, f32)
+    hi = np.asarray(hi, f32)
+    g = (lo + hi) * f32(0.5)
+    e = hi - lo
+    h2 = f32(0.25) * ((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2])
+    c = com[cells, :3].astype(f32)
+    d = c - g
+    r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    phi2 = f32(far) * f32(far)
+    return h2 < phi2 * r2, g
+
+
 def potential_fp64(x, box, com, link, quad, extent, theta, eps2, mode):
     """Potential (without -G) at point x for the interaction set of `box`:
     sum m / r over monopoles; quadrupole cells (mode 1) add
@@ -124,26 +218,41 @@ def features(x, com, oct_, idx, extent, link):
 
 
 def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_ratio=0.0,
-              oct_=None):
+              oct_=None, far=0.0):
     """Acceleration (without G) at point x for the interaction set of `box`.
     mode 0: all accepted nodes as monopoles; 1: cells as quadrupoles;
     2: as 1 plus the network's residual, evaluated at the box centre, scaled by
     m s^3 / r^5 with r from x (net: a callable (rows, 8) -> (rows, 3); oct_:
-    the third moments, (nodes, 10+))."""
+    the third moments, (nodes, 10+)).  far > 0 (modes 1 and 2): the cells of
+    far_split take their monopole and quadrupole terms from the local
+    expansion about the box centre, except, in mode 2, the cells the network
+    corrects (their terms stay exact; the network's term is unchanged);
+    leaves stay exact."""
     lo, hi = box
     leaves, cells = accepted(com, link, extent, lo, hi, theta)
     x = np.asarray(x, np.float64)
     if mode == 0:
         return monopole(x, com, np.concatenate([leaves, cells]), eps2), leaves.size + cells.size
-    a = monopole(x, com, leaves, eps2) + quadrupole(x, com, quad, cells, eps2)
+    corr = np.zeros(cells.size, bool)
     if mode == 2 and cells.size:
         ctr = 0.5 * (np.asarray(lo, np.float64) + np.asarray(hi, np.float64))
         X, k = features(ctr, com, oct_, cells, extent, link)
-        ok = (k > 0) & (X[:, 3] > min_ratio)
-        o = net(X[ok])
-        d = com[cells[ok], :3].astype(np.float64) - x
+        corr = (k > 0) & (X[:, 3] > min_ratio)
+    if far > 0:
+        fm, g = far_split(cells, com, lo, hi, far)
+        fm &= ~corr
+        a = monopole(x, com, leaves, eps2) + quadrupole(x, com, quad, cells[~fm], eps2)
+        if fm.any():
+            g = g.astype(np.float64)
+            C = local_coefficients(g, com, quad, cells[fm], eps2)
+            a = a + local_eval(C, (x - g)[None, :])[0]
+    else:
+        a = monopole(x, com, leaves, eps2) + quadrupole(x, com, quad, cells, eps2)
+    if corr.any():
+        o = net(X[corr])
+        d = com[cells[corr], :3].astype(np.float64) - x
         r2 = (d * d).sum(1)
-        a = a + ((k[ok] / (r2 * r2 * np.sqrt(r2)))[:, None] * o).sum(0)
+        a = a + ((k[corr] / (r2 * r2 * np.sqrt(r2)))[:, None] * o).sum(0)
     return a, leaves.size + cells.size
 
 
