@@ -58,16 +58,20 @@ sys.path.insert(0, ROOT)
 METRIC = "body-steps/s at ≤1e-3 median force err vs FP64 sum: learned vs plain walk"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the evaluation kernel, from two rocprofv3 --pmc runs
+# HBM bytes per walk of the evaluation kernels (far-field kernel + list
+# evaluation when the far field is on), from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
-# (walk, theta, bodies) of the workload they were measured on.
+# (walk, theta, bodies, far field) of the workload they were measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75, 1 << 21): {"bytes": 695638019,
-                                 "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.675, 1 << 24): {"bytes": 7069520245,
-                                  "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
-    ("learned", 0.6875, 1 << 24): {"bytes": 6812063984,
-                                   "profile": "profiles/pmc_walk_eval_learned_theta0.6875_merger16M.md"},
+    ("learned", 0.75, 1 << 21, 0.0): {
+        "bytes": 695638019, "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
+    ("learned", 0.675, 1 << 24, 0.0): {
+        "bytes": 7069520245, "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
+    ("learned", 0.6875, 1 << 24, 0.0): {
+        "bytes": 6812063984, "profile": "profiles/pmc_walk_eval_learned_theta0.6875_merger16M.md"},
+    ("learned", 0.675, 1 << 24, 0.5): {
+        "bytes": 12219001121,
+        "profile": "profiles/pmc_walk_far_eval_learned_theta0.675_far0.5_merger16M.md"},
 }
 BUDGET = 1e-3
 # Each walk's opening angle is calibrated against this share of the budget on
@@ -294,7 +298,7 @@ def eval_roofline(walks, mode, theta, n):
     # opened a node (the tests run in the traversal kernel: ~0.3 % here)
     algo = FLOP_INTERACTION * inter + FLOP_OPEN * st["opened"]
     achieved = algo / (ev_ms * 1e-3) / 1e12
-    tr = MEASURED_TRAFFIC.get((mode, theta, n)) or {}
+    tr = MEASURED_TRAFFIC.get((mode, theta, n, walks.sim.far if mode != "plain" else 0.0)) or {}
     return {
         "bound": "mfma",
         "compute": ("FP32 vector ALU (the walk has no matrix-core work except the learned "
@@ -310,7 +314,7 @@ def eval_roofline(walks, mode, theta, n):
         "frac": achieved / FP32_PEAK_TFLOPS,
         "frac_of_unpacked_fp32": achieved / (FP32_PEAK_TFLOPS / 2),
         "traffic": tr.get("bytes"),
-        "traffic_unit": "bytes per launch (HBM read + write)",
+        "traffic_unit": "bytes per walk (HBM read + write of the kernels timed)",
         "traffic_source": tr.get("profile"),
         "numerator": ("20 FLOP x accepted body-node interactions + 9 FLOP x opening tests that "
                       "opened a node, per walk"),

@@ -41,6 +41,8 @@ def main():
                     help="fixed first-pass list capacity per group (0: adaptive)")
     ap.add_argument("--group-split", type=float, default=-1.0,
                     help="volume fraction for splitting 64-body groups (< 0: by theta)")
+    ap.add_argument("--far", type=float, default=0.0,
+                    help="quadrupole and learned walks: far-field ratio (gd_walk_set_far_field)")
     ap.add_argument("--phases", action="store_true",
                     help="time the walk's phases with HIP events (last step's walk)")
     ap.add_argument("--copy-mib", type=int, default=0)
@@ -57,6 +59,7 @@ def main():
         if a.list_cap:
             sim.lib.gd_walk_set_list_capacity(a.list_cap, 0)
         sim.lib.gd_walk_set_group_split(a.group_split)
+        sim.set_far_field(a.far)
         if a.walk in ("quad", "learned"):
             sim.set_quadrupole(2 if a.walk == "learned" else 1)
         sim.sync()
@@ -75,7 +78,8 @@ def main():
                 sim.step(dt, a.theta)
         sim.sync()
         t1 = time.perf_counter()
-        print(f"profile_step: {n} bodies, {a.steps} steps, walk={a.walk}, theta={a.theta}: "
+        print(f"profile_step: {n} bodies, {a.steps} steps, walk={a.walk}, theta={a.theta}, "
+              f"far field {a.far}: "
               f"{1e3 * (t1 - t0) / a.steps:.3f} ms/step", flush=True)
         if a.phases and a.walk != "reference":
             import ctypes as C
