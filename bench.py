@@ -61,15 +61,16 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per walk of the evaluation kernels (far-field kernel + list
 # evaluation when the far field is on), from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by what they read for a 1 GiB copy; keyed by
-# (walk, theta, bodies, far field) of the workload they were measured on.
+# (walk, theta, bodies, far field, opening rule) of the workload they were
+# measured on.
 MEASURED_TRAFFIC = {
-    ("learned", 0.75, 1 << 21, 0.0): {
+    ("learned", 0.75, 1 << 21, 0.0, "size"): {
         "bytes": 695638019, "profile": "profiles/pmc_walk_eval_learned_theta0.75_merger2M.md"},
-    ("learned", 0.675, 1 << 24, 0.0): {
+    ("learned", 0.675, 1 << 24, 0.0, "size"): {
         "bytes": 7069520245, "profile": "profiles/pmc_walk_eval_learned_theta0.675_merger16M.md"},
-    ("learned", 0.6875, 1 << 24, 0.0): {
+    ("learned", 0.6875, 1 << 24, 0.0, "size"): {
         "bytes": 6812063984, "profile": "profiles/pmc_walk_eval_learned_theta0.6875_merger16M.md"},
-    ("learned", 0.675, 1 << 24, 0.5): {
+    ("learned", 0.675, 1 << 24, 0.5, "size"): {
         "bytes": 12219001121,
         "profile": "profiles/pmc_walk_far_eval_learned_theta0.675_far0.5_merger16M.md"},
 }
@@ -110,6 +111,9 @@ def parse():
                     help="the corrector acts on cells with size / distance above this")
     ap.add_argument("--train-targets", type=int, default=8192,
                     help="target bodies per snapshot for the corrector's samples")
+    ap.add_argument("--opening", choices=("size", "radius"), default="size",
+                    help="opening rule of the three walks (gd_walk_set_opening_rule): cell "
+                         "size or mass radius")
     ap.add_argument("--far", type=float, default=0.5,
                     help="quadrupole and learned walks: cells with h / R below this go "
                          "through their group's local expansion (0: off)")
@@ -298,7 +302,8 @@ def eval_roofline(walks, mode, theta, n):
     # opened a node (the tests run in the traversal kernel: ~0.3 % here)
     algo = FLOP_INTERACTION * inter + FLOP_OPEN * st["opened"]
     achieved = algo / (ev_ms * 1e-3) / 1e12
-    tr = MEASURED_TRAFFIC.get((mode, theta, n, walks.sim.far if mode != "plain" else 0.0)) or {}
+    tr = MEASURED_TRAFFIC.get((mode, theta, n, walks.sim.far if mode != "plain" else 0.0,
+                               walks.sim.opening)) or {}
     return {
         "bound": "mfma",
         "compute": ("FP32 vector ALU (the walk has no matrix-core work except the learned "
@@ -436,6 +441,8 @@ def main():
     sim.set_node_records(False)
     sim.far = args.far
     sim.set_far_field(args.far)
+    sim.set_opening_rule(args.opening)
+    sim.opening = args.opening
     net = cr.Corrector(seed=1)
     walks = Walks(sim, net, args.min_ratio)
     cal = ics.subsample(n, 4096, seed=CAL_SEED)
@@ -520,6 +527,7 @@ def main():
         log("CPU baseline (host only)")
         cpu = cpu_baseline(sim, final, eps, ref["theta"] if ref else 0.5, dt, args.cpu_groups)
     sim.set_far_field(0.0)
+    sim.set_opening_rule("size")
     net.close()
     sim.close()
 
@@ -550,6 +558,7 @@ def main():
             "theta": th_l,
             "force_error_budget": BUDGET,
             "far_field": args.far,
+            "opening_rule": args.opening,
             "calibration_budget": CALIBRATION_SHARE * BUDGET,
             "judge": (f"FP64 direct sum; calibration sample seed {CAL_SEED}, judge sample "
                       f"seed {JUDGE_SEED} (4096 bodies each)"),

@@ -115,6 +115,7 @@ _PROTOS = {
     "gd_build_timing": (None, [i32]),
     "gd_build_phase_ms": (i32, [vp]),
     "gd_tree_walk_arrays": (None, [C.POINTER(TreeState), vp, vp, vp, i32]),
+    "gd_tree_mass_radii": (None, [C.POINTER(TreeState), vp, i32]),
     "gd_tree_third_moments": (None, [C.POINTER(TreeState), vp, i32]),
     "gd_walk": (None, [C.POINTER(TreeState), vp, vp, vp, f32, f32, f32, i32,
                        C.POINTER(WalkStats)]),
@@ -122,6 +123,7 @@ _PROTOS = {
     "gd_walk_set_list_capacity": (i32, [i32, i32]),
     "gd_walk_set_group_split": (f32, [f32]),
     "gd_walk_set_far_field": (f32, [f32]),
+    "gd_walk_set_opening_rule": (i32, [i32]),
     "gd_walk_set_async": (i32, [i32]),
     "gd_walk_poll": (i32, []),
     "gd_walk_kernel_ms": (f32, []),

@@ -191,9 +191,11 @@ static void gd_builder_walk_arrays(gd_builder* b, int maxNodes) {
     }
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
+    if (b->wrad) GD_HIP(hipFree(b->wrad));
     if (b->meta) GD_HIP(hipFree(b->meta));
     GD_HIP(hipMalloc(&b->wcom, sizeof(float4) * (size_t)maxNodes));
     GD_HIP(hipMalloc(&b->wlink, sizeof(int2) * (size_t)maxNodes));
+    GD_HIP(hipMalloc(&b->wrad, sizeof(float) * (size_t)maxNodes));
     GD_HIP(hipMalloc(&b->meta, sizeof(int4) * (size_t)maxNodes));
     b->maxNodes = maxNodes;
 }
@@ -273,6 +275,7 @@ void sumDestroyer(treeBuilder* sum) {
     GD_HIP(hipFree(b->scanTiles));
     if (b->wcom) GD_HIP(hipFree(b->wcom));
     if (b->wlink) GD_HIP(hipFree(b->wlink));
+    if (b->wrad) GD_HIP(hipFree(b->wrad));
     if (b->meta) GD_HIP(hipFree(b->meta));
     if (b->wmom) GD_HIP(hipFree(b->wmom));
     memset(b, 0, sizeof(*b));
@@ -533,6 +536,19 @@ void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* qua
             out[2 * (size_t)i + 1] = make_float4(m1.x, m1.y, tr, 0.f);
         }
     }
+}
+
+void gd_tree_mass_radii(const treeState* tree, float* rad, int count) {
+    if (!tree || !tree->builder) {
+        GD_FAIL("gd_tree_mass_radii: uninitialised tree state");
+        return;
+    }
+    gd_builder* b = gd_builder_of(tree->builder);
+    if (count < 0 || count > b->maxNodes) {
+        GD_FAIL("gd_tree_mass_radii: count %d outside [0, %d]", count, b->maxNodes);
+        return;
+    }
+    if (rad) GD_HIP(hipMemcpy(rad, b->wrad, sizeof(float) * (size_t)count, hipMemcpyDeviceToHost));
 }
 
 void gd_tree_levels(treeState* tree) {

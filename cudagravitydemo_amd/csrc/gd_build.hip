@@ -546,6 +546,11 @@ __device__ __forceinline__ void gd_leaf_third(const float4* __restrict__ pos, in
  * A single-body leaf has zero moments; its moment record is not written.
  *
  * MOM: 0 centre of mass only, 1 also second moments, 2 also third moments.
+ * Every build also writes each node's mass radius wrad, a bound on the
+ * distance from its centre of mass to its bodies: the largest such distance
+ * for a leaf (0 for one body), max over the children of |c_child - c| +
+ * wrad_child for an internal node (the opening rule of
+ * gd_walk_set_opening_rule).
  * Input: the per-node meta of k_build_nodes (first body, count, parent,
  * first child or -1).  Output: the walk arrays of level L.
  */
@@ -555,15 +560,17 @@ __device__ __forceinline__ void gd_leaf_third(const float4* __restrict__ pos, in
 /* Finishes the level-L nodes [x0, x1) (step 4): leaves from their bodies,
  * internal nodes from the tile's results at the tile slot h of their first
  * child (tileBase = that level's index of slot 0): centre of mass sC[h],
- * children sNc[h], moment sums sTerm[e * 256 + h]. */
+ * children sNc[h], moment sums sTerm[e * 256 + h], mass radius sRad[h]. */
 template <int MOM>
 __device__ __forceinline__ int gd_finish_range(int x0, int x1, int level,
                                                const float4* __restrict__ pos,
                                                const int4* __restrict__ meta, long long tileBase,
                                                const float4* sC, const float* sTerm,
-                                               const int* sNc, float4* __restrict__ wcom,
+                                               const float* sRad, const int* sNc,
+                                               float4* __restrict__ wcom,
                                                int2* __restrict__ wlink,
-                                               float4* __restrict__ wmom) {
+                                               float4* __restrict__ wmom,
+                                               float* __restrict__ wrad) {
     int zeros = 0;
     /* a long range (leaf-heavy levels) is walked with the next chunk's meta
        in flight while this chunk's leaves load their bodies */
@@ -579,6 +586,7 @@ __device__ __forceinline__ int gd_finish_range(int x0, int x1, int level,
             const float4 c = sC[h];
             wcom[X] = c;
             wlink[X] = make_int2(mt.w, (sNc[h] << 5) | level);
+            wrad[X] = sRad[h];
             if (MOM >= 1) {
                 float v[16];
 #pragma unroll
@@ -605,6 +613,13 @@ __device__ __forceinline__ int gd_finish_range(int x0, int x1, int level,
             else zeros++;
             wcom[X] = c;
             wlink[X] = make_int2(-(first + 1), (count << 5) | level);
+            float rad = 0.f;   /* one body: its own centre of mass */
+            for (int b = 0; count > 1 && b < count; ++b) {
+                const float4 x = pos[first + b];
+                const float dx = x.x - c.x, dy = x.y - c.y, dz = x.z - c.z;
+                rad = fmaxf(rad, sqrtf((dx * dx + dy * dy) + dz * dz));
+            }
+            wrad[X] = rad;
             if (MOM >= 1 && count > 1) {   /* a single-body leaf's moments are zero */
                 float4 q0, q1, o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0;
                 gd_leaf_moments(pos, first, count, c, &q0, &q1);
@@ -623,11 +638,15 @@ void k_build_mass(const float4* __restrict__ pos,
                                                              gd_tree_info* info, int level,
                                                              float4* __restrict__ wcom,
                                                              int2* __restrict__ wlink,
-                                                             float4* __restrict__ wmom) {
+                                                             float4* __restrict__ wmom,
+                                                             float* __restrict__ wrad) {
     constexpr int TERMS = MOM == 2 ? 16 : (MOM == 1 ? 6 : 1);
     __shared__ float4 sCom[GD_MASS_TILE];     /* children's centres of mass */
     __shared__ float4 sC[GD_MASS_TILE];       /* their parent's */
     __shared__ float sTerm[TERMS * GD_MASS_TILE];   /* [term][child] */
+    /* once the parents' centres of mass are formed, sCom holds the children's
+       radius terms, then each run's radius at its first child */
+    float* sRad = reinterpret_cast<float*>(sCom);
     __shared__ int sPar[GD_MASS_TILE + 1];    /* parent of tile child t - 1 */
     __shared__ int sNc[GD_MASS_TILE];         /* a run's children, at its first child */
     __shared__ int sRange[2];
@@ -644,8 +663,8 @@ void k_build_mass(const float4* __restrict__ pos,
         for (long long x0 = lvStart + (long long)blockIdx.x * GD_MASS_TILE; x0 < lvEnd;
              x0 += (long long)gridDim.x * GD_MASS_TILE) {
             const int x1 = x0 + GD_MASS_TILE < lvEnd ? (int)x0 + GD_MASS_TILE : lvEnd;
-            zeros += gd_finish_range<MOM>((int)x0, x1, level, pos, meta, 0, sC, sTerm, sNc,
-                                          wcom, wlink, wmom);
+            zeros += gd_finish_range<MOM>((int)x0, x1, level, pos, meta, 0, sC, sTerm, sRad,
+                                          sNc, wcom, wlink, wmom, wrad);
         }
         if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
         return;
@@ -659,9 +678,11 @@ void k_build_mass(const float4* __restrict__ pos,
         int par = -2;
         bool hasMom = false;
         float4 ck = z4, m0 = z4, m1 = z4, m2 = z4, m3 = z4;
+        float rk = 0.f;
         if (valid) {
             const int4 mt = meta[j];
             ck = wcom[j];
+            rk = wrad[j];
             par = mt.z;
             hasMom = mt.w >= 0 || (mt.y & 0x1fffffff) > 1;
             if (MOM >= 1 && hasMom) {
@@ -709,10 +730,16 @@ void k_build_mass(const float4* __restrict__ pos,
             sRange[0] = a;
             sRange[1] = b;
         }
+        __syncthreads();
+        /* 3. each child's terms about its parent's centre of mass (a child
+           whose run another tile finishes writes unused ones): the radius
+           bound, and with MOM the moments' parallel-axis shifts */
+        {
+            const float4 cp = sC[tid];
+            const float dx = ck.x - cp.x, dy = ck.y - cp.y, dz = ck.z - cp.z;
+            sRad[tid] = sqrtf((dx * dx + dy * dy) + dz * dz) + rk;
+        }
         if (MOM >= 1) {
-            __syncthreads();
-            /* 3. each child's terms about its parent's centre of mass (a
-               child whose run another tile finishes writes unused ones) */
             {
                 const float4 cp = sC[tid];
                 const float4 qa = m0;
@@ -758,11 +785,19 @@ void k_build_mass(const float4* __restrict__ pos,
                 for (int e = 0; e < TERMS; ++e) sTerm[e * GD_MASS_TILE + tid] = sum[e];
             }
         }
-        if (head) sNc[tid] = nc;
+        if (MOM == 0) __syncthreads();
+        if (head) {
+            /* the run's radius, into its first child's slot (only this run's
+               first thread reads its children's slots) */
+            float r = 0.f;
+            for (int k = 0; k < nc; ++k) r = fmaxf(r, sRad[tid + k]);
+            sRad[tid] = r;
+            sNc[tid] = nc;
+        }
         __syncthreads();
         /* 4. the tile's level-L nodes */
         zeros += gd_finish_range<MOM>(sRange[0], sRange[1], level, pos, meta, coff + t0, sC,
-                                      sTerm, sNc, wcom, wlink, wmom);
+                                      sTerm, sRad, sNc, wcom, wlink, wmom, wrad);
         __syncthreads();   /* the next tile reuses the LDS */
     }
     if (zeros) atomicAdd(&info->zeroMassNodes, zeros);
@@ -897,13 +932,13 @@ void gd_build_octree(gd_builder* b, const uint64_t* sortedKeys, const float4* so
         if (lb > fit) lb = fit;
         if (b->quad == 2)
             k_build_mass<2><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
-                                                                  b->wcom, b->wlink, b->wmom);
+                                                                  b->wcom, b->wlink, b->wmom, b->wrad);
         else if (b->quad == 1)
             k_build_mass<1><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
-                                                                  b->wcom, b->wlink, b->wmom);
+                                                                  b->wcom, b->wlink, b->wmom, b->wrad);
         else
             k_build_mass<0><<<lb, GD_MASS_TILE, 0, gd_stream()>>>(sortedPos, b->meta, b->dinfo, L,
-                                                                  b->wcom, b->wlink, nullptr);
+                                                                  b->wcom, b->wlink, nullptr, b->wrad);
         GD_LAUNCHED();
     }
     if (records) gd_build_node_records(b, sortedKeys, nodes, maxNodes);

@@ -134,6 +134,8 @@ typedef struct {
     float4* wcom;                    /* centre of mass + mass */
     int2* wlink;                     /* .x first child (>= 0) or -(first body + 1) for a leaf,
                                         .y = (children or bodies) << 5 | level */
+    float* wrad;                     /* mass radius: bound on the distance from the centre of
+                                        mass to the node's bodies */
     int maxNodes;
     int noRecords;                   /* 1: builds leave the 80-byte node array alone */
     int lastBodies;                  /* bodies of the last octree build */
@@ -146,6 +148,10 @@ typedef struct {
 
 static inline gd_sorter* gd_sorter_of(radixSorter* s) { return (gd_sorter*)s; }
 static inline gd_builder* gd_builder_of(treeBuilder* b) { return (gd_builder*)b; }
+
+/* The mass radii the production walks open cells by (b->wrad under the
+ * mass-radius rule of gd_walk_set_opening_rule), or NULL for the size rule. */
+const float* gd_walk_radii(const gd_builder* b);
 
 /* ---- launchers shared between translation units (all on gd_stream()) */
 void gd_launch_bbox(const float4* pos, const float3* lo, int n, gd_builder* b, float3* outMin,

@@ -560,7 +560,8 @@ const float* gd_mlp_bias(const gd_mlp* net) { return net->bias; }
 __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
     const uint64_t* __restrict__ keys, const float4* __restrict__ pos, int n,
     const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const gd_tree_info* __restrict__ info,
     const int* __restrict__ targets, int nt, float theta, float ratio, float minRatio,
     int2* __restrict__ pairs,
     long long cap, unsigned long long* __restrict__ count, int* __restrict__ dropped,
@@ -594,7 +595,8 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
         gd_probe pr = {0, 0, 0, false, false, false};
         if ((int)lane < take) {
             const int idx = stack[base + lane];
-            pr = gd_test_node(wcom[idx], wlink[idx], idx, ext, th2, lx, ly, lz, hx, hy, hz);
+            pr = gd_test_node(wcom[idx], wlink[idx], idx, ext, th2, lx, ly, lz, hx, hy, hz,
+                              wrad != nullptr, wrad ? wrad[idx] : 0.f);
         }
         __builtin_amdgcn_wave_barrier();
         top = base;
@@ -756,7 +758,8 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
     long long n = 0;
     if (!gd_has_error()) {
         k_samples_list<<<gd_div_up(nt, GD_SL_WAVES), 64 * GD_SL_WAVES, 0, gd_stream()>>>(
-            tree->mortonOut, pos, tree->starCount, b->wcom, b->wlink, b->dinfo, targets, nt, theta,
+            tree->mortonOut, pos, tree->starCount, b->wcom, b->wlink, gd_walk_radii(b), b->dinfo,
+            targets, nt, theta,
             gd_walk_split_ratio(theta), minRatio, pairs, cap, count, dropped, centre);
         GD_LAUNCHED();
         unsigned long long total = 0;

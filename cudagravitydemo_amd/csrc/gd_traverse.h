@@ -100,11 +100,15 @@ struct gd_probe {
     bool accept, open, leaf;
 };
 
+/* The opening test: a leaf, or b^2 < theta^2 d^2 with d the distance from the
+ * box to the centre of mass c and b the node's size ext / 2^level or, with
+ * useRad (the mass-radius rule), its mass radius rad. */
 __device__ __forceinline__ gd_probe gd_test_node(float4 c, int2 lk, int idx, float ext,
                                                  float th2, float lx, float ly, float lz,
-                                                 float hx, float hy, float hz) {
+                                                 float hx, float hy, float hz, bool useRad,
+                                                 float rad) {
     gd_probe r;
-    const float size = ldexpf(ext, -(lk.y & 31));
+    const float size = useRad ? rad : ldexpf(ext, -(lk.y & 31));
     const float ddx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
     const float ddy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
     const float ddz = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
@@ -165,10 +169,12 @@ __device__ __forceinline__ void gd_wave_scan_dpp(int v, int& excl, int& total) {
  * many as hold at most 128 nodes -- and expands them into pop (128 + 8 ints
  * of LDS); lane l then tests nodes l and 64 + l, so that two node loads per
  * lane are in flight.  Children of the upper nodes go on top: the walk stays
- * depth-first and the stack shallow. */
+ * depth-first and the stack shallow.  wrad: the nodes' mass radii for the
+ * mass-radius opening rule, or NULL for the size rule. */
 template <bool SPLIT>
 __device__ __forceinline__ bool gd_traverse_box(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
+    const float* __restrict__ wrad,
     const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
     float hy, float hz, int* stack, int stackCap, int* pop, int* __restrict__ list, int cap,
     int limit, int* nFrontOut, int* nBackOut, unsigned long long* opened, int* depth) {
@@ -213,8 +219,10 @@ __device__ __forceinline__ bool gd_traverse_box(
         const int ib = hasB ? pb : 0;
         const float4 ca = wcom[ia], cb = wcom[ib];
         const int2 la = wlink[ia], lb = wlink[ib];
-        gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz);
-        gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz);
+        const bool useRad = wrad != nullptr;
+        const float ra = useRad ? wrad[ia] : 0.f, rb = useRad ? wrad[ib] : 0.f;
+        gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz, useRad, ra);
+        gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz, useRad, rb);
         if (!hasA) A.accept = A.open = false, A.nk = 0;
         if (!hasB) B.accept = B.open = false, B.nk = 0;
         __builtin_amdgcn_wave_barrier();

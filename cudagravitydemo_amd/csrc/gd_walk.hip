@@ -211,7 +211,8 @@ __host__ __device__ __forceinline__ int gd_seg_count(int groups) {
 template <bool SPLIT, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const gd_tree_info* __restrict__ info, float theta, int n,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList, int slots,
     const long long* __restrict__ offsets,
     const int* __restrict__ caps, int cap, const int* __restrict__ slotCap,
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_traverse(
     }
     const int limit = slotCap ? min(slotCap[s], size) : size;
     int depth = 0;
-    const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, lx, ly, lz, hx, hy, hz,
+    const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, wrad, info, theta, lx, ly, lz, hx, hy, hz,
                                            sStack[wave], STACK, sStack[wave] + STACK, list, size,
                                            limit, &nf, &nb,
                                            &nOpen, &depth);
@@ -414,7 +415,8 @@ __global__ __launch_bounds__(1024) void k_walk_place(const int2* __restrict__ sl
 template <bool SPLIT>
 __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const gd_tree_info* __restrict__ info, float theta, int n,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const gd_tree_info* __restrict__ info, float theta, int n,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList, int slots, int cap2,
     int floor, int* __restrict__ slotCap, const int* __restrict__ slotsDev = nullptr) {
     __shared__ int sStack[GD_TR_WAVES][GD_TR_STACK + GD_POP];
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(64 * GD_TR_WAVES) void k_walk_estimate(
     int nf = 0, nb = 0, depth = 0;
     unsigned long long nOpen = 0;
     const int bound = cap2 / GD_BODY_RATIO;
-    const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, info, theta, p.x, p.y, p.z, p.x, p.y,
+    const bool ok = gd_traverse_box<SPLIT>(wcom, wlink, wrad, info, theta, p.x, p.y, p.z, p.x, p.y,
                                            p.z, sStack[wave], GD_TR_STACK,
                                            sStack[wave] + GD_TR_STACK, nullptr, bound,
                                            bound, &nf, &nb, &nOpen, &depth);
@@ -1411,7 +1413,8 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
 template <int MODE, bool PHI, int STACK>
 __device__ __forceinline__ void gd_walk_one_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, long long i, int* sStack,
     int* __restrict__ overflowBodies, int* __restrict__ overflowCount,
     float3* __restrict__ acc, float* __restrict__ phi, float eps2, float G, float theta,
@@ -1438,10 +1441,11 @@ __device__ __forceinline__ void gd_walk_one_body(
             c = wcom[idx];
             const int2 lk = wlink[idx];
             size = ldexpf(ext, -(lk.y & 31));
+            const float b = wrad ? wrad[idx] : size;   /* the opening rule's measure */
             const float dx = c.x - p.x, dy = c.y - p.y, dz = c.z - p.z;
             d2 = dx * dx + dy * dy + dz * dz;
             leaf = lk.x < 0;
-            accept = leaf || (size * size < th2 * d2);
+            accept = leaf || (b * b < th2 * d2);
             open = !accept;
             nk = open ? (lk.y >> 5) : 0;
             first = lk.x;
@@ -1530,7 +1534,8 @@ __device__ __forceinline__ void gd_walk_one_body(
 template <int MODE, bool PHI, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info,
     const int2* __restrict__ slotRange, const int* __restrict__ groupList,
     const int* __restrict__ bodyList, long long count,
@@ -1560,7 +1565,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk_body(
             i = rg.x + (q & 63);
         }
         if (i >= n) continue;   /* wave-uniform */
-        gd_walk_one_body<MODE, PHI, STACK>(pos, wcom, wlink, wmom, info, i, sStack,
+        gd_walk_one_body<MODE, PHI, STACK>(pos, wcom, wlink, wrad, wmom, info, i, sStack,
                                            overflowBodies, overflowCount, acc, phi, eps2, G,
                                            theta, stats);
         __builtin_amdgcn_wave_barrier();
@@ -1637,6 +1642,10 @@ static int g_lastRuns = 0;   /* 64-body runs of the last asynchronous walk */
  * and learned walks without potential, a cell with h / R below it is summed
  * through its group's local expansion (gd_far_half). */
 static float g_farRatio = 0.f;
+/* Opening rule of the production walks: 0 cell size, 1 mass radius. */
+static int g_openRule = 0;
+
+const float* gd_walk_radii(const gd_builder* b) { return g_openRule == 1 ? b->wrad : nullptr; }
 
 /* optional HIP-event timing of the walk's phases (library stream) */
 static int g_walkTiming = 0;
@@ -1785,6 +1794,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     constexpr bool SPLIT = MODE != 0;
     const int runs = gd_div_up(n, 64);
     const int groups = 2 * runs;   /* group slots: two per 64-body run */
+    const float* wrad = gd_walk_radii(b);
     gd_walk_status* st = g_walk.status;
     k_walk_split<<<gd_div_up(runs, 4), 256, 0, gd_stream()>>>(keys, pos, n, runs,
                                               gd_walk_split_ratio(theta),
@@ -1810,7 +1820,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
 #define GD_PASS1(STK)                                                                          \
     k_walk_traverse<SPLIT, GD_TR_WAVES, STK><<<gd_div_up(groups, GD_TR_WAVES),                  \
                                                64 * GD_TR_WAVES, 0, gd_stream()>>>(                             \
-        pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.order, groups,    \
+        pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.order, groups,    \
         g_walk.offsets,                                                                        \
         g_walk.caps, 0, nullptr, g_walk.lists, g_walk.counts, g_walk.sizes, g_walk.overflow1,  \
         &st->over1, g_walk.fixedCap > 0 ? nullptr : g_walk.overflow2, &st->over2, &st->deep1,   \
@@ -1864,12 +1874,12 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
                                   &st->over2);
         GD_LAUNCHED();
         k_walk_estimate<SPLIT><<<gd_div_up(S2, GD_TR_WAVES), 64 * GD_TR_WAVES, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
+            pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
             cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap,
             &st->over1);
         GD_LAUNCHED();
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<S2, 64, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
+            pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1, S2,
             nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
@@ -1885,12 +1895,12 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         const long long bodies = 64ll * groups;
         k_walk_body<BMODE0, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
             <<<GD_ASYNC_BODY_BLOCKS, 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
-                pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
+                pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
                 nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
                 g_walk.stats, &st->over2, 64);
         GD_LAUNCHED();
         k_walk_body<BMODE0, PHI, 1, GD_BODY_STACK><<<GD_ASYNC_DEEP_BLOCKS, 64, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, nullptr,
+            pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, nullptr,
             g_walk.bodyOver, bodies, nullptr, nullptr, acc, phi, eps2, G, theta, n,
             g_walk.stats, &st->over3, 1);
         GD_LAUNCHED();
@@ -1928,11 +1938,11 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     for (int b0 = 0; b0 < over1; b0 += slots) {
         const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
         k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
+            pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap);
         GD_LAUNCHED();
         k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
+            pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats);
         GD_LAUNCHED();
@@ -1971,7 +1981,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
     k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
         <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
+            pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
             nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
             g_walk.stats);
     GD_LAUNCHED();
@@ -1979,7 +1989,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
     k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64, 0, gd_stream()>>>(
-        pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
+        pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
         h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, g_walk.stats);
     GD_LAUNCHED();
 }
@@ -2150,6 +2160,17 @@ extern "C" float gd_walk_set_far_field(float ratio) {
         return prev;
     }
     g_farRatio = ratio;
+    return prev;
+}
+
+extern "C" int gd_walk_set_opening_rule(int rule) {
+    if (rule != 0 && rule != 1) {
+        GD_FAIL("gd_walk_set_opening_rule: rule %d is neither 0 (cell size) nor 1 (mass radius)",
+                rule);
+        return -1;
+    }
+    const int prev = g_openRule;
+    g_openRule = rule;
     return prev;
 }
 

@@ -179,6 +179,15 @@ class Simulation:
         check(self.lib)
         return float(prev)
 
+    def set_opening_rule(self, rule):
+        """Opening rule of the production walks (gd_walk_set_opening_rule):
+        "size" (cell size, the default) or "radius" (the cells' mass radii).
+        Returns the previous rule.  The setting belongs to the library."""
+        code = {"size": 0, "radius": 1}[rule]
+        prev = self.lib.gd_walk_set_opening_rule(code)
+        check(self.lib)
+        return ("size", "radius")[prev]
+
     def step(self, dt, theta):
         self.build_tree_async()
         self.walk(theta)
@@ -305,6 +314,13 @@ class Simulation:
                                      quad.ctypes.data, count)
         check(self.lib)
         return com, link, quad
+
+    def mass_radii(self, count):
+        """Mass radii (count,) of the first `count` nodes (gd_tree_mass_radii)."""
+        out = np.empty(count, np.float32)
+        self.lib.gd_tree_mass_radii(C.byref(self.state), out.ctypes.data, count)
+        check(self.lib)
+        return out
 
     def third_moments(self, count):
         """Third moments (count, 12) of the first `count` nodes: Oxxx Oyyy Ozzz

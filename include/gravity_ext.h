@@ -95,6 +95,14 @@ void gd_tree_third_moments(const treeState* tree, void* oct, int count);
  * (2 float4: Sxx Syy Szz Sxy, Sxz Syz trace 0).  Synchronises. */
 void gd_tree_walk_arrays(const treeState* tree, void* com, void* link, void* quad, int count);
 
+/* Copies the mass radii of nodes [0, count) to host memory: for a leaf the
+ * largest distance from its centre of mass to its bodies (0 for one body),
+ * for an internal node the largest |c_child - c| + radius_child over its
+ * children (float32, sqrt((dx dx + dy dy) + dz dz) + radius), so a bound on
+ * the distance from c to every body below.  Every build writes them.
+ * Synchronises. */
+void gd_tree_mass_radii(const treeState* tree, float* rad, int count);
+
 /* Copies the device level tables into tree->levelOffsets / levelCount /
  * deepestLevel (synchronises). */
 void gd_tree_levels(treeState* tree);
@@ -186,6 +194,18 @@ float gd_walk_set_group_split(float ratio);
  * every cell body by body.  Second-pass and per-body walks always do.
  * Returns the previous setting. */
 float gd_walk_set_far_field(float ratio);
+
+/* Opening rule of the production walks (list walks, second pass, per-body
+ * walk and the training samples; not the reference walk calculateAcceleration).
+ * 0 (the default): an internal cell is accepted for a box when
+ * size^2 < theta^2 d^2, size = extent / 2^level and d the distance from the
+ * box to the cell's centre of mass.  1: b^2 < theta^2 d^2 with b the cell's
+ * mass radius (gd_tree_mass_radii), a bound on the distance from its centre
+ * of mass to its bodies, so that a cell whose bodies lie close to their
+ * centre of mass is accepted nearer; the same theta then means a different
+ * (smaller) error.  Returns the previous rule, or -1 after reporting an
+ * error (a rule other than 0 or 1). */
+int gd_walk_set_opening_rule(int rule);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
  * library's stream (enable = 1) or stop (0).  The walk is a traversal kernel that

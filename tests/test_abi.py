@@ -117,9 +117,9 @@ def test_public_headers_compile_as_c_and_cxx(tmp_path):
 
 
 def test_walk_settings_are_checked_on_the_host():
-    """gd_walk_set_group_split, gd_walk_set_list_capacity and
-    gd_walk_set_far_field return the previous setting and reject values out of
-    range without touching the GPU."""
+    """gd_walk_set_group_split, gd_walk_set_list_capacity,
+    gd_walk_set_far_field and gd_walk_set_opening_rule return the previous
+    setting and reject values out of range without touching the GPU."""
     lib = _lib.load_library()
     lib.gd_set_exit_on_error(0)
     try:
@@ -148,6 +148,14 @@ def test_walk_settings_are_checked_on_the_host():
         lib.gd_clear_error()
         assert lib.gd_walk_set_far_field(-0.1) == 0.25
         assert lib.gd_walk_set_far_field(0.0) == 0.25            # the rejected values left no trace
+        lib.gd_clear_error()
+        assert lib.gd_walk_set_opening_rule(1) == 0              # default: the cell-size rule
+        assert lib.gd_walk_set_opening_rule(2) == -1
+        assert b"neither 0" in lib.gd_last_error()
+        lib.gd_clear_error()
+        assert lib.gd_walk_set_opening_rule(-1) == -1
+        lib.gd_clear_error()
+        assert lib.gd_walk_set_opening_rule(0) == 1              # the rejected values left no trace
     finally:
         lib.gd_clear_error()
         lib.gd_set_exit_on_error(1)

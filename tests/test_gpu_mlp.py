@@ -8,7 +8,7 @@ import pytest
 from cudagravitydemo_amd import Simulation, ics
 from cudagravitydemo_amd import corrector as cr
 from oracle import mlp
-from tests.walk_check import group_slots, split_ratio
+from tests.walk_check import accepted, group_slots, split_ratio, world_extent
 
 pytestmark = pytest.mark.gpu
 
@@ -180,6 +180,53 @@ def test_samples_match_direct_sums(ratio, min_ratio):
             assert ok, row
             found += 1
         assert found >= 30
+
+
+@pytest.mark.parametrize("rule", ["size", "radius"])
+def test_sample_count_follows_the_opening_rule(rule):
+    """The sampler's traversal opens cells by the walks' rule: under either
+    rule the number of rows equals, target by target, the accepted internal
+    cells of the target's group (tests/walk_check.accepted under the same rule)
+    with size^2 > minRatio^2 |c - g|^2 from the group centre g, restated in
+    float32 -- and the two rules give different sets."""
+    pos, vel = ics.exponential_disk(8192, seed=13)
+    theta, min_ratio = 0.8, 0.5
+    f32 = np.float32
+    with Simulation(pos, vel, eps=0.01) as sim:
+        sim.set_quadrupole(2)
+        sim.build_tree_async()
+        targets = ics.subsample(8192, 64, seed=2)
+        total = sim.tree_summary()[0]
+        com, link, _ = sim.walk_arrays(total)
+        rad = sim.mass_radii(total)
+        ps = sim.positions()
+        ext = world_extent(*sim.bbox())
+        slots, slot_of = group_slots(sim.keys_sorted(), ps, split_ratio(theta))
+        want = {}
+        for r in ("size", "radius"):
+            cnt = 0
+            for t in targets:
+                a0, c0 = slots[slot_of[t]]
+                blk = ps[a0:a0 + c0, :3].astype(f32)
+                lo, hi = blk.min(0), blk.max(0)
+                _, cells = accepted(com, link, ext, lo, hi, theta, rad if r == "radius" else None)
+                g = (lo + hi) * f32(0.5)
+                size = np.ldexp(f32(ext), -(link[cells, 1] & 31)).astype(f32)
+                d = com[cells, :3].astype(f32) - g
+                d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+                mr2 = f32(min_ratio) * f32(min_ratio)
+                cnt += int((size * size > mr2 * d2).sum())
+            want[r] = cnt
+        assert want["size"] != want["radius"]
+        s = cr.SampleSet(sim.lib, 400000)
+        prev = sim.set_opening_rule(rule)
+        try:
+            n, dropped = s.add_from(sim, targets, theta, min_ratio)
+        finally:
+            sim.set_opening_rule(prev)
+            s.close()
+        assert dropped == 0
+        assert n == want[rule], (n, want)
 
 
 def test_trained_corrector_gains_over_quadrupole_walk():

@@ -17,8 +17,8 @@ import pytest
 from cudagravitydemo_amd import Simulation, ics, load_library
 from cudagravitydemo_amd import corrector as cr
 from oracle import mlp
-from tests.walk_check import (accepted, group_slots, potential_fp64, split_ratio, walk_fp64,
-                               world_extent)
+from tests.walk_check import (accepted, group_slots, mass_radii, potential_fp64, split_ratio,
+                               walk_fp64, world_extent)
 
 pytestmark = pytest.mark.gpu
 
@@ -74,7 +74,7 @@ def _sample(n, extra=()):
     return np.unique(s)
 
 
-def _check(sim, mode, per_body_of, params, extra=(), ratio=None, far_of=None):
+def _check(sim, mode, per_body_of, params, extra=(), ratio=None, far_of=None, rad=None):
     ps = sim.positions()
     acc = sim.accelerations()
     slots, slot_of = _slots(sim, mode, ratio=ratio)
@@ -93,12 +93,12 @@ def _check(sim, mode, per_body_of, params, extra=(), ratio=None, far_of=None):
         m = 1 if (mode == 2 and pb) else mode
         far = 0.0 if (pb or far_of is None) else far_of(i)
         want, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
-                            sim.eps * sim.eps, m, net, MIN_RATIO, oct_=oct_, far=far)
+                            sim.eps * sim.eps, m, net, MIN_RATIO, oct_=oct_, far=far, rad=rad)
         want = sim.G * want
         tol = 1e-5
         if m == 2:
             base, _ = walk_fp64(x.astype(np.float64), box, com, link, quad, ext, THETA,
-                                sim.eps * sim.eps, 1, far=far)
+                                sim.eps * sim.eps, 1, far=far, rad=rad)
             tol += 1e-2 * np.linalg.norm(want - sim.G * base) / np.linalg.norm(want)
         err = np.linalg.norm(acc[i] - want) / np.linalg.norm(want)
         worst = max(worst, err / tol)
@@ -114,19 +114,19 @@ def _tree(sim):
     return ps, com, link, world_extent(lo, hi)
 
 
-def _group_sizes(sim, mode):
+def _group_sizes(sim, mode, rad=None):
     """List length of every group slot (0 for empty slots)."""
     ps, com, link, ext = _tree(sim)
     slots, _ = _slots(sim, mode)
     sizes = np.zeros(slots.shape[0], np.int64)
     for s in range(slots.shape[0]):
         if slots[s, 1]:
-            lv, cl = accepted(com, link, ext, *_slot_box(ps, slots, s), THETA)
+            lv, cl = accepted(com, link, ext, *_slot_box(ps, slots, s), THETA, rad)
             sizes[s] = lv.size + cl.size
     return sizes
 
 
-def _predict(sim, mode, sizes, cap1, cap2):
+def _predict(sim, mode, sizes, cap1, cap2, rad=None):
     """Routing of the walk's passes, from the exact list lengths: a group leaves
     the first pass when its list exceeds cap1; the second pass gives it up when
     its list exceeds 8x the middle body's own list and min(16384, cap2 / 4)
@@ -138,7 +138,7 @@ def _predict(sim, mode, sizes, cap1, cap2):
     for g in over:
         a, c = slots[g]
         x = ps[a + (c - 1) // 2, :3]
-        lv, cl = accepted(com, link, ext, x, x, THETA)
+        lv, cl = accepted(com, link, ext, x, x, THETA, rad)
         lb = lv.size + cl.size
         lim = max(8 * lb, min(16384, cap2 // 4)) if lb <= cap2 // 8 else cap2
         if sizes[g] > min(lim, cap2):
@@ -253,6 +253,86 @@ def test_walk_with_set_group_split(mode, ratio):
         assert prev < 0
         assert st["overflow_groups"] == 0
         _check(sim, mode, lambda i: False, params, ratio=ratio)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_build_mass_radii_match_restatement(mode):
+    """The build's mass radii equal tests/walk_check.mass_radii on the
+    build's own centres of mass and bodies, bit for bit, with every moment
+    setting (the radius rides in the same mass pass)."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        total = sim.tree_summary()[0]
+        com, link, _ = sim.walk_arrays(total)
+        rad = sim.mass_radii(total)
+        want = mass_radii(com, link, sim.positions())
+        assert np.array_equal(rad, want), np.flatnonzero(rad != want)[:10]
+        internal = link[:, 0] >= 0
+        assert (rad[internal] > 0).all()
+
+
+@pytest.mark.parametrize("mode,path", [(0, "lists"), (1, "lists"), (2, "lists"),
+                                       (1, "second_pass"), (1, "per_body"), (1, "far"),
+                                       (2, "far")])
+def test_radius_rule_walk_matches_interaction_set_fp64(mode, path):
+    """gd_walk_set_opening_rule(1): every walk path (list walk, second pass,
+    per-body walk, far field) opens cells by their mass radii -- against the
+    restated sets under the same rule, summed in FP64, 1e-5.  The rule changes
+    the sets: fewer leaves than the size rule at the same angle."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    params = _net_params()
+    caps = {"lists": (16384, 0), "second_pass": (1, 0), "per_body": (1, 1),
+            "far": (16384, 0)}[path]
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(mode)
+        sim.build_tree_async()
+        total = sim.tree_summary()[0]
+        rad = sim.mass_radii(total)
+        sizes = _group_sizes(sim, mode, rad)
+        assert sizes.sum() < _group_sizes(sim, mode).sum()
+        over, body = _predict(sim, mode, sizes, caps[0], caps[1] or 262144, rad)
+        slots, slot_of = _slots(sim, mode)
+        far = 0.5 if path == "far" else 0.0
+        prev = sim.set_opening_rule("radius")
+        prev_far = sim.set_far_field(far)
+        try:
+            st = _run(sim, mode, caps[0], params, caps[1])
+        finally:
+            sim.set_far_field(prev_far)
+            assert sim.set_opening_rule(prev) == "radius"
+        assert prev == "size"
+        assert st["overflow_groups"] == len(over)
+        assert st["body_groups"] == len(body)
+        _check(sim, mode, lambda i: slot_of[i] in body, params, rad=rad,
+               far_of=(lambda i: far) if far else None)
+
+
+def test_radius_rule_potential_matches_interaction_set_fp64():
+    """The walk's potential under the mass-radius rule, list walk."""
+    pos, vel = ics.exponential_disk(N, seed=1234)
+    with Simulation(pos, vel, eps=EPS) as sim:
+        sim.set_quadrupole(1)
+        sim.build_tree_async()
+        prev = sim.set_opening_rule("radius")
+        try:
+            sim.walk(THETA, potential=True)
+        finally:
+            sim.set_opening_rule(prev)
+        phi = sim.potentials()
+        ps = sim.positions()
+        slots, slot_of = _slots(sim, 1)
+        total = sim.tree_summary()[0]
+        com, link, quad = sim.walk_arrays(total)
+        rad = sim.mass_radii(total)
+        lo, hi = sim.bbox()
+        ext = world_extent(lo, hi)
+        for i in _sample(sim.n):
+            want = -sim.G * potential_fp64(ps[i, :3].astype(np.float64),
+                                            _slot_box(ps, slots, slot_of[i]), com, link, quad,
+                                            ext, THETA, sim.eps * sim.eps, 1, rad)
+            assert abs(phi[i] - want) < 1e-5 * abs(want), (i, phi[i], want)
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -15,7 +15,7 @@ from cudagravitydemo_amd import ics
 from oracle import binding as ob
 from oracle import mlp
 from tests.walk_check import (accepted, far_split, features, group_slots, kernel_derivative,
-                               local_coefficients, local_eval, monopole, quadrupole,
+                               local_coefficients, local_eval, mass_radii, monopole, quadrupole,
                                split_ratio, walk_fp64, world_extent)
 
 
@@ -36,7 +36,9 @@ def tree():
             # children of a node are consecutive in the level-wise layout
             assert np.array_equal(ch, np.arange(ch[0], ch[0] + ch.size))
             link[j, 0] = ch[0]
-        link[j, 1] = (ch.size << 5) | nodes["level"][j]
+        # count: children of an internal node, bodies of a leaf (the walk arrays' layout)
+        k = ch.size if ch.size else int(nodes["count"][j])
+        link[j, 1] = (k << 5) | nodes["level"][j]
     ext = world_extent(t["bmin"], t["bmax"])
     return dict(t=t, nodes=nodes, com=com, link=link, ext=ext)
 
@@ -94,6 +96,87 @@ def test_accepted_nodes_partition_bodies_and_pass_the_test(tree, theta):
             if p >= 0 and j != 0:
                 s2, rhs = crit(p)
                 assert s2 >= rhs * (1 - 1e-5), (j, p)
+
+
+def _exact_radii(nodes, com, ps):
+    """Largest distance from each node's centre of mass to its bodies (FP64)."""
+    out = np.zeros(nodes.shape[0])
+    for j in range(nodes.shape[0]):
+        a, k = int(nodes["first"][j]), int(nodes["count"][j])
+        d = ps[a:a + k, :3].astype(np.float64) - com[j, :3].astype(np.float64)
+        out[j] = np.sqrt((d * d).sum(1)).max() if k else 0.0
+    return out
+
+
+def test_mass_radii_bound_every_body(tree):
+    """Each radius bounds the distance from the node's centre of mass to
+    every one of its bodies (up to float32 rounding of the coordinates: a
+    one-body leaf's centre of mass, (x m) / m, sits an ulp or so off its body,
+    and its radius is 0), and a node whose children are one-body leaves gets
+    the exact largest distance.  A recursion that drops the children's own
+    radii (a plausible slip) breaks the bound."""
+    t, nodes, com, link = tree["t"], tree["nodes"], tree["com"], tree["link"]
+    ps = t["pos"]
+    rad = mass_radii(com, link, ps).astype(np.float64)
+    exact = _exact_radii(nodes, com, ps)
+    ulps = 4e-7 * (1.0 + np.abs(com[:, :3].astype(np.float64)).max(1))
+    assert (rad >= exact * (1 - 1e-6) - ulps).all()
+    cnt = link[:, 1] >> 5
+    single = (link[:, 0] < 0) & (cnt == 1)
+    assert (rad[single] == 0).all()
+    internal = np.flatnonzero(link[:, 0] >= 0)
+    flat = [j for j in internal
+            if (link[link[j, 0]:link[j, 0] + cnt[j], 0] < 0).all()
+            and (cnt[link[j, 0]:link[j, 0] + cnt[j]] == 1).all()]
+    assert len(flat) > 20
+    np.testing.assert_allclose(rad[flat], exact[flat], rtol=2e-6, atol=4e-7 * 30)
+    # the bound is not loose everywhere: the root's is within 2x of the truth
+    assert rad[0] < 2 * exact[0]
+    # planted slip: children's radii left out of the recursion
+    slip = np.zeros_like(rad)
+    for j in internal[::-1]:
+        ch = np.arange(link[j, 0], link[j, 0] + cnt[j])
+        d = com[ch, :3].astype(np.float64) - com[j, :3].astype(np.float64)
+        slip[j] = np.sqrt((d * d).sum(1)).max()
+    assert (slip < exact * (1 - 1e-3) - ulps).any()
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.6, 1.0])
+def test_radius_rule_partitions_bodies_and_passes_the_test(tree, theta):
+    """Mass-radius opening rule: the accepted nodes still partition the
+    bodies, every accepted cell passes rad^2 < theta^2 d^2 and every ancestor
+    of an accepted node fails it; at equal theta it accepts no more leaves
+    than the size rule (a radius never exceeds the cell's half diagonal by
+    much, and small clumps are accepted sooner)."""
+    t, nodes, com, link, ext = tree["t"], tree["nodes"], tree["com"], tree["link"], tree["ext"]
+    ps = t["pos"]
+    n = ps.shape[0]
+    rad = mass_radii(com, link, ps)
+    fewer = 0
+    for lo, hi in _boxes(ps):
+        lv, cl = accepted(com, link, ext, lo, hi, theta, rad)
+        acc = np.concatenate([lv, cl])
+        cover = np.zeros(n, np.int64)
+        for a, k in zip(nodes["first"][acc], nodes["count"][acc]):
+            cover[a:a + k] += 1
+        assert (cover == 1).all()
+
+        def crit(j):
+            c = com[j, :3].astype(np.float64)
+            d = np.maximum(np.maximum(lo - c, c - hi), 0.0)
+            return float(rad[j]) ** 2, theta * theta * (d @ d)
+
+        for j in cl:
+            s2, rhs = crit(j)
+            assert s2 <= rhs * (1 + 1e-5)
+        for j in acc:
+            p = int(nodes["parent"][j])
+            if p >= 0 and j != 0:
+                s2, rhs = crit(p)
+                assert s2 >= rhs * (1 - 1e-5), (j, p)
+        lv0, _ = accepted(com, link, ext, lo, hi, theta)
+        fewer += lv.size < lv0.size
+    assert fewer >= len(_boxes(ps)) // 2
 
 
 def test_quadrupole_term_against_brute_force():

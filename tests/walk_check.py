@@ -3,7 +3,9 @@
 The production walk accepts node j for a body box [lo, hi] (the 64-body
 group's bounding box, or the body itself for the per-body walk) when j is a
 leaf or size_j^2 < theta^2 d^2, with d the distance from the box to j's
-centre of mass and size_j = extent / 2^level_j.  The test is restated here in
+centre of mass and size_j = extent / 2^level_j -- or, under the mass-radius
+opening rule, size_j replaced by the node's mass radius (mass_radii).  The
+test is restated here in
 float32 with the kernels' operation order (no fused multiply-add), so the
 accepted set is the kernels' own; the forces of that set are then summed in
 float64.  A GPU walk that matches this sum to FP32 rounding has the right
@@ -23,8 +25,10 @@ def world_extent(lo, hi):
     return f32(max(w, max(h, d)))
 
 
-def accepted(com, link, extent, lo, hi, theta):
-    """Accepted node indices for the box [lo, hi]: (leaves, cells), sorted."""
+def accepted(com, link, extent, lo, hi, theta, rad=None):
+    """Accepted node indices for the box [lo, hi]: (leaves, cells), sorted.
+    rad: the nodes' mass radii for the mass-radius opening rule (b = rad in
+    place of size in b^2 < theta^2 d^2), None for the size rule."""
     lo = np.asarray(lo, f32)
     hi = np.asarray(hi, f32)
     th2 = f32(theta) * f32(theta)
@@ -37,6 +41,8 @@ def accepted(com, link, extent, lo, hi, theta):
         lk = link[frontier]
         level = lk[:, 1] & 31
         size = np.ldexp(f32(extent), -level).astype(f32)
+        if rad is not None:
+            size = np.asarray(rad, f32)[frontier]
         leaf = lk[:, 0] < 0
         acc = leaf | (size * size < th2 * d2)
         leaves.append(frontier[acc & leaf])
@@ -49,6 +55,40 @@ def accepted(com, link, extent, lo, hi, theta):
         else:
             frontier = np.zeros(0, np.int64)
     return np.sort(np.concatenate(leaves)), np.sort(np.concatenate(cells))
+
+
+def mass_radii(com, link, pos):
+    """Each node's mass radius as the build forms it, in float32 with its
+    operation order: a leaf of one body 0, of several the largest
+    sqrt((dx dx + dy dy) + dz dz) from its centre of mass to its bodies; an
+    internal node the largest sqrt(...) + radius over its children, d from
+    its centre of mass to the child's.  Nodes are level-major, so children
+    follow their parents and a reverse sweep finishes them first."""
+    com = np.asarray(com, f32)
+    p = np.asarray(pos[:, :3], f32)
+    M = com.shape[0]
+    rad = np.zeros(M, f32)
+    first = link[:, 0].astype(np.int64)
+    cnt = (link[:, 1] >> 5).astype(np.int64)
+    level = link[:, 1] & 31
+    for L in np.unique(level)[::-1]:
+        idx = np.flatnonzero(level == L)
+        lf = idx[(first[idx] < 0) & (cnt[idx] > 1)]
+        for j in lf:
+            b0 = -first[j] - 1
+            d = p[b0:b0 + cnt[j]] - com[j, :3]
+            r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+            rad[j] = r.max()
+        it = idx[first[idx] >= 0]
+        if it.size:
+            k = cnt[it]
+            par = np.repeat(it, k)
+            ch = np.concatenate([np.arange(a, a + b) for a, b in zip(first[it], k)])
+            d = com[ch, :3] - com[par, :3]
+            t = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) + rad[ch]
+            starts = np.concatenate([[0], np.cumsum(k)[:-1]])
+            rad[it] = np.maximum.reduceat(t.astype(f32), starts)
+    return rad
 
 
 def monopole(x, com, idx, eps2):
@@ -170,12 +210,12 @@ def far_split(cells, com, lo, hi, far):
     return h2 < phi2 * r2, g
 
 
-def potential_fp64(x, box, com, link, quad, extent, theta, eps2, mode):
+def potential_fp64(x, box, com, link, quad, extent, theta, eps2, mode, rad=None):
     """Potential (without -G) at point x for the interaction set of `box`:
     sum m / r over monopoles; quadrupole cells (mode 1) add
     (3 d.S d - r^2 tr S) / (2 r^5); r softened, as the walk evaluates it."""
     lo, hi = box
-    leaves, cells = accepted(com, link, extent, lo, hi, theta)
+    leaves, cells = accepted(com, link, extent, lo, hi, theta, rad)
     x = np.asarray(x, np.float64)
     mono = leaves if mode else np.concatenate([leaves, cells])
     d = com[mono, :3].astype(np.float64) - x
@@ -218,7 +258,7 @@ def features(x, com, oct_, idx, extent, link):
 
 
 def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_ratio=0.0,
-              oct_=None, far=0.0):
+              oct_=None, far=0.0, rad=None):
     """Acceleration (without G) at point x for the interaction set of `box`.
     mode 0: all accepted nodes as monopoles; 1: cells as quadrupoles;
     2: as 1 plus the network's residual, evaluated at the box centre, scaled by
@@ -227,9 +267,9 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
     far_split take their monopole and quadrupole terms from the local
     expansion about the box centre, except, in mode 2, the cells the network
     corrects (their terms stay exact; the network's term is unchanged);
-    leaves stay exact."""
+    leaves stay exact.  rad: mass radii for the mass-radius opening rule."""
     lo, hi = box
-    leaves, cells = accepted(com, link, extent, lo, hi, theta)
+    leaves, cells = accepted(com, link, extent, lo, hi, theta, rad)
     x = np.asarray(x, np.float64)
     if mode == 0:
         return monopole(x, com, np.concatenate([leaves, cells]), eps2), leaves.size + cells.size
