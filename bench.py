@@ -23,6 +23,8 @@ The quadrupole and learned walks sum the cells far from each group (h / R
 below --far, h the group box's half diagonal, R the distance to the cell)
 through one local expansion per group instead of body by body
 (gd_walk_set_far_field); the plain walk evaluates every node body by body.
+All three walks open cells by their mass radii (--opening radius,
+gd_walk_set_opening_rule), which is also the learned walk's cell measure.
 `value` is the learned run's body-steps/s with its judge steps included;
 `ms_per_step_without_judge` leaves them out.  Each leg is judged again on the
 held-out sample after its timed steps.
@@ -111,7 +113,7 @@ def parse():
                     help="the corrector acts on cells with size / distance above this")
     ap.add_argument("--train-targets", type=int, default=8192,
                     help="target bodies per snapshot for the corrector's samples")
-    ap.add_argument("--opening", choices=("size", "radius"), default="size",
+    ap.add_argument("--opening", choices=("size", "radius"), default="radius",
                     help="opening rule of the three walks (gd_walk_set_opening_rule): cell "
                          "size or mass radius")
     ap.add_argument("--far", type=float, default=0.5,

@@ -602,8 +602,10 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
         top = base;
         bool cell = pr.accept && !pr.leaf;
         if (cell) {
+            /* the walk's correction test, with its cell measure (the mass
+               radius under the mass-radius rule, else the cell size) */
             const float4 c = wcom[pr.idx];
-            const float size = ldexpf(ext, -(wlink[pr.idx].y & 31));
+            const float size = wrad ? wrad[pr.idx] : ldexpf(ext, -(wlink[pr.idx].y & 31));
             const float dx = c.x - gx, dy = c.y - gy, dz = c.z - gz;
             cell = size * size > mr2 * (dx * dx + dy * dy + dz * dz);
         }
@@ -635,7 +637,7 @@ __global__ __launch_bounds__(64 * GD_SL_WAVES) void k_samples_list(
  * in FP64: what the quadrupole cell misses, which the network learns. */
 __global__ __launch_bounds__(256) void k_samples_eval(
     const float4* __restrict__ pos, const int4* __restrict__ meta,
-    const int2* __restrict__ wlink,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
     const float4* __restrict__ wcom, const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int* __restrict__ targets,
     const float4* __restrict__ centre, const int2* __restrict__ pairs, long long npairs,
@@ -669,7 +671,9 @@ __global__ __launch_bounds__(256) void k_samples_eval(
             const float4 c = wcom[idx];
             float4 q0, q1;
             gd_mom_quad(wmom, idx, q0, q1);
-            const double size = ldexp((double)info->box.extent, -(wlink[idx].y & 31));
+            /* the cell measure s, as the walk takes it */
+            const double size = wrad ? (double)wrad[idx]
+                                     : ldexp((double)info->box.extent, -(wlink[idx].y & 31));
             const double dx = (double)c.x - x.x, dy = (double)c.y - x.y, dz = (double)c.z - x.z;
             const double rc2 = dx * dx + dy * dy + dz * dz;
             const double rs2 = rc2 + eps2;
@@ -772,7 +776,8 @@ extern "C" long long gd_corrector_samples(const treeState* tree, const float4* p
         if (rows > 0) {
             long long blocks = (rows + 3) / 4;
             if (blocks > 65536) blocks = 65536;
-            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, b->meta, b->wlink, b->wcom, b->wmom,
+            k_samples_eval<<<(int)blocks, 256, 0, gd_stream()>>>(pos, b->meta, b->wlink,
+                                                 gd_walk_radii(b), b->wcom, b->wmom,
                                                  b->dinfo, targets, centre, pairs, rows,
                                                  (double)eps2, X, Y);
             GD_LAUNCHED();

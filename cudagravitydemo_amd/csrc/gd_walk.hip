@@ -592,6 +592,14 @@ __device__ __forceinline__ int gd_mlp_compact(const gd_bf16x4* __restrict__ W,
     return m;
 }
 
+/* The learned walk's cell measure s (its features, its correction test
+ * s / r > minRatio and the training samples'): the mass radius under the
+ * mass-radius opening rule (wrad given), else the cell size ext / 2^level. */
+__device__ __forceinline__ float gd_cell_measure(const float* __restrict__ wrad, float ext,
+                                                 const int2* __restrict__ wlink, int idx) {
+    return wrad ? wrad[idx] : ldexpf(ext, -(wlink[idx].y & 31));
+}
+
 /*
  * The network's inputs for a cell seen from point x: with d = c - x, r = |d|,
  * u = d / r, s the cell's edge, m its mass, O its third moments about c,
@@ -649,7 +657,8 @@ __device__ __forceinline__ void gd_cell_features(float4 c, float4 o0, float4 o1,
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_group(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int* __restrict__ list, int cap, int2 cnt,
     long long start, int count, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sE, float4* sQ0, float4* sQ1,
@@ -716,7 +725,7 @@ __device__ __forceinline__ void gd_eval_group(
             float size = 1.f;
             if (idx >= 0) {
                 gd_mom_oct(wmom, idx, o0, o1, o2);
-                size = ldexpf(ext, -(wlink[idx].y & 31));
+                size = gd_cell_measure(wrad, ext, wlink, idx);
             }
             float f[8], k;
             gd_cell_features(c, o0, o1, o2, size, gx, gy, gz, idx >= 0, minRatio, f, k);
@@ -753,7 +762,8 @@ __device__ __forceinline__ void gd_eval_group(
 template <int MODE, bool PHI>
 __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange,
     const int* __restrict__ groupList, int slots,
     const int* __restrict__ lists, const long long* __restrict__ offsets,
@@ -790,7 +800,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
     unsigned long long nn = 0;
     const int* list = offsets ? lists + offsets[g] : lists + (size_t)s * cap;
     const int size = offsets ? caps[g] : cap;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wmom, info, list, size, cnt, rg.x, rg.y,
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wrad, wmom, info, list, size, cnt, rg.x, rg.y,
                              eps2,
                              mlpW, mlpB, minRatio, sE[wave],
                              sQ0[MODE != 0 ? wave : 0], sQ1[MODE != 0 ? wave : 0],
@@ -816,7 +826,8 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval(
 template <int MODE, bool PHI>
 __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange,
     const int* __restrict__ groupList, int slots, int segs,
     const int* __restrict__ lists, const int2* __restrict__ counts, int cap,
@@ -858,7 +869,7 @@ __global__ __launch_bounds__(64 * GD_EV_WAVES) void k_walk_eval_seg(
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
     unsigned long long nn = 0;
-    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wmom, info,
+    gd_eval_group<MODE, PHI>(pos, wcom, wlink, wrad, wmom, info,
                              lists + (size_t)s * cap + f0, cap - f0 - b0,
                              make_int2(f1 - f0, b1 - b0), rg.x, rg.y, eps2, mlpW, mlpB,
                              minRatio, sE[wave], sQ0[MODE != 0 ? wave : 0],
@@ -916,6 +927,7 @@ __global__ __launch_bounds__(256) void k_walk_eval_sum(const int2* __restrict__ 
  */
 __device__ __forceinline__ int gd_flush_corr(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
+    const float* __restrict__ wrad,
     const float4* __restrict__ wmom, float ext, const gd_bf16x4* __restrict__ W,
     const float* __restrict__ Bv, float minRatio, const int* queue, int qn, int* map,
     float gax, float gay, float gaz, float gbx, float gby, float gbz, bool inB, float px,
@@ -930,7 +942,7 @@ __device__ __forceinline__ int gd_flush_corr(
     if (have) {
         c = wcom[idx];
         gd_mom_oct(wmom, idx, o0, o1, o2);
-        size = ldexpf(ext, -(wlink[idx].y & 31));
+        size = gd_cell_measure(wrad, ext, wlink, idx);
     }
     float f[8], k;
     gd_cell_features(c, o0, o1, o2, size, half ? gbx : gax, half ? gby : gay,
@@ -994,6 +1006,7 @@ __device__ __forceinline__ int gd_flush_corr(
 template <bool KEEP>
 __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
                                              const int2* __restrict__ wlink,
+                                             const float* __restrict__ wrad,
                                              const float4* __restrict__ wmom, float ext,
                                              int* __restrict__ list, int cap, int n, float gx,
                                              float gy, float gz, float h2, float far2, float eps2,
@@ -1030,7 +1043,7 @@ __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
         bool isFar = idx >= 0 && h2 < far2 * r2;
         if (KEEP && isFar && c.w > 0.f) {
             const float rq = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-            const float size = ldexpf(ext, -(wlink[idx].y & 31));
+            const float size = gd_cell_measure(wrad, ext, wlink, idx);
             if (size * __builtin_amdgcn_rsqf(rq) > minRatio) isFar = false;
         }
         const bool isNear = idx >= 0 && !isFar;
@@ -1082,6 +1095,7 @@ __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
 template <bool KEEP>
 __device__ __forceinline__ void gd_far_half(const float4* __restrict__ wcom,
                                             const int2* __restrict__ wlink,
+                                            const float* __restrict__ wrad,
                                             const float4* __restrict__ wmom, float ext,
                                             int* __restrict__ list, int cap, int nf, int nb,
                                             float gx, float gy, float gz, float h2, float far2,
@@ -1095,7 +1109,7 @@ __device__ __forceinline__ void gd_far_half(const float4* __restrict__ wcom,
     /* leaves stay in the per-body loop: they are mostly near, and streaming
        them through the split cost more than the far ones saved */
     nearF = nf;
-    nearB = gd_far_region<KEEP>(wcom, wlink, wmom, ext, list, cap, nb, gx, gy, gz, h2, far2,
+    nearB = gd_far_region<KEEP>(wcom, wlink, wrad, wmom, ext, list, cap, nb, gx, gy, gz, h2, far2,
                                 eps2, minRatio, sRec, L, fq, nFar);
     if (fq > 0) {
         if ((int)lane < fq) {
@@ -1130,7 +1144,8 @@ __device__ __forceinline__ void gd_far_half(const float4* __restrict__ wcom,
 template <int EVW, bool KEEP>
 __global__ __launch_bounds__(64 * EVW) void k_walk_far(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom,
     const gd_tree_info* __restrict__ info, const int2* __restrict__ slotRange, int runs,
     int* __restrict__ lists, const long long* __restrict__ offsets,
     const int* __restrict__ caps, int2* __restrict__ counts, float3* __restrict__ acc,
@@ -1166,7 +1181,7 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_far(
         const float h2 = 0.25f * ((ex * ex + ey * ey) + ez * ez);
         const long long s = 2 * r + h;
         int nearF, nearB;
-        gd_far_half<KEEP>(wcom, wlink, wmom, ext, lists + offsets[s], caps[s], cn.x, cn.y, gx, gy,
+        gd_far_half<KEEP>(wcom, wlink, wrad, wmom, ext, lists + offsets[s], caps[s], cn.x, cn.y, gx, gy,
                           gz, h2, far2, eps2, minRatio, valid && (h ? inB : !inB), p.x, p.y, p.z,
                           sRec[wave], nearF, nearB, fx, fy, fz);
         if (lane == 0) {
@@ -1200,7 +1215,8 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_far(
 template <int MODE, bool PHI>
 __device__ __forceinline__ void gd_eval_pair(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
     const int* __restrict__ listA, int capA, int2 cA, const int* __restrict__ listB, int capB,
     int2 cB, long long start, int nA, int nB, float eps2, const gd_bf16x4* __restrict__ mlpW,
     const float* __restrict__ mlpB, float minRatio, float4* sA, float4* sB, float4* sQ,
@@ -1293,14 +1309,14 @@ __device__ __forceinline__ void gd_eval_pair(
                     const float dx = cc.x - (h ? gbx : gax), dy = cc.y - (h ? gby : gay),
                                 dz = cc.z - (h ? gbz : gaz);
                     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                    const float size = ldexpf(ext, -(wlink[id].y & 31));
+                    const float size = gd_cell_measure(wrad, ext, wlink, id);
                     need = size * __builtin_amdgcn_rsqf(r2) > minRatio;
                 }
                 const unsigned long long bal = __ballot(need);
                 const int m = __popcll(bal);
                 if (qn + m > 64) {   /* wave-uniform */
                     *nnEvals += (unsigned long long)gd_flush_corr(
-                        wcom, wlink, wmom, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax,
+                        wcom, wlink, wrad, wmom, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax,
                         gay, gaz, gbx, gby, gbz, inB, px, py, pz, sQ, lx, ly, lz);
                     qn = 0;
                 }
@@ -1318,7 +1334,7 @@ __device__ __forceinline__ void gd_eval_pair(
     if (MODE == 2) {
         if (qn > 0)
             *nnEvals += (unsigned long long)gd_flush_corr(
-                wcom, wlink, wmom, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax, gay, gaz,
+                wcom, wlink, wrad, wmom, ext, mlpW, mlpB, minRatio, sQueue, qn, sMap, gax, gay, gaz,
                 gbx, gby, gbz, inB, px, py, pz, sQ, lx, ly, lz);
         gd_kahan_add(tot.x, tot.cx, lx);
         gd_kahan_add(tot.y, tot.cy, ly);
@@ -1333,7 +1349,8 @@ __device__ __forceinline__ void gd_eval_pair(
 template <int MODE, bool PHI, int EVW, bool FAR>
 __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     const float4* __restrict__ pos, const float4* __restrict__ wcom,
-    const int2* __restrict__ wlink, const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
+    const int2* __restrict__ wlink, const float* __restrict__ wrad,
+    const float4* __restrict__ wmom, const gd_tree_info* __restrict__ info,
     const int2* __restrict__ slotRange, int runs, const int* __restrict__ lists,
     const long long* __restrict__ offsets, const int* __restrict__ caps,
     const int2* __restrict__ counts, float3* __restrict__ acc, float* __restrict__ phi,
@@ -1372,7 +1389,7 @@ __global__ __launch_bounds__(64 * EVW) void k_walk_eval_pair(
     gd_kahan3 tot = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float phs = 0.f, phc = 0.f;
     unsigned long long nn = 0;
-    gd_eval_pair<MODE, PHI>(pos, wcom, wlink, wmom, info, listA, capA, cA, listB, capB,
+    gd_eval_pair<MODE, PHI>(pos, wcom, wlink, wrad, wmom, info, listA, capA, cA, listB, capB,
                             cB, rgA.x, rgA.y, rgB.y, eps2, mlpW, mlpB, minRatio, sE[wave],
                             sE[wave] + 64, sE[wave], sMap[MODE == 2 ? wave : 0],
                             sQueue[MODE == 2 ? wave : 0], tot, phs, phc, &nn);
@@ -1782,8 +1799,9 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            float* phi, float eps2, float G, int n, const gd_bf16x4* W,
                            const float* Bv, float minRatio) {
     k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
-        pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, groupList, slots,
-        lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio, g_walk.kstats);
+        pos, b->wcom, b->wlink, gd_walk_radii(b), b->wmom, b->dinfo, g_walk.slotRange, groupList,
+        slots, lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio,
+        g_walk.kstats);
 }
 
 template <int MODE, bool PHI>
@@ -1838,13 +1856,13 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         if constexpr (MODE != 0 && !PHI) {
             if (far > 0.f) {
                 k_walk_far<EVW, MODE == 2><<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
-                    pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
+                    pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, runs,
                     g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, eps2,
                     far * far, minRatio, g_walk.kstats);
                 GD_LAUNCHED();
                 k_walk_eval_pair<MODE, PHI, EVW, true>
                     <<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
-                        pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
+                        pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, runs,
                         g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2,
                         G, W, Bv, minRatio, g_walk.kstats);
                 farDone = true;
@@ -1853,7 +1871,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         if (!farDone) {
             k_walk_eval_pair<MODE, PHI, EVW, false>
                 <<<gd_div_up(runs, EVW), 64 * EVW, 0, gd_stream()>>>(
-                    pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange, runs,
+                    pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, runs,
                     g_walk.lists, g_walk.offsets, g_walk.caps, g_walk.counts, acc, phi, eps2, G,
                     W, Bv, minRatio, g_walk.kstats);
         }
@@ -1884,7 +1902,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
         k_walk_eval_seg<MODE, PHI><<<GD_SEG_WAVES / GD_EV_WAVES, 64 * GD_EV_WAVES, 0, gd_stream()>>>(
-            pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange,
+            pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange,
             g_walk.overflow1, S2, 0, g_walk.pool, g_walk.counts2, cap2, g_walk.segPart, eps2, W,
             Bv, minRatio, g_walk.kstats, &st->over1);
         GD_LAUNCHED();
@@ -1952,7 +1970,7 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         if (segs > 1) {
             k_walk_eval_seg<MODE, PHI>
                 <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
-                    pos, b->wcom, b->wlink, b->wmom, b->dinfo, g_walk.slotRange,
+                    pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange,
                     g_walk.overflow1 + b0, cnt, segs, g_walk.pool, g_walk.counts2, cap2,
                     g_walk.segPart, eps2, W, Bv, minRatio, g_walk.kstats);
             GD_LAUNCHED();

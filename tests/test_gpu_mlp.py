@@ -111,12 +111,14 @@ def test_canonical_sample_order_is_reproducible():
         assert np.array_equal(rows[0][1], rows[1][1])
 
 
-@pytest.mark.parametrize("ratio,min_ratio", [(-1.0, 0.5), (0.0, 0.0)])
-def test_samples_match_direct_sums(ratio, min_ratio):
+@pytest.mark.parametrize("ratio,min_ratio,rule", [(-1.0, 0.5, "size"), (0.0, 0.0, "size"),
+                                                  (-1.0, 0.4, "radius")])
+def test_samples_match_direct_sums(ratio, min_ratio, rule):
     """ratio -1: the learned walk's default groups at the sampling angle 0.8
     (every run split at its largest key jump), cells with s/r > 0.5 only; 0: whole
     64-body runs, as gd_walk_set_group_split(0) makes the walk keep them, and
-    every accepted cell."""
+    every accepted cell.  rule "radius": the mass-radius opening rule, whose
+    cell measure s (features, target scale) is the mass radius."""
     pos, vel = ics.exponential_disk(8192, seed=13)
     with Simulation(pos, vel, eps=0.01) as sim:
         sim.set_quadrupole(2)
@@ -124,12 +126,14 @@ def test_samples_match_direct_sums(ratio, min_ratio):
         targets = ics.subsample(8192, 64, seed=2)
         s = cr.SampleSet(sim.lib, 200000)
         prev = sim.lib.gd_walk_set_group_split(ratio)
+        prev_rule = sim.set_opening_rule(rule)
         try:
             n, dropped = s.add_from(sim, targets, 0.8, min_ratio)
             assert n > 64 and dropped == 0
             X, Y = s.host()
         finally:
             sim.lib.gd_walk_set_group_split(prev)
+            sim.set_opening_rule(prev_rule)
             s.close()
         assert (X[:, 3] > min_ratio * (1 - 1e-6)).all()
         if min_ratio == 0:
@@ -138,6 +142,7 @@ def test_samples_match_direct_sums(ratio, min_ratio):
         total = sim.tree_summary()[0]
         com, link, quad = sim.walk_arrays(total)
         oct_ = sim.third_moments(total)
+        rad = sim.mass_radii(total)
         from oracle.binding import NODE_DTYPE
         nodes = sim.nodes_raw(total).view(NODE_DTYPE).reshape(-1)
         ps = sim.positions()
@@ -163,6 +168,8 @@ def test_samples_match_direct_sums(ratio, min_ratio):
                 u = d / np.linalg.norm(d, axis=1)[:, None]
                 for idx in cand[np.all(np.abs(u - X[row, :3]) < 1e-6, axis=1)]:
                     size = ext * 2.0 ** -int(nodes[idx]["level"])
+                    if rule == "radius":
+                        size = float(rad[idx])
                     S = np.array([[quad[idx, 0], quad[idx, 3], quad[idx, 4]],
                                   [quad[idx, 3], quad[idx, 1], quad[idx, 5]],
                                   [quad[idx, 4], quad[idx, 5], quad[idx, 2]]])
@@ -187,8 +194,9 @@ def test_sample_count_follows_the_opening_rule(rule):
     """The sampler's traversal opens cells by the walks' rule: under either
     rule the number of rows equals, target by target, the accepted internal
     cells of the target's group (tests/walk_check.accepted under the same rule)
-    with size^2 > minRatio^2 |c - g|^2 from the group centre g, restated in
-    float32 -- and the two rules give different sets."""
+    with s^2 > minRatio^2 |c - g|^2 from the group centre g, s the rule's cell
+    measure (size, or mass radius), restated in float32 -- and the two rules
+    give different sets."""
     pos, vel = ics.exponential_disk(8192, seed=13)
     theta, min_ratio = 0.8, 0.5
     f32 = np.float32
@@ -212,6 +220,8 @@ def test_sample_count_follows_the_opening_rule(rule):
                 _, cells = accepted(com, link, ext, lo, hi, theta, rad if r == "radius" else None)
                 g = (lo + hi) * f32(0.5)
                 size = np.ldexp(f32(ext), -(link[cells, 1] & 31)).astype(f32)
+                if r == "radius":
+                    size = rad[cells].astype(f32)
                 d = com[cells, :3].astype(f32) - g
                 d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
                 mr2 = f32(min_ratio) * f32(min_ratio)

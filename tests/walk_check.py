@@ -234,14 +234,17 @@ def potential_fp64(x, box, com, link, quad, extent, theta, eps2, mode, rad=None)
     return phi
 
 
-def features(x, com, oct_, idx, extent, link):
+def features(x, com, oct_, idx, extent, link, rad=None):
     """The network's 8 inputs for cells idx seen from point x, and k = m s^3:
     u (3), s/r, (7.5 w - 1.5 T) / (m s^3) (3), u.w / (m s^3), with
-    w_i = O_ijk u_j u_k and T_i = O_ijj, from the 10 stored components."""
+    w_i = O_ijk u_j u_k and T_i = O_ijj, from the 10 stored components.
+    s is the cell size, or its mass radius under the mass-radius rule (rad)."""
     d = com[idx, :3].astype(np.float64) - x
     r = np.sqrt((d * d).sum(1))
     u = d / r[:, None]
     size = np.ldexp(np.float64(extent), -(link[idx, 1] & 31))
+    if rad is not None:
+        size = np.asarray(rad, np.float64)[idx]
     m = com[idx, 3].astype(np.float64)
     k = m * size ** 3
     o = oct_[idx].astype(np.float64)
@@ -267,7 +270,8 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
     far_split take their monopole and quadrupole terms from the local
     expansion about the box centre, except, in mode 2, the cells the network
     corrects (their terms stay exact; the network's term is unchanged);
-    leaves stay exact.  rad: mass radii for the mass-radius opening rule."""
+    leaves stay exact.  rad: mass radii for the mass-radius opening rule (then
+    also the network's cell measure s)."""
     lo, hi = box
     leaves, cells = accepted(com, link, extent, lo, hi, theta, rad)
     x = np.asarray(x, np.float64)
@@ -276,7 +280,7 @@ def walk_fp64(x, box, com, link, quad, extent, theta, eps2, mode, net=None, min_
     corr = np.zeros(cells.size, bool)
     if mode == 2 and cells.size:
         ctr = 0.5 * (np.asarray(lo, np.float64) + np.asarray(hi, np.float64))
-        X, k = features(ctr, com, oct_, cells, extent, link)
+        X, k = features(ctr, com, oct_, cells, extent, link, rad)
         corr = (k > 0) & (X[:, 3] > min_ratio)
     if far > 0:
         fm, g = far_split(cells, com, lo, hi, far)
