@@ -75,6 +75,9 @@ MEASURED_TRAFFIC = {
     ("learned", 0.675, 1 << 24, 0.5, "size"): {
         "bytes": 12219001121,
         "profile": "profiles/pmc_walk_far_eval_learned_theta0.675_far0.5_merger16M.md"},
+    ("learned", 0.65, 1 << 24, 0.5, "radius"): {
+        "bytes": 10309010432,
+        "profile": "profiles/pmc_walk_far_eval_learned_theta0.65_far0.5_radius_merger16M.md"},
 }
 BUDGET = 1e-3
 # Each walk's opening angle is calibrated against this share of the budget on

@@ -43,6 +43,8 @@ def main():
                     help="volume fraction for splitting 64-body groups (< 0: by theta)")
     ap.add_argument("--far", type=float, default=0.0,
                     help="quadrupole and learned walks: far-field ratio (gd_walk_set_far_field)")
+    ap.add_argument("--opening", choices=["size", "radius"], default="size",
+                    help="opening rule of the production walks (gd_walk_set_opening_rule)")
     ap.add_argument("--phases", action="store_true",
                     help="time the walk's phases with HIP events (last step's walk)")
     ap.add_argument("--copy-mib", type=int, default=0)
@@ -60,6 +62,7 @@ def main():
             sim.lib.gd_walk_set_list_capacity(a.list_cap, 0)
         sim.lib.gd_walk_set_group_split(a.group_split)
         sim.set_far_field(a.far)
+        sim.set_opening_rule(a.opening)
         if a.walk in ("quad", "learned"):
             sim.set_quadrupole(2 if a.walk == "learned" else 1)
         sim.sync()
@@ -79,7 +82,7 @@ def main():
         sim.sync()
         t1 = time.perf_counter()
         print(f"profile_step: {n} bodies, {a.steps} steps, walk={a.walk}, theta={a.theta}, "
-              f"far field {a.far}: "
+              f"far field {a.far}, opening rule {a.opening}: "
               f"{1e3 * (t1 - t0) / a.steps:.3f} ms/step", flush=True)
         if a.phases and a.walk != "reference":
             import ctypes as C

@@ -203,8 +203,11 @@ float gd_walk_set_far_field(float ratio);
  * mass radius (gd_tree_mass_radii), a bound on the distance from its centre
  * of mass to its bodies, so that a cell whose bodies lie close to their
  * centre of mass is accepted nearer; the same theta then means a different
- * (smaller) error.  Returns the previous rule, or -1 after reporting an
- * error (a rule other than 0 or 1). */
+ * (smaller) error.  Under rule 1 the learned walk and its training samples
+ * also take the mass radius as the cell measure s of the network's inputs,
+ * of its correction test s / r > minRatio and of the target's scale (a
+ * network trained under one rule is for that rule).  Returns the previous
+ * rule, or -1 after reporting an error (a rule other than 0 or 1). */
 int gd_walk_set_opening_rule(int rule);
 
 /* Time every later gd_walk / gd_walk_learned call with HIP events on the
@@ -251,8 +254,9 @@ void gd_mlp_adam_state(const gd_mlp* net, float* hostM, float* hostV);
  * moments, gd_set_quadrupole(tree, 2)): for each target body (device
  * indices, sorted order), every internal cell that the traversal of the
  * body's group (as the list walks form groups, gd_walk_set_group_split) at
- * opening angle theta accepts and whose size / distance from the group's
- * centre exceeds minRatio (the cells gd_walk_learned corrects) gives one row
+ * opening angle theta accepts (under the walks' opening rule) and whose
+ * s / distance from the group's centre exceeds minRatio (s as in
+ * gd_walk_learned: the cells it corrects) gives one row
  * of X (the features at the group's centre) and the exact scaled residual in
  * Y (FP64 sum over the cell's bodies minus the softened monopole and
  * quadrupole terms, from the body).  Returns the number of rows the targets
@@ -266,9 +270,10 @@ long long gd_corrector_samples(const treeState* tree, const float4* pos, const i
                                int nt, float theta, float minRatio, float eps2, float* X,
                                float* Y, long long cap, int* droppedTargets);
 /* gd_walk with quadrupole cells plus the network's residual correction of
- * every accepted cell whose size / r (from the group's centre) exceeds
+ * every accepted cell whose s / r (from the group's centre) exceeds
  * minRatio, evaluated once per (64-body group, cell) on the matrix cores and
- * scaled per body by m s^3 / r^5.  Needs a tree built with second and third
+ * scaled per body by m s^3 / r^5 (s: the cell size, or its mass radius under
+ * the mass-radius opening rule).  Needs a tree built with second and third
  * moments (gd_set_quadrupole(tree, 2)).  phi, when given, receives the
  * quadrupole walk's potential: the network corrects accelerations only. */
 void gd_walk_learned(const treeState* tree, const float4* pos, float3* acc, float* phi,
