@@ -93,7 +93,7 @@ FLOP_CORR = 15
 FLOP_FAR = 316             # per far (group, cell) pair: its local-expansion terms
 FLOP_MLP = 17792
 THETAS = (1.0, 0.9, 0.85, 0.8, 0.75, 0.725, 0.7, 0.6875, 0.675, 0.65, 0.625, 0.6, 0.55, 0.5)
-PLAIN_THETAS = (0.3, 0.25, 0.2, 0.175, 0.15, 0.1)
+PLAIN_THETAS = (0.3, 0.275, 0.25, 0.225, 0.2, 0.175, 0.15, 0.1)
 REF_THETAS = (0.5, 0.4, 0.3, 0.25, 0.2, 0.175, 0.15)
 
 
