@@ -993,16 +993,18 @@ __device__ __forceinline__ int gd_flush_corr(
 /*
  * The cells of a half's list in the far split (entries 0..n-1 at
  * list[cap - 1 - e]).  The 64 lanes take 64 entries at a time (indices two
- * chunks ahead, records one chunk ahead, so the gathers overlap the work); a
- * cell whose centre of mass is far from the half's box (h2 < far2 R^2, R
- * from the box centre g) goes with its record to the LDS queue sRec (fq
- * entries of 3 float4, < 64 between calls), and each time 64 are queued every
- * lane adds one of them to its local-expansion coefficients L (gd_far.h);
- * the near cells are written back, in order, to the front of the region, for
- * the evaluation's per-body loop.  KEEP (learned walk): a cell the network
- * corrects (s / r > minRatio from g, the evaluation's own test) stays near,
- * so the evaluation finds it.  Returns the near count.
+ * chunks ahead, records one chunk ahead, so the gathers overlap the work;
+ * two chunks per iteration in alternating registers); a cell whose centre of
+ * mass is far from the half's box (h2 < far2 R^2, R from the box centre g)
+ * goes with its record to the LDS ring sRec (128 entries of 3 float4: fq
+ * pending from entry fh on, < 64 between calls), and each time 64 are
+ * pending every lane adds one of them to its local-expansion coefficients L
+ * (gd_far.h); the near cells are written back, in order, to the front of the
+ * region, for the evaluation's per-body loop.  KEEP (learned walk): a cell
+ * the network corrects (s / r > minRatio from g, the evaluation's own test)
+ * stays near, so the evaluation finds it.  Returns the near count.
  */
+#define GD_FAR_RING 128
 template <bool KEEP>
 __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
                                              const int2* __restrict__ wlink,
@@ -1011,33 +1013,23 @@ __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
                                              int* __restrict__ list, int cap, int n, float gx,
                                              float gy, float gz, float h2, float far2, float eps2,
                                              float minRatio, float4* sRec, gd_local& L, int& fq,
-                                             int& nFar) {
+                                             int& fh, int& nFar) {
     const unsigned lane = gd_lane();
     const unsigned long long lt = gd_lanemask_lt();
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     /* entry e of the region */
     auto at = [&](int e) -> int& { return list[cap - 1 - e]; };
+    /* the entry's index (-1 past the end); a lane past the end gathers node
+       0's records (never used), so the gathers need no branch */
+    auto index = [&](int e) -> int { return e < n ? at(e) : -1; };
+    auto gather = [&](int idx, float4& c, float4& q0, float4& q1) {
+        const int safe = idx >= 0 ? idx : 0;
+        c = wcom[safe];
+        gd_mom_quad(wmom, safe, q0, q1);
+    };
     int nNear = 0;
-    /* chunk 0's records and chunk 1's indices in flight */
-    int idxN = (int)lane < n ? at((int)lane) : -1;
-    float4 cN = zero, q0N = zero, q1N = zero;
-    if (idxN >= 0) {
-        cN = wcom[idxN];
-        gd_mom_quad(wmom, idxN, q0N, q1N);
-    }
-    int idxNN = 64 + (int)lane < n ? at(64 + (int)lane) : -1;
-    for (int c0 = 0; c0 < n; c0 += 64) {
-        const int idx = idxN;
-        const float4 c = cN, q0 = q0N, q1 = q1N;
-        /* next chunk's records, the chunk after's indices (entries >= c0 + 64:
-           the write-backs below go to entries < c0 + 64) */
-        idxN = idxNN;
-        cN = zero; q0N = zero; q1N = zero;
-        if (idxN >= 0) {
-            cN = wcom[idxN];
-            gd_mom_quad(wmom, idxN, q0N, q1N);
-        }
-        idxNN = c0 + 128 + (int)lane < n ? at(c0 + 128 + (int)lane) : -1;
+    /* one chunk: far test, near write-back, far records to the ring, and a
+       wave of expansion adds when 64 are pending */
+    auto chunk = [&](int idx, float4 c, float4 q0, float4 q1) {
         const float dx = c.x - gx, dy = c.y - gy, dz = c.z - gz;
         const float r2 = (dx * dx + dy * dy) + dz * dz;
         bool isFar = idx >= 0 && h2 < far2 * r2;
@@ -1053,7 +1045,7 @@ __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
         nNear += __popcll(nbal);
         const unsigned long long fbal = __ballot(isFar);
         if (isFar) {
-            float4* q = sRec + 3 * (fq + __popcll(fbal & lt));
+            float4* q = sRec + 3 * ((fh + fq + __popcll(fbal & lt)) & (GD_FAR_RING - 1));
             q[0] = c;
             q[1] = q0;
             q[2] = q1;
@@ -1061,25 +1053,31 @@ __device__ __forceinline__ int gd_far_region(const float4* __restrict__ wcom,
         fq += __popcll(fbal);
         __builtin_amdgcn_wave_barrier();
         if (fq >= 64) {   /* wave-uniform: a full wave of far cells */
-            const float4 cj = sRec[3 * lane], a = sRec[3 * lane + 1], b = sRec[3 * lane + 2];
+            const float4* q = sRec + 3 * ((fh + (int)lane) & (GD_FAR_RING - 1));
+            const float4 cj = q[0], a = q[1], b = q[2];
             gd_local_add(L, gx - cj.x, gy - cj.y, gz - cj.z, cj.w, a, b, eps2);
             nFar += 64;
-            const int rest = fq - 64;
-            float4 v0 = zero, v1 = zero, v2 = zero;
-            if ((int)lane < rest) {
-                v0 = sRec[3 * (64 + lane)];
-                v1 = sRec[3 * (64 + lane) + 1];
-                v2 = sRec[3 * (64 + lane) + 2];
-            }
-            __builtin_amdgcn_wave_barrier();
-            if ((int)lane < rest) {
-                sRec[3 * lane] = v0;
-                sRec[3 * lane + 1] = v1;
-                sRec[3 * lane + 2] = v2;
-            }
-            fq = rest;
+            fh = (fh + 64) & (GD_FAR_RING - 1);
+            fq -= 64;
             __builtin_amdgcn_wave_barrier();
         }
+    };
+    /* chunk 0's records, chunk 1's indices */
+    int iA = index((int)lane), iB = index(64 + (int)lane);
+    float4 cA, qA0, qA1, cB, qB0, qB1;
+    gather(iA, cA, qA0, qA1);
+    for (int c0 = 0; c0 < n; c0 += 128) {
+        /* indices of chunk c0 + 128 and records of chunk c0 + 64 (entries >=
+           c0 + 64: the write-backs of chunk c0 go to entries < c0 + 64) */
+        const int iA2 = index(c0 + 128 + (int)lane);
+        gather(iB, cB, qB0, qB1);
+        chunk(iA, cA, qA0, qA1);
+        if (c0 + 64 >= n) break;   /* wave-uniform */
+        const int iB2 = index(c0 + 192 + (int)lane);
+        gather(iA2, cA, qA0, qA1);
+        chunk(iB, cB, qB0, qB1);
+        iA = iA2;
+        iB = iB2;
     }
     return nNear;
 }
@@ -1105,15 +1103,16 @@ __device__ __forceinline__ void gd_far_half(const float4* __restrict__ wcom,
     const unsigned lane = gd_lane();
     gd_local L;
     gd_local_zero(L);
-    int fq = 0, nFar = 0;
+    int fq = 0, fh = 0, nFar = 0;
     /* leaves stay in the per-body loop: they are mostly near, and streaming
        them through the split cost more than the far ones saved */
     nearF = nf;
     nearB = gd_far_region<KEEP>(wcom, wlink, wrad, wmom, ext, list, cap, nb, gx, gy, gz, h2, far2,
-                                eps2, minRatio, sRec, L, fq, nFar);
+                                eps2, minRatio, sRec, L, fq, fh, nFar);
     if (fq > 0) {
         if ((int)lane < fq) {
-            const float4 cj = sRec[3 * lane], a = sRec[3 * lane + 1], b = sRec[3 * lane + 2];
+            const float4* q = sRec + 3 * ((fh + (int)lane) & (GD_FAR_RING - 1));
+            const float4 cj = q[0], a = q[1], b = q[2];
             gd_local_add(L, gx - cj.x, gy - cj.y, gz - cj.z, cj.w, a, b, eps2);
         }
         nFar += fq;
