@@ -1635,6 +1635,9 @@ struct gd_walk_scratch {
     gd_walk_status* hstatus;   /* pinned copy of the last asynchronous walk's status */
     hipEvent_t statusEv;   /* recorded after that copy */
     int statusPending;     /* hstatus not read yet */
+    gd_walk_status* hstatus1;  /* synchronous walks: pinned copy of the status after the
+                                  first pass's traversal */
+    hipEvent_t status1Ev;  /* recorded after that copy */
 };
 
 #define GD_LIST2 262144
@@ -1710,6 +1713,11 @@ static void gd_walk_ensure(int groups) {
         k_walk_fill<<<gd_div_up(groups, 256), 256, 0, gd_stream()>>>(g_walk.sizes, groups, -1);
         GD_LAUNCHED();
         g_walk.groupCap = groups;
+    }
+    if (!g_walk.hstatus1) {
+        GD_HIP(hipHostMalloc((void**)&g_walk.hstatus1, sizeof(gd_walk_status),
+                             hipHostMallocDefault));
+        GD_HIP(hipEventCreateWithFlags(&g_walk.status1Ev, hipEventDisableTiming));
     }
     if (!g_walk.lists) {
         /* first guess: the minimum room for every group; grown to what walks ask for */
@@ -1846,6 +1854,15 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     else GD_PASS1(GD_TR_STACK);
 #undef GD_PASS1
     GD_LAUNCHED();
+    if (!g_walk.async) {
+        /* the status (overflow counts, deepest stack, requested room) is final
+           once the first pass has traversed: its copy to the host travels
+           while the evaluation runs, so the host's wait below leaves the GPU
+           no gap */
+        GD_HIP(hipMemcpyAsync(g_walk.hstatus1, st, sizeof(gd_walk_status), hipMemcpyDeviceToHost,
+                              gd_stream()));
+        GD_HIP(hipEventRecord(g_walk.status1Ev, gd_stream()));
+    }
     if (g_walkTiming) GD_HIP(hipEventRecord(g_walkEv[1], gd_stream()));
     {
         /* one wavefront per run: both halves' lists side by side */
@@ -1934,8 +1951,8 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
         *requested = 0;
         return;
     }
-    gd_walk_status h;
-    GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    GD_HIP(hipEventSynchronize(g_walk.status1Ev));
+    gd_walk_status h = *g_walk.hstatus1;
     if (h.treeOverflow) {
         /* the traversal and evaluation kernels exited at once (basicGravity.cu:282
            stops the run on the same condition) */
