@@ -64,6 +64,16 @@ hipStream_t gd_stream(void) {
     if (!g_stream) GD_HIP(hipStreamCreate(&g_stream));
     return g_stream;
 }
+
+/* A second stream, also with the default flags and priority, for the few
+ * groups a synchronous walk sends past its first pass: their latency-bound
+ * kernels run beside the first pass's evaluation, and the library stream
+ * waits for them (gd_walk.hip). */
+static hipStream_t g_sideStream = nullptr;
+hipStream_t gd_side_stream(void) {
+    if (!g_sideStream) GD_HIP(hipStreamCreate(&g_sideStream));
+    return g_sideStream;
+}
 int gd_capturing(void) { return g_capturing; }
 int gd_graphs_hold_buffers(void) { return g_capturing || g_liveGraphs > 0; }
 

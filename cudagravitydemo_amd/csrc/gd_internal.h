@@ -27,6 +27,7 @@ float gd_walk_split_ratio(float theta);
 
 /* The stream of every launch, asynchronous copy and event of the library. */
 hipStream_t gd_stream(void);
+hipStream_t gd_side_stream(void);
 /* 1 while a gd_graph_begin capture is open. */
 int gd_capturing(void);
 /* 1 while a capture is open or an instantiated graph lives: the buffers a

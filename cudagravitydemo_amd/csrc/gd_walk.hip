@@ -1638,6 +1638,7 @@ struct gd_walk_scratch {
     gd_walk_status* hstatus1;  /* synchronous walks: pinned copy of the status after the
                                   first pass's traversal */
     hipEvent_t status1Ev;  /* recorded after that copy */
+    hipEvent_t sideEv;     /* end of a synchronous walk's passes 2 and 3 (side stream) */
 };
 
 #define GD_LIST2 262144
@@ -1804,8 +1805,8 @@ static void gd_launch_eval(int slots, const gd_builder* b, const float4* pos,
                            const int* groupList, const int* lists, const long long* offsets,
                            const int* caps, const int2* counts, int cap, float3* acc,
                            float* phi, float eps2, float G, int n, const gd_bf16x4* W,
-                           const float* Bv, float minRatio) {
-    k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
+                           const float* Bv, float minRatio, hipStream_t stream) {
+    k_walk_eval<MODE, PHI><<<gd_div_up(slots, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, stream>>>(
         pos, b->wcom, b->wlink, gd_walk_radii(b), b->wmom, b->dinfo, g_walk.slotRange, groupList,
         slots, lists, offsets, caps, counts, cap, acc, phi, eps2, G, n, W, Bv, minRatio,
         g_walk.kstats);
@@ -1969,13 +1970,28 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
     const int slots = over1 < fit ? over1 : (int)fit;
     if (over1 > 0) gd_walk_ensure_pool(slots, cap2);
     if (gd_has_error()) return;
+    if (over1 <= 0 && h.over2 <= 0) return;
+    /* passes 2 and 3 run on the side stream, beside the first pass's
+       evaluation (their groups' bodies are disjoint from its); it starts
+       after the traversal (the status copy above follows it) and the library
+       stream waits for it at the end */
+    hipStream_t side = gd_side_stream();
+    GD_HIP(hipStreamWaitEvent(side, g_walk.status1Ev, 0));
+    struct Join {
+        hipStream_t side;
+        ~Join() {
+            if (!g_walk.sideEv) GD_HIP(hipEventCreateWithFlags(&g_walk.sideEv, hipEventDisableTiming));
+            GD_HIP(hipEventRecord(g_walk.sideEv, side));
+            GD_HIP(hipStreamWaitEvent(gd_stream(), g_walk.sideEv, 0));
+        }
+    } join{side};
     for (int b0 = 0; b0 < over1; b0 += slots) {
         const int cnt = over1 - b0 < slots ? over1 - b0 : slots;
-        k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES, 0, gd_stream()>>>(
+        k_walk_estimate<SPLIT><<<gd_div_up(cnt, GD_TR_WAVES), 64 * GD_TR_WAVES, 0, side>>>(
             pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, cap2, cap2 / 4 < GD_PASS2_FLOOR ? cap2 / 4 : GD_PASS2_FLOOR, g_walk.slotCap);
         GD_LAUNCHED();
-        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64, 0, gd_stream()>>>(
+        k_walk_traverse<SPLIT, 1, GD_TR2_STACK><<<cnt, 64, 0, side>>>(
             pos, b->wcom, b->wlink, wrad, b->dinfo, theta, n, g_walk.slotRange, g_walk.overflow1 + b0,
             cnt, nullptr, nullptr, cap2, g_walk.slotCap, g_walk.pool, g_walk.counts2, g_walk.sizes,
             g_walk.overflow2, &st->over2, nullptr, nullptr, nullptr, g_walk.kstats);
@@ -1985,23 +2001,27 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
                                                  : 1;
         if (segs > 1) {
             k_walk_eval_seg<MODE, PHI>
-                <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, gd_stream()>>>(
+                <<<gd_div_up((long long)cnt * segs, GD_EV_WAVES), 64 * GD_EV_WAVES, 0, side>>>(
                     pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange,
                     g_walk.overflow1 + b0, cnt, segs, g_walk.pool, g_walk.counts2, cap2,
                     g_walk.segPart, eps2, W, Bv, minRatio, g_walk.kstats);
             GD_LAUNCHED();
-            k_walk_eval_sum<PHI><<<gd_div_up(cnt, 4), 256, 0, gd_stream()>>>(g_walk.slotRange,
+            k_walk_eval_sum<PHI><<<gd_div_up(cnt, 4), 256, 0, side>>>(g_walk.slotRange,
                                                              g_walk.overflow1 + b0, cnt, segs,
                                                              g_walk.counts2, g_walk.segPart,
                                                              acc, phi, G);
         } else {
             gd_launch_eval<MODE, PHI>(cnt, b, pos, g_walk.overflow1 + b0, g_walk.pool, nullptr,
                                       nullptr, g_walk.counts2, cap2, acc, phi, eps2, G, n, W,
-                                      Bv, minRatio);
+                                      Bv, minRatio, side);
         }
         GD_LAUNCHED();
     }
-    if (over1 > 0) GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    if (over1 > 0) {   /* the side stream's own status (the null stream would wait for both) */
+        GD_HIP(hipMemcpyAsync(g_walk.hstatus1, st, sizeof(h), hipMemcpyDeviceToHost, side));
+        GD_HIP(hipStreamSynchronize(side));
+        h = *g_walk.hstatus1;
+    }
     const int over2 = h.over2;
     overOut[2] = over2;
     if (over2 <= 0) return;
@@ -2014,15 +2034,17 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
        ones), which the point-based opening test already serves better. */
     constexpr int BMODE = MODE == 2 ? 1 : MODE;
     k_walk_body<BMODE, PHI, GD_BODY_WAVES, GD_BODY_STACK1>
-        <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES, 0, gd_stream()>>>(
+        <<<(unsigned)gd_div_up(bodies, GD_BODY_WAVES), 64 * GD_BODY_WAVES, 0, side>>>(
             pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, g_walk.overflow2,
             nullptr, bodies, g_walk.bodyOver, &st->over3, acc, phi, eps2, G, theta, n,
             g_walk.stats);
     GD_LAUNCHED();
-    GD_HIP(hipMemcpy(&h, st, sizeof(h), hipMemcpyDeviceToHost));
+    GD_HIP(hipMemcpyAsync(g_walk.hstatus1, st, sizeof(h), hipMemcpyDeviceToHost, side));
+    GD_HIP(hipStreamSynchronize(side));
+    h = *g_walk.hstatus1;
     overOut[3] = h.over3;
     if (h.over3 <= 0) return;
-    k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64, 0, gd_stream()>>>(
+    k_walk_body<BMODE, PHI, 1, GD_BODY_STACK><<<(unsigned)h.over3, 64, 0, side>>>(
         pos, b->wcom, b->wlink, wrad, b->wmom, b->dinfo, g_walk.slotRange, nullptr, g_walk.bodyOver,
         h.over3, nullptr, nullptr, acc, phi, eps2, G, theta, n, g_walk.stats);
     GD_LAUNCHED();
