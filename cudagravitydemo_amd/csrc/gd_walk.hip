@@ -371,19 +371,33 @@ __global__ __launch_bounds__(1024) void k_walk_place(const int2* __restrict__ sl
                                                      unsigned long long* __restrict__ total) {
     __shared__ long long sBase, sSlotBase, sSlots;
     __shared__ long long sFlag[16];
-    if (threadIdx.x == 0) {
+    {
+        /* the preceding blocks' rooms and non-empty slots, and all blocks'
+           slots: block-wide sums (one term per thread) */
         long long base = 0, slotBase = 0, slots = 0;
-        for (unsigned k = 0; k < blockIdx.x; ++k) base += blockSums[k];
-        for (unsigned k = 0; k < gridDim.x; ++k) {
+        for (unsigned k = threadIdx.x; k < gridDim.x; k += 1024) {
             const long long c = blockSums[gridDim.x + k];
-            if (k < blockIdx.x) slotBase += c;
+            if (k < blockIdx.x) {
+                base += blockSums[k];
+                slotBase += c;
+            }
             slots += c;
         }
-        sBase = base;
-        sSlotBase = slotBase;
-        sSlots = slots;
-        if (blockIdx.x == gridDim.x - 1)
-            *total = (unsigned long long)(base + blockSums[blockIdx.x]);
+        long long t0, t1, t2;
+        /* one shared buffer: every scan's reads end before the next one writes */
+        gd_block_scan1024(base, sFlag, &t0);
+        __syncthreads();
+        gd_block_scan1024(slotBase, sFlag, &t1);
+        __syncthreads();
+        gd_block_scan1024(slots, sFlag, &t2);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            sBase = t0;
+            sSlotBase = t1;
+            sSlots = t2;
+            if (blockIdx.x == gridDim.x - 1)
+                *total = (unsigned long long)(t0 + blockSums[blockIdx.x]);
+        }
     }
     __syncthreads();
     const int g = blockIdx.x * 1024 + threadIdx.x;
