@@ -65,7 +65,7 @@ int gd_graphs_hold_buffers(void);
 #define GD_SORT_TILE (GD_SORT_THREADS * GD_SORT_ITEMS)
 #define GD_SORT_PASSES 8
 #define GD_SORT_BINS 256
-#define GD_SORT_GROUPS 8            /* tile ranges, one per XCD */
+#define GD_SORT_GROUPS 64           /* tile ranges, eight per XCD */
 
 /* exclusive scan (prefixSum): one tile = 256 threads x 8 items */
 #define GD_SCAN_THREADS 256
