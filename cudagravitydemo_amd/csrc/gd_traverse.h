@@ -97,7 +97,7 @@ __device__ __forceinline__ int gd_split_run(const uint64_t* __restrict__ keys,
 /* One stack entry tested against the box [l, h]. */
 struct gd_probe {
     int idx, nk, first;
-    bool accept, open, leaf;
+    bool accept, open, leaf, mac;   /* mac: the node passes the size test */
 };
 
 /* The opening test: a leaf, or b^2 < theta^2 d^2 with d the distance from the
@@ -115,11 +115,19 @@ __device__ __forceinline__ gd_probe gd_test_node(float4 c, int2 lk, int idx, flo
     const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
     r.idx = idx;
     r.leaf = lk.x < 0;
-    r.accept = r.leaf | (size * size < th2 * d2);   /* no branch: both are cheap */
+    r.mac = size * size < th2 * d2;
+    r.accept = r.leaf | r.mac;   /* no branch: both are cheap */
     r.open = !r.accept;
     r.nk = r.open ? (lk.y >> 5) : 0;
     r.first = lk.x;
     return r;
+}
+
+/* Lanes below this one whose bit is set in m (the hardware's masked bit
+ * count: two instructions, where a popcount of m & lanemask_lt takes four). */
+__device__ __forceinline__ int gd_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 /* Exclusive prefix and total over the wavefront of v in [0, 15]
@@ -153,6 +161,16 @@ __device__ __forceinline__ void gd_wave_scan_dpp(int v, int& excl, int& total) {
     total = __builtin_amdgcn_readlane(s, 63);
 }
 
+/* Nodes each lane tests per step of a traversal (the pop buffer holds
+ * 64 * GD_TR_PER + 8 ints).  Four (256 nodes a step, with the first pass's
+ * stack cut to 896 ranges to keep eight wavefronts per SIMD) measured 2.51 ->
+ * 3.21 ms for the first pass of the 16M-body merger at theta 0.65 and 10.95 ->
+ * 10.50 ms at theta 0.225: at the larger angle few steps fill 256 slots and
+ * the tests of the empty ones cost more than the steps saved. */
+#ifndef GD_TR_PER
+#define GD_TR_PER 2
+#endif
+
 /* Traversal by one wavefront for the box [l, h]: writes the accepted nodes to
  * list[0 .. cap) (leaves / all nodes from the front, cells from the back when
  * SPLIT) unless list is NULL (counting only), and returns false as soon as
@@ -166,13 +184,15 @@ __device__ __forceinline__ void gd_wave_scan_dpp(int v, int& excl, int& total) {
  * later passes walk the group).  A range takes one LDS word where its nodes
  * would take up to eight, which keeps the stack small enough for eight
  * wavefronts per SIMD.  Each step pops up to 64 ranges from the top -- as
- * many as hold at most 128 nodes -- and expands them into pop (128 + 8 ints
- * of LDS); lane l then tests nodes l and 64 + l, so that two node loads per
- * lane are in flight.  Children of the upper nodes go on top: the walk stays
- * depth-first and the stack shallow.  wrad: the nodes' mass radii for the
- * mass-radius opening rule, or NULL for the size rule. */
-template <bool SPLIT>
-__device__ __forceinline__ bool gd_traverse_box(
+ * many as hold at most 64 PER nodes -- and expands them into pop (64 PER + 8
+ * ints of LDS); lane l then tests nodes l, 64 + l, ..., so that PER node
+ * loads per lane are in flight.  The walk is bound by the latency of each
+ * step's chain (stack, pop, node records, test), not by its instructions:
+ * more nodes per step means fewer steps.  Children of the upper nodes go on
+ * top: the walk stays depth-first and the stack shallow.  wrad: the nodes'
+ * mass radii for the mass-radius opening rule (RAD), unused otherwise. */
+template <bool SPLIT, bool RAD, int PER>
+__device__ __forceinline__ bool gd_traverse_rule(
     const float4* __restrict__ wcom, const int2* __restrict__ wlink,
     const float* __restrict__ wrad,
     const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
@@ -181,22 +201,26 @@ __device__ __forceinline__ bool gd_traverse_box(
     const unsigned lane = gd_lane();
     const float ext = info->box.extent;
     const float th2 = theta * theta;
-    const unsigned long long lt = gd_lanemask_lt();
     int nFront = 0, nBack = 0, top = 1, deepest = 1;
     unsigned long long nOpen = 0;
     if (lane == 0) stack[0] = 0;   /* the root: range (0, 1) */
     while (top > 0) {
         /* pop: the top ranges (lane l reads entry top-1-l), as many as hold
-           <= 128 nodes -- at least one, as a range holds at most 8 */
+           <= 64 PER nodes -- at least one, as a range holds at most 8 */
         const int avail = top < 64 ? top : 64;
         const bool has = (int)lane < avail;
         const unsigned rg = has ? (unsigned)stack[top - 1 - (int)lane] : 0u;
         const int cnt = has ? (int)(rg & 7u) + 1 : 0;
         int ex, all;
         gd_wave_scan_dpp(cnt, ex, all);
-        const bool take = has && ex + cnt <= 128;
-        const int nr = __popcll(__ballot(take));
-        const int nn = __shfl(ex + cnt, nr - 1);
+        const bool take = has && ex + cnt <= 64 * PER;
+        /* the taken ranges are a prefix of the lanes: their count from a
+           ballot of the raw compare masked by the first avail lanes (scalar
+           shifts), and the node count nn read from the last taken lane into
+           a scalar register */
+        const unsigned long long availMask = avail >= 64 ? ~0ull : (1ull << avail) - 1ull;
+        const int nr = __popcll(__ballot(ex + cnt <= 64 * PER) & availMask);
+        const int nn = __builtin_amdgcn_readlane(ex + cnt, nr - 1);
         if (take) {
             /* all 8 slots of each range, from the last: a range shorter than 8
                writes past its end into the next ranges' slots (or the 8 ints
@@ -211,54 +235,89 @@ __device__ __forceinline__ bool gd_traverse_box(
         }
         __builtin_amdgcn_wave_barrier();
         top -= nr;
-        /* lanes without a node read the root's records (no branch) and are
-           masked below */
-        const bool hasA = (int)lane < nn, hasB = 64 + (int)lane < nn;
-        const int pa = pop[lane], pb = pop[64 + lane];
-        const int ia = hasA ? pa : 0;
-        const int ib = hasB ? pb : 0;
-        const float4 ca = wcom[ia], cb = wcom[ib];
-        const int2 la = wlink[ia], lb = wlink[ib];
-        const bool useRad = wrad != nullptr;
-        const float ra = useRad ? wrad[ia] : 0.f, rb = useRad ? wrad[ib] : 0.f;
-        gd_probe A = gd_test_node(ca, la, ia, ext, th2, lx, ly, lz, hx, hy, hz, useRad, ra);
-        gd_probe B = gd_test_node(cb, lb, ib, ext, th2, lx, ly, lz, hx, hy, hz, useRad, rb);
-        if (!hasA) A.accept = A.open = false, A.nk = 0;
-        if (!hasB) B.accept = B.open = false, B.nk = 0;
+        /* slot p of lane l holds node 64 p + l of the step; lanes without a
+           node read the root's records (no branch) and are masked below */
+        bool hv[PER];
+        int id[PER];
+        float4 cm[PER];
+        int2 lk[PER];
+        float rd[PER];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            hv[p] = 64 * p + (int)lane < nn;
+            const int pv = pop[64 * p + (int)lane];
+            id[p] = hv[p] ? pv : 0;
+        }
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            cm[p] = wcom[id[p]];
+            lk[p] = wlink[id[p]];
+            rd[p] = RAD ? wrad[id[p]] : 0.f;
+        }
         __builtin_amdgcn_wave_barrier();
-        const bool fA = SPLIT ? (A.accept && A.leaf) : A.accept;
-        const bool fB = SPLIT ? (B.accept && B.leaf) : B.accept;
-        const bool kA = SPLIT && A.accept && !A.leaf;
-        const bool kB = SPLIT && B.accept && !B.leaf;
-        const unsigned long long fba = __ballot(fA), fbb = __ballot(fB);
-        const unsigned long long bba = __ballot(kA), bbb = __ballot(kB);
-        const unsigned long long oba = __ballot(A.open), obb = __ballot(B.open);
-        const int nfa = __popcll(fba), nfb = __popcll(fbb);
-        const int nba = __popcll(bba), nbb = __popcll(bbb);
-        const int noa = __popcll(oba), nob = __popcll(obb);
-        nOpen += (unsigned long long)(noa + nob);
-        deepest = max(deepest, top + noa + nob);
-        const bool wide = (A.open && A.first >= (1 << 29)) || (B.open && B.first >= (1 << 29));
-        if (nFront + nfa + nfb + nBack + nba + nbb > limit || top + noa + nob > stackCap ||
-            __ballot(wide)) {
+        /* The step's lane masks come from ballots of the raw compares (leaf,
+           size test) masked by the slots holding a node (scalar shifts of
+           nn), and are combined by scalar operations; a ballot of a combined
+           per-lane flag would cost two vector instructions more.  The
+           per-lane flags that guard the stores are the same combinations of
+           the same compares. */
+        gd_probe pr[PER];
+        unsigned long long fbm[PER], bbm[PER], obm[PER];
+        int nf[PER], nbk[PER], no[PER];
+        int sumF = 0, sumB = 0, sumO = 0;
+        unsigned long long wide = 0ull;
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            pr[p] = gd_test_node(cm[p], lk[p], id[p], ext, th2, lx, ly, lz, hx, hy, hz, RAD,
+                                 rd[p]);
+            const int left = nn - 64 * p;
+            const unsigned long long hm =
+                left <= 0 ? 0ull : left >= 64 ? ~0ull : (1ull << left) - 1ull;
+            const unsigned long long lf = __ballot(pr[p].leaf) & hm;
+            const unsigned long long ac = (__ballot(pr[p].mac) & hm) | lf;
+            obm[p] = hm & ~ac;
+            fbm[p] = SPLIT ? lf : ac;
+            bbm[p] = SPLIT ? ac & ~lf : 0ull;
+            nf[p] = __popcll(fbm[p]);
+            nbk[p] = __popcll(bbm[p]);
+            no[p] = __popcll(obm[p]);
+            sumF += nf[p];
+            sumB += nbk[p];
+            sumO += no[p];
+            wide |= __ballot(pr[p].first >= (1 << 29)) & obm[p];
+        }
+        nOpen += (unsigned long long)sumO;
+        deepest = max(deepest, top + sumO);
+        if (nFront + sumF + nBack + sumB > limit || top + sumO > stackCap || wide != 0ull) {
             *depth = deepest;
             return false;
         }
         if (list) {
-            if (fA) list[nFront + __popcll(fba & lt)] = A.idx;
-            if (fB) list[nFront + nfa + __popcll(fbb & lt)] = B.idx;
-            if (kA) list[cap - 1 - (nBack + __popcll(bba & lt))] = A.idx;
-            if (kB) list[cap - 1 - (nBack + nba + __popcll(bbb & lt))] = B.idx;
+            int f0 = nFront, b0 = nBack;
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                const bool fl = hv[p] && (SPLIT ? pr[p].leaf : (pr[p].leaf || pr[p].mac));
+                const bool kc = SPLIT && hv[p] && !pr[p].leaf && pr[p].mac;
+                if (fl) list[f0 + gd_rank(fbm[p])] = pr[p].idx;
+                if (kc) list[cap - 1 - (b0 + gd_rank(bbm[p]))] = pr[p].idx;
+                f0 += nf[p];
+                b0 += nbk[p];
+            }
         }
-        nFront += nfa + nfb;
-        nBack += nba + nbb;
-        /* children of the upper nodes (A) go on top */
-        if (B.open)
-            stack[top + __popcll(obb & lt)] = (int)(((unsigned)B.first << 3) | (unsigned)(B.nk - 1));
-        if (A.open)
-            stack[top + nob + __popcll(oba & lt)] =
-                (int)(((unsigned)A.first << 3) | (unsigned)(A.nk - 1));
-        top += noa + nob;
+        nFront += sumF;
+        nBack += sumB;
+        /* children of the upper nodes (lower slots) go on top; an opened
+           node's child count is its link's count field */
+        int below = top;
+#pragma unroll
+        for (int p = PER - 1; p >= 0; --p) {
+            const bool op = hv[p] && !pr[p].leaf && !pr[p].mac;
+            if (op)
+                stack[below + gd_rank(obm[p])] =
+                    (int)(((unsigned)pr[p].first << 3) | (unsigned)((lk[p].y >> 5) - 1));
+            below += no[p];
+        }
+        top += sumO;
         __builtin_amdgcn_wave_barrier();
     }
     *nFrontOut = nFront;
@@ -266,6 +325,25 @@ __device__ __forceinline__ bool gd_traverse_box(
     *opened = nOpen;
     *depth = deepest;
     return true;
+}
+
+/* The traversal with the opening rule fixed per copy (wrad NULL: cell size),
+ * so the node test neither computes both sizes nor branches round the radius
+ * loads. */
+template <bool SPLIT>
+__device__ __forceinline__ bool gd_traverse_box(
+    const float4* __restrict__ wcom, const int2* __restrict__ wlink,
+    const float* __restrict__ wrad,
+    const gd_tree_info* __restrict__ info, float theta, float lx, float ly, float lz, float hx,
+    float hy, float hz, int* stack, int stackCap, int* pop, int* __restrict__ list, int cap,
+    int limit, int* nFrontOut, int* nBackOut, unsigned long long* opened, int* depth) {
+    if (wrad)
+        return gd_traverse_rule<SPLIT, true, GD_TR_PER>(wcom, wlink, wrad, info, theta, lx, ly, lz, hx, hy,
+                                             hz, stack, stackCap, pop, list, cap, limit,
+                                             nFrontOut, nBackOut, opened, depth);
+    return gd_traverse_rule<SPLIT, false, GD_TR_PER>(wcom, wlink, nullptr, info, theta, lx, ly, lz, hx, hy,
+                                          hz, stack, stackCap, pop, list, cap, limit, nFrontOut,
+                                          nBackOut, opened, depth);
 }
 
 #endif

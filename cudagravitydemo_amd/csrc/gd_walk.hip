@@ -154,13 +154,15 @@ __device__ __forceinline__ void gd_kahan_add(float& s, float& c, float v) {
  * stack) and is left to the per-body walk.
  * ------------------------------------------------------------------------- */
 #ifndef GD_TR_STACK
-#define GD_TR_STACK 1024          /* LDS stack ranges per wavefront, first pass */
+#define GD_TR_STACK 1024          /* LDS stack ranges per wavefront, first pass: with the
+                                     pop buffer, 4640 bytes, eight wavefronts per SIMD */
 #endif
 #define GD_TR_WAVES 4
 #define GD_EV_WAVES 4
 #define GD_TR2_STACK 4096         /* second pass, one wavefront per block (16 KB of LDS) */
-#define GD_POP (128 + 8)          /* LDS pop buffer per traversing wavefront: 128 nodes
-                                     and the 8-int slack of the range expansion */
+#define GD_POP (64 * GD_TR_PER + 8)   /* LDS pop buffer per traversing wavefront: the
+                                        step's nodes and the 8-int slack of the range
+                                        expansion */
 #define GD_BODY_STACK1 2048       /* per-body walk, first try (nodes) */
 
 /* Walk statistics: 64 copies of the counters, 128 bytes apart, picked by
@@ -1851,12 +1853,13 @@ static void gd_walk_passes(const gd_builder* b, const uint64_t* keys, const floa
                                     g_walk.blockSums, g_walk.offsets, g_walk.caps,
                                     g_walk.order, &st->requested);
     GD_LAUNCHED();
-    /* The first pass's LDS stack: 1024 ranges per wavefront (8 wavefronts
-       per SIMD), or 2048 (4 per SIMD) while more than 1/16 of the runs needed
-       more than 7/8 of 1024 in the last walk -- small opening angles on deep
-       trees.  A group that overflows either goes to the second pass, which
-       walks one group per wavefront at a fraction of the first pass's rate:
-       worth it for a few groups, not for a large share. */
+    /* The first pass's LDS stack: GD_TR_STACK ranges per wavefront (8
+       wavefronts per SIMD), or twice that (4 per SIMD) while more than 1/16
+       of the runs needed more than 7/8 of GD_TR_STACK in the last walk --
+       small opening angles on deep trees.  A group that overflows either
+       goes to the second pass, which walks one group per wavefront at a
+       fraction of the first pass's rate: worth it for a few groups, not for
+       a large share. */
 #define GD_PASS1(STK)                                                                          \
     k_walk_traverse<SPLIT, GD_TR_WAVES, STK><<<gd_div_up(groups, GD_TR_WAVES),                  \
                                                64 * GD_TR_WAVES, 0, gd_stream()>>>(                             \
